@@ -1,0 +1,185 @@
+/*
+ * hipsph.h — C ABI of the MI355X (gfx950) WCSPH step engine.
+ *
+ * The reference (KoningJasper/Offshore-SPH) has no FFI: its hot path is a set
+ * of numba dispatchers called from Python. Each entry point below replaces one
+ * of those dispatchers and cites it. Host code binds this header through
+ * ctypes (offshore-sph_amd/hipsph/_lib.py).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative SPH_E* code on error;
+ *     sph_last_error() gives a message for the calling thread. No C++
+ *     exception crosses this boundary.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Every call is stream-ordered; calls that return host values synchronise
+ *     that stream and say so.
+ *   - Device pointers are borrowed: the library never frees memory it did not
+ *     allocate. A context owns its workspace (cell-ordered state, keys,
+ *     permutation, cell offsets); step calls allocate nothing.
+ *   - A context is not thread-safe: use one per host thread.
+ *   - All arrays are structure-of-arrays; indices are positions in the
+ *     active-particle array, the reference's `pA[indexes]` order.
+ */
+#ifndef HIPSPH_H
+#define HIPSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define SPH_OK 0
+#define SPH_EINVAL (-1)      /* bad argument */
+#define SPH_EHIP (-2)        /* HIP runtime error */
+#define SPH_ENOMEM (-3)      /* allocation failed */
+#define SPH_ECAPACITY (-4)   /* cell grid larger than the context's capacity */
+#define SPH_ESTATE (-5)      /* call out of order (e.g. compute before nn_update) */
+#define SPH_ENUMERIC (-6)    /* non-finite value met (position, dt) */
+#define SPH_EUNSUPPORTED (-7)
+
+/* ---- device status flags (sph_status) ---- */
+#define SPH_FLAG_CAPACITY 1u     /* cell grid exceeded capacity; step halted */
+#define SPH_FLAG_NONFINITE 2u    /* NaN/Inf position or bound; step halted */
+#define SPH_FLAG_OOB_KEY 4u      /* a particle's flat cell key fell outside [0, n_cells) */
+#define SPH_FLAG_DT_SMALL 8u     /* dt < 1e-6 (src/Solver.py:220-224) */
+#define SPH_FLAG_NO_FLUID 16u    /* dt reduction over zero fluid particles */
+
+/* ---- enums ---- */
+enum { SPH_KERNEL_CUBIC = 0, SPH_KERNEL_GAUSSIAN = 1, SPH_KERNEL_WENDLAND = 2 };
+/* FP64: pair arithmetic in double, neighbours in the reference's order, no FMA
+ * contraction (bit-reproducible against the CPU oracle for the cubic spline).
+ * FP32C: pair arithmetic in float from cell-local offsets, compensated
+ * (Neumaier) sums; integrator, time and dt stay double. */
+enum { SPH_PREC_FP64 = 0, SPH_PREC_FP32C = 1 };
+enum { SPH_INTEGRATOR_PEC = 0, SPH_INTEGRATOR_EULER = 1 };
+enum { SPH_FLUID = 0, SPH_BOUNDARY = 1, SPH_TEMPBOUNDARY = 2, SPH_COUPLED = 3 };
+
+typedef struct sph_ctx sph_ctx_t;
+
+/* Constants of WCSPH (src/Methods/WCSPH.py:35-79), of the integrator
+ * (src/Integrators/PEC.py:13-27), of the neighbour grid
+ * (src/Tools/NNLinkedList.py:23-35, src/Solver.py:109) and of the time step
+ * (src/Solver.py:210-230). */
+typedef struct {
+    int64_t n;               /* active particles */
+    int32_t kernel;          /* SPH_KERNEL_* */
+    int32_t precision;       /* SPH_PREC_* */
+    int32_t integrator;      /* SPH_INTEGRATOR_* */
+    int32_t strict;          /* PEC(strict=...) : clamp rho at 0 */
+    int32_t use_xsph;        /* must be 0 (XSPH is not on this path) */
+    int32_t use_summation_density; /* must be 0 */
+    double rho0, gamma, co, B, Pb, alpha, beta;
+    double r0, D, p1, p2;    /* Lennard-Jones wall force */
+    double gravity;          /* 9.81, subtracted from a_y */
+    double scale;            /* cell size = scale * min h (2.0) */
+    double qmax;             /* neighbour cut-off q <= qmax (3.0) */
+    double gamma_c, gamma_f; /* Courant and force factors (0.25, 0.25) */
+    double damping;          /* PEC damping */
+    double fixed_dt;         /* > 0: use this dt (Solver(timeStep=...)); else dynamic */
+    double duration;         /* device-side stepping stops once t >= duration */
+    int64_t cell_capacity;   /* max cells of the grid; 0 = automatic */
+} sph_params_t;
+
+/* Structure-of-arrays state in the active-particle order. Device pointers. */
+typedef struct {
+    double *x, *y, *vx, *vy, *rho, *ax, *ay, *drho, *m, *h;
+    double *p;     /* download only: Tait pressure + Pb (src/Tools/SolverTools.py:122-127); may be NULL */
+    int8_t *label;
+} sph_state_t;
+
+/* ---- context ---- */
+/* Replaces Solver.__init__ + setup's device side (src/Solver.py:46-141,173-208). */
+int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out);
+int sph_ctx_destroy(sph_ctx_t *ctx);
+/* Update constants that may change between steps (damping, alpha, beta, co,
+ * B, duration, fixed_dt). n, kernel and precision are fixed at creation. */
+int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params);
+
+/* Copy the state in (reference index order) / out (un-permuted back to that
+ * order). Replaces the masked copies `pA[indexes]` / scatter-back of
+ * src/Solver.py:238,254. upload also resets t and the step count when
+ * reset_clock != 0. */
+int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock, void *stream);
+int sph_state_download(sph_ctx_t *ctx, const sph_state_t *st, void *stream);
+
+/* ---- neighbour search: src/Tools/NNLinkedList.py ---- */
+/* NNLinkedList.update (:37-39 -> _init :86-101, _get_cell_size :103-110,
+ * _get_number_of_cells :112-127, _bin :129-141): bounds, grid, counting sort
+ * of cell keys with a prefix scan, reorder of the state into cell order. */
+int sph_nn_update(sph_ctx_t *ctx, void *stream);
+/* Grid of the last update: out_d = {xmin, xmax, ymin, ymax, cell_size},
+ * out_i = {ncx, ncy, n_cells}. Synchronises `stream`. */
+int sph_nn_grid(sph_ctx_t *ctx, double out_d[5], int64_t out_i[3], void *stream);
+/* For parity: NNLinkedList.near(i) (:82-84 -> nearPos :41-80) for every
+ * particle i, as CSR in the reference's order (cells x-shift outer, y-shift
+ * inner; descending index inside a cell), plus each particle's flat key
+ * cx + ncx*cy (:136-137). Device outputs: keys[n], off[n+1], idx[cap],
+ * q[cap], r[cap] (q, r may be NULL). *total receives the entry count; when it
+ * exceeds cap nothing past cap is written and SPH_ECAPACITY is returned.
+ * Synchronises `stream`. */
+int sph_nn_dump(sph_ctx_t *ctx, int64_t *keys, int64_t *off, int32_t *idx, double *q, double *r,
+                int64_t cap, int64_t *total, void *stream);
+
+/* ---- interaction loop: src/Tools/SolverTools.py:120-175 ---- */
+/* One fused pass over the fluid particles: Tait pressure, continuity
+ * (src/Equations/Continuity.py:4-16), momentum with artificial viscosity and
+ * gravity (src/Equations/Momentum.py:5-57, src/Methods/WCSPH.py:151-169) and
+ * the Lennard-Jones wall force (src/Equations/BoundaryForce.py:6-42).
+ * Requires a prior sph_nn_update. */
+int sph_compute(sph_ctx_t *ctx, void *stream);
+
+/* ---- time step and integrators ---- */
+/* Solver._minTimeStep + TimeStep.compute (src/Solver.py:210-230,
+ * src/Equations/TimeStep.py:11-90), evaluated on the device into the
+ * context's control block; also starts a step (t += dt, step += 1) unless
+ * t >= duration, in which case the step's later calls are no-ops. */
+int sph_dt(sph_ctx_t *ctx, void *stream);
+/* PEC.predict (src/Integrators/PEC.py:32-59) on the fluid; no-op for Euler
+ * (src/Integrators/Euler.py:13-15). */
+int sph_predict(sph_ctx_t *ctx, void *stream);
+/* PEC.correct (src/Integrators/PEC.py:61-89) / Euler.correct
+ * (src/Integrators/Euler.py:17-26) on the fluid. */
+int sph_correct(sph_ctx_t *ctx, void *stream);
+/* nsteps iterations of the Solver.run loop body (src/Solver.py:366-410):
+ * dt -> predict -> nn_update -> compute -> correct, all on the device.
+ * use_graph != 0 captures the step in a hipGraph and replays it. */
+int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream);
+
+/* Device clock and flags: out_d = {t, dt, dt_c, dt_f}, out_i = {steps, flags}.
+ * Synchronises `stream`. */
+int sph_status(sph_ctx_t *ctx, double out_d[4], int64_t out_i[2], void *stream);
+/* The dt series of the last `count` steps (dt_a, dt_c, dt_f interleaved) from
+ * the device ring of the most recent SPH_DT_HISTORY steps. Synchronises. */
+#define SPH_DT_HISTORY 65536
+int sph_dt_history(sph_ctx_t *ctx, double *out3, int64_t count, void *stream);
+
+/* ---- element-wise plugin functions on device arrays ---- */
+/* Kernel.evaluate / Kernel.gradient (src/Kernels/Kernel.py:6-12) for
+ * CubicSpline (src/Kernels/CubicSpline.py:10-70), Gaussian
+ * (src/Kernels/Gaussian.py:16-59, incl. derivative :27-37 via
+ * sph_kernel_derivative) and Wendland (src/Kernels/Wendland.py:9-64). */
+int sph_kernel_evaluate(int32_t kernel, int64_t n, const double *r, const double *h, double *out, void *stream);
+int sph_kernel_gradient(int32_t kernel, int64_t n, const double *x, const double *r, const double *h, double *out,
+                        void *stream);
+int sph_kernel_derivative(int32_t kernel, int64_t n, const double *r, const double *h, double *out, void *stream);
+/* TaitEOS (src/Equations/TaitEOS.py:6-31) + Pb. */
+int sph_tait_eos(int64_t n, double gamma, double B, double rho0, double Pb, const double *rho, const int8_t *label,
+                 double *p_out, void *stream);
+
+/* ---- diagnostics ---- */
+const char *sph_last_error(void);
+/* Library build string (compile flags, target). */
+const char *sph_version(void);
+/* Average duration (ms) of the launches of the dominant kernel (the fused
+ * pair pass) since the last reset, measured with HIP events on the stream it
+ * ran on; *count receives the number of timed launches. Timing is enabled by
+ * sph_timing_enable(ctx, 1). */
+int sph_timing_enable(sph_ctx_t *ctx, int32_t on);
+int sph_timing_read(sph_ctx_t *ctx, double *avg_ms_pair, double *avg_ms_nn, int64_t *count, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

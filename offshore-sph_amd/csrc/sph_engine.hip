@@ -1,0 +1,1287 @@
+// sph_engine.hip — neighbour search, integrators, time step and the C ABI.
+//
+// Built with -ffp-contract=off: the neighbour predicate, cell keys, bounds,
+// PEC/Euler updates and the dt reduction are IEEE-exact jitclass code in the
+// reference (no fastmath), so every + - * / sqrt here is one correctly
+// rounded double operation in the reference's order, and the results are
+// bit-identical to it given identical inputs.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sph_internal.h"
+
+using namespace hipsph;
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) return fail(SPH_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct sph_ctx {
+    sph_params_t P;
+    int n = 0;
+    int dev = 0;
+    State buf[2];
+    void *arena[2] = {nullptr, nullptr};
+    int cur = 0;
+    Ctrl *ctrl = nullptr;
+    double *dt_hist = nullptr;
+    int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
+    int2 *cxy_old = nullptr, *cxy_s = nullptr;
+    int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr;
+    long long cap_cells = 0;
+    double *red = nullptr;
+    double *prs = nullptr;
+    float *f32 = nullptr;
+    float *xl = nullptr, *yl = nullptr, *vxf = nullptr, *vyf = nullptr, *rhof = nullptr, *prsf = nullptr,
+          *mf = nullptr, *hf = nullptr;
+    int *slot_of = nullptr;
+    hipStream_t own = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // timing of the pair pass and of the neighbour update
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pair, ev_nn;
+    size_t n_pair = 0, n_nn = 0;
+};
+
+static Phys make_phys(const sph_params_t &P) {
+    Phys ph;
+    ph.rho0 = P.rho0;
+    ph.gamma = P.gamma;
+    ph.B = P.B;
+    ph.Pb = P.Pb;
+    ph.co = P.co;
+    ph.alpha = P.alpha;
+    ph.beta = P.beta;
+    ph.r0 = P.r0;
+    ph.D = P.D;
+    ph.p1 = P.p1;
+    ph.p2 = P.p2;
+    ph.g = P.gravity;
+    ph.qmax = P.qmax;
+    ph.kernel = P.kernel;
+    return ph;
+}
+
+// ---------------------------------------------------------------------------
+// reductions (wavefront shuffles, then LDS across the 4 waves of a block)
+// ---------------------------------------------------------------------------
+template <typename T, typename Op>
+__device__ inline T wave_reduce(T v, Op op) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <typename T, typename Op>
+__device__ inline T block_reduce(T v, Op op, T *lds) {
+    v = wave_reduce(v, op);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = (lane < (int)(blockDim.x >> 6)) ? lds[lane] : Op::identity();
+        v = wave_reduce(v, op);
+    }
+    __syncthreads();
+    return v;  // valid in wave 0
+}
+
+struct MinOp {
+    __device__ static double identity() { return INFINITY; }
+    __device__ double operator()(double a, double b) const { return b < a ? b : a; }
+};
+struct MaxOp {
+    __device__ static double identity() { return -INFINITY; }
+    __device__ double operator()(double a, double b) const { return b > a ? b : a; }
+};
+struct AddOp {
+    __device__ static double identity() { return 0.0; }
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+
+// ---------------------------------------------------------------------------
+// time step: src/Equations/TimeStep.py:58-90 computeVars over the fluid
+// ---------------------------------------------------------------------------
+__global__ void k_dt_partial(const int8_t *label, const double *h, const double *ax, const double *ay, int n,
+                             double *red) {
+    __shared__ double lds[8];
+    double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (label[i] != kFluid) continue;
+        double a2 = ax[i] * ax[i] + ay[i] * ay[i];
+        if (h[i] < mn) mn = h[i];
+        if (a2 > mx) mx = a2;
+        if (!(a2 == a2)) bad += 1.0;
+        cnt += 1.0;
+    }
+    mn = block_reduce(mn, MinOp(), lds);
+    mx = block_reduce(mx, MaxOp(), lds);
+    cnt = block_reduce(cnt, AddOp(), lds);
+    bad = block_reduce(bad, AddOp(), lds);
+    if (threadIdx.x == 0) {
+        red[blockIdx.x * 4 + 0] = mn;
+        red[blockIdx.x * 4 + 1] = mx;
+        red[blockIdx.x * 4 + 2] = cnt;
+        red[blockIdx.x * 4 + 3] = bad;
+    }
+}
+
+// Solver._minTimeStep (src/Solver.py:210-230): courant (TimeStep.py:47-49),
+// force (:51-56), min(c, f) (:45). Also opens the step: the reference's loop
+// condition `while self.t < self.duration` (src/Solver.py:366) and
+// `self.t += self.dt` (:409) move here, so a captured step needs no host.
+__global__ void k_dt_final(Ctrl *ctrl, const double *red, int nb, double gamma_c, double gamma_f, double *hist) {
+    __shared__ double lds[8];
+    double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        mn = MinOp()(mn, red[b * 4 + 0]);
+        mx = MaxOp()(mx, red[b * 4 + 1]);
+        cnt += red[b * 4 + 2];
+        bad += red[b * 4 + 3];
+    }
+    mn = block_reduce(mn, MinOp(), lds);
+    mx = block_reduce(mx, MaxOp(), lds);
+    cnt = block_reduce(cnt, AddOp(), lds);
+    bad = block_reduce(bad, AddOp(), lds);
+    if (threadIdx.x != 0) return;
+    const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID;
+    if ((ctrl->flags & halt) || !(ctrl->t < ctrl->duration)) {
+        ctrl->active = 0;
+        return;
+    }
+    if (cnt == 0.0) {
+        ctrl->flags |= SPH_FLAG_NO_FLUID;
+        ctrl->active = 0;
+        return;
+    }
+    if (bad != 0.0) ctrl->flags |= SPH_FLAG_NONFINITE;
+    double dt, c, f;
+    if (ctrl->fixed_dt > 0.0) {
+        dt = ctrl->fixed_dt;  // src/Solver.py:370-372
+        c = 0.0;
+        f = 0.0;
+    } else {
+        double max_c = ctrl->co;  // c = co for every fluid particle (WCSPH.py:110-129)
+        c = gamma_c * mn / max_c;
+        f = (mx < 1e-12) ? 1e10 : gamma_f * sqrt(mn / mx);
+        dt = (f < c) ? f : c;
+        if (dt < 1e-6) ctrl->flags |= SPH_FLAG_DT_SMALL;  // src/Solver.py:220-224
+    }
+    long long k = ctrl->step % SPH_DT_HISTORY;
+    hist[3 * k + 0] = dt;
+    hist[3 * k + 1] = c;
+    hist[3 * k + 2] = f;
+    ctrl->dt = dt;
+    ctrl->dt_c = c;
+    ctrl->dt_f = f;
+    if (ctrl->advance_t) ctrl->t = ctrl->t + dt;
+    ctrl->step = ctrl->step + 1;
+    ctrl->active = 1;
+}
+
+// ---------------------------------------------------------------------------
+// integrators (fluid only: src/Solver.py:380,396 pass pA[f_indexes])
+// ---------------------------------------------------------------------------
+// src/Integrators/PEC.py:32-59 (useXSPH = False branch)
+__global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
+    if (!ctrl->active) return;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || s.label[j] != kFluid) return;
+    const double dt = ctrl->dt, damping = ctrl->damping;
+    double x = s.x[j], y = s.y[j], vx = s.vx[j], vy = s.vy[j], rho = s.rho[j];
+    s.x0[j] = x;
+    s.y0[j] = y;
+    s.vx0[j] = vx;
+    s.vy0[j] = vy;
+    s.x[j] = x + 0.5 * dt * vx;
+    s.y[j] = y + 0.5 * dt * vy;
+    s.vx[j] = (vx + 0.5 * dt * s.ax[j]) / (1 + 0.5 * damping);
+    s.vy[j] = (vy + 0.5 * dt * s.ay[j]) / (1 + 0.5 * damping);
+    s.rho0[j] = rho;
+    double r = rho + 0.5 * dt * s.drho[j];
+    if (strict && r < 0.0) r = 0.0;
+    s.rho[j] = r;
+}
+
+// src/Integrators/PEC.py:61-89 and src/Integrators/Euler.py:17-26
+__global__ void k_correct(const Ctrl *ctrl, State s, int n, int integrator, int strict) {
+    if (!ctrl->active) return;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || s.label[j] != kFluid) return;
+    const double dt = ctrl->dt, damping = ctrl->damping;
+    if (integrator == SPH_INTEGRATOR_PEC) {
+        double x0 = s.x0[j], y0 = s.y0[j], vx0 = s.vx0[j], vy0 = s.vy0[j], rho0 = s.rho0[j];
+        double mid_x = x0 + 0.5 * dt * s.vx[j];
+        double mid_y = y0 + 0.5 * dt * s.vy[j];
+        double mid_vx = (vx0 + 0.5 * dt * s.ax[j]) / (1 + 0.5 * damping);
+        double mid_vy = (vy0 + 0.5 * dt * s.ay[j]) / (1 + 0.5 * damping);
+        s.x[j] = 2 * mid_x - x0;
+        s.y[j] = 2 * mid_y - y0;
+        s.vx[j] = 2 * mid_vx - vx0;
+        s.vy[j] = 2 * mid_vy - vy0;
+        double mid_rho = rho0 + 0.5 * dt * s.drho[j];
+        double r = 2 * mid_rho - rho0;
+        if (strict && r < 0.0) r = 0.0;
+        s.rho[j] = r;
+    } else {
+        double vx = s.vx[j], vy = s.vy[j], ax = s.ax[j], ay = s.ay[j];
+        s.x[j] = s.x[j] + dt * vx + 0.5 * dt * dt * ax;
+        s.y[j] = s.y[j] + dt * vy + 0.5 * dt * dt * ay;
+        s.vx[j] = vx + dt * ax;
+        s.vy[j] = vy + dt * ay;
+        s.rho[j] = s.rho[j] + dt * s.drho[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour search: src/Tools/NNLinkedList.py
+// ---------------------------------------------------------------------------
+// _init :86-101 bounds and _get_cell_size :103-110 min h, over all active.
+__global__ void k_bounds_partial(const Ctrl *ctrl, const double *x, const double *y, const double *h, int n,
+                                 double *red) {
+    if (!ctrl->active) return;
+    __shared__ double lds[8];
+    double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, bad = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double xi = x[i], yi = y[i], hi = h[i];
+        if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;  // NaN or Inf
+        xmn = MinOp()(xmn, xi);
+        xmx = MaxOp()(xmx, xi);
+        ymn = MinOp()(ymn, yi);
+        ymx = MaxOp()(ymx, yi);
+        hmn = MinOp()(hmn, hi);
+    }
+    xmn = block_reduce(xmn, MinOp(), lds);
+    xmx = block_reduce(xmx, MaxOp(), lds);
+    ymn = block_reduce(ymn, MinOp(), lds);
+    ymx = block_reduce(ymx, MaxOp(), lds);
+    hmn = block_reduce(hmn, MinOp(), lds);
+    bad = block_reduce(bad, AddOp(), lds);
+    if (threadIdx.x == 0) {
+        double *r = red + blockIdx.x * 8;
+        r[0] = xmn;
+        r[1] = xmx;
+        r[2] = ymn;
+        r[3] = ymx;
+        r[4] = hmn;
+        r[5] = bad;
+    }
+}
+
+// _get_cell_size :103-110 and _get_number_of_cells :112-127.
+__global__ void k_bounds_final(Ctrl *ctrl, const double *red, int nb, double scale, long long cap) {
+    if (!ctrl->active) return;
+    __shared__ double lds[8];
+    double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, bad = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double *r = red + b * 8;
+        xmn = MinOp()(xmn, r[0]);
+        xmx = MaxOp()(xmx, r[1]);
+        ymn = MinOp()(ymn, r[2]);
+        ymx = MaxOp()(ymx, r[3]);
+        hmn = MinOp()(hmn, r[4]);
+        bad += r[5];
+    }
+    xmn = block_reduce(xmn, MinOp(), lds);
+    xmx = block_reduce(xmx, MaxOp(), lds);
+    ymn = block_reduce(ymn, MinOp(), lds);
+    ymx = block_reduce(ymx, MaxOp(), lds);
+    hmn = block_reduce(hmn, MinOp(), lds);
+    bad = block_reduce(bad, AddOp(), lds);
+    if (threadIdx.x != 0) return;
+    Grid g;
+    g.xmin = xmn;
+    g.xmax = xmx;
+    g.ymin = ymn;
+    g.ymax = ymx;
+    double cell_size = hmn * scale;
+    if (cell_size < 1e-6) cell_size = 1.0;
+    g.cs = cell_size;
+    double cell_size1 = 1. / cell_size;
+    double fx = ceil(cell_size1 * (xmx - xmn));
+    double fy = ceil(cell_size1 * (ymx - ymn));
+    if (bad != 0.0 || !(fx < 2.0e9) || !(fy < 2.0e9)) {
+        ctrl->flags |= SPH_FLAG_NONFINITE;
+        ctrl->active = 0;
+        ctrl->nn_valid = 0;
+        return;
+    }
+    long long ncx = (long long)fx, ncy = (long long)fy;
+    if (ncx < 1) ncx = 1;
+    if (ncy < 1) ncy = 1;
+    g.ncx = ncx;
+    g.ncy = ncy;
+    g.ncells = ncx * ncy;
+    ctrl->grid = g;
+    if (g.ncells > cap) {
+        ctrl->flags |= SPH_FLAG_CAPACITY;
+        ctrl->active = 0;
+        ctrl->nn_valid = 0;
+    }
+}
+
+// _bin :129-141: cid = floor((x - xmin) / cs) per axis (:146-176), flat key
+// cx + ncx * cy (:200-208). Counts per bin, rank inside the bin from the
+// atomic (the order is fixed afterwards by k_cellsort).
+__global__ void k_keys(Ctrl *ctrl, const double *x, const double *y, int n, int *counts, int *bin, int *rank,
+                       int2 *cxy) {
+    if (!ctrl->active) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Grid g = ctrl->grid;
+    double fx = floor((x[t] - g.xmin) / g.cs);
+    double fy = floor((y[t] - g.ymin) / g.cs);
+    // clamp only for the int conversion; any clamped value is far outside the grid
+    fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
+    fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
+    long long cx = (long long)fx, cy = (long long)fy;
+    long long key = cx + g.ncx * cy;
+    int b;
+    if (key >= 0 && key < g.ncells) {
+        b = (int)key;
+    } else {
+        b = (int)g.ncells;
+        atomicOr(&ctrl->flags, (unsigned)SPH_FLAG_OOB_KEY);
+    }
+    bin[t] = b;
+    rank[t] = atomicAdd(&counts[b], 1);
+    cxy[t] = make_int2((int)cx, (int)cy);
+}
+
+// Exclusive prefix scan of counts[0 .. ncells] into cellStart, in tiles of
+// kScanTile bins: tile sums, a scan of the tile sums, then the tiles.
+__device__ inline int block_scan_excl(int v, int *lds, int *total) {
+    // inclusive scan inside each wave
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int s = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(s, o, 64);
+        if (lane >= o) s += u;
+    }
+    if (lane == 63) lds[w] = s;
+    __syncthreads();
+    int nw = blockDim.x >> 6;
+    int off = 0, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+        int ws = lds[k];
+        if (k < w) off += ws;
+        tot += ws;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + s - v;
+}
+
+__global__ void k_scan_reduce(const Ctrl *ctrl, const int *counts, int *tileSums) {
+    if (!ctrl->active) return;
+    long long nbins = ctrl->grid.ncells + 1;
+    long long ntiles = (nbins + kScanTile - 1) / kScanTile;
+    __shared__ double lds[8];
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        long long base = tile * kScanTile;
+        double s = 0.0;
+        for (int k = 0; k < kScanItems; ++k) {
+            long long b = base + (long long)threadIdx.x * kScanItems + k;
+            if (b < nbins) s += (double)counts[b];
+        }
+        s = block_reduce(s, AddOp(), lds);
+        if (threadIdx.x == 0) tileSums[tile] = (int)s;
+    }
+}
+
+__global__ void k_scan_tiles(const Ctrl *ctrl, int *tileSums) {
+    if (!ctrl->active) return;
+    __shared__ int lds[16];
+    long long nbins = ctrl->grid.ncells + 1;
+    int ntiles = (int)((nbins + kScanTile - 1) / kScanTile);
+    int carry = 0;
+    for (int base = 0; base < ntiles; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int v = i < ntiles ? tileSums[i] : 0;
+        int tot;
+        int ex = block_scan_excl(v, lds, &tot);
+        if (i < ntiles) tileSums[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_apply(const Ctrl *ctrl, int *counts, const int *tileSums, int *cellStart) {
+    if (!ctrl->active) return;
+    long long nbins = ctrl->grid.ncells + 1;
+    long long ntiles = (nbins + kScanTile - 1) / kScanTile;
+    __shared__ int lds[16];
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        long long base = tile * kScanTile;
+        int v[kScanItems];
+        int sum = 0;
+        long long b0 = base + (long long)threadIdx.x * kScanItems;
+        for (int k = 0; k < kScanItems; ++k) {
+            long long b = b0 + k;
+            v[k] = b < nbins ? counts[b] : 0;
+            sum += v[k];
+        }
+        int tot;
+        int ex = block_scan_excl(sum, lds, &tot) + tileSums[tile];
+        for (int k = 0; k < kScanItems; ++k) {
+            long long b = b0 + k;
+            if (b < nbins) {
+                cellStart[b] = ex;
+                counts[b] = 0;  // ready for the next update (no memset node in a graph)
+            }
+            ex += v[k];
+        }
+        if (tile == ntiles - 1 && threadIdx.x == blockDim.x - 1) cellStart[nbins] = ex;
+    }
+}
+
+__global__ void k_scatter(const Ctrl *ctrl, const int *cellStart, const int *bin, const int *rank, const int *id,
+                          int *perm, int *permKey, int n) {
+    if (!ctrl->active) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int pos = cellStart[bin[t]] + rank[t];
+    perm[pos] = t;
+    permKey[pos] = id[t];
+}
+
+// Order each bin by the reference index, which is what the reference's
+// head-inserted lists are ordered by (descending there; the pair pass walks
+// the bin backwards).
+__global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, int *permKey) {
+    if (!ctrl->active) return;
+    long long nbins = ctrl->grid.ncells + 1;
+    for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nbins;
+         b += (long long)gridDim.x * blockDim.x) {
+        int s = cellStart[b], e = cellStart[b + 1];
+        for (int a = s + 1; a < e; ++a) {
+            int k = permKey[a], v = perm[a];
+            int p = a - 1;
+            while (p >= s && permKey[p] > k) {
+                permKey[p + 1] = permKey[p];
+                perm[p + 1] = perm[p];
+                --p;
+            }
+            permKey[p + 1] = k;
+            perm[p + 1] = v;
+        }
+    }
+}
+
+// Tait pressure + Pb (src/Equations/TaitEOS.py:6-31; WCSPH.py:131-149).
+__device__ inline double tait_p(double rho, int8_t label, double gamma, double B, double rho0, double Pb) {
+    double p = 0.0;
+    if (label == kFluid) {
+        double ratio = ipow(rho / rho0, gamma);
+        p = (ratio - 1.0) * B;
+    }
+    return p + Pb;
+}
+
+// Permute the whole state into the new cell order and derive the pair-pass
+// inputs: p / rho^2 per particle (the momentum equation's slf/othr terms,
+// src/Equations/Momentum.py:28,36) and, for FP32C, float copies with
+// positions as offsets from the raw cell origin.
+__global__ void k_reorder(const Ctrl *ctrl, State a, State b, const int *perm, const int2 *cxy_old, int2 *cxy_s,
+                          double *prs, Phys ph, int n, int fp32, float *xl, float *yl, float *vxf, float *vyf,
+                          float *rhof, float *prsf, float *mf, float *hf) {
+    // An inactive step (halted, or past the duration) still flips the
+    // buffers on the host, so copy the state unpermuted.
+    const int act = ctrl->active;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int s = act ? perm[t] : t;
+    double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
+    int8_t lab = a.label[s];
+    b.x[t] = x;
+    b.y[t] = y;
+    b.vx[t] = vx;
+    b.vy[t] = vy;
+    b.rho[t] = rho;
+    b.m[t] = m;
+    b.h[t] = h;
+    b.label[t] = lab;
+    b.id[t] = a.id[s];
+    b.ax[t] = a.ax[s];
+    b.ay[t] = a.ay[s];
+    b.drho[t] = a.drho[s];
+    b.x0[t] = a.x0[s];
+    b.y0[t] = a.y0[s];
+    b.vx0[t] = a.vx0[s];
+    b.vy0[t] = a.vy0[s];
+    b.rho0[t] = a.rho0[s];
+    if (!act) return;
+    int2 c = cxy_old[s];
+    cxy_s[t] = c;
+    double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
+    double pr = p / (rho * rho);
+    prs[t] = pr;
+    if (fp32) {
+        const Grid g = ctrl->grid;
+        double ox = g.xmin + (double)c.x * g.cs;
+        double oy = g.ymin + (double)c.y * g.cs;
+        xl[t] = (float)(x - ox);
+        yl[t] = (float)(y - oy);
+        vxf[t] = (float)vx;
+        vyf[t] = (float)vy;
+        rhof[t] = (float)rho;
+        prsf[t] = (float)pr;
+        mf[t] = (float)m;
+        hf[t] = (float)h;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// upload / download (reference index order <-> cell order)
+// ---------------------------------------------------------------------------
+__global__ void k_upload(State d, sph_state_t in, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d.x[i] = d.x0[i] = in.x[i];
+    d.y[i] = d.y0[i] = in.y[i];
+    d.vx[i] = d.vx0[i] = in.vx[i];
+    d.vy[i] = d.vy0[i] = in.vy[i];
+    d.rho[i] = d.rho0[i] = in.rho[i];
+    d.ax[i] = in.ax ? in.ax[i] : 0.0;
+    d.ay[i] = in.ay ? in.ay[i] : 0.0;
+    d.drho[i] = in.drho ? in.drho[i] : 0.0;
+    d.m[i] = in.m[i];
+    d.h[i] = in.h[i];
+    d.label[i] = in.label[i];
+    d.id[i] = i;
+}
+
+__global__ void k_download(State s, sph_state_t out, Phys ph, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = s.id[t];
+    if (out.x) out.x[i] = s.x[t];
+    if (out.y) out.y[i] = s.y[t];
+    if (out.vx) out.vx[i] = s.vx[t];
+    if (out.vy) out.vy[i] = s.vy[t];
+    if (out.rho) out.rho[i] = s.rho[t];
+    if (out.ax) out.ax[i] = s.ax[t];
+    if (out.ay) out.ay[i] = s.ay[t];
+    if (out.drho) out.drho[i] = s.drho[t];
+    if (out.m) out.m[i] = s.m[t];
+    if (out.h) out.h[i] = s.h[t];
+    if (out.label) out.label[i] = s.label[t];
+    if (out.p) out.p[i] = tait_p(s.rho[t], s.label[t], ph.gamma, ph.B, ph.rho0, ph.Pb);
+}
+
+__global__ void k_mark_nn(Ctrl *ctrl, int v) {
+    if (v == 0 || ctrl->active) ctrl->nn_valid = v;
+}
+
+// ---------------------------------------------------------------------------
+// parity dump: NNLinkedList.near for every particle, reference order
+// ---------------------------------------------------------------------------
+__global__ void k_inverse(const int *id, int *slot_of, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) slot_of[id[t]] = t;
+}
+
+// nearPos (src/Tools/NNLinkedList.py:41-80) for particle i. mode 0 counts,
+// mode 1 writes indices (and q, r) from off[i].
+__global__ void k_nn_dump(const Ctrl *ctrl, State s, const int *cellStart, const int *slot_of, int n, int mode,
+                          long long *keys, long long *off, int *idx, double *qo, double *ro, double qmax) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Grid g = ctrl->grid;
+    int t = slot_of[i];
+    double x = s.x[t], y = s.y[t], h = s.h[t];
+    double fx = floor((x - g.xmin) / g.cs);
+    double fy = floor((y - g.ymin) / g.cs);
+    fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
+    fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
+    long long cx0 = (long long)fx, cy0 = (long long)fy;
+    if (mode == 0) keys[i] = cx0 + g.ncx * cy0;
+    long long k = 0, base = mode ? off[i] : 0;
+    for (int a = -1; a <= 1; ++a) {
+        for (int b = -1; b <= 1; ++b) {
+            long long cx = cx0 + a, cy = cy0 + b;
+            if (!(g.ncx > cx && cx > -1) || !(g.ncy > cy && cy > -1)) continue;
+            long long cell = cx + g.ncx * cy;
+            for (int u = cellStart[cell + 1] - 1; u >= cellStart[cell]; --u) {
+                double dx = x - s.x[u], dy = y - s.y[u];
+                double xij2 = dx * dx + dy * dy;
+                double r = sqrt(xij2);
+                double h_ij = 0.5 * (h + s.h[u]);
+                double q = r / h_ij;
+                if (q <= qmax) {
+                    if (mode) {
+                        idx[base + k] = s.id[u];
+                        if (qo) qo[base + k] = q;
+                        if (ro) ro[base + k] = r;
+                    }
+                    ++k;
+                }
+            }
+        }
+    }
+    if (mode == 0) off[i] = k;
+}
+
+// ---------------------------------------------------------------------------
+// element-wise plugin kernels (Kernel.evaluate/gradient, TaitEOS)
+// ---------------------------------------------------------------------------
+__device__ inline double dev_kernel_w(int kernel, double r, double h) {
+    const double PI = 3.141592653589793;
+    double q = r / h;
+    if (kernel == SPH_KERNEL_GAUSSIAN) {  // src/Kernels/Gaussian.py:16-25
+        if (q <= 3) return (1.0 / PI) / (h * h) * exp(-q * q);
+        return 0.0;
+    }
+    if (kernel == SPH_KERNEL_WENDLAND) {  // src/Kernels/Wendland.py:10-30
+        double alpha = 9.0 / (4.0 * PI) / (h * h);
+        if (q >= 2.0) return 0.0;
+        double inner = 1.0 - 0.5 * q;
+        return alpha * (ipow(inner, 6.0) * (35.0 / 12.0 * q * q + 3.0 * q + 1.0));
+    }
+    double alpha = 10.0 / (7.0 * PI) / (h * h);  // src/Kernels/CubicSpline.py:10-36
+    if (q > 2.0) return 0.0;
+    if (q > 1.0) return alpha * 0.25 * ipow(2.0 - q, 3.0);
+    return alpha * (1.0 - 1.5 * ipow(q, 2.0) * (1.0 - 0.5 * q));
+}
+
+__device__ inline double dev_kernel_g(int kernel, double x, double r, double h) {
+    const double PI = 3.141592653589793;
+    if (kernel == SPH_KERNEL_GAUSSIAN) {  // src/Kernels/Gaussian.py:39-60
+        double tmp = r * h;
+        if (tmp > 1e-12) {
+            double q = r / h;
+            if (q <= 3) {
+                double alpha_c = (1.0 / PI) / (h * h);
+                double dwdq = -2 * q * alpha_c * exp(-q * q);
+                return dwdq / tmp * x;
+            }
+        }
+        return 0.0;
+    }
+    double q = r / h;
+    if (kernel == SPH_KERNEL_WENDLAND) {  // src/Kernels/Wendland.py:32-64
+        double alpha = 9.0 / (4.0 * PI) / (h * h);
+        if ((q >= 2.0) || (r < 1e-10)) return 0.0;
+        double inner = 1.0 - 0.5 * q;
+        double grad = ipow(inner, 5.0) * (-14.0 / 3.0) * q * (1 + 2.5 * q);
+        return alpha * grad / (h * r) * x;
+    }
+    double alpha = 10.0 / (7.0 * PI) / (h * h);  // src/Kernels/CubicSpline.py:38-70
+    if ((q > 2.0) || (r < 1e-10)) return 0.0;
+    double grad = (q > 1.0) ? -0.75 * ipow(2.0 - q, 2.0) : -3.0 * q * (1.0 - 0.75 * q);
+    double norm = alpha * grad / (h * r);
+    return norm * x;
+}
+
+__global__ void k_kernel_eval(int kernel, int n, const double *r, const double *h, double *out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = dev_kernel_w(kernel, r[j], h[j]);
+}
+
+__global__ void k_kernel_grad(int kernel, int n, const double *x, const double *r, const double *h, double *out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = dev_kernel_g(kernel, x[j], r[j], h[j]);
+}
+
+// src/Kernels/Gaussian.py:27-37
+__global__ void k_kernel_deriv(int n, const double *r, const double *h, double *out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double PI = 3.141592653589793;
+    double q = r[j] / h[j];
+    double v = 0.0;
+    if (q <= 3) v = -2 * q * ((1.0 / PI) / (h[j] * h[j])) * exp(-q * q);
+    out[j] = v;
+}
+
+__global__ void k_tait(int n, double gamma, double B, double rho0, double Pb, const double *rho, const int8_t *label,
+                       double *p) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) p[j] = tait_p(rho[j], label[j], gamma, B, rho0, Pb);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+static inline int nblk(long long n, int bs = kBlock) {
+    long long b = (n + bs - 1) / bs;
+    return (int)(b < 1 ? 1 : b);
+}
+
+static inline int redblk(long long n) {
+    int b = nblk(n);
+    return b > kMaxRedBlocks ? kMaxRedBlocks : b;
+}
+
+static hipStream_t S(void *s) { return (hipStream_t)s; }
+
+static void free_ctx(sph_ctx *c) {
+    if (!c) return;
+    for (int k = 0; k < 2; ++k) {
+        if (c->arena[k]) hipFree(c->arena[k]);
+        if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
+    }
+    void *ptrs[] = {c->ctrl, c->dt_hist, c->bin, c->rank, c->perm, c->permKey, c->cxy_old, c->cxy_s,
+                    c->counts, c->cellStart, c->tileSums, c->red, c->prs, c->f32, c->slot_of};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto e : c->ev_pair) hipEventDestroy(e);
+    for (auto e : c->ev_nn) hipEventDestroy(e);
+    if (c->own) hipStreamDestroy(c->own);
+    delete c;
+}
+
+static int alloc_cells(sph_ctx *c, long long cap) {
+    if (c->counts) hipFree(c->counts);
+    if (c->cellStart) hipFree(c->cellStart);
+    if (c->tileSums) hipFree(c->tileSums);
+    c->counts = nullptr;
+    c->cellStart = nullptr;
+    c->tileSums = nullptr;
+    long long tiles = (cap + 1 + kScanTile - 1) / kScanTile;
+    HIPCHK(hipMalloc(&c->counts, sizeof(int) * (cap + 1)));
+    HIPCHK(hipMalloc(&c->cellStart, sizeof(int) * (cap + 2)));
+    HIPCHK(hipMalloc(&c->tileSums, sizeof(int) * (tiles + 1)));
+    HIPCHK(hipMemset(c->counts, 0, sizeof(int) * (cap + 1)));
+    HIPCHK(hipMemset(c->cellStart, 0, sizeof(int) * (cap + 2)));
+    c->cap_cells = cap;
+    return 0;
+}
+
+static int check_params(const sph_params_t *P) {
+    if (!P) return fail(SPH_EINVAL, "params is NULL");
+    if (P->n <= 0 || P->n > (1LL << 30)) return fail(SPH_EINVAL, "n=%lld out of range (1 .. 2^30)", (long long)P->n);
+    if (P->kernel < 0 || P->kernel > 2) return fail(SPH_EINVAL, "unknown kernel id %d", P->kernel);
+    if (P->precision != SPH_PREC_FP64 && P->precision != SPH_PREC_FP32C)
+        return fail(SPH_EINVAL, "unknown precision %d", P->precision);
+    if (P->integrator != SPH_INTEGRATOR_PEC && P->integrator != SPH_INTEGRATOR_EULER)
+        return fail(SPH_EINVAL, "unknown integrator %d", P->integrator);
+    if (P->use_xsph || P->use_summation_density)
+        return fail(SPH_EUNSUPPORTED, "XSPH and summation density are not implemented on the device path");
+    if (!(P->scale > 0) || !(P->qmax > 0) || !(P->rho0 > 0))
+        return fail(SPH_EINVAL, "scale, qmax and rho0 must be positive");
+    return 0;
+}
+
+static void ctrl_from_params(Ctrl &h, const sph_params_t &P) {
+    h.duration = P.duration;
+    h.fixed_dt = P.fixed_dt;
+    h.damping = P.damping;
+    h.alpha = P.alpha;
+    h.beta = P.beta;
+    h.co = P.co;
+    h.B = P.B;
+    h.Pb = P.Pb;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *sph_last_error(void) { return g_err.c_str(); }
+
+const char *sph_version(void) {
+    return "hipsph 1.0 gfx950 (engine: -ffp-contract=off; pair fp64: -ffp-contract=off; pair fp32c: "
+           "-ffp-contract=fast)";
+}
+
+int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
+    if (!out) return fail(SPH_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(params);
+    if (rc) return rc;
+    sph_ctx *c = new (std::nothrow) sph_ctx;
+    if (!c) return fail(SPH_ENOMEM, "host allocation failed");
+    c->P = *params;
+    c->n = (int)params->n;
+    const size_t n = (size_t)c->n;
+    hipGetDevice(&c->dev);
+    // Two cell-ordered buffer sets: 15 double arrays + int8 label + int id.
+    const size_t nd = 15;
+    const size_t bytes = n * (nd * sizeof(double) + sizeof(int) + 8) + 1024;
+    for (int k = 0; k < 2; ++k) {
+        if (hipMalloc(&c->arena[k], bytes) != hipSuccess) {
+            free_ctx(c);
+            return fail(SPH_ENOMEM, "device allocation of %zu bytes failed", bytes);
+        }
+        char *p = (char *)c->arena[k];
+        double **fields[] = {&c->buf[k].x,  &c->buf[k].y,   &c->buf[k].vx,  &c->buf[k].vy,  &c->buf[k].rho,
+                             &c->buf[k].ax, &c->buf[k].ay,  &c->buf[k].drho, &c->buf[k].x0, &c->buf[k].y0,
+                             &c->buf[k].vx0, &c->buf[k].vy0, &c->buf[k].rho0, &c->buf[k].m, &c->buf[k].h};
+        for (double **f : fields) {
+            *f = (double *)p;
+            p += n * sizeof(double);
+        }
+        c->buf[k].id = (int *)p;
+        p += n * sizeof(int);
+        c->buf[k].label = (int8_t *)p;
+    }
+    bool ok = hipMalloc(&c->ctrl, sizeof(Ctrl)) == hipSuccess &&
+              hipMalloc(&c->dt_hist, sizeof(double) * 3 * SPH_DT_HISTORY) == hipSuccess &&
+              hipMalloc(&c->bin, sizeof(int) * n) == hipSuccess && hipMalloc(&c->rank, sizeof(int) * n) == hipSuccess &&
+              hipMalloc(&c->perm, sizeof(int) * n) == hipSuccess &&
+              hipMalloc(&c->permKey, sizeof(int) * n) == hipSuccess &&
+              hipMalloc(&c->cxy_old, sizeof(int2) * n) == hipSuccess &&
+              hipMalloc(&c->cxy_s, sizeof(int2) * n) == hipSuccess &&
+              hipMalloc(&c->red, sizeof(double) * 8 * kMaxRedBlocks) == hipSuccess &&
+              hipMalloc(&c->prs, sizeof(double) * n) == hipSuccess &&
+              hipMalloc(&c->slot_of, sizeof(int) * n) == hipSuccess;
+    if (ok && params->precision == SPH_PREC_FP32C) {
+        ok = hipMalloc(&c->f32, sizeof(float) * n * 8) == hipSuccess;
+        if (ok) {
+            float **f[] = {&c->xl, &c->yl, &c->vxf, &c->vyf, &c->rhof, &c->prsf, &c->mf, &c->hf};
+            for (int k = 0; k < 8; ++k) *f[k] = c->f32 + (size_t)k * n;
+        }
+    }
+    if (!ok) {
+        free_ctx(c);
+        return fail(SPH_ENOMEM, "device workspace allocation failed");
+    }
+    // Cell capacity: caller's value, else room for a grid 8x the particle
+    // count (the dam-break tanks use ~0.4 cells per particle).
+    long long cap = params->cell_capacity > 0 ? params->cell_capacity : (long long)n * 8 + 65536;
+    if (cap > (1LL << 31) - 16) cap = (1LL << 31) - 16;
+    if ((rc = alloc_cells(c, cap))) {
+        free_ctx(c);
+        return rc;
+    }
+    Ctrl h;
+    memset(&h, 0, sizeof h);
+    ctrl_from_params(h, c->P);
+    h.advance_t = 1;
+    h.active = 1;
+    if (hipMemcpy(c->ctrl, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->dt_hist, 0, sizeof(double) * 3 * SPH_DT_HISTORY) != hipSuccess) {
+        free_ctx(c);
+        return fail(SPH_EHIP, "control block init failed");
+    }
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        free_ctx(c);
+        return fail(SPH_EHIP, "stream creation failed");
+    }
+    *out = c;
+    return 0;
+}
+
+int sph_ctx_destroy(sph_ctx_t *ctx) {
+    if (!ctx) return 0;
+    hipDeviceSynchronize();
+    free_ctx(ctx);
+    return 0;
+}
+
+int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params) {
+    if (!ctx || !params) return fail(SPH_EINVAL, "NULL argument");
+    if (params->n != ctx->P.n || params->kernel != ctx->P.kernel || params->precision != ctx->P.precision)
+        return fail(SPH_EINVAL, "n, kernel and precision are fixed at creation");
+    int rc = check_params(params);
+    if (rc) return rc;
+    ctx->P = *params;
+    Ctrl h;
+    HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    ctrl_from_params(h, ctx->P);
+    HIPCHK(hipMemcpy(ctx->ctrl, &h, sizeof h, hipMemcpyHostToDevice));
+    for (int k = 0; k < 2; ++k)
+        if (ctx->gexec[k]) {
+            hipGraphExecDestroy(ctx->gexec[k]);
+            ctx->gexec[k] = nullptr;
+        }
+    return 0;
+}
+
+int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock, void *stream) {
+    if (!ctx || !st || !st->x || !st->y || !st->vx || !st->vy || !st->rho || !st->m || !st->h || !st->label)
+        return fail(SPH_EINVAL, "upload needs x, y, vx, vy, rho, m, h and label");
+    hipStream_t s = S(stream);
+    ctx->cur = 0;
+    k_upload<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->buf[0], *st, ctx->n);
+    HIPCHK(hipGetLastError());
+    Ctrl h;
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    if (reset_clock) {
+        h.t = 0.0;
+        h.step = 0;
+        h.dt = h.dt_c = h.dt_f = 0.0;
+    }
+    h.flags = 0;
+    h.active = 1;
+    h.nn_valid = 0;
+    ctrl_from_params(h, ctx->P);
+    HIPCHK(hipMemcpy(ctx->ctrl, &h, sizeof h, hipMemcpyHostToDevice));
+    for (int k = 0; k < 2; ++k)
+        if (ctx->gexec[k]) {
+            hipGraphExecDestroy(ctx->gexec[k]);
+            ctx->gexec[k] = nullptr;
+        }
+    return 0;
+}
+
+int sph_state_download(sph_ctx_t *ctx, const sph_state_t *st, void *stream) {
+    if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
+    k_download<<<nblk(ctx->n), kBlock, 0, S(stream)>>>(ctx->buf[ctx->cur], *st, make_phys(ctx->P), ctx->n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipStream_t s) {
+    if (used >= v.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        v.push_back(e);
+    }
+    HIPCHK(hipEventRecord(v[used++], s));
+    return 0;
+}
+
+static int nn_update(sph_ctx *c, hipStream_t s) {
+    const int n = c->n;
+    State a = c->buf[c->cur];
+    State b = c->buf[1 - c->cur];
+    if (c->timing) {
+        int rc = ev_record(c, c->ev_nn, c->n_nn, s);
+        if (rc) return rc;
+    }
+    int rb = redblk(n);
+    k_bounds_partial<<<rb, kBlock, 0, s>>>(c->ctrl, a.x, a.y, a.h, n, c->red);
+    k_bounds_final<<<1, kBlock, 0, s>>>(c->ctrl, c->red, rb, c->P.scale, c->cap_cells);
+    k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a.x, a.y, n, c->counts, c->bin, c->rank, c->cxy_old);
+    long long tiles_cap = (c->cap_cells + 1 + kScanTile - 1) / kScanTile;
+    int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
+    k_scan_reduce<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums);
+    k_scan_tiles<<<1, 1024, 0, s>>>(c->ctrl, c->tileSums);
+    k_scan_apply<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums, c->cellStart);
+    k_scatter<<<nblk(n), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->bin, c->rank, a.id, c->perm, c->permKey, n);
+    k_cellsort<<<redblk(c->cap_cells + 1), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->perm, c->permKey);
+    k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_old, c->cxy_s, c->prs, make_phys(c->P),
+                                         n, c->P.precision == SPH_PREC_FP32C, c->xl, c->yl, c->vxf, c->vyf,
+                                         c->rhof, c->prsf, c->mf, c->hf);
+    k_mark_nn<<<1, 1, 0, s>>>(c->ctrl, 1);
+    HIPCHK(hipGetLastError());
+    if (c->timing) {
+        int rc = ev_record(c, c->ev_nn, c->n_nn, s);
+        if (rc) return rc;
+    }
+    c->cur = 1 - c->cur;
+    return 0;
+}
+
+static int compute(sph_ctx *c, hipStream_t s) {
+    State st = c->buf[c->cur];
+    PairIn in;
+    in.x = st.x;
+    in.y = st.y;
+    in.vx = st.vx;
+    in.vy = st.vy;
+    in.rho = st.rho;
+    in.m = st.m;
+    in.h = st.h;
+    in.prs = c->prs;
+    in.label = st.label;
+    in.cxy = c->cxy_s;
+    in.cellStart = c->cellStart;
+    in.xl = c->xl;
+    in.yl = c->yl;
+    in.vxf = c->vxf;
+    in.vyf = c->vyf;
+    in.rhof = c->rhof;
+    in.prsf = c->prsf;
+    in.mf = c->mf;
+    in.hf = c->hf;
+    PairOut out{st.ax, st.ay, st.drho};
+    if (c->timing) {
+        int rc = ev_record(c, c->ev_pair, c->n_pair, s);
+        if (rc) return rc;
+    }
+    hipError_t e = c->P.precision == SPH_PREC_FP32C ? launch_pair_fp32c(c->ctrl, in, out, make_phys(c->P), c->n, s)
+                                                   : launch_pair_fp64(c->ctrl, in, out, make_phys(c->P), c->n, s);
+    if (e != hipSuccess) return fail(SPH_EHIP, "pair launch: %s", hipGetErrorString(e));
+    if (c->timing) {
+        int rc = ev_record(c, c->ev_pair, c->n_pair, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+static int dt_step(sph_ctx *c, hipStream_t s) {
+    State st = c->buf[c->cur];
+    int rb = redblk(c->n);
+    k_dt_partial<<<rb, kBlock, 0, s>>>(st.label, st.h, st.ax, st.ay, c->n, c->red);
+    k_dt_final<<<1, kBlock, 0, s>>>(c->ctrl, c->red, rb, c->P.gamma_c, c->P.gamma_f, c->dt_hist);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int predict(sph_ctx *c, hipStream_t s) {
+    if (c->P.integrator != SPH_INTEGRATOR_PEC) return 0;  // Euler.predict is a no-op
+    k_pec_predict<<<nblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, c->P.strict);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int correct(sph_ctx *c, hipStream_t s) {
+    k_correct<<<nblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, c->P.integrator, c->P.strict);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int one_step(sph_ctx *c, hipStream_t s) {
+    int rc;
+    if ((rc = dt_step(c, s))) return rc;
+    if ((rc = predict(c, s))) return rc;
+    if ((rc = nn_update(c, s))) return rc;
+    if ((rc = compute(c, s))) return rc;
+    return correct(c, s);
+}
+
+int sph_nn_update(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    return nn_update(ctx, S(stream));
+}
+
+int sph_compute(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    return compute(ctx, S(stream));
+}
+
+int sph_dt(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    return dt_step(ctx, S(stream));
+}
+
+int sph_predict(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    return predict(ctx, S(stream));
+}
+
+int sph_correct(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    return correct(ctx, S(stream));
+}
+
+int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (nsteps < 0) return fail(SPH_EINVAL, "nsteps < 0");
+    hipStream_t s = S(stream);
+    if (!use_graph || ctx->timing) {
+        for (int64_t k = 0; k < nsteps; ++k) {
+            int rc = one_step(ctx, s);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    // Graph path: one captured step per starting buffer (the step flips
+    // `cur`), replayed alternately on the context's own stream, ordered after
+    // and before the caller's stream with events.
+    hipStream_t g = ctx->own;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(hipStreamWaitEvent(g, e0, 0));
+    for (int64_t k = 0; k < nsteps; ++k) {
+        int start = ctx->cur;
+        if (!ctx->gexec[start]) {
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+            int rc = one_step(ctx, g);
+            hipError_t ce = hipStreamEndCapture(g, &graph);
+            if (rc) return rc;
+            if (ce != hipSuccess) return fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
+            ce = hipGraphInstantiate(&ctx->gexec[start], graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ce != hipSuccess) return fail(SPH_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+            ctx->cur = start;  // capture flipped it without running anything
+        }
+        HIPCHK(hipGraphLaunch(ctx->gexec[start], g));
+        ctx->cur = 1 - start;
+    }
+    HIPCHK(hipEventRecord(e1, g));
+    HIPCHK(hipStreamWaitEvent(s, e1, 0));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return 0;
+}
+
+int sph_status(sph_ctx_t *ctx, double out_d[4], int64_t out_i[2], void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    Ctrl h;
+    HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    if (out_d) {
+        out_d[0] = h.t;
+        out_d[1] = h.dt;
+        out_d[2] = h.dt_c;
+        out_d[3] = h.dt_f;
+    }
+    if (out_i) {
+        out_i[0] = h.step;
+        out_i[1] = h.flags;
+    }
+    return 0;
+}
+
+int sph_dt_history(sph_ctx_t *ctx, double *out3, int64_t count, void *stream) {
+    if (!ctx || !out3) return fail(SPH_EINVAL, "NULL argument");
+    if (count < 0 || count > SPH_DT_HISTORY) return fail(SPH_EINVAL, "count out of range");
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    Ctrl h;
+    HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    if (count > h.step) return fail(SPH_EINVAL, "only %lld steps taken", (long long)h.step);
+    std::vector<double> ring(3 * (size_t)SPH_DT_HISTORY);
+    HIPCHK(hipMemcpy(ring.data(), ctx->dt_hist, sizeof(double) * ring.size(), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < count; ++k) {
+        long long step = h.step - count + k;
+        long long slot = step % SPH_DT_HISTORY;
+        for (int c = 0; c < 3; ++c) out3[3 * k + c] = ring[3 * slot + c];
+    }
+    return 0;
+}
+
+int sph_nn_grid(sph_ctx_t *ctx, double out_d[5], int64_t out_i[3], void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    Ctrl h;
+    HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    if (!h.nn_valid) return fail(SPH_ESTATE, "no valid neighbour update (flags=0x%x)", h.flags);
+    if (out_d) {
+        out_d[0] = h.grid.xmin;
+        out_d[1] = h.grid.xmax;
+        out_d[2] = h.grid.ymin;
+        out_d[3] = h.grid.ymax;
+        out_d[4] = h.grid.cs;
+    }
+    if (out_i) {
+        out_i[0] = h.grid.ncx;
+        out_i[1] = h.grid.ncy;
+        out_i[2] = h.grid.ncells;
+    }
+    return 0;
+}
+
+int sph_nn_dump(sph_ctx_t *ctx, int64_t *keys, int64_t *off, int32_t *idx, double *q, double *r, int64_t cap,
+                int64_t *total, void *stream) {
+    if (!ctx || !keys || !off || !total) return fail(SPH_EINVAL, "NULL argument");
+    hipStream_t s = S(stream);
+    int rc = sph_nn_grid(ctx, nullptr, nullptr, stream);
+    if (rc) return rc;
+    const int n = ctx->n;
+    State st = ctx->buf[ctx->cur];
+    k_inverse<<<nblk(n), kBlock, 0, s>>>(st.id, ctx->slot_of, n);
+    k_nn_dump<<<nblk(n), kBlock, 0, s>>>(ctx->ctrl, st, ctx->cellStart, ctx->slot_of, n, 0, (long long *)keys,
+                                          (long long *)off, nullptr, nullptr, nullptr, ctx->P.qmax);
+    HIPCHK(hipGetLastError());
+    std::vector<long long> cnt((size_t)n + 1);
+    HIPCHK(hipMemcpyAsync(cnt.data(), off, sizeof(long long) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    long long acc = 0;
+    for (int i = 0; i < n; ++i) {
+        long long k = cnt[i];
+        cnt[i] = acc;
+        acc += k;
+    }
+    cnt[n] = acc;
+    *total = acc;
+    HIPCHK(hipMemcpyAsync(off, cnt.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, s));
+    if (acc > cap) {
+        HIPCHK(hipStreamSynchronize(s));
+        return fail(SPH_ECAPACITY, "dump needs %lld entries, cap is %lld", acc, (long long)cap);
+    }
+    if (!idx) return fail(SPH_EINVAL, "idx is NULL");
+    k_nn_dump<<<nblk(n), kBlock, 0, s>>>(ctx->ctrl, st, ctx->cellStart, ctx->slot_of, n, 1, nullptr,
+                                          (long long *)off, idx, q, r, ctx->P.qmax);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int sph_kernel_evaluate(int32_t kernel, int64_t n, const double *r, const double *h, double *out, void *stream) {
+    if (n < 0 || n > (1LL << 31) - 1 || kernel < 0 || kernel > 2) return fail(SPH_EINVAL, "bad n or kernel");
+    if (n == 0) return 0;
+    k_kernel_eval<<<nblk(n), kBlock, 0, S(stream)>>>(kernel, (int)n, r, h, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sph_kernel_gradient(int32_t kernel, int64_t n, const double *x, const double *r, const double *h, double *out,
+                        void *stream) {
+    if (n < 0 || n > (1LL << 31) - 1 || kernel < 0 || kernel > 2) return fail(SPH_EINVAL, "bad n or kernel");
+    if (n == 0) return 0;
+    k_kernel_grad<<<nblk(n), kBlock, 0, S(stream)>>>(kernel, (int)n, x, r, h, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sph_kernel_derivative(int32_t kernel, int64_t n, const double *r, const double *h, double *out, void *stream) {
+    if (kernel != SPH_KERNEL_GAUSSIAN) return fail(SPH_EUNSUPPORTED, "derivative exists for the Gaussian only");
+    if (n < 0 || n > (1LL << 31) - 1) return fail(SPH_EINVAL, "bad n");
+    if (n == 0) return 0;
+    k_kernel_deriv<<<nblk(n), kBlock, 0, S(stream)>>>((int)n, r, h, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sph_tait_eos(int64_t n, double gamma, double B, double rho0, double Pb, const double *rho, const int8_t *label,
+                 double *p_out, void *stream) {
+    if (n < 0 || n > (1LL << 31) - 1) return fail(SPH_EINVAL, "bad n");
+    if (n == 0) return 0;
+    k_tait<<<nblk(n), kBlock, 0, S(stream)>>>((int)n, gamma, B, rho0, Pb, rho, label, p_out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sph_timing_enable(sph_ctx_t *ctx, int32_t on) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    ctx->timing = on != 0;
+    ctx->n_pair = ctx->n_nn = 0;
+    return 0;
+}
+
+int sph_timing_read(sph_ctx_t *ctx, double *avg_ms_pair, double *avg_ms_nn, int64_t *count, int32_t reset) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    HIPCHK(hipDeviceSynchronize());
+    double tp = 0.0, tn = 0.0;
+    for (size_t k = 0; k + 1 < ctx->n_pair; k += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pair[k], ctx->ev_pair[k + 1]));
+        tp += ms;
+    }
+    for (size_t k = 0; k + 1 < ctx->n_nn; k += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_nn[k], ctx->ev_nn[k + 1]));
+        tn += ms;
+    }
+    size_t np = ctx->n_pair / 2, nn = ctx->n_nn / 2;
+    if (avg_ms_pair) *avg_ms_pair = np ? tp / np : 0.0;
+    if (avg_ms_nn) *avg_ms_nn = nn ? tn / nn : 0.0;
+    if (count) *count = (int64_t)np;
+    if (reset) ctx->n_pair = ctx->n_nn = 0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// sph_internal.h — layouts shared by the translation units of libhipsph.
+//
+// Device memory layout (one context, n active particles):
+//   * State is structure-of-arrays in CELL ORDER: after every neighbour update
+//     the whole state is permuted so that particles of one grid cell are
+//     contiguous and, inside a cell, sorted by their index in the reference's
+//     active array (`id`). Two such buffer sets exist (ping-pong); `cur` names
+//     the live one. Nothing on the step path reads the reference order; only
+//     upload/download convert.
+//   * Cell offsets: cellStart[0 .. ncells+1], an exclusive prefix sum of the
+//     per-cell counts. Bin `ncells` collects particles whose flat key lies
+//     outside the grid (the reference writes those out of bounds); no query
+//     ever visits it.
+//   * Control block (Ctrl): time, dt, step count, flags and the grid of the
+//     last update live on the device, so a whole step runs without a host
+//     round trip and can be captured in a hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hipsph.h"
+
+namespace hipsph {
+
+constexpr int kFluid = 0;
+constexpr int kBlock = 256;        // threads per block of streaming kernels
+constexpr int kScanItems = 8;      // bins per thread in the cell scan
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr int kMaxRedBlocks = 1024;
+
+struct Grid {
+    double xmin, xmax, ymin, ymax, cs;
+    long long ncx, ncy, ncells;
+};
+
+struct Ctrl {
+    double t, dt, dt_c, dt_f;
+    long long step;
+    int active;          // 1 while the current step may run
+    unsigned flags;      // SPH_FLAG_* bits
+    Grid grid;
+    double duration, fixed_dt, damping;
+    double alpha, beta, co, B, Pb;
+    int advance_t;       // 0 while settling (t does not advance)
+    int nn_valid;        // cell order and offsets match the positions
+};
+
+// One cell-ordered buffer set.
+struct State {
+    double *x, *y, *vx, *vy, *rho, *ax, *ay, *drho, *x0, *y0, *vx0, *vy0, *rho0, *m, *h;
+    int8_t *label;
+    int *id;
+};
+
+// Inputs of the pair pass, cell ordered (written by the reorder kernel).
+struct PairIn {
+    const double *x, *y, *vx, *vy, *rho, *m, *h, *prs;  // prs = p / rho^2 (Tait + Pb)
+    const int8_t *label;
+    const int2 *cxy;          // raw cell coordinates (cx, cy) of each particle
+    const int *cellStart;
+    // FP32C inputs: offsets from the particle's raw cell origin, velocities,
+    // density, p/rho^2, mass and h in float.
+    const float *xl, *yl, *vxf, *vyf, *rhof, *prsf, *mf, *hf;
+};
+
+struct PairOut {
+    double *ax, *ay, *drho;
+};
+
+struct Phys {
+    double rho0, gamma, B, Pb, co, alpha, beta, r0, D, p1, p2, g, qmax;
+    int kernel;
+};
+
+// Launchers implemented in the pair translation units.
+hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, hipStream_t s);
+hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, hipStream_t s);
+
+// Number of blocks for a grid over n items, padded to a multiple of the XCD
+// count so that the block remap below is a bijection.
+constexpr int kXcds = 8;
+inline int xcd_blocks(long long n, int bs) {
+    long long b = (n + bs - 1) / bs;
+    if (b < 1) b = 1;
+    return (int)((b + kXcds - 1) / kXcds * kXcds);
+}
+
+// The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and
+// each XCD has its own L2. Remap so that each XCD walks one contiguous range
+// of logical blocks: neighbouring cells (which share neighbour data) then
+// meet in the same L2.
+__device__ inline int xcd_remap(int b, int nb) {
+    int per = nb / kXcds;
+    return (b % kXcds) * per + (b / kXcds);
+}
+
+// Integer power as a left-to-right product for small integer exponents
+// (what the reference's pow(x, 4.0) etc. evaluate to up to rounding).
+template <typename R>
+__device__ inline R ipow(R x, R p) {
+    if (p >= R(1) && p <= R(16) && p == floor(p)) {
+        int n = (int)p;
+        R r = x;
+        for (int k = 1; k < n; ++k) r = r * x;
+        return r;
+    }
+    if (p == R(0)) return R(1);
+    return pow(x, p);
+}
+
+}  // namespace hipsph

@@ -1,0 +1,232 @@
+"""Device engine: one libhipsph context on one GPU.
+
+PyTorch-ROCm supplies the device buffers the state is uploaded from and
+downloaded to, and the stream every call is ordered on
+(``torch.cuda.current_stream()``). The WCSPH step itself runs in the HIP
+kernels of libhipsph; nothing here computes physics.
+"""
+import ctypes as C
+from dataclasses import dataclass, asdict
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, sph_params_t, sph_state_t
+
+IO_F64 = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p')
+
+KERNELS = {'cubic': _lib.SPH_KERNEL_CUBIC, 'gaussian': _lib.SPH_KERNEL_GAUSSIAN, 'wendland': _lib.SPH_KERNEL_WENDLAND}
+PRECISIONS = {'fp64': _lib.SPH_PREC_FP64, 'fp32c': _lib.SPH_PREC_FP32C}
+INTEGRATORS = {'pec': _lib.SPH_INTEGRATOR_PEC, 'euler': _lib.SPH_INTEGRATOR_EULER}
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.HipSphUnavailable('no GPU visible to torch: the WCSPH engine runs on the device only')
+    return torch
+
+
+@dataclass
+class EngineParams:
+    """Constants of one run (mirrors sph_params_t; defaults are the reference's)."""
+    n: int
+    kernel: str = 'cubic'
+    precision: str = 'fp64'
+    integrator: str = 'pec'
+    strict: bool = False
+    rho0: float = 1000.0
+    gamma: float = 7.0
+    co: float = 0.0
+    B: float = 0.0
+    Pb: float = 0.0
+    alpha: float = 0.01
+    beta: float = 0.0
+    r0: float = 0.0
+    D: float = 0.0
+    p1: float = 4.0
+    p2: float = 2.0
+    gravity: float = 9.81
+    scale: float = 2.0
+    qmax: float = 3.0
+    gamma_c: float = 0.25
+    gamma_f: float = 0.25
+    damping: float = 0.0
+    fixed_dt: float = 0.0
+    duration: float = math.inf
+    cell_capacity: int = 0
+
+    def to_c(self):
+        p = sph_params_t()
+        p.n = int(self.n)
+        p.kernel = KERNELS[self.kernel] if isinstance(self.kernel, str) else int(self.kernel)
+        p.precision = PRECISIONS[self.precision] if isinstance(self.precision, str) else int(self.precision)
+        p.integrator = INTEGRATORS[self.integrator] if isinstance(self.integrator, str) else int(self.integrator)
+        p.strict = int(bool(self.strict))
+        p.use_xsph = 0
+        p.use_summation_density = 0
+        for f in ('rho0', 'gamma', 'co', 'B', 'Pb', 'alpha', 'beta', 'r0', 'D', 'p1', 'p2', 'gravity', 'scale',
+                  'qmax', 'gamma_c', 'gamma_f', 'damping', 'fixed_dt', 'duration'):
+            setattr(p, f, float(getattr(self, f)))
+        p.cell_capacity = int(self.cell_capacity)
+        return p
+
+    @classmethod
+    def from_method(cls, n, method, kernel='cubic', integrator='pec', precision='fp64', **kw):
+        """Take the WCSPH constants from a method object (src/Methods/WCSPH.py:35-79)."""
+        if getattr(method, 'useXSPH', False) or getattr(method, 'useSummationDensity', False):
+            raise NotImplementedError('XSPH and summation density are not on the device path')
+        return cls(n=n, kernel=kernel, integrator=integrator, precision=precision, rho0=method.rho0,
+                   gamma=method.gamma, co=method.co, B=method.B, Pb=method.Pb, alpha=method.alpha,
+                   beta=method.beta, r0=method.r0, D=method.D, p1=method.p1, p2=method.p2, **kw)
+
+
+class Engine:
+    """A libhipsph context plus the torch buffers its state moves through."""
+
+    def __init__(self, params: EngineParams, device='cuda:0'):
+        torch = _torch()
+        _lib.load()
+        self.params = params
+        self.device = torch.device(device)
+        self.n = int(params.n)
+        self._ctx = C.c_void_p()
+        with torch.cuda.device(self.device):
+            call('sph_ctx_create', C.byref(params.to_c()), C.byref(self._ctx))
+            self.io = {k: torch.zeros(self.n, dtype=torch.float64, device=self.device) for k in IO_F64}
+            self.io['label'] = torch.zeros(self.n, dtype=torch.int8, device=self.device)
+
+    # -- plumbing --------------------------------------------------------
+    def _stream(self):
+        import torch
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _state(self, fields=None):
+        st = sph_state_t()
+        for f, _ in sph_state_t._fields_:
+            if fields is None or f in fields:
+                setattr(st, f, self.io[f].data_ptr())
+        return st
+
+    def close(self):
+        if self._ctx:
+            _lib.load().sph_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, **changes):
+        for k, v in changes.items():
+            setattr(self.params, k, v)
+        call('sph_ctx_set_params', self._ctx, C.byref(self.params.to_c()))
+
+    # -- state -----------------------------------------------------------
+    def upload(self, arrays, reset_clock=True):
+        """Copy x, y, vx, vy, rho, m, h, label (and ax, ay, drho if given) in."""
+        import torch
+        need = ('x', 'y', 'vx', 'vy', 'rho', 'm', 'h', 'label')
+        for k in need:
+            if k not in arrays:
+                raise KeyError(f'upload needs {k}')
+        fields = set(need)
+        for k, v in arrays.items():
+            if k not in self.io:
+                continue
+            t = torch.as_tensor(np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v)
+            if t.numel() != self.n:
+                raise ValueError(f'{k} has {t.numel()} entries, engine has n={self.n}')
+            self.io[k].copy_(t.to(self.io[k].dtype).reshape(-1))
+            fields.add(k)
+        call('sph_state_upload', self._ctx, C.byref(self._state(fields)), int(bool(reset_clock)), self._stream())
+
+    def download(self, to_numpy=True):
+        """State in reference index order (plus p = Tait + Pb)."""
+        import torch
+        call('sph_state_download', self._ctx, C.byref(self._state()), self._stream())
+        if not to_numpy:
+            return {k: v.clone() for k, v in self.io.items()}
+        torch.cuda.current_stream(self.device).synchronize()
+        return {k: v.cpu().numpy() for k, v in self.io.items()}
+
+    # -- the step ----------------------------------------------------------
+    def dt(self):
+        call('sph_dt', self._ctx, self._stream())
+
+    def predict(self):
+        call('sph_predict', self._ctx, self._stream())
+
+    def nn_update(self):
+        call('sph_nn_update', self._ctx, self._stream())
+
+    def compute(self):
+        call('sph_compute', self._ctx, self._stream())
+
+    def correct(self):
+        call('sph_correct', self._ctx, self._stream())
+
+    def step(self, nsteps=1, graph=False):
+        call('sph_step', self._ctx, int(nsteps), int(bool(graph)), self._stream())
+
+    def status(self):
+        d = (C.c_double * 4)()
+        i = (C.c_int64 * 2)()
+        call('sph_status', self._ctx, d, i, self._stream())
+        return {'t': d[0], 'dt': d[1], 'dt_c': d[2], 'dt_f': d[3], 'steps': int(i[0]), 'flags': int(i[1])}
+
+    def dt_history(self, count):
+        out = np.zeros((int(count), 3), dtype=np.float64)
+        call('sph_dt_history', self._ctx, out.ctypes.data_as(C.POINTER(C.c_double)), int(count), self._stream())
+        return out
+
+    # -- neighbour search --------------------------------------------------
+    def grid(self):
+        d = (C.c_double * 5)()
+        i = (C.c_int64 * 3)()
+        call('sph_nn_grid', self._ctx, d, i, self._stream())
+        return {'xmin': d[0], 'xmax': d[1], 'ymin': d[2], 'ymax': d[3], 'cell_size': d[4],
+                'ncx': int(i[0]), 'ncy': int(i[1]), 'n_cells': int(i[2])}
+
+    def nn_dump(self, with_qr=True):
+        """CSR of NNLinkedList.near(i) for every i: keys, off, idx, q, r (numpy)."""
+        import torch
+        dev = self.device
+        keys = torch.zeros(self.n, dtype=torch.int64, device=dev)
+        off = torch.zeros(self.n + 1, dtype=torch.int64, device=dev)
+        total = C.c_int64(0)
+        rc = _lib.load().sph_nn_dump(self._ctx, keys.data_ptr(), off.data_ptr(), None, None, None, 0,
+                                     C.byref(total), self._stream())
+        if rc not in (0, -4):
+            _lib.check(rc)
+        cap = max(int(total.value), 1)
+        idx = torch.zeros(cap, dtype=torch.int32, device=dev)
+        q = torch.zeros(cap, dtype=torch.float64, device=dev) if with_qr else None
+        r = torch.zeros(cap, dtype=torch.float64, device=dev) if with_qr else None
+        call('sph_nn_dump', self._ctx, keys.data_ptr(), off.data_ptr(), idx.data_ptr(),
+             q.data_ptr() if with_qr else None, r.data_ptr() if with_qr else None, cap, C.byref(total),
+             self._stream())
+        out = {'keys': keys.cpu().numpy(), 'off': off.cpu().numpy(), 'idx': idx[:total.value].cpu().numpy()}
+        if with_qr:
+            out['q'] = q[:total.value].cpu().numpy()
+            out['r'] = r[:total.value].cpu().numpy()
+        return out
+
+    # -- timing --------------------------------------------------------------
+    def timing_enable(self, on=True):
+        call('sph_timing_enable', self._ctx, int(bool(on)))
+
+    def timing_read(self, reset=True):
+        p = C.c_double()
+        q = C.c_double()
+        k = C.c_int64()
+        call('sph_timing_read', self._ctx, C.byref(p), C.byref(q), C.byref(k), int(bool(reset)))
+        return {'pair_ms': p.value, 'nn_ms': q.value, 'count': int(k.value)}
+
+
+def soa_from_particles(pA, fields=('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'label')):
+    """Structure-of-arrays view of a particle_dtype array (active records only)."""
+    return {f: np.ascontiguousarray(pA[f]) for f in fields}

@@ -1,0 +1,133 @@
+/*
+ * refcpu — CPU oracle for the WCSPH step of KoningJasper/Offshore-SPH.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP engine
+ * (tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it);
+ * the product never links, loads or calls it.
+ *
+ * It is a plain-C, FP64, serial restatement of the reference algorithm. It
+ * keeps the reference's data layout (the 154-byte `particle_dtype` record and
+ * the 113-byte `computed_dtype` neighbour record, both packed exactly like
+ * numpy's unaligned structured dtypes) and its cost structure (a head/next
+ * linked-list grid rebuilt every step, one neighbour vector per query, a
+ * gathered record per neighbour, per-axis kernel gradients, a serial loop).
+ * Every function cites the reference file:line it restates.
+ *
+ * Floating point: compiled with -ffp-contract=off and without fast-math, so
+ * every + - * / sqrt is one IEEE-754 double operation in the order the
+ * reference writes it. Integer powers that the reference evaluates with pow()
+ * (Tait exponent 7, Lennard-Jones exponents 4 and 2, (2-q)^2) are evaluated as
+ * left-to-right products when the exponent is a small integer (ref_ipow);
+ * the numba reference runs those call sites with fastmath, so they are only
+ * reproducible to a few ulp there anyway.
+ */
+#ifndef REFCPU_H
+#define REFCPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#pragma pack(push, 1)
+/* src/Common.py:26-57 particle_dtype (154 bytes, field order preserved). */
+typedef struct {
+    uint8_t deleted;
+    int8_t label;
+    double m, rho, p, c, drho, h, x, y, vx, vy, ax, ay, xsphx, xsphy, x0, y0, vx0, vy0, rho0;
+} ref_particle_t;
+
+/* src/Common.py:60-98 computed_dtype (113 bytes, field order preserved). */
+typedef struct {
+    int8_t label;
+    double m, p, rho, h, q, c, r, w, dw_x, dw_y, x, y, vx, vy;
+} ref_computed_t;
+#pragma pack(pop)
+
+enum { REF_FLUID = 0, REF_BOUNDARY = 1, REF_TEMPBOUNDARY = 2, REF_COUPLED = 3 };
+enum { REF_KERNEL_CUBIC = 0, REF_KERNEL_GAUSSIAN = 1 };
+enum { REF_INTEGRATOR_PEC = 0, REF_INTEGRATOR_EULER = 1 };
+
+/* src/Methods/WCSPH.py:8-80 — the WCSPH jitclass fields. */
+typedef struct {
+    int32_t useXSPH, useSummationDensity;
+    double height, rho0, r0, D, p1, p2, epsilon, gamma, co, B, Pb, alpha, beta;
+} ref_wcsph_t;
+
+/* src/Tools/NNLinkedList.py:7-20 — the NNLinkedList jitclass fields. */
+typedef struct {
+    double scale;
+    double xmin, xmax, ymin, ymax, cell_size;
+    int64_t ncx, ncy, n_cells;
+    int64_t *heads, *nexts;
+    int64_t cap_cells, cap_n;
+    int64_t oob_binned; /* particles whose flat key fell outside [0, n_cells) */
+} ref_nn_t;
+
+/* Solver-level loop state (src/Solver.py:29-44,366-410) for the parity harness. */
+typedef struct {
+    int32_t kernel;      /* REF_KERNEL_* */
+    int32_t integrator;  /* REF_INTEGRATOR_* */
+    int32_t pec_useXSPH, pec_strict;
+    double h_fixed;      /* Solver(h=...) ; <= 0 means dynamic h (computeH) */
+    double damping;
+    double gamma_c, gamma_f;
+    double t;
+    int64_t t_step;
+    int32_t nthreads;    /* 1 = serial like the reference; >1 = OpenMP over fluid */
+} ref_solver_t;
+
+/* --- L1: kernels (src/Kernels/CubicSpline.py, src/Kernels/Gaussian.py) --- */
+double ref_cubic_evaluate(double r, double h);
+double ref_cubic_gradient(double x, double r, double h);
+double ref_gaussian_evaluate(double r, double h);
+double ref_gaussian_derivative(double r, double h);
+double ref_gaussian_gradient(double x, double r, double h);
+void ref_kernel_evaluate_n(int32_t kernel, int64_t n, const double *r, const double *h, double *out);
+void ref_kernel_gradient_n(int32_t kernel, int64_t n, const double *x, const double *r, const double *h, double *out);
+double ref_ipow(double x, double p);
+
+/* --- L1: equations --- */
+double ref_tait_eos(double gamma, double B, double rho0, double rho, int8_t label);
+double ref_tait_co(double H);
+double ref_tait_B(double co, double rho0, double gamma);
+double ref_tait_height(double rho0, double H, double B, double gamma, double y);
+double ref_continuity(const ref_computed_t *comp, int64_t J);
+void ref_momentum(double alpha, double beta, const ref_particle_t *p, const ref_computed_t *comp, int64_t J, double out[2]);
+void ref_boundary_force(double r0, double D, double p1, double p2, const ref_computed_t *comp, int64_t J, double out[2]);
+void ref_timestep(int64_t J, const ref_particle_t *pA, double gamma_c, double gamma_f, double out[3]);
+
+/* --- L4: method --- */
+void ref_wcsph_init(ref_wcsph_t *m, double height, double r0, double rho0, int32_t useXSPH, double Pb, int32_t useSD);
+
+/* --- L5: integrators (src/Integrators/PEC.py, Euler.py) --- */
+void ref_pec_predict(int32_t useXSPH, int32_t strict, double dt, ref_particle_t *pA, int64_t J, double damping);
+void ref_pec_correct(int32_t useXSPH, int32_t strict, double dt, ref_particle_t *pA, int64_t J, double damping);
+void ref_euler_correct(double dt, ref_particle_t *pA, int64_t J, double damping);
+
+/* --- L2: neighbour search (src/Tools/NNLinkedList.py) --- */
+void ref_nn_init(ref_nn_t *nn, double scale);
+void ref_nn_free(ref_nn_t *nn);
+int ref_nn_update(ref_nn_t *nn, const ref_particle_t *pA, int64_t n);
+int64_t ref_nn_near_pos(const ref_nn_t *nn, double x, double y, double h, const ref_particle_t *pA,
+                        double *h_out, double *q_out, double *r_out, int64_t *idx_out, int64_t cap);
+int64_t ref_nn_near(const ref_nn_t *nn, int64_t i, const ref_particle_t *pA,
+                    double *h_out, double *q_out, double *r_out, int64_t *idx_out, int64_t cap);
+int64_t ref_nn_key(const ref_nn_t *nn, double x, double y);
+/* CSR dump of near(i) for every i, for parity. Returns total entries or -needed when cap too small. */
+int64_t ref_nn_dump(const ref_nn_t *nn, const ref_particle_t *pA, int64_t n, int64_t *keys,
+                    int64_t *off, int64_t *idx, double *q, double *r, int64_t cap);
+
+/* --- L3: interaction loop (src/Tools/SolverTools.py:120-175) --- */
+int ref_loop(ref_particle_t *pA, int64_t n, int32_t kernel, const ref_wcsph_t *m, const ref_nn_t *nn, int32_t nthreads);
+
+/* --- L6: one Solver.run loop body (src/Solver.py:366-410, damping given, no settling) --- */
+int ref_solver_step(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn, double dt_out[3]);
+int ref_solver_run(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn,
+                   int64_t nsteps, double *dt_hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,338 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Bars (stated per test):
+* neighbour search: grid, flat keys and every near(i) list bit-exact,
+  including order, q and r;
+* FP64 pair pass, cubic spline: bit-exact (same operations, same order, no
+  contraction on either side); Gaussian and Wendland: within 1e-12 of the
+  largest magnitude (exp/pow come from different math libraries);
+* FP32C pair pass: within 2e-5 of the largest magnitude per field;
+* multi-step trajectories: FP64 cubic bit-exact after 1,000 steps.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from gpu_util import FIELDS, make_engine, oracle_loop, perturbed, rel_err, say
+from hipsph import scenarios
+from hipsph.Kernels.CubicSpline import CubicSpline
+from hipsph.Kernels.Gaussian import Gaussian
+from hipsph.Kernels.Wendland import Wendland
+
+pytestmark = pytest.mark.gpu
+
+
+# --------------------------------------------------------------------------
+# element-wise plugin kernels
+# --------------------------------------------------------------------------
+def test_kernels_match_oracle_and_golden():
+    rng = np.random.default_rng(0)
+    n = 20000
+    h = rng.uniform(0.01, 0.05, n)
+    r = rng.uniform(0, 3.2, n) * h
+    r[:50] = 0.0
+    x = rng.uniform(-1, 1, n) * r
+    for K, name, exact in ((CubicSpline(), 'cubic', True), (Wendland(), 'wendland', True),
+                           (Gaussian(), 'gaussian', False)):
+        say('kernel', name)
+        w = K.evaluate(r, h)
+        g = K.gradient(x, r, h)
+        wr = oracle.kernel_evaluate(name, r, h)
+        gr = oracle.kernel_gradient(name, x, r, h)
+        if exact:
+            np.testing.assert_array_equal(w, wr)
+            np.testing.assert_array_equal(g, gr)
+        else:
+            np.testing.assert_allclose(w, wr, rtol=1e-14, atol=0)
+            np.testing.assert_allclose(g, gr, rtol=1e-14, atol=0)
+    with open(os.path.join(GOLDEN, 'cubic_kat.json')) as f:
+        kat = json.load(f)
+    K = CubicSpline()
+    for e in kat['evaluate']:
+        assert K.evaluate([e['r']], [e['h']])[0] == pytest.approx(e['w'], rel=1e-14, abs=1e-7)
+    for e in kat['gradient']:
+        assert K.gradient([e['x']], [e['r']], [e['h']])[0] == pytest.approx(e['g'], rel=1e-14, abs=1e-7)
+    z = np.load(os.path.join(GOLDEN, 'gaussian_pysph.npz'))['data']
+    G = Gaussian()
+    np.testing.assert_allclose(G.evaluate(z[:, 2], z[:, 3]), z[:, 4], rtol=1e-13, atol=1e-7)
+    np.testing.assert_allclose(G.derivative(z[:, 2], z[:, 3]), z[:, 5], rtol=1e-13, atol=1e-7)
+    np.testing.assert_allclose(G.gradient(z[:, 0], z[:, 2], z[:, 3]), z[:, 6], rtol=1e-13, atol=1e-7)
+
+
+def test_tait_matches_oracle_and_golden():
+    from hipsph.Equations.TaitEOS import TaitEOS
+    z = np.load(os.path.join(GOLDEN, 'tait_kat.npz'))
+    gamma, B, rho0, _ = z['params']
+    lab = np.zeros(len(z['rho']), np.int8)
+    lab[::7] = 1
+    p = TaitEOS(gamma, B, rho0, z['rho'], lab)
+    np.testing.assert_array_equal(p, oracle.tait(gamma, B, rho0, z['rho'], lab))
+    f = lab == 0
+    np.testing.assert_allclose(p[f], z['p'][f], rtol=1e-12, atol=1e-3)
+
+
+# --------------------------------------------------------------------------
+# neighbour search
+# --------------------------------------------------------------------------
+def _nn_parity(pA, method, scale=2.0):
+    e = make_engine(pA, method, scale=scale)
+    e.nn_update()
+    g = e.grid()
+    d = e.nn_dump()
+    nn = oracle.NN(scale)
+    a = oracle.as_particles(pA)
+    nn.update(a)
+    gr = nn.grid()
+    for k in ('xmin', 'xmax', 'ymin', 'ymax', 'cell_size', 'ncx', 'ncy', 'n_cells'):
+        assert g[k] == gr[k], k
+    dr = nn.dump(a)
+    np.testing.assert_array_equal(d['keys'], dr['keys'])
+    np.testing.assert_array_equal(d['off'], dr['off'])
+    np.testing.assert_array_equal(d['idx'], dr['idx'])
+    np.testing.assert_array_equal(d['q'], dr['q'])
+    np.testing.assert_array_equal(d['r'], dr['r'])
+    e.close()
+    return d
+
+
+def test_nn_dam_break_c1():
+    sc = scenarios.dam_break_c1()
+    say('nn c1', sc.n)
+    d = _nn_parity(sc.pA, sc.method())
+    f = sc.pA['label'] == 0
+    cnt = np.diff(d['off'])
+    assert cnt[f].min() >= 1 and 35 < cnt[f].mean() < 50
+
+
+@pytest.mark.parametrize('seed', [1, 2])
+def test_nn_random_mixed_h(seed):
+    rng = np.random.default_rng(seed)
+    n = 3000
+    sc = scenarios.dam_break_c1()
+    pA = np.zeros(n, dtype=sc.pA.dtype)
+    pA['x'] = rng.uniform(0, 1.0, n)
+    pA['y'] = rng.uniform(0, 0.7, n)
+    pA['h'] = rng.uniform(0.01, 0.02, n)
+    pA['rho'] = 1000.0
+    pA['m'] = 0.1
+    pA['label'] = (rng.uniform(0, 1, n) < 0.1).astype(np.int8)
+    say('nn random', seed)
+    _nn_parity(pA, sc.method())
+
+
+def test_nn_edge_alias_and_out_of_grid():
+    """x on the far edge gives cx = ncx: the key aliases to cell (0, cy+1); the corner particle
+    gets a key past the grid and is seen by no query (the reference writes it out of bounds)."""
+    sc = scenarios.dam_break_c1()
+    x = np.array([0.0, 1.5, 2.5, 4.0, 0.2, 4.0, 3.9])
+    y = np.array([0.0, 0.5, 1.5, 0.5, 1.6, 2.0, 1.9])
+    pA = np.zeros(len(x), dtype=sc.pA.dtype)
+    pA['x'], pA['y'], pA['h'], pA['rho'], pA['m'] = x, y, 0.5, 1000.0, 1.0
+    say('nn alias')
+    d = _nn_parity(pA, sc.method())
+    assert d['keys'][3] == 4          # (4, 0) -> cell (0, 1)
+    assert d['keys'][5] == 4 + 4 * 2  # (4, 2) -> past the 4 x 2 grid
+
+
+def test_nn_scale3_kat():
+    """test/test_linked_list.py contract through the device search (scale 3)."""
+    with open(os.path.join(GOLDEN, 'linked_list_kat.json')) as f:
+        kat = json.load(f)
+    sc = scenarios.dam_break_c1()
+    xv = np.linspace(0, 25, kat['N'] + 1)
+    X, Y = np.meshgrid(xv, xv, indexing='ij')
+    pA = np.zeros(X.size, dtype=sc.pA.dtype)
+    pA['x'], pA['y'], pA['h'], pA['rho'], pA['m'] = X.ravel(), Y.ravel(), kat['h'], 1000.0, 1.0
+    d = _nn_parity(pA, sc.method(), scale=kat['scale'])
+    assert sorted(d['idx'][d['off'][0]:d['off'][1]].tolist()) == kat['near0']
+
+
+def test_nn_stress_lattice_small():
+    sc = scenarios.stress_lattice(nside=300)
+    say('nn stress 300^2', sc.n)
+    _nn_parity(sc.pA, sc.method())
+
+
+# --------------------------------------------------------------------------
+# fused pair pass
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
+def test_compute_fp64_vs_oracle(kernel):
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 3)
+    m = sc.method()
+    say('compute fp64', kernel)
+    e = make_engine(pA, m, kernel=kernel)
+    e.nn_update()
+    e.compute()
+    out = e.download()
+    ref, _ = oracle_loop(pA, m, kernel)
+    f = pA['label'] == 0
+    for k in ('drho', 'ax', 'ay'):
+        if kernel == 'cubic':
+            np.testing.assert_array_equal(out[k][f], ref[k][f], err_msg=k)
+        else:
+            assert rel_err(out[k], ref[k], f) < 1e-12, k
+    # walls are not touched by the pass
+    np.testing.assert_array_equal(out['ax'][~f], pA['ax'][~f])
+    e.close()
+
+
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian'])
+def test_compute_fp32c_vs_oracle(kernel):
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 4)
+    m = sc.method()
+    say('compute fp32c', kernel)
+    e = make_engine(pA, m, kernel=kernel, precision='fp32c')
+    e.nn_update()
+    e.compute()
+    out = e.download()
+    ref, _ = oracle_loop(pA, m, kernel)
+    f = pA['label'] == 0
+    errs = {k: rel_err(out[k], ref[k], f) for k in ('drho', 'ax', 'ay')}
+    say('fp32c rel errors', kernel, errs)
+    for k, v in errs.items():
+        assert v < 2e-5, (k, v)
+    e.close()
+
+
+def test_compute_stress_lattice_fp64():
+    sc = scenarios.stress_lattice(nside=200)
+    m = sc.method()
+    say('compute stress 200^2')
+    e = make_engine(sc.pA, m)
+    e.nn_update()
+    e.compute()
+    out = e.download()
+    ref, _ = oracle_loop(sc.pA, m, 'cubic')
+    f = sc.pA['label'] == 0
+    for k in ('drho', 'ax', 'ay'):
+        np.testing.assert_array_equal(out[k][f], ref[k][f], err_msg=k)
+    e.close()
+
+
+# --------------------------------------------------------------------------
+# whole steps
+# --------------------------------------------------------------------------
+def _run_both(sc, steps, kernel='cubic', integrator='pec', precision='fp64', graph=False):
+    m = sc.method()
+    e = make_engine(sc.pA, m, kernel=kernel, integrator=integrator, precision=precision)
+    e.step(steps, graph=graph)
+    st = e.status()
+    out = e.download()
+    hist = e.dt_history(steps)
+    e.close()
+    s = oracle.Solver(sc.pA, m, kernel, integrator, h=sc.h, nthreads=8)
+    rhist = s.run(steps)
+    return out, hist, st, s.pA, rhist, s
+
+
+def test_step_c1_fp64_bitexact_1000():
+    """C1 (dx 0.02 dam break), cubic, PEC, 1,000 steps: every field bit-identical to the oracle."""
+    sc = scenarios.dam_break_c1()
+    say('c1 1000 steps fp64')
+    out, hist, st, ref, rhist, s = _run_both(sc, 1000)
+    assert st['steps'] == 1000 and st['flags'] & 3 == 0
+    np.testing.assert_array_equal(hist, rhist)
+    assert st['t'] == s.t
+    for k in FIELDS:
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+def test_step_euler_and_gaussian():
+    sc = scenarios.dam_break_c1()
+    say('c1 euler 100')
+    out, hist, st, ref, rhist, _ = _run_both(sc, 100, integrator='euler')
+    for k in FIELDS:
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+    say('c1 gaussian 100')
+    out, hist, st, ref, rhist, _ = _run_both(sc, 100, kernel='gaussian')
+    np.testing.assert_allclose(hist, rhist, rtol=1e-9)
+    f = sc.pA['label'] == 0
+    assert np.max(np.abs(out['x'] - ref['x'])) < 1e-9 * sc.dx
+    assert rel_err(out['rho'], ref['rho'], f) < 1e-9
+
+
+def test_graph_replay_matches_eager():
+    """hipGraph replay of the step gives the eager result bit for bit, replay after replay, with
+    eager work and synchronisation between replays."""
+    sc = scenarios.dam_break_c1()
+    m = sc.method()
+    a = make_engine(sc.pA, m)
+    b = make_engine(sc.pA, m)
+    for chunk in range(4):
+        say('graph chunk', chunk)
+        a.step(5, graph=False)
+        b.step(5, graph=True)
+        oa, ob = a.download(), b.download()
+        for k in FIELDS:
+            np.testing.assert_array_equal(ob[k], oa[k], err_msg=f'{k} chunk {chunk}')
+    assert a.status() == b.status()
+    a.close()
+    b.close()
+
+
+def test_fp32c_density_error_after_1000_steps():
+    """Precision form: FP32 pairs + FP64 integrator vs the FP64 engine, C1, 1,000 steps."""
+    sc = scenarios.dam_break_c1()
+    m = sc.method()
+    a = make_engine(sc.pA, m, precision='fp64')
+    b = make_engine(sc.pA, m, precision='fp32c')
+    a.step(1000)
+    b.step(1000)
+    oa, ob = a.download(), b.download()
+    f = sc.pA['label'] == 0
+    drho = np.max(np.abs(ob['rho'][f] - oa['rho'][f]) / oa['rho'][f])
+    dx = np.max(np.hypot(ob['x'][f] - oa['x'][f], ob['y'][f] - oa['y'][f])) / sc.dx
+    say('fp32c after 1000 steps: max rel rho', drho, 'max |dx|/dx', dx)
+    assert drho < 1e-5
+    assert dx < 1e-3
+    a.close()
+    b.close()
+
+
+# --------------------------------------------------------------------------
+# host-forced failure handling (every kernel still exits normally)
+# --------------------------------------------------------------------------
+def test_capacity_flag_halts_step():
+    from hipsph import _lib
+    sc = scenarios.dam_break_c1()
+    e = make_engine(sc.pA, sc.method(), cell_capacity=100)
+    e.step(2)
+    st = e.status()
+    assert st['flags'] & _lib.SPH_FLAG_CAPACITY
+    with pytest.raises(_lib.HipSphError):
+        e.grid()
+    out = e.download()
+    f = sc.pA['label'] == 0
+    assert np.all(np.isfinite(out['x'][f]))
+    e.close()
+
+
+def test_duration_stops_the_clock():
+    sc = scenarios.dam_break_c1()
+    e = make_engine(sc.pA, sc.method(), duration=3e-4)
+    e.step(20)
+    st = e.status()
+    x1 = e.download()['x']
+    e.step(5)
+    x2 = e.download()['x']
+    np.testing.assert_array_equal(x1, x2)
+    assert st['t'] >= 3e-4 and st['steps'] < 20
+    e.close()
+
+
+def test_nonfinite_position_flags():
+    from hipsph import _lib
+    sc = scenarios.dam_break_c1()
+    pA = sc.pA.copy()
+    pA['x'][10] = np.nan
+    e = make_engine(pA, sc.method())
+    e.step(1)
+    assert e.status()['flags'] & _lib.SPH_FLAG_NONFINITE
+    e.close()

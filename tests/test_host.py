@@ -1,0 +1,62 @@
+"""Host-side logic: data model, lattice generator, scenarios, parameter plumbing."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from hipsph import Common, scenarios
+from hipsph.Helpers import Helpers
+from hipsph.Methods.WCSPH import WCSPH
+from hipsph.device import EngineParams
+
+
+def test_dtypes_match_reference_layout():
+    assert Common.particle_dtype.itemsize == 154
+    assert Common.computed_dtype.itemsize == 113
+    assert Common.particle_dtype == oracle.particle_dtype
+    assert Common.computed_dtype == oracle.computed_dtype
+    assert Common.get_label_code('boundary') == Common.ParticleType.Boundary
+    with pytest.raises(Exception):
+        Common.get_label_code('nope')
+
+
+def test_rect_counts_and_pack():
+    """src/Helpers.py:45-51: N = ceil(range / r0) points per axis, inclusive ends."""
+    a = Helpers.rect(0, 1.0, 0, 2.0, 0.02)
+    assert len(a) == 50 * 100
+    assert a['x'].min() == 0 and a['x'].max() == 1.0 and a['h'].max() == 0
+    b = Helpers.rect(0, 1.0, 0, 1.0, 0.25, pack=True)
+    assert len(b) == 16
+    assert np.allclose(b['y'][0:4] - a['y'][0:4] * 0, np.linspace(0, 1, 4) + 0.125)
+    line = Helpers.rect(-0.06, 4.06, -0.02, -0.02, 0.02)
+    assert len(line) == math.ceil(4.12 / 0.02)
+
+
+def test_scenario_sizes():
+    c1 = scenarios.dam_break_c1()
+    assert c1.n_fluid == 5000
+    assert 1200 < c1.n - c1.n_fluid < 1700
+    assert np.all(c1.pA['h'] == 1.3 * 0.02)
+    m = c1.method()
+    f = c1.pA['label'] == 0
+    # hydrostatic density decreases with height, equals rho0 at the free surface y = H
+    top = np.argmax(c1.pA['y'][f])
+    assert c1.pA['rho'][f][top] == pytest.approx(1000.0)
+    assert m.co == pytest.approx(10 * math.sqrt(2 * 9.81 * 2.0))
+
+
+def test_wcsph_constants_match_oracle():
+    m = WCSPH(height=2.0, r0=0.02, rho0=1000.0, useXSPH=False, Pb=0, useSummationDensity=False)
+    r = oracle.wcsph(height=2.0, r0=0.02, rho0=1000.0)
+    for f in ('co', 'B', 'D', 'p1', 'p2', 'gamma', 'alpha', 'beta', 'epsilon'):
+        assert getattr(m, f) == getattr(r, f), f
+
+
+def test_engine_params_to_c():
+    m = WCSPH(2.0, 0.02, 1000.0)
+    p = EngineParams.from_method(123, m, kernel='gaussian', precision='fp32c', integrator='euler').to_c()
+    assert (p.n, p.kernel, p.precision, p.integrator) == (123, 1, 1, 1)
+    assert p.co == m.co and p.B == m.B and p.D == m.D and p.qmax == 3.0 and p.scale == 2.0
+    with pytest.raises(NotImplementedError):
+        EngineParams.from_method(1, WCSPH(2.0, 0.02, 1000.0, useXSPH=True))

@@ -1,0 +1,306 @@
+"""Pinning the composed oracle step where the reference's tests are silent.
+
+The reference has no working test of NNLinkedList (current API), _loop,
+TimeStep or the Solver step. These checks pin the oracle by mathematics and
+by an independent formulation:
+
+* neighbour sets against an O(N^2) brute force of the stated rule
+  (3x3 block of cells of size 2 min h around the query's cell, q <= 3),
+  including the reference's order (x-shift outer, y-shift inner, descending
+  index inside a cell), and completeness of the cubic support (r <= 2h);
+* the fused per-particle sums (continuity, momentum + artificial viscosity +
+  gravity, Lennard-Jones wall force) against a dense all-pairs numpy model;
+* closed forms (two particles), exact zeros (uniform velocity), momentum
+  conservation (antisymmetric pair forces), mirror and rotation symmetry.
+A dropped term, a wrong sign, a swapped x/y or a wrong neighbour rule fails
+at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+G = 9.81
+
+
+def _particles(x, y, h, label=None, vx=None, vy=None, rho=None, m=None):
+    n = len(x)
+    pA = np.zeros(n, dtype=oracle.particle_dtype)
+    pA['x'], pA['y'], pA['h'] = x, y, h
+    pA['label'] = 0 if label is None else label
+    pA['vx'] = 0 if vx is None else vx
+    pA['vy'] = 0 if vy is None else vy
+    pA['rho'] = 1000.0 if rho is None else rho
+    pA['m'] = 0.4 if m is None else m
+    return pA
+
+
+def _random_set(seed, n=400, mixed_h=False):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1.0, n)
+    y = rng.uniform(0, 0.7, n)
+    h = np.full(n, 0.03) if not mixed_h else rng.uniform(0.02, 0.04, n)
+    return _particles(x, y, h)
+
+
+def brute_near(pA, scale=2.0):
+    """near(i) for all i by brute force: returns list of index arrays in reference order."""
+    x, y, h = pA['x'], pA['y'], pA['h']
+    xmin, ymin = x.min(), y.min()
+    cs = h.min() * scale
+    if cs < 1e-6:
+        cs = 1.0
+    ncx = max(1, math.ceil((1.0 / cs) * (x.max() - xmin)))
+    ncy = max(1, math.ceil((1.0 / cs) * (y.max() - ymin)))
+    cx = np.floor((x - xmin) / cs).astype(np.int64)
+    cy = np.floor((y - ymin) / cs).astype(np.int64)
+    key = cx + ncx * cy
+    kx, ky = key % ncx, key // ncx
+    inside = (key >= 0) & (key < ncx * ncy)
+    out = []
+    for i in range(len(x)):
+        dx = x[i] - x
+        dy = y[i] - y
+        r = np.sqrt(dx * dx + dy * dy)
+        q = r / (0.5 * (h[i] + h))
+        a = kx - cx[i]
+        b = ky - cy[i]
+        ok = inside & (np.abs(a) <= 1) & (np.abs(b) <= 1) & (q <= 3.0)
+        j = np.nonzero(ok)[0]
+        order = np.lexsort((-j, b[j], a[j]))  # x-shift outer, y-shift inner, index descending
+        out.append(j[order])
+    return out
+
+
+@pytest.mark.parametrize('seed,mixed', [(1, False), (2, True), (3, False)])
+def test_nn_bruteforce(seed, mixed):
+    pA = _random_set(seed, mixed_h=mixed)
+    nn = oracle.NN(2.0)
+    nn.update(pA)
+    d = nn.dump(pA)
+    ref = brute_near(pA)
+    for i in range(len(pA)):
+        got = d['idx'][d['off'][i]:d['off'][i + 1]]
+        np.testing.assert_array_equal(got, ref[i])
+
+
+def test_nn_cubic_support_complete():
+    """With uniform h the 3x3 block (cells 2h wide) holds every pair with r <= 2h."""
+    pA = _random_set(7, n=600)
+    nn = oracle.NN(2.0)
+    nn.update(pA)
+    d = nn.dump(pA)
+    x, y, h = pA['x'], pA['y'], pA['h']
+    for i in range(len(pA)):
+        r = np.hypot(x[i] - x, y[i] - y)
+        must = set(np.nonzero(r <= 2 * h[i])[0].tolist())
+        got = set(d['idx'][d['off'][i]:d['off'][i + 1]].tolist())
+        assert must <= got
+
+
+def test_nn_edge_alias():
+    """A particle whose x lands exactly on xmin + ncx*cs gets cx = ncx: its flat key aliases to
+    cell (0, cy + 1) (src/Tools/NNLinkedList.py:129-141 has no validity check)."""
+    # h = 0.5 -> cell size 1.0; x spans exactly 0..4 -> ncx = 4; the particle at x = 4 has cx = 4
+    x = np.array([0.0, 1.5, 2.5, 4.0, 0.2])
+    y = np.array([0.0, 0.5, 1.5, 0.5, 1.6])
+    pA = _particles(x, y, 0.5)
+    nn = oracle.NN(2.0)
+    nn.update(pA)
+    g = nn.grid()
+    assert (g['ncx'], g['ncy'], g['cell_size']) == (4, 2, 1.0)
+    d = nn.dump(pA)
+    assert d['keys'][3] == 4 + 4 * 0  # == cell (0, 1)
+    ref = brute_near(pA)
+    for i in range(len(pA)):
+        np.testing.assert_array_equal(d['idx'][d['off'][i]:d['off'][i + 1]], ref[i])
+    # particle 4 sits in cell (0, 1): it sees the aliased particle 3 as a candidate but q > 3
+    assert 3 not in d['idx'][d['off'][4]:d['off'][5]]
+
+
+def dense_rates(pA, method, kernel='cubic', scale=2.0):
+    """Continuity, momentum + AV + gravity, LJ wall force for every fluid particle, all pairs at once."""
+    x, y, h, lab = pA['x'], pA['y'], pA['h'], pA['label']
+    rho, vx, vy, m = pA['rho'], pA['vx'], pA['vy'], pA['m']
+    n = len(x)
+    xmin, ymin = x.min(), y.min()
+    cs = h.min() * scale
+    ncx = max(1, math.ceil((1.0 / cs) * (x.max() - xmin)))
+    ncy = max(1, math.ceil((1.0 / cs) * (y.max() - ymin)))
+    cx = np.floor((x - xmin) / cs).astype(np.int64)
+    cy = np.floor((y - ymin) / cs).astype(np.int64)
+    key = cx + ncx * cy
+    kx, ky = key % ncx, key // ncx
+    inside = (key >= 0) & (key < ncx * ncy)
+    block = inside[None, :] & (np.abs(kx[None, :] - cx[:, None]) <= 1) & (np.abs(ky[None, :] - cy[:, None]) <= 1)
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    r = np.sqrt(dx * dx + dy * dy)
+    hij = 0.5 * (h[:, None] + h[None, :])
+    q = r / hij
+    nb = block & (q <= 3.0)
+    p = np.where(lab == 0, ((rho / method.rho0) ** method.gamma - 1.0) * method.B, 0.0) + method.Pb
+    prs = p / (rho * rho)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        if kernel == 'cubic':
+            alpha = 10.0 / (7.0 * np.pi) / (hij * hij)
+            g = np.where(q <= 1.0, -3.0 * q * (1.0 - 0.75 * q), -0.75 * (2.0 - q) ** 2)
+            norm = np.where((q > 2.0) | (r < 1e-10), 0.0, alpha * g / (hij * r))
+        elif kernel == 'gaussian':
+            dwdq = -2.0 * q / (np.pi * hij * hij) * np.exp(-q * q)
+            norm = np.where((r * hij > 1e-12) & (q <= 3.0), dwdq / (r * hij), 0.0)
+        else:
+            alpha = 9.0 / (4.0 * np.pi) / (hij * hij)
+            inner = 1.0 - 0.5 * q
+            g = inner ** 5 * (-14.0 / 3.0) * q * (1 + 2.5 * q)
+            norm = np.where((q >= 2.0) | (r < 1e-10), 0.0, alpha * g / (hij * r))
+        dwx, dwy = norm * dx, norm * dy
+        vxij = vx[:, None] - vx[None, :]
+        vyij = vy[:, None] - vy[None, :]
+        W = nb & (lab[None, :] == 0)
+        drho = np.sum(np.where(W, m[None, :] * (vxij * dwx + vyij * dwy), 0.0), axis=1)
+        dv = vxij * dx + vyij * dy
+        hm = 0.5 * (h[:, None] + hij)
+        mu = hm * dv / (r * r + 0.01 * hm * hm)
+        cij = 0.5 * method.co
+        PI = np.where(dv < 0, mu * (method.beta * mu - method.alpha * cij) / (0.5 * (rho[:, None] + rho[None, :])),
+                      0.0)
+        fac = -m[None, :] * (prs[:, None] + prs[None, :] + PI)
+        ax = np.sum(np.where(W, fac * dwx, 0.0), axis=1)
+        ay = np.sum(np.where(W, fac * dwy, 0.0), axis=1)
+        Bm = nb & (lab[None, :] != 0) & (r <= method.r0) & (r > 1e-12)
+        fr = method.r0 / r
+        lj = method.D * (fr ** method.p1 - fr ** method.p2) / (r * r)
+        fx = np.sum(np.where(Bm, lj * dx, 0.0), axis=1)
+        fy = np.sum(np.where(Bm, lj * dy, 0.0), axis=1)
+    return drho, ax + fx, ay - G + fy
+
+
+class _M:
+    """WCSPH constants (src/Methods/WCSPH.py:35-79) for a column height H."""
+
+    def __init__(self, H=1.0, r0=0.02, rho0=1000.0):
+        self.height, self.r0, self.rho0 = H, r0, rho0
+        self.useXSPH = False
+        self.useSummationDensity = False
+        self.gamma = 7.0
+        self.co = 10.0 * math.sqrt(2 * 9.81 * H)
+        self.B = self.co * self.co * rho0 / self.gamma
+        self.Pb = 0.0
+        self.alpha, self.beta = 0.01, 0.0
+        self.epsilon = 0.5
+        self.D = 5 * 9.81 * H
+        self.p1, self.p2 = 4, 2
+
+
+def _block_with_walls(seed, nx=16, ny=12, dx=0.02):
+    """A jittered fluid block resting on a wall row, random velocities and densities."""
+    rng = np.random.default_rng(seed)
+    gx, gy = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dx, indexing='ij')
+    xf = gx.ravel() + rng.uniform(-0.15 * dx, 0.15 * dx, gx.size)
+    yf = gy.ravel() + rng.uniform(-0.15 * dx, 0.15 * dx, gx.size)
+    xw = np.arange(-3, nx + 3) * dx
+    yw = np.full(xw.size, -0.9 * dx)
+    x = np.concatenate([xf, xw])
+    y = np.concatenate([yf, yw])
+    lab = np.concatenate([np.zeros(xf.size, np.int8), np.ones(xw.size, np.int8)])
+    vx = np.concatenate([rng.normal(0, 0.5, xf.size), np.zeros(xw.size)])
+    vy = np.concatenate([rng.normal(0, 0.5, xf.size), np.zeros(xw.size)])
+    rho = 1000.0 * (1 + rng.uniform(-0.01, 0.01, x.size))
+    return _particles(x, y, 1.3 * dx, lab, vx, vy, rho, 1000.0 * dx * dx)
+
+
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
+@pytest.mark.parametrize('seed', [11, 12])
+def test_loop_vs_dense_pairs(kernel, seed):
+    pA = _block_with_walls(seed)
+    m = _M()
+    ref = dense_rates(pA, m, kernel)
+    oracle.loop(pA, m, kernel)
+    f = pA['label'] == 0
+    for got, want in ((pA['drho'], ref[0]), (pA['ax'], ref[1]), (pA['ay'], ref[2])):
+        scale = np.max(np.abs(want[f]))
+        assert scale > 0
+        np.testing.assert_allclose(got[f], want[f], rtol=0, atol=1e-12 * scale)
+    # the AV branch and the wall force were both exercised
+    assert np.any(np.abs(ref[1][f]) > 0)
+
+
+def test_two_particles_closed_form():
+    """Two fluid particles at rest, d = 0.6 h apart along x (q = 0.6, cubic inner branch)."""
+    h, d, m0 = 0.026, 0.6 * 0.026, 0.4
+    pA = _particles(np.array([0.0, d]), np.array([0.0, 0.0]), h, rho=np.array([1002.0, 1001.0]), m=m0)
+    m = _M()
+    oracle.loop(pA, m, 'cubic')
+    p = ((pA['rho'] / m.rho0) ** 7 - 1) * m.B
+    q = d / h
+    dwdx = 10 / (7 * math.pi * h * h) * (-3 * q * (1 - 0.75 * q)) / (h * d) * (0.0 - d)  # x_0 - x_1 = -d
+    a0 = -m0 * (p[0] / pA['rho'][0] ** 2 + p[1] / pA['rho'][1] ** 2) * dwdx
+    assert pA['ax'][0] == pytest.approx(a0, rel=1e-13)
+    assert pA['ax'][1] == pytest.approx(-a0, rel=1e-13)
+    assert pA['ay'][0] == pytest.approx(-G, rel=1e-15)
+    assert pA['drho'][0] == 0.0
+
+
+def test_uniform_velocity_gives_zero_density_rate():
+    pA = _block_with_walls(5)
+    f = pA['label'] == 0
+    pA['vx'][f] = 0.7
+    pA['vy'][f] = -0.2
+    pA['vx'][~f] = 0.7
+    pA['vy'][~f] = -0.2
+    oracle.loop(pA, _M(), 'cubic')
+    assert np.all(pA['drho'][f] == 0.0)
+
+
+def test_momentum_conservation_fluid_only():
+    """Pair forces are antisymmetric (uniform h, m): sum m (a + g) over a fluid-only set is ~0."""
+    pA = _block_with_walls(9)
+    pA = pA[pA['label'] == 0].copy()
+    oracle.loop(pA, _M(), 'cubic')
+    tot_x = np.sum(pA['m'] * pA['ax'])
+    tot_y = np.sum(pA['m'] * (pA['ay'] + G))
+    scale = np.sum(pA['m'] * np.abs(pA['ax']))
+    assert abs(tot_x) < 1e-12 * scale
+    assert abs(tot_y) < 1e-12 * scale
+
+
+def test_mirror_and_rotation_symmetry():
+    pA = _block_with_walls(4)
+    m = _M()
+    a = pA.copy()
+    oracle.loop(a, m, 'cubic')
+    # mirror x -> -x
+    b = pA.copy()
+    b['x'] = -b['x']
+    b['vx'] = -b['vx']
+    oracle.loop(b, m, 'cubic')
+    f = pA['label'] == 0
+    sc = np.max(np.abs(a['ax'][f]))
+    np.testing.assert_allclose(b['ax'][f], -a['ax'][f], atol=1e-10 * sc)
+    np.testing.assert_allclose(b['ay'][f], a['ay'][f], atol=1e-10 * sc)
+    np.testing.assert_allclose(b['drho'][f], a['drho'][f], atol=1e-10 * np.max(np.abs(a['drho'][f])))
+    # rotate by +90 degrees: (x, y) -> (-y, x); gravity stays along -y, so compare a + g
+    c = pA.copy()
+    c['x'], c['y'] = -pA['y'], pA['x']
+    c['vx'], c['vy'] = -pA['vy'], pA['vx']
+    oracle.loop(c, m, 'cubic')
+    np.testing.assert_allclose(c['ax'][f], -(a['ay'][f] + G), atol=1e-10 * sc)
+    np.testing.assert_allclose(c['ay'][f] + G, a['ax'][f], atol=1e-10 * sc)
+
+
+def test_solver_step_masks_and_walls_fixed():
+    """The loop body moves fluid only and the first dt is the Courant step (a = 0)."""
+    pA = _block_with_walls(3)
+    m = _M()
+    pA['c'][pA['label'] == 0] = m.co  # Solver.setup (src/Solver.py:196)
+    s = oracle.Solver(pA, m, 'cubic', 'pec', h=0.026)
+    w = s.pA['label'] != 0
+    xw = s.pA['x'][w].copy()
+    hist = s.run(3)
+    assert hist[0][0] == pytest.approx(0.25 * 0.026 / m.co, rel=1e-15)
+    assert hist[0][2] == 1e10
+    np.testing.assert_array_equal(s.pA['x'][w], xw)
+    assert s.t == pytest.approx(hist[:, 0].sum(), rel=1e-15)
