@@ -50,8 +50,8 @@ extern "C" {
 enum { SPH_KERNEL_CUBIC = 0, SPH_KERNEL_GAUSSIAN = 1, SPH_KERNEL_WENDLAND = 2 };
 /* FP64: pair arithmetic in double, neighbours in the reference's order, no FMA
  * contraction (bit-reproducible against the CPU oracle for the cubic spline).
- * FP32C: pair arithmetic in float from cell-local offsets, compensated
- * (Neumaier) sums; integrator, time and dt stay double. */
+ * FP32C: pair arithmetic in float (separations rounded once from double
+ * positions), TwoSum-compensated sums; integrator, time and dt stay double. */
 enum { SPH_PREC_FP64 = 0, SPH_PREC_FP32C = 1 };
 enum { SPH_INTEGRATOR_PEC = 0, SPH_INTEGRATOR_EULER = 1 };
 enum { SPH_FLUID = 0, SPH_BOUNDARY = 1, SPH_TEMPBOUNDARY = 2, SPH_COUPLED = 3 };
@@ -130,6 +130,12 @@ int sph_nn_dump(sph_ctx_t *ctx, int64_t *keys, int64_t *off, int32_t *idx, doubl
  * Requires a prior sph_nn_update. */
 int sph_compute(sph_ctx_t *ctx, void *stream);
 
+/* For parity: one pair pass (as sph_compute) that also reports, per particle
+ * in reference order, the number of neighbours the pass itself found
+ * (len(near(i)) of src/Tools/SolverTools.py:163; -1 for non-fluid).
+ * counts is a device int32[n]. Synchronises `stream`. */
+int sph_compute_counts(sph_ctx_t *ctx, int32_t *counts, void *stream);
+
 /* ---- time step and integrators ---- */
 /* Solver._minTimeStep + TimeStep.compute (src/Solver.py:210-230,
  * src/Equations/TimeStep.py:11-90), evaluated on the device into the
@@ -167,6 +173,14 @@ int sph_kernel_derivative(int32_t kernel, int64_t n, const double *r, const doub
 /* TaitEOS (src/Equations/TaitEOS.py:6-31) + Pb. */
 int sph_tait_eos(int64_t n, double gamma, double B, double rho0, double Pb, const double *rho, const int8_t *label,
                  double *p_out, void *stream);
+
+/* ---- launch shape of the pair pass ---- */
+/* No reference counterpart (numba picks no launch shape): the workgroup size
+ * of the pair pass (64, 128 or 256 threads; each thread walks one particle's
+ * 3x3 cells and keeps a 32-entry contributor list in LDS). Set by the
+ * autotuner (offshore-sph_amd/hipsph/autotune.py). */
+int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads);
+int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads);
 
 /* ---- diagnostics ---- */
 const char *sph_last_error(void);

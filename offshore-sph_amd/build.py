@@ -6,7 +6,8 @@ Compiles each translation unit of offshore-sph_amd/csrc with hipcc for
 gfx950 and links them into offshore-sph_amd/hipsph/libhipsph.so:
   sph_engine.hip    -ffp-contract=off  (neighbour search, integrators, dt, C ABI)
   sph_pair_fp64.hip -ffp-contract=off  (double pair pass, reference operation order)
-  sph_pair_fp32.hip -ffp-contract=fast (float pair pass with compensated sums)
+  sph_pair_fp32.hip -ffp-contract=fast-honor-pragmas (float pair pass; the exact
+                    neighbour re-test and the TwoSum steps switch contraction off)
 Packed FP32 instructions are disabled for every device compile
 (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
@@ -31,7 +32,7 @@ COMMON = ['-O3', '-fPIC', '-std=c++17', f'--offload-arch={ARCH}', '-fno-fast-mat
 UNITS = [
     ('sph_engine.hip', ['-ffp-contract=off']),
     ('sph_pair_fp64.hip', ['-ffp-contract=off']),
-    ('sph_pair_fp32.hip', ['-ffp-contract=fast']),
+    ('sph_pair_fp32.hip', ['-ffp-contract=fast-honor-pragmas']),
 ]
 
 

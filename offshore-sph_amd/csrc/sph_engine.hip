@@ -54,13 +54,11 @@ struct sph_ctx {
     int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr;
     long long cap_cells = 0;
     double *red = nullptr;
-    double *prs = nullptr;
-    float *f32 = nullptr;
-    float *xl = nullptr, *yl = nullptr, *vxf = nullptr, *vyf = nullptr, *rhof = nullptr, *prsf = nullptr,
-          *mf = nullptr, *hf = nullptr;
+    void *recA = nullptr, *recB = nullptr;  // packed pair-pass records (Rec<double> / Rec<float>)
     int *slot_of = nullptr;
     hipStream_t own = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int pair_threads = 256;
     // timing of the pair pass and of the neighbour update
     bool timing = false;
     std::vector<hipEvent_t> ev_pair, ev_nn;
@@ -502,13 +500,12 @@ __device__ inline double tait_p(double rho, int8_t label, double gamma, double B
     return p + Pb;
 }
 
-// Permute the whole state into the new cell order and derive the pair-pass
-// inputs: p / rho^2 per particle (the momentum equation's slf/othr terms,
-// src/Equations/Momentum.py:28,36) and, for FP32C, float copies with
-// positions as offsets from the raw cell origin.
+// Permute the whole state into the new cell order and write the packed
+// pair-pass records: p / rho^2 per particle (the momentum equation's slf and
+// othr terms, src/Equations/Momentum.py:28,36) and w = m for fluid, -label
+// otherwise.
 __global__ void k_reorder(const Ctrl *ctrl, State a, State b, const int *perm, const int2 *cxy_old, int2 *cxy_s,
-                          double *prs, Phys ph, int n, int fp32, float *xl, float *yl, float *vxf, float *vyf,
-                          float *rhof, float *prsf, float *mf, float *hf) {
+                          Phys ph, int n, int fp32, void *recA, void *recB) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
     const int act = ctrl->active;
@@ -535,23 +532,23 @@ __global__ void k_reorder(const Ctrl *ctrl, State a, State b, const int *perm, c
     b.vy0[t] = a.vy0[s];
     b.rho0[t] = a.rho0[s];
     if (!act) return;
-    int2 c = cxy_old[s];
-    cxy_s[t] = c;
-    double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
-    double pr = p / (rho * rho);
-    prs[t] = pr;
+    cxy_s[t] = cxy_old[s];
+    const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
+    const double pr = p / (rho * rho);
+    const double w = lab == kFluid ? m : -(double)lab;
     if (fp32) {
-        const Grid g = ctrl->grid;
-        double ox = g.xmin + (double)c.x * g.cs;
-        double oy = g.ymin + (double)c.y * g.cs;
-        xl[t] = (float)(x - ox);
-        yl[t] = (float)(y - oy);
-        vxf[t] = (float)vx;
-        vyf[t] = (float)vy;
-        rhof[t] = (float)rho;
-        prsf[t] = (float)pr;
-        mf[t] = (float)m;
-        hf[t] = (float)h;
+        Rec<float>::A ra;
+        ra.x = x;
+        ra.y = y;
+        ra.h = (float)h;
+        ra.w = (float)w;
+        ra.vx = (float)vx;
+        ra.vy = (float)vy;
+        ((Rec<float>::A *)recA)[t] = ra;
+        ((Rec<float>::B *)recB)[t] = Rec<float>::B{(float)rho, (float)pr};
+    } else {
+        ((Rec<double>::A *)recA)[t] = Rec<double>::A{x, y, h, w};
+        ((Rec<double>::B *)recB)[t] = Rec<double>::B{vx, vy, rho, pr};
     }
 }
 
@@ -746,7 +743,7 @@ static void free_ctx(sph_ctx *c) {
         if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
     }
     void *ptrs[] = {c->ctrl, c->dt_hist, c->bin, c->rank, c->perm, c->permKey, c->cxy_old, c->cxy_s,
-                    c->counts, c->cellStart, c->tileSums, c->red, c->prs, c->f32, c->slot_of};
+                    c->counts, c->cellStart, c->tileSums, c->red, c->recA, c->recB, c->slot_of};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : c->ev_pair) hipEventDestroy(e);
@@ -807,7 +804,7 @@ const char *sph_last_error(void) { return g_err.c_str(); }
 
 const char *sph_version(void) {
     return "hipsph 1.0 gfx950 (engine: -ffp-contract=off; pair fp64: -ffp-contract=off; pair fp32c: "
-           "-ffp-contract=fast)";
+           "-ffp-contract=fast-honor-pragmas)";
 }
 
 int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
@@ -849,14 +846,12 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
               hipMalloc(&c->cxy_old, sizeof(int2) * n) == hipSuccess &&
               hipMalloc(&c->cxy_s, sizeof(int2) * n) == hipSuccess &&
               hipMalloc(&c->red, sizeof(double) * 8 * kMaxRedBlocks) == hipSuccess &&
-              hipMalloc(&c->prs, sizeof(double) * n) == hipSuccess &&
               hipMalloc(&c->slot_of, sizeof(int) * n) == hipSuccess;
-    if (ok && params->precision == SPH_PREC_FP32C) {
-        ok = hipMalloc(&c->f32, sizeof(float) * n * 8) == hipSuccess;
-        if (ok) {
-            float **f[] = {&c->xl, &c->yl, &c->vxf, &c->vyf, &c->rhof, &c->prsf, &c->mf, &c->hf};
-            for (int k = 0; k < 8; ++k) *f[k] = c->f32 + (size_t)k * n;
-        }
+    if (ok) {
+        const bool f32 = params->precision == SPH_PREC_FP32C;
+        const size_t ra = f32 ? sizeof(Rec<float>::A) : sizeof(Rec<double>::A);
+        const size_t rb = f32 ? sizeof(Rec<float>::B) : sizeof(Rec<double>::B);
+        ok = hipMalloc(&c->recA, ra * n) == hipSuccess && hipMalloc(&c->recB, rb * n) == hipSuccess;
     }
     if (!ok) {
         free_ctx(c);
@@ -978,9 +973,8 @@ static int nn_update(sph_ctx *c, hipStream_t s) {
     k_scan_apply<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums, c->cellStart);
     k_scatter<<<nblk(n), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->bin, c->rank, a.id, c->perm, c->permKey, n);
     k_cellsort<<<redblk(c->cap_cells + 1), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->perm, c->permKey);
-    k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_old, c->cxy_s, c->prs, make_phys(c->P),
-                                         n, c->P.precision == SPH_PREC_FP32C, c->xl, c->yl, c->vxf, c->vyf,
-                                         c->rhof, c->prsf, c->mf, c->hf);
+    k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_old, c->cxy_s, make_phys(c->P), n,
+                                         c->P.precision == SPH_PREC_FP32C, c->recA, c->recB);
     k_mark_nn<<<1, 1, 0, s>>>(c->ctrl, 1);
     HIPCHK(hipGetLastError());
     if (c->timing) {
@@ -991,35 +985,17 @@ static int nn_update(sph_ctx *c, hipStream_t s) {
     return 0;
 }
 
-static int compute(sph_ctx *c, hipStream_t s) {
+static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr) {
     State st = c->buf[c->cur];
-    PairIn in;
-    in.x = st.x;
-    in.y = st.y;
-    in.vx = st.vx;
-    in.vy = st.vy;
-    in.rho = st.rho;
-    in.m = st.m;
-    in.h = st.h;
-    in.prs = c->prs;
-    in.label = st.label;
-    in.cxy = c->cxy_s;
-    in.cellStart = c->cellStart;
-    in.xl = c->xl;
-    in.yl = c->yl;
-    in.vxf = c->vxf;
-    in.vyf = c->vyf;
-    in.rhof = c->rhof;
-    in.prsf = c->prsf;
-    in.mf = c->mf;
-    in.hf = c->hf;
-    PairOut out{st.ax, st.ay, st.drho};
+    PairIn in{c->recA, c->recB, st.h, c->cxy_s, c->cellStart};
+    PairOut out{st.ax, st.ay, st.drho, count};
     if (c->timing) {
         int rc = ev_record(c, c->ev_pair, c->n_pair, s);
         if (rc) return rc;
     }
-    hipError_t e = c->P.precision == SPH_PREC_FP32C ? launch_pair_fp32c(c->ctrl, in, out, make_phys(c->P), c->n, s)
-                                                   : launch_pair_fp64(c->ctrl, in, out, make_phys(c->P), c->n, s);
+    hipError_t e = c->P.precision == SPH_PREC_FP32C
+                       ? launch_pair_fp32c(c->ctrl, in, out, make_phys(c->P), c->n, c->pair_threads, s)
+                       : launch_pair_fp64(c->ctrl, in, out, make_phys(c->P), c->n, c->pair_threads, s);
     if (e != hipSuccess) return fail(SPH_EHIP, "pair launch: %s", hipGetErrorString(e));
     if (c->timing) {
         int rc = ev_record(c, c->ev_pair, c->n_pair, s);
@@ -1220,6 +1196,24 @@ int sph_nn_dump(sph_ctx_t *ctx, int64_t *keys, int64_t *off, int32_t *idx, doubl
     return 0;
 }
 
+__global__ void k_unpermute_int(const int *src, const int *id, int *dst, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[id[t]] = src[t];
+}
+
+int sph_compute_counts(sph_ctx_t *ctx, int32_t *counts, void *stream) {
+    if (!ctx || !counts) return fail(SPH_EINVAL, "NULL argument");
+    hipStream_t s = S(stream);
+    int rc = sph_nn_grid(ctx, nullptr, nullptr, stream);
+    if (rc) return rc;
+    // ctx->bin is free between updates: use it as the cell-order count buffer
+    if ((rc = compute(ctx, s, ctx->bin))) return rc;
+    k_unpermute_int<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->bin, ctx->buf[ctx->cur].id, counts, ctx->n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
 int sph_kernel_evaluate(int32_t kernel, int64_t n, const double *r, const double *h, double *out, void *stream) {
     if (n < 0 || n > (1LL << 31) - 1 || kernel < 0 || kernel > 2) return fail(SPH_EINVAL, "bad n or kernel");
     if (n == 0) return 0;
@@ -1252,6 +1246,25 @@ int sph_tait_eos(int64_t n, double gamma, double B, double rho0, double Pb, cons
     if (n == 0) return 0;
     k_tait<<<nblk(n), kBlock, 0, S(stream)>>>((int)n, gamma, B, rho0, Pb, rho, label, p_out);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (pair_threads != 64 && pair_threads != 128 && pair_threads != 256)
+        return fail(SPH_EINVAL, "pair_threads must be 64, 128 or 256");
+    ctx->pair_threads = pair_threads;
+    for (int k = 0; k < 2; ++k)
+        if (ctx->gexec[k]) {
+            hipGraphExecDestroy(ctx->gexec[k]);
+            ctx->gexec[k] = nullptr;
+        }
+    return 0;
+}
+
+int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads) {
+    if (!ctx || !pair_threads) return fail(SPH_EINVAL, "NULL argument");
+    *pair_threads = ctx->pair_threads;
     return 0;
 }
 

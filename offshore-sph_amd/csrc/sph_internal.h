@@ -52,19 +52,36 @@ struct State {
     int *id;
 };
 
-// Inputs of the pair pass, cell ordered (written by the reorder kernel).
+// Packed pair-pass records, one per particle in cell order (written by the
+// reorder kernel). w = m for fluid and -label otherwise; prs = p / rho^2 with
+// p the Tait pressure + Pb (src/Equations/Momentum.py:28,36).
+template <typename R>
+struct Rec;
+template <>
+struct Rec<double> {
+    struct __align__(16) A { double x, y, h, w; };
+    struct __align__(16) B { double vx, vy, rho, prs; };
+};
+template <>
+struct Rec<float> {
+    struct __align__(16) A {
+        double x, y;
+        float h, w, vx, vy;
+    };
+    struct __align__(8) B { float rho, prs; };
+};
+
+// Inputs of the pair pass, cell ordered.
 struct PairIn {
-    const double *x, *y, *vx, *vy, *rho, *m, *h, *prs;  // prs = p / rho^2 (Tait + Pb)
-    const int8_t *label;
-    const int2 *cxy;          // raw cell coordinates (cx, cy) of each particle
+    const void *A, *B;     // Rec<double> or Rec<float> records
+    const double *H;       // double h
+    const int2 *cxy;       // raw cell coordinates (cx, cy) of each particle
     const int *cellStart;
-    // FP32C inputs: offsets from the particle's raw cell origin, velocities,
-    // density, p/rho^2, mass and h in float.
-    const float *xl, *yl, *vxf, *vyf, *rhof, *prsf, *mf, *hf;
 };
 
 struct PairOut {
     double *ax, *ay, *drho;
+    int *count;  // optional: neighbour count per particle (cell order), -1 for walls
 };
 
 struct Phys {
@@ -72,9 +89,10 @@ struct Phys {
     int kernel;
 };
 
-// Launchers implemented in the pair translation units.
-hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, hipStream_t s);
-hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, hipStream_t s);
+// Launchers implemented in the pair translation units (threads: workgroup
+// size of the pair pass, 64, 128 or 256; chosen by the autotuner).
+hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
+hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
 
 // Number of blocks for a grid over n items, padded to a multiple of the XCD
 // count so that the block remap below is a bijection.

@@ -1,0 +1,352 @@
+// sph_pair_impl.h — the fused pair pass (Tait-derived p/rho^2, continuity,
+// momentum + artificial viscosity + gravity, Lennard-Jones wall force).
+// Included by sph_pair_fp64.hip (R = double, reference order,
+// -ffp-contract=off) and sph_pair_fp32.hip (R = float, -ffp-contract=fast-honor-pragmas).
+//
+// The pass is VALU-bound (rocprof: ~93% of the VALU issue slots busy), so the
+// design minimises instructions per candidate:
+//   * One thread per particle of the cell-ordered state; fluid particles are
+//     targets. Consecutive threads hold neighbouring particles, so the
+//     candidates of a wavefront overlap and are served by L1/L2.
+//   * Each particle's pair inputs are two packed records written by the
+//     reorder kernel (one address computation, wide loads):
+//       double: A = (x, y, h, w), B = (vx, vy, rho, p/rho^2)
+//       float:  A = (x, y as double; h, w, vx, vy as float), B = (rho, p/rho^2)
+//     with w = m for fluid and w = -label for walls (the pair equations use no
+//     wall mass). The float path forms x_i - x_j in double and rounds it once:
+//     on this GPU a double add costs the same issue slot as a float add, and
+//     the separation keeps full accuracy at any tank size.
+//   * Pass 1 walks the 3x3 cells and appends, per target, the candidates that
+//     are neighbours and contribute (inside the kernel support, or within r0
+//     of a wall) to a list in LDS (interleaved by lane: conflict-free). Pass 2
+//     runs the costly pair arithmetic over the list only, so lanes of a wave
+//     stay busy instead of idling through candidates that add nothing. The list
+//     is flushed when full, which keeps the order of the sums.
+#pragma once
+#include "sph_internal.h"
+
+namespace hipsph {
+namespace {
+
+constexpr int kPairThreads = 256;
+constexpr int kListLen = 32;  // list entries per thread before a flush
+constexpr double kPI = 3.141592653589793;
+
+__device__ inline double rec_vx(const Rec<double>::A &, const Rec<double>::B &b) { return b.vx; }
+__device__ inline double rec_vy(const Rec<double>::A &, const Rec<double>::B &b) { return b.vy; }
+__device__ inline float rec_vx(const Rec<float>::A &a, const Rec<float>::B &) { return a.vx; }
+__device__ inline float rec_vy(const Rec<float>::A &a, const Rec<float>::B &) { return a.vy; }
+
+template <typename R>
+struct PairArgs {
+    const typename Rec<R>::A *A;
+    const typename Rec<R>::B *B;
+    const double *H;  // double h (exact neighbour re-test of the float path)
+    const int2 *cxy;  // raw cell of each particle
+    const int *cellStart;
+    double *ax, *ay, *drho;
+    int *count;  // optional diagnostic output
+    int n;
+};
+
+template <typename R>
+struct Acc;
+
+template <>
+struct Acc<double> {  // plain sums in the reference's order
+    double s = 0.0;
+    __device__ void add(double v) { s += v; }
+    __device__ double get() const { return s; }
+};
+
+template <>
+struct Acc<float> {  // TwoSum (Knuth) compensated float sum
+    float s = 0.f, c = 0.f;
+    __device__ void add(float v) {
+#pragma clang fp contract(off)
+        float t = s + v;
+        float z = t - s;
+        c += (s - (t - z)) + (v - z);
+        s = t;
+    }
+    __device__ double get() const { return (double)s + (double)c; }
+};
+
+// Kernel support radius in units of h (the gradient is exactly zero beyond).
+template <int KERNEL>
+__device__ constexpr double support_q() {
+    return KERNEL == SPH_KERNEL_GAUSSIAN ? 3.0 : 2.0;
+}
+
+// Double gradient in the reference's expression order
+// (src/Kernels/CubicSpline.py:38-70, Gaussian.py:39-60, Wendland.py:32-64),
+// with h = h_ij, r = r_ij and q = r / h (the reference recomputes q = r / h in
+// the kernel: the same value). Returns false where the reference returns 0.
+template <int KERNEL>
+__device__ inline bool grad_exact(double dx, double dy, double r, double q, double h, double &dwx, double &dwy) {
+    if (KERNEL == SPH_KERNEL_GAUSSIAN) {
+        double tmp = r * h;
+        if (!(tmp > 1e-12 && q <= 3)) return false;
+        double alpha_c = (1.0 / kPI) / (h * h);
+        double dwdq = -2 * q * alpha_c * exp(-q * q);
+        double f = dwdq / tmp;
+        dwx = f * dx;
+        dwy = f * dy;
+        return true;
+    } else if (KERNEL == SPH_KERNEL_WENDLAND) {
+        double alpha = 9.0 / (4.0 * kPI) / (h * h);
+        if ((q >= 2.0) || (r < 1e-10)) return false;
+        double inner = 1.0 - 0.5 * q;
+        double grad = ipow(inner, 5.0) * (-14.0 / 3.0) * q * (1 + 2.5 * q);
+        double norm = alpha * grad / (h * r);
+        dwx = norm * dx;
+        dwy = norm * dy;
+        return true;
+    } else {
+        double alpha = 10.0 / (7.0 * kPI) / (h * h);
+        if ((q > 2.0) || (r < 1e-10)) return false;
+        double grad = (q > 1.0) ? -0.75 * ((2.0 - q) * (2.0 - q)) : -3.0 * q * (1.0 - 0.75 * q);
+        double norm = alpha * grad / (h * r);
+        dwx = norm * dx;
+        dwy = norm * dy;
+        return true;
+    }
+}
+
+// Float gradient factor (dW/dr) / r with native rcp/rsq (~1 ulp).
+template <int KERNEL>
+__device__ inline float grad_norm_f(float r, float rinv, float q, float hinv) {
+    if (KERNEL == SPH_KERNEL_GAUSSIAN) {
+        if (!(q <= 3.0f) || !(r > 1e-12f)) return 0.0f;
+        return -2.0f * q * ((float)(1.0 / kPI) * hinv * hinv) * __expf(-q * q) * rinv * hinv;
+    } else if (KERNEL == SPH_KERNEL_WENDLAND) {
+        if (q >= 2.0f || r < 1e-10f) return 0.0f;
+        const float inner = 1.0f - 0.5f * q;
+        const float i2 = inner * inner;
+        const float grad = i2 * i2 * inner * (-14.0f / 3.0f) * q * (1.0f + 2.5f * q);
+        return (float)(9.0 / (4.0 * kPI)) * hinv * hinv * grad * hinv * rinv;
+    } else {
+        if (q > 2.0f || r < 1e-10f) return 0.0f;
+        const float grad = (q > 1.0f) ? -0.75f * (2.0f - q) * (2.0f - q) : -3.0f * q * (1.0f - 0.75f * q);
+        return (float)(10.0 / (7.0 * kPI)) * hinv * hinv * grad * hinv * rinv;
+    }
+}
+
+// The reference's neighbour test in double (src/Tools/NNLinkedList.py:63-71),
+// contraction off whatever the translation unit's default.
+__device__ inline bool exact_q_test(double xi, double yi, double hi, double xj, double yj, double hj, double qmax) {
+#pragma clang fp contract(off)
+    const double dx = xi - xj, dy = yi - yj;
+    const double xij2 = dx * dx + dy * dy;
+    const double r = sqrt(xij2);
+    const double h_ij = 0.5 * (hi + hj);
+    const double q = r / h_ij;
+    return q <= qmax;
+}
+
+template <typename R>
+struct Target {
+    double X, Y, H;            // double position and h
+    R h, vx, vy, rho, slf;     // slf = p_i / rho_i^2 (Momentum.py:28)
+};
+
+template <typename R>
+struct Sums {
+    Acc<R> arho, a0, a1, f0, f1;
+};
+
+// Pass 2: the pair arithmetic for candidate u.
+template <typename R, int KERNEL>
+__device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int u, Sums<R> &S) {
+    const typename Rec<R>::A a = P.A[u];
+    if constexpr (sizeof(R) == 8) {
+        // Reference expression order: Continuity.py:13-16, Momentum.py:36-56,
+        // BoundaryForce.py:31-41.
+        const double dx = T.X - a.x, dy = T.Y - a.y;
+        const double xij2 = dx * dx + dy * dy;
+        const double r = sqrt(xij2);
+        const double hij = 0.5 * ((double)T.h + (double)a.h);
+        if (a.w >= 0) {
+            const double q = r / hij;
+            double dwx, dwy;
+            if (!grad_exact<KERNEL>(dx, dy, r, q, hij, dwx, dwy)) return;  // the reference adds exactly +-0
+            const typename Rec<R>::B b = P.B[u];
+            const double vx = (double)T.vx - (double)rec_vx(a, b), vy = (double)T.vy - (double)rec_vy(a, b);
+            const double mj = (double)a.w;
+            const double dot = vx * dwx + vy * dwy;
+            S.arho.add((R)(mj * dot));
+            const double dv = vx * dx + vy * dy;
+            double PI_ij = 0.0;
+            if (dv < 0) {
+                const double hm = 0.5 * ((double)T.h + hij);
+                const double cij = 0.5 * (ph.co + 0.0);  // c_j is never filled: c_ij = c_i / 2
+                const double rhoij = 0.5 * ((double)T.rho + (double)b.rho);
+                const double muij = hm * dv / (r * r + 0.01 * hm * hm);
+                const double ppij = muij * (ph.beta * muij - ph.alpha * cij);
+                PI_ij = ppij / rhoij;
+            }
+            const double factor = (double)T.slf + (double)b.prs + PI_ij;
+            S.a0.add((R)(-mj * factor * dwx));
+            S.a1.add((R)(-mj * factor * dwy));
+        } else if (!(r > ph.r0) && r > 1e-12) {
+            const double frac = ph.r0 / r;
+            const double tmp = ipow(frac, ph.p1) - ipow(frac, ph.p2);
+            const double fac = ph.D * tmp;
+            S.f0.add((R)(fac * dx / (r * r)));
+            S.f1.add((R)(fac * dy / (r * r)));
+        }
+    } else {
+        const float dx = (float)(T.X - a.x), dy = (float)(T.Y - a.y);
+        const float r2 = dx * dx + dy * dy;
+        const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+        const float r = r2 * rinv;
+        const float hij = 0.5f * ((float)T.h + (float)a.h);
+        if (a.w >= 0.f) {
+            const float hinv = __builtin_amdgcn_rcpf(hij);
+            const float q = r * hinv;
+            const float norm = grad_norm_f<KERNEL>(r, rinv, q, hinv);
+            if (norm == 0.0f) return;
+            const typename Rec<R>::B b = P.B[u];
+            const float dwx = norm * dx, dwy = norm * dy;
+            const float vx = (float)T.vx - (float)rec_vx(a, b), vy = (float)T.vy - (float)rec_vy(a, b);
+            const float mj = (float)a.w;
+            S.arho.add(mj * (vx * dwx + vy * dwy));
+            const float dv = vx * dx + vy * dy;
+            float PI_ij = 0.0f;
+            if (dv < 0.f) {
+                // mu (beta mu - alpha c_ij) / rho_ij with mu = hm dv / Dn, with one reciprocal:
+                // hm dv (beta hm dv - alpha c_ij Dn) / (Dn^2 rho_ij)
+                const float hm = 0.5f * ((float)T.h + hij);
+                const float rhoij = 0.5f * ((float)T.rho + (float)b.rho);
+                const float Dn = r * r + 0.01f * hm * hm;
+                const float hv = hm * dv;
+                PI_ij = hv * ((float)ph.beta * hv - (float)ph.alpha * (0.5f * (float)ph.co) * Dn) *
+                        __builtin_amdgcn_rcpf(Dn * Dn * rhoij);
+            }
+            const float fm = -mj * ((float)T.slf + (float)b.prs + PI_ij);
+            S.a0.add(fm * dwx);
+            S.a1.add(fm * dwy);
+        } else if (!(r > (float)ph.r0) && r > 1e-12f) {
+            const float frac = (float)ph.r0 * rinv;
+            const float tmp = ipow(frac, (float)ph.p1) - ipow(frac, (float)ph.p2);
+            const float fac = (float)ph.D * tmp * (rinv * rinv);
+            S.f0.add(fac * dx);
+            S.f1.add(fac * dy);
+        }
+    }
+}
+
+template <typename R, int KERNEL>
+__device__ inline void flush(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, const unsigned *list, int nlist,
+                             Sums<R> &S) {
+    for (int j = 0; j < nlist; ++j) contribute<R, KERNEL>(P, T, ph, (int)list[j * blockDim.x], S);
+}
+
+// Pass 1 over the candidates [s, e) of one cell (descending for the reference
+// order): counts neighbours and appends contributors to this thread's list,
+// flushing it through pass 2 when full.
+template <typename R, int KERNEL, bool DESC>
+__device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int s, int e, unsigned *list,
+                                int &nlist, Sums<R> &S) {
+    constexpr double kEps = sizeof(R) == 8 ? 1e-12 : 1e-5;
+    const R qmax = (R)ph.qmax;
+    const R ks = (R)support_q<KERNEL>();
+    const R r0sq = (R)ph.r0 * (R)ph.r0 * R(1.0 + kEps);
+    int cnt = 0;
+    for (int k = 0; k < e - s; ++k) {
+        const int u = DESC ? e - 1 - k : s + k;
+        const typename Rec<R>::A a = P.A[u];
+        const R dx = (R)(T.X - a.x), dy = (R)(T.Y - a.y);
+        const R r2 = dx * dx + dy * dy;
+        const R hij = R(0.5) * (T.h + (R)a.h);
+        const R qh = qmax * hij;
+        const R lim = qh * qh;
+        const R sh = ks * hij;
+        const R sup2 = a.w >= R(0) ? sh * sh * R(1.0 + kEps) : r0sq;
+        bool in = r2 < lim * R(1.0 - kEps);
+        if (!in && !(r2 > lim * R(1.0 + kEps))) {
+            // within the rounding band of the cut-off: the reference's exact test
+            const double hj = sizeof(R) == 8 ? (double)a.h : P.H[u];
+            in = exact_q_test(T.X, T.Y, T.H, a.x, a.y, hj, ph.qmax);
+        }
+        cnt += in;
+        if (in && r2 > R(0) && !(r2 > sup2)) {  // r = 0 pairs contribute exactly nothing
+            list[nlist * blockDim.x] = (unsigned)u;
+            if (++nlist == kListLen) {
+                flush<R, KERNEL>(P, T, ph, list, nlist, S);
+                nlist = 0;
+            }
+        }
+    }
+    return cnt;
+}
+
+template <typename R, int KERNEL, bool EXACT>
+__global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArgs<R> P, Phys ph) {
+    __shared__ unsigned s_list[kListLen * kPairThreads];
+    if (!ctrl->active) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const typename Rec<R>::A at = P.A[t];
+    if (!(at.w >= R(0))) {  // walls are not targets
+        if (P.count) P.count[t] = -1;
+        return;
+    }
+    const typename Rec<R>::B bt = P.B[t];
+    const Grid g = ctrl->grid;
+    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
+    Target<R> T;
+    T.X = at.x;
+    T.Y = at.y;
+    T.H = sizeof(R) == 8 ? (double)at.h : P.H[t];
+    T.h = (R)at.h;
+    T.vx = (R)rec_vx(at, bt);
+    T.vy = (R)rec_vy(at, bt);
+    T.rho = (R)bt.rho;
+    T.slf = (R)bt.prs;
+    const int2 c = P.cxy[t];
+    unsigned *list = s_list + threadIdx.x;
+    int nlist = 0, cnt = 0;
+    Sums<R> S;
+    // Cells in the reference's order: x-shift outer, y-shift inner
+    // (src/Tools/NNLinkedList.py:50-56), descending index inside a cell for
+    // the double pass (head-inserted lists, :129-141). The float pass only
+    // needs a fixed order.
+    for (int da = -1; da <= 1; ++da) {
+        const int cx = c.x + da;
+        if (!(ncx > cx && cx > -1)) continue;
+        for (int db = -1; db <= 1; ++db) {
+            const int cy = c.y + db;
+            if (!(ncy > cy && cy > -1)) continue;
+            const int k = cx + ncx * cy;
+            cnt += scan_cell<R, KERNEL, EXACT>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list, nlist, S);
+        }
+    }
+    flush<R, KERNEL>(P, T, ph, list, nlist, S);
+    if (P.count) P.count[t] = cnt;
+    if (cnt == 0) return;  // SolverTools.py:166-169: no neighbours, keep the previous values
+    P.drho[t] = S.arho.get();
+    const double a0 = S.a0.get(), a1 = S.a1.get(), f0 = S.f0.get(), f1 = S.f1.get();
+    P.ax[t] = a0 + f0;
+    P.ay[t] = (a1 - ph.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+}
+
+template <typename R, bool EXACT>
+hipError_t launch_pair(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, int threads, hipStream_t s) {
+    if (threads != 64 && threads != 128 && threads != 256) threads = kPairThreads;
+    const int nb = xcd_blocks(P.n, threads);
+    switch (ph.kernel) {
+        case SPH_KERNEL_GAUSSIAN:
+            k_pair<R, SPH_KERNEL_GAUSSIAN, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph);
+            break;
+        case SPH_KERNEL_WENDLAND:
+            k_pair<R, SPH_KERNEL_WENDLAND, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph);
+            break;
+        default:
+            k_pair<R, SPH_KERNEL_CUBIC, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace hipsph

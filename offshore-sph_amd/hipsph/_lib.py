@@ -69,6 +69,7 @@ _PROTOS = {
     'sph_nn_grid': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), _P]),
     'sph_nn_dump': (C.c_int, [_P, _P, _P, _P, _P, _P, _I64, C.POINTER(_I64), _P]),
     'sph_compute': (C.c_int, [_P, _P]),
+    'sph_compute_counts': (C.c_int, [_P, _P, _P]),
     'sph_dt': (C.c_int, [_P, _P]),
     'sph_predict': (C.c_int, [_P, _P]),
     'sph_correct': (C.c_int, [_P, _P]),
@@ -81,6 +82,8 @@ _PROTOS = {
     'sph_tait_eos': (C.c_int, [_I64, _D, _D, _D, _D, _P, _P, _P, _P]),
     'sph_last_error': (C.c_char_p, []),
     'sph_version': (C.c_char_p, []),
+    'sph_ctx_set_tuning': (C.c_int, [_P, _I32]),
+    'sph_ctx_get_tuning': (C.c_int, [_P, C.POINTER(_I32)]),
     'sph_timing_enable': (C.c_int, [_P, _I32]),
     'sph_timing_read': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I64), _I32]),
 }

@@ -142,6 +142,10 @@ class Engine:
                 raise ValueError(f'{k} has {t.numel()} entries, engine has n={self.n}')
             self.io[k].copy_(t.to(self.io[k].dtype).reshape(-1))
             fields.add(k)
+        # The pair records carry the label in the sign of the mass slot.
+        bad = (self.io['label'] == 0) & ~(self.io['m'] >= 0)
+        if bool(bad.any()):
+            raise ValueError('fluid particles need a finite, non-negative mass')
         call('sph_state_upload', self._ctx, C.byref(self._state(fields)), int(bool(reset_clock)), self._stream())
 
     def download(self, to_numpy=True):
@@ -165,6 +169,13 @@ class Engine:
 
     def compute(self):
         call('sph_compute', self._ctx, self._stream())
+
+    def compute_counts(self):
+        """One pair pass; returns the neighbour count it found per particle (-1 for walls)."""
+        import torch
+        out = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        call('sph_compute_counts', self._ctx, out.data_ptr(), self._stream())
+        return out.cpu().numpy()
 
     def correct(self):
         call('sph_correct', self._ctx, self._stream())

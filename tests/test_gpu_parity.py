@@ -336,3 +336,41 @@ def test_nonfinite_position_flags():
     e.step(1)
     assert e.status()['flags'] & _lib.SPH_FLAG_NONFINITE
     e.close()
+
+
+def test_boundary_pairs_dump_and_pair_pass_counts():
+    """Pairs within a few ulp of q = 3: the device search and both pair passes reach the
+    reference's verdict (a fused dx*dx + dy*dy would flip every one of these pairs)."""
+    from test_oracle_invariants import boundary_system
+    pA, z = boundary_system()
+    sc = scenarios.dam_break_c1()
+    full = np.zeros(len(pA), dtype=sc.pA.dtype)
+    for f in ('x', 'y', 'h', 'm', 'rho', 'label'):
+        full[f] = pA[f]
+    d = _nn_parity(full, sc.method(), scale=4.0)
+    for k in range(len(z['xi'])):
+        assert (2 * k + 1 in d['idx'][d['off'][2 * k]:d['off'][2 * k + 1]]) == bool(z['ref_in'][k])
+    want = np.diff(d['off']).astype(np.int32)
+    for prec in ('fp64', 'fp32c'):
+        e = make_engine(full, sc.method(), precision=prec, scale=4.0)
+        e.nn_update()
+        got = e.compute_counts()
+        e.close()
+        np.testing.assert_array_equal(got, want, err_msg=prec)
+
+
+@pytest.mark.parametrize('prec', ['fp64', 'fp32c'])
+def test_pair_pass_counts_match_near(prec):
+    """The pair pass's own neighbour sets have the sizes of the reference's near(i)."""
+    sc = scenarios.stress_lattice(nside=150)
+    e = make_engine(sc.pA, sc.method(), precision=prec)
+    e.nn_update()
+    got = e.compute_counts()
+    e.close()
+    nn = oracle.NN(2.0)
+    a = oracle.as_particles(sc.pA)
+    nn.update(a)
+    want = np.diff(nn.dump(a)['off'])
+    f = sc.pA['label'] == 0
+    np.testing.assert_array_equal(got[f], want[f])
+    assert np.all(got[~f] == -1)

@@ -304,3 +304,30 @@ def test_solver_step_masks_and_walls_fixed():
     assert hist[0][2] == 1e10
     np.testing.assert_array_equal(s.pA['x'][w], xw)
     assert s.t == pytest.approx(hist[:, 0].sum(), rel=1e-15)
+
+
+def boundary_system():
+    """The fixture's boundary pairs as one particle set: (i_k, j_k) at indices (2k, 2k+1), h = 1."""
+    import os
+    from conftest import GOLDEN
+    z = np.load(os.path.join(GOLDEN, 'boundary_pairs.npz'))
+    n = 2 * len(z['xi'])
+    x = np.empty(n)
+    y = np.empty(n)
+    x[0::2], y[0::2] = z['xi'], z['yi']
+    x[1::2], y[1::2] = z['xj'], z['yj']
+    return _particles(x, y, float(z['h']), m=1.0), z
+
+
+def test_nn_boundary_pairs_follow_ieee_expression():
+    """Near q = 3 the verdict is the reference's unfused expression, not a fused one."""
+    pA, z = boundary_system()
+    nn = oracle.NN(4.0)  # cell 4 h wide: every pair at r ~ 3h is a candidate
+    nn.update(pA)
+    d = nn.dump(pA)
+    for k in range(len(z['xi'])):
+        near_i = d['idx'][d['off'][2 * k]:d['off'][2 * k + 1]]
+        near_j = d['idx'][d['off'][2 * k + 1]:d['off'][2 * k + 2]]
+        assert (2 * k + 1 in near_i) == bool(z['ref_in'][k])
+        assert (2 * k in near_j) == bool(z['ref_in'][k])  # x_j - x_i = -(x_i - x_j) exactly
+    assert z['ref_in'].any() and (~z['ref_in']).any() and np.any(z['ref_in'] != z['fused_in'])
