@@ -51,9 +51,10 @@ struct sph_ctx {
     double *dt_hist = nullptr;
     int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
     int2 *cxy_old = nullptr, *cxy_s = nullptr;
-    int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr;
+    int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
+    double *part_b = nullptr, *part_d = nullptr;  // per-workgroup partials: bounds, dt
     long long cap_cells = 0;
-    double *red = nullptr;
+    bool dt_fresh = false;  // part_d holds the dt partials of the current state (left by the corrector)
     void *recA = nullptr, *recB = nullptr;  // packed pair-pass records (Rec<double> / Rec<float>)
     int *slot_of = nullptr;
     hipStream_t own = nullptr;
@@ -124,59 +125,74 @@ struct AddOp {
 // ---------------------------------------------------------------------------
 // time step: src/Equations/TimeStep.py:58-90 computeVars over the fluid
 // ---------------------------------------------------------------------------
-__global__ void k_dt_partial(const int8_t *label, const double *h, const double *ax, const double *ay, int n,
-                             double *red) {
-    __shared__ double lds[8];
+struct DtVars {
     double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (label[i] != kFluid) continue;
-        double a2 = ax[i] * ax[i] + ay[i] * ay[i];
-        if (h[i] < mn) mn = h[i];
+    __device__ void add(double h, double ax, double ay) {
+        double a2 = ax * ax + ay * ay;
+        if (h < mn) mn = h;
         if (a2 > mx) mx = a2;
         if (!(a2 == a2)) bad += 1.0;
         cnt += 1.0;
     }
-    mn = block_reduce(mn, MinOp(), lds);
-    mx = block_reduce(mx, MaxOp(), lds);
-    cnt = block_reduce(cnt, AddOp(), lds);
-    bad = block_reduce(bad, AddOp(), lds);
-    if (threadIdx.x == 0) {
-        red[blockIdx.x * 4 + 0] = mn;
-        red[blockIdx.x * 4 + 1] = mx;
-        red[blockIdx.x * 4 + 2] = cnt;
-        red[blockIdx.x * 4 + 3] = bad;
+    // block reduction, then this workgroup's partials (plain stores)
+    __device__ void publish(double *part, double *lds) {
+        mn = block_reduce(mn, MinOp(), lds);
+        mx = block_reduce(mx, MaxOp(), lds);
+        cnt = block_reduce(cnt, AddOp(), lds);
+        bad = block_reduce(bad, AddOp(), lds);
+        if (threadIdx.x == 0) {
+            const int R = gridDim.x, k = blockIdx.x;
+            part[0 * R + k] = mn;
+            part[1 * R + k] = mx;
+            part[2 * R + k] = cnt;
+            part[3 * R + k] = bad;
+        }
     }
+};
+
+// Dt reduction over the fluid of the current state (used when no corrector
+// has left the reduction of this state behind). Grid-stride over R workgroups.
+__global__ void k_dt_partial(const int8_t *label, const double *h, const double *ax, const double *ay, int n,
+                             double *part) {
+    __shared__ double lds[8];
+    DtVars v;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (label[i] == kFluid) v.add(h[i], ax[i], ay[i]);
+    v.publish(part, lds);
 }
 
 // Solver._minTimeStep (src/Solver.py:210-230): courant (TimeStep.py:47-49),
-// force (:51-56), min(c, f) (:45). Also opens the step: the reference's loop
-// condition `while self.t < self.duration` (src/Solver.py:366) and
-// `self.t += self.dt` (:409) move here, so a captured step needs no host.
-__global__ void k_dt_final(Ctrl *ctrl, const double *red, int nb, double gamma_c, double gamma_f, double *hist) {
-    __shared__ double lds[8];
-    double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        mn = MinOp()(mn, red[b * 4 + 0]);
-        mx = MaxOp()(mx, red[b * 4 + 1]);
-        cnt += red[b * 4 + 2];
-        bad += red[b * 4 + 3];
+// force (:51-56), min(c, f) (:45), from the R partials of the fluid
+// reduction. Also opens the step: the reference's loop condition
+// `while self.t < self.duration` (src/Solver.py:366) and `self.t += self.dt`
+// (:409) move here, so a captured step needs no host. One workgroup.
+__global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *hist, const double *part, int R) {
+    __shared__ double lds[16];
+    double mn = INFINITY, mx = -INFINITY, cnt_d = 0.0, bad_d = 0.0;
+    for (int k = threadIdx.x; k < R; k += blockDim.x) {
+        const double a = part[0 * R + k], b = part[1 * R + k];
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+        cnt_d += part[2 * R + k];
+        bad_d += part[3 * R + k];
     }
     mn = block_reduce(mn, MinOp(), lds);
     mx = block_reduce(mx, MaxOp(), lds);
-    cnt = block_reduce(cnt, AddOp(), lds);
-    bad = block_reduce(bad, AddOp(), lds);
+    cnt_d = block_reduce(cnt_d, AddOp(), lds);
+    bad_d = block_reduce(bad_d, AddOp(), lds);
     if (threadIdx.x != 0) return;
+    const double cnt = cnt_d, bad = bad_d;
     const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID;
     if ((ctrl->flags & halt) || !(ctrl->t < ctrl->duration)) {
         ctrl->active = 0;
         return;
     }
-    if (cnt == 0.0) {
+    if (cnt == 0) {
         ctrl->flags |= SPH_FLAG_NO_FLUID;
         ctrl->active = 0;
         return;
     }
-    if (bad != 0.0) ctrl->flags |= SPH_FLAG_NONFINITE;
+    if (bad != 0) ctrl->flags |= SPH_FLAG_NONFINITE;
     double dt, c, f;
     if (ctrl->fixed_dt > 0.0) {
         dt = ctrl->fixed_dt;  // src/Solver.py:370-372
@@ -204,7 +220,18 @@ __global__ void k_dt_final(Ctrl *ctrl, const double *red, int nb, double gamma_c
 // ---------------------------------------------------------------------------
 // integrators (fluid only: src/Solver.py:380,396 pass pA[f_indexes])
 // ---------------------------------------------------------------------------
-// src/Integrators/PEC.py:32-59 (useXSPH = False branch)
+// src/Integrators/PEC.py:32-59 (useXSPH = False branch), one particle. The
+// fused step evaluates the same expressions inside the neighbour update.
+__device__ inline double pec_half_pos(double x, double v, double dt) { return x + 0.5 * dt * v; }
+__device__ inline double pec_half_vel(double v, double a, double dt, double damping) {
+    return (v + 0.5 * dt * a) / (1 + 0.5 * damping);
+}
+__device__ inline double pec_half_rho(double rho, double drho, double dt, int strict) {
+    double r = rho + 0.5 * dt * drho;
+    if (strict && r < 0.0) r = 0.0;
+    return r;
+}
+
 __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
     if (!ctrl->active) return;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -215,102 +242,111 @@ __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
     s.y0[j] = y;
     s.vx0[j] = vx;
     s.vy0[j] = vy;
-    s.x[j] = x + 0.5 * dt * vx;
-    s.y[j] = y + 0.5 * dt * vy;
-    s.vx[j] = (vx + 0.5 * dt * s.ax[j]) / (1 + 0.5 * damping);
-    s.vy[j] = (vy + 0.5 * dt * s.ay[j]) / (1 + 0.5 * damping);
+    s.x[j] = pec_half_pos(x, vx, dt);
+    s.y[j] = pec_half_pos(y, vy, dt);
+    s.vx[j] = pec_half_vel(vx, s.ax[j], dt, damping);
+    s.vy[j] = pec_half_vel(vy, s.ay[j], dt, damping);
     s.rho0[j] = rho;
-    double r = rho + 0.5 * dt * s.drho[j];
-    if (strict && r < 0.0) r = 0.0;
-    s.rho[j] = r;
+    s.rho[j] = pec_half_rho(rho, s.drho[j], dt, strict);
 }
 
-// src/Integrators/PEC.py:61-89 and src/Integrators/Euler.py:17-26
-__global__ void k_correct(const Ctrl *ctrl, State s, int n, int integrator, int strict) {
+// src/Integrators/PEC.py:61-89 and src/Integrators/Euler.py:17-26. Also
+// leaves the next step's dt partials (h and a of the fluid do not change
+// until the next pair pass) in `part`. Grid-stride over R workgroups.
+__global__ void k_correct(Ctrl *ctrl, State s, int n, int integrator, int strict, double *part) {
     if (!ctrl->active) return;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n || s.label[j] != kFluid) return;
+    __shared__ double lds[8];
     const double dt = ctrl->dt, damping = ctrl->damping;
-    if (integrator == SPH_INTEGRATOR_PEC) {
-        double x0 = s.x0[j], y0 = s.y0[j], vx0 = s.vx0[j], vy0 = s.vy0[j], rho0 = s.rho0[j];
-        double mid_x = x0 + 0.5 * dt * s.vx[j];
-        double mid_y = y0 + 0.5 * dt * s.vy[j];
-        double mid_vx = (vx0 + 0.5 * dt * s.ax[j]) / (1 + 0.5 * damping);
-        double mid_vy = (vy0 + 0.5 * dt * s.ay[j]) / (1 + 0.5 * damping);
-        s.x[j] = 2 * mid_x - x0;
-        s.y[j] = 2 * mid_y - y0;
-        s.vx[j] = 2 * mid_vx - vx0;
-        s.vy[j] = 2 * mid_vy - vy0;
-        double mid_rho = rho0 + 0.5 * dt * s.drho[j];
-        double r = 2 * mid_rho - rho0;
-        if (strict && r < 0.0) r = 0.0;
-        s.rho[j] = r;
-    } else {
-        double vx = s.vx[j], vy = s.vy[j], ax = s.ax[j], ay = s.ay[j];
-        s.x[j] = s.x[j] + dt * vx + 0.5 * dt * dt * ax;
-        s.y[j] = s.y[j] + dt * vy + 0.5 * dt * dt * ay;
-        s.vx[j] = vx + dt * ax;
-        s.vy[j] = vy + dt * ay;
-        s.rho[j] = s.rho[j] + dt * s.drho[j];
+    DtVars v;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        if (s.label[j] != kFluid) continue;
+        const double ax = s.ax[j], ay = s.ay[j];
+        v.add(s.h[j], ax, ay);
+        if (integrator == SPH_INTEGRATOR_PEC) {
+            double x0 = s.x0[j], y0 = s.y0[j], vx0 = s.vx0[j], vy0 = s.vy0[j], rho0 = s.rho0[j];
+            double mid_x = x0 + 0.5 * dt * s.vx[j];
+            double mid_y = y0 + 0.5 * dt * s.vy[j];
+            double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
+            double mid_vy = (vy0 + 0.5 * dt * ay) / (1 + 0.5 * damping);
+            s.x[j] = 2 * mid_x - x0;
+            s.y[j] = 2 * mid_y - y0;
+            s.vx[j] = 2 * mid_vx - vx0;
+            s.vy[j] = 2 * mid_vy - vy0;
+            double mid_rho = rho0 + 0.5 * dt * s.drho[j];
+            double r = 2 * mid_rho - rho0;
+            if (strict && r < 0.0) r = 0.0;
+            s.rho[j] = r;
+        } else {
+            double vx = s.vx[j], vy = s.vy[j];
+            s.x[j] = s.x[j] + dt * vx + 0.5 * dt * dt * ax;
+            s.y[j] = s.y[j] + dt * vy + 0.5 * dt * dt * ay;
+            s.vx[j] = vx + dt * ax;
+            s.vy[j] = vy + dt * ay;
+            s.rho[j] = s.rho[j] + dt * s.drho[j];
+        }
     }
+    v.publish(part, lds);
 }
 
 // ---------------------------------------------------------------------------
 // neighbour search: src/Tools/NNLinkedList.py
 // ---------------------------------------------------------------------------
-// _init :86-101 bounds and _get_cell_size :103-110 min h, over all active.
-__global__ void k_bounds_partial(const Ctrl *ctrl, const double *x, const double *y, const double *h, int n,
-                                 double *red) {
-    if (!ctrl->active) return;
-    __shared__ double lds[8];
-    double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, bad = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double xi = x[i], yi = y[i], hi = h[i];
-        if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;  // NaN or Inf
-        xmn = MinOp()(xmn, xi);
-        xmx = MaxOp()(xmx, xi);
-        ymn = MinOp()(ymn, yi);
-        ymx = MaxOp()(ymx, yi);
-        hmn = MinOp()(hmn, hi);
-    }
-    xmn = block_reduce(xmn, MinOp(), lds);
-    xmx = block_reduce(xmx, MaxOp(), lds);
-    ymn = block_reduce(ymn, MinOp(), lds);
-    ymx = block_reduce(ymx, MaxOp(), lds);
-    hmn = block_reduce(hmn, MinOp(), lds);
-    bad = block_reduce(bad, AddOp(), lds);
-    if (threadIdx.x == 0) {
-        double *r = red + blockIdx.x * 8;
-        r[0] = xmn;
-        r[1] = xmx;
-        r[2] = ymn;
-        r[3] = ymx;
-        r[4] = hmn;
-        r[5] = bad;
+// Position the neighbour update sees: the predicted one when the PEC
+// predictor is fused into the update (fluid only), else the stored one.
+__device__ inline void upd_pos(const State &a, int i, int pred, double dt, double &x, double &y) {
+    x = a.x[i];
+    y = a.y[i];
+    if (pred && a.label[i] == kFluid) {
+        x = pec_half_pos(x, a.vx[i], dt);
+        y = pec_half_pos(y, a.vy[i], dt);
     }
 }
 
-// _get_cell_size :103-110 and _get_number_of_cells :112-127.
-__global__ void k_bounds_final(Ctrl *ctrl, const double *red, int nb, double scale, long long cap) {
+// _init :86-101 bounds and _get_cell_size :103-110 min h, over all active
+// particles: per-workgroup partials, folded by k_grid. Grid-stride over R
+// workgroups.
+__global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *part) {
     if (!ctrl->active) return;
     __shared__ double lds[8];
-    double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, bad = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        const double *r = red + b * 8;
-        xmn = MinOp()(xmn, r[0]);
-        xmx = MaxOp()(xmx, r[1]);
-        ymn = MinOp()(ymn, r[2]);
-        ymx = MaxOp()(ymx, r[3]);
-        hmn = MinOp()(hmn, r[4]);
-        bad += r[5];
+    double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, hmx = -INFINITY;
+    double bad = 0.0;
+    const double dt = ctrl->dt;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double xi, yi;
+        upd_pos(a, i, pred, dt, xi, yi);
+        const double hi = a.h[i];
+        if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;  // NaN or Inf
+        xmn = xi < xmn ? xi : xmn;
+        xmx = xi > xmx ? xi : xmx;
+        ymn = yi < ymn ? yi : ymn;
+        ymx = yi > ymx ? yi : ymx;
+        hmn = hi < hmn ? hi : hmn;
+        hmx = hi > hmx ? hi : hmx;
     }
     xmn = block_reduce(xmn, MinOp(), lds);
     xmx = block_reduce(xmx, MaxOp(), lds);
     ymn = block_reduce(ymn, MinOp(), lds);
     ymx = block_reduce(ymx, MaxOp(), lds);
     hmn = block_reduce(hmn, MinOp(), lds);
+    hmx = block_reduce(hmx, MaxOp(), lds);
     bad = block_reduce(bad, AddOp(), lds);
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x == 0) {
+        const int R = gridDim.x, k = blockIdx.x;
+        part[0 * R + k] = xmn;
+        part[1 * R + k] = xmx;
+        part[2 * R + k] = ymn;
+        part[3 * R + k] = ymx;
+        part[4 * R + k] = hmn;
+        part[5 * R + k] = hmx;
+        part[6 * R + k] = bad;
+    }
+}
+
+// _get_cell_size :103-110 and _get_number_of_cells :112-127 from the folded
+// bounds; writes the grid (or the halt flags) to the control block. Returns
+// false when the step must halt.
+__device__ bool derive_grid(Ctrl *ctrl, double scale, long long cap, const double *v) {
+    const double xmn = v[0], xmx = v[1], ymn = v[2], ymx = v[3], hmn = v[4], hmx = v[5], bad = v[6];
     Grid g;
     g.xmin = xmn;
     g.xmax = xmx;
@@ -319,6 +355,7 @@ __global__ void k_bounds_final(Ctrl *ctrl, const double *red, int nb, double sca
     double cell_size = hmn * scale;
     if (cell_size < 1e-6) cell_size = 1.0;
     g.cs = cell_size;
+    g.uniform_h = hmn == hmx;
     double cell_size1 = 1. / cell_size;
     double fx = ceil(cell_size1 * (xmx - xmn));
     double fy = ceil(cell_size1 * (ymx - ymn));
@@ -326,7 +363,7 @@ __global__ void k_bounds_final(Ctrl *ctrl, const double *red, int nb, double sca
         ctrl->flags |= SPH_FLAG_NONFINITE;
         ctrl->active = 0;
         ctrl->nn_valid = 0;
-        return;
+        return false;
     }
     long long ncx = (long long)fx, ncy = (long long)fy;
     if (ncx < 1) ncx = 1;
@@ -339,35 +376,87 @@ __global__ void k_bounds_final(Ctrl *ctrl, const double *red, int nb, double sca
         ctrl->flags |= SPH_FLAG_CAPACITY;
         ctrl->active = 0;
         ctrl->nn_valid = 0;
+        return false;
     }
+    return true;
+}
+
+// Fold the R bounds partials and derive the grid. One workgroup.
+__global__ void k_grid(Ctrl *ctrl, const double *part, int R, double scale, long long cap) {
+    if (!ctrl->active) return;
+    __shared__ double lds[16];
+    __shared__ double v[kBoundsQ];
+    for (int q = 0; q < kBoundsQ; ++q) {
+        const bool is_max = q == 1 || q == 3 || q == 5;
+        double acc = q == 6 ? 0.0 : (is_max ? -INFINITY : INFINITY);
+        for (int k = threadIdx.x; k < R; k += blockDim.x) {
+            const double u = part[q * R + k];
+            acc = q == 6 ? acc + u : (is_max ? (u > acc ? u : acc) : (u < acc ? u : acc));
+        }
+        if (q == 6) acc = block_reduce(acc, AddOp(), lds);
+        else if (is_max) acc = block_reduce(acc, MaxOp(), lds);
+        else acc = block_reduce(acc, MinOp(), lds);
+        if (threadIdx.x == 0) v[q] = acc;
+    }
+    if (threadIdx.x == 0) derive_grid(ctrl, scale, cap, v);
 }
 
 // _bin :129-141: cid = floor((x - xmin) / cs) per axis (:146-176), flat key
-// cx + ncx * cy (:200-208). Counts per bin, rank inside the bin from the
-// atomic (the order is fixed afterwards by k_cellsort).
-__global__ void k_keys(Ctrl *ctrl, const double *x, const double *y, int n, int *counts, int *bin, int *rank,
-                       int2 *cxy) {
+// cx + ncx * cy (:200-208). Counts per bin; the rank inside a bin comes from
+// one atomic per run of equal bins within a wavefront (the state is already
+// in the previous step's cell order, so ~7 consecutive lanes share a bin).
+// k_cellsort fixes the order inside each bin afterwards.
+__global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *bin, int *rank, int2 *cxy,
+                       int *binflag) {
     if (!ctrl->active) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const Grid g = ctrl->grid;
-    double fx = floor((x[t] - g.xmin) / g.cs);
-    double fy = floor((y[t] - g.ymin) / g.cs);
-    // clamp only for the int conversion; any clamped value is far outside the grid
-    fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
-    fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
-    long long cx = (long long)fx, cy = (long long)fy;
-    long long key = cx + g.ncx * cy;
-    int b;
-    if (key >= 0 && key < g.ncells) {
-        b = (int)key;
-    } else {
-        b = (int)g.ncells;
-        atomicOr(&ctrl->flags, (unsigned)SPH_FLAG_OOB_KEY);
+    int b = -1;
+    if (t < n) {
+        double x, y;
+        upd_pos(a, t, pred, ctrl->dt, x, y);
+        double fx = floor((x - g.xmin) / g.cs);
+        double fy = floor((y - g.ymin) / g.cs);
+        // clamp only for the int conversion; any clamped value is far outside the grid
+        fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
+        fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
+        long long cx = (long long)fx, cy = (long long)fy;
+        long long key = cx + g.ncx * cy;
+        if (key >= 0 && key < g.ncells) {
+            b = (int)key;
+        } else {
+            b = (int)g.ncells;
+            atomicOr(&ctrl->flags, (unsigned)SPH_FLAG_OOB_KEY);
+        }
+        cxy[t] = make_int2((int)cx, (int)cy);
     }
-    bin[t] = b;
-    rank[t] = atomicAdd(&counts[b], 1);
-    cxy[t] = make_int2((int)cx, (int)cy);
+    // Runs of equal bins in lane order, one atomic per run. A run also breaks
+    // where the reference index drops (consecutive slots from two old cells),
+    // so every run is in reference-index order; a bin that receives two runs
+    // is listed for k_cellsort below.
+    const int id = t < n ? a.id[t] : 0x7fffffff;
+    const int prev = __shfl_up(b, 1, 64);
+    const int prev_id = __shfl_up(id, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || b != prev || id < prev_id);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int leader = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int next = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (lane == leader && b >= 0) {
+        base = atomicAdd(&counts[b], next - leader);
+        // A bin filled by more than one run may be out of reference-index
+        // order: flag it for k_cellsort, which clears the flag again. Every
+        // run but the first to arrive sees base != 0, so such a bin is always
+        // flagged; the store is idempotent, so no atomic is needed.
+        if (base != 0) binflag[b] = 1;
+    }
+    base = __shfl(base, leader, 64);
+    if (t < n) {
+        bin[t] = b;
+        rank[t] = base + (lane - leader);
+    }
 }
 
 // Exclusive prefix scan of counts[0 .. ncells] into cellStart, in tiles of
@@ -467,25 +556,64 @@ __global__ void k_scatter(const Ctrl *ctrl, const int *cellStart, const int *bin
     permKey[pos] = id[t];
 }
 
+// Sort the slots [s, e) of one bin by key (the reference index; keys are
+// distinct) with one wavefront: each element's destination is the number of
+// smaller keys in the bin. Bins of up to 64 sort in registers (one load and
+// one store round); larger ones rank through global memory and stage the
+// result in tmpKey/tmpVal before copying it back.
+__device__ void wave_sort_bin(int s, int e, int *perm, int *permKey, int *tmpKey, int *tmpVal) {
+    const int lane = threadIdx.x & 63;
+    const int len = e - s;
+    if (len <= 64) {
+        const int key = lane < len ? permKey[s + lane] : 0x7fffffff;
+        const int val = lane < len ? perm[s + lane] : 0;
+        int r = 0;
+        for (int j = 0; j < len; ++j) r += __shfl(key, j, 64) < key;
+        if (lane < len) {
+            permKey[s + r] = key;
+            perm[s + r] = val;
+        }
+        return;
+    }
+    for (int i = s + lane; i < e; i += 64) {
+        const int key = permKey[i];
+        int r = 0;
+        for (int j = s; j < e; ++j) r += permKey[j] < key;
+        tmpKey[s + r] = key;
+        tmpVal[s + r] = perm[i];
+    }
+    __threadfence_block();
+    for (int i = s + lane; i < e; i += 64) {
+        permKey[i] = tmpKey[i];
+        perm[i] = tmpVal[i];
+    }
+}
+
 // Order each bin by the reference index, which is what the reference's
 // head-inserted lists are ordered by (descending there; the pair pass walks
-// the bin backwards).
-__global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, int *permKey) {
+// the bin backwards). Only bins flagged by k_keys (filled by more than one
+// run) can be out of order. Each wavefront reads the flags of 64 consecutive
+// bins, clears them and sorts the flagged ones one after another.
+__global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, int *permKey, int *binflag,
+                           int *tmpKey, int *tmpVal) {
     if (!ctrl->active) return;
-    long long nbins = ctrl->grid.ncells + 1;
-    for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nbins;
-         b += (long long)gridDim.x * blockDim.x) {
-        int s = cellStart[b], e = cellStart[b + 1];
-        for (int a = s + 1; a < e; ++a) {
-            int k = permKey[a], v = perm[a];
-            int p = a - 1;
-            while (p >= s && permKey[p] > k) {
-                permKey[p + 1] = permKey[p];
-                perm[p + 1] = perm[p];
-                --p;
-            }
-            permKey[p + 1] = k;
-            perm[p + 1] = v;
+    const long long nbins = ctrl->grid.ncells + 1;
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long b0 = wave * 64; b0 < nbins; b0 += nwaves * 64) {
+        const long long bl = b0 + lane;
+        int f = 0;
+        if (bl < nbins) {
+            f = binflag[bl];
+            if (f) binflag[bl] = 0;
+        }
+        unsigned long long m = __ballot(f != 0);
+        while (m) {
+            const int k = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const long long b = b0 + k;
+            wave_sort_bin(cellStart[b], cellStart[b + 1], perm, permKey, tmpKey, tmpVal);
         }
     }
 }
@@ -504,16 +632,38 @@ __device__ inline double tait_p(double rho, int8_t label, double gamma, double B
 // pair-pass records: p / rho^2 per particle (the momentum equation's slf and
 // othr terms, src/Equations/Momentum.py:28,36) and w = m for fluid, -label
 // otherwise.
-__global__ void k_reorder(const Ctrl *ctrl, State a, State b, const int *perm, const int2 *cxy_old, int2 *cxy_s,
-                          Phys ph, int n, int fp32, void *recA, void *recB) {
+__global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const int2 *cxy_old, int2 *cxy_s, Phys ph,
+                          int n, int fp32, void *recA, void *recB, int pred, int strict) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
     const int act = ctrl->active;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    if (t == 0 && act) ctrl->nn_valid = 1;
     int s = act ? perm[t] : t;
     double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
+    const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
     int8_t lab = a.label[s];
+    if (act && pred && lab == kFluid) {
+        // PEC.predict (src/Integrators/PEC.py:32-59) fused into the permutation
+        const double dt = ctrl->dt, damping = ctrl->damping;
+        b.x0[t] = x;
+        b.y0[t] = y;
+        b.vx0[t] = vx;
+        b.vy0[t] = vy;
+        b.rho0[t] = rho;
+        x = pec_half_pos(x, vx, dt);
+        y = pec_half_pos(y, vy, dt);
+        vx = pec_half_vel(vx, ax, dt, damping);
+        vy = pec_half_vel(vy, ay, dt, damping);
+        rho = pec_half_rho(rho, drho, dt, strict);
+    } else {
+        b.x0[t] = a.x0[s];
+        b.y0[t] = a.y0[s];
+        b.vx0[t] = a.vx0[s];
+        b.vy0[t] = a.vy0[s];
+        b.rho0[t] = a.rho0[s];
+    }
     b.x[t] = x;
     b.y[t] = y;
     b.vx[t] = vx;
@@ -523,14 +673,9 @@ __global__ void k_reorder(const Ctrl *ctrl, State a, State b, const int *perm, c
     b.h[t] = h;
     b.label[t] = lab;
     b.id[t] = a.id[s];
-    b.ax[t] = a.ax[s];
-    b.ay[t] = a.ay[s];
-    b.drho[t] = a.drho[s];
-    b.x0[t] = a.x0[s];
-    b.y0[t] = a.y0[s];
-    b.vx0[t] = a.vx0[s];
-    b.vy0[t] = a.vy0[s];
-    b.rho0[t] = a.rho0[s];
+    b.ax[t] = ax;
+    b.ay[t] = ay;
+    b.drho[t] = drho;
     if (!act) return;
     cxy_s[t] = cxy_old[s];
     const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
@@ -588,10 +733,6 @@ __global__ void k_download(State s, sph_state_t out, Phys ph, int n) {
     if (out.h) out.h[i] = s.h[t];
     if (out.label) out.label[i] = s.label[t];
     if (out.p) out.p[i] = tait_p(s.rho[t], s.label[t], ph.gamma, ph.B, ph.rho0, ph.Pb);
-}
-
-__global__ void k_mark_nn(Ctrl *ctrl, int v) {
-    if (v == 0 || ctrl->active) ctrl->nn_valid = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -731,7 +872,7 @@ static inline int nblk(long long n, int bs = kBlock) {
 
 static inline int redblk(long long n) {
     int b = nblk(n);
-    return b > kMaxRedBlocks ? kMaxRedBlocks : b;
+    return b > kRedBlocks ? kRedBlocks : b;
 }
 
 static hipStream_t S(void *s) { return (hipStream_t)s; }
@@ -743,7 +884,7 @@ static void free_ctx(sph_ctx *c) {
         if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
     }
     void *ptrs[] = {c->ctrl, c->dt_hist, c->bin, c->rank, c->perm, c->permKey, c->cxy_old, c->cxy_s,
-                    c->counts, c->cellStart, c->tileSums, c->red, c->recA, c->recB, c->slot_of};
+                    c->counts, c->cellStart, c->tileSums, c->binflag, c->part_b, c->part_d, c->recA, c->recB, c->slot_of};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : c->ev_pair) hipEventDestroy(e);
@@ -756,15 +897,19 @@ static int alloc_cells(sph_ctx *c, long long cap) {
     if (c->counts) hipFree(c->counts);
     if (c->cellStart) hipFree(c->cellStart);
     if (c->tileSums) hipFree(c->tileSums);
+    if (c->binflag) hipFree(c->binflag);
     c->counts = nullptr;
     c->cellStart = nullptr;
     c->tileSums = nullptr;
+    c->binflag = nullptr;
     long long tiles = (cap + 1 + kScanTile - 1) / kScanTile;
     HIPCHK(hipMalloc(&c->counts, sizeof(int) * (cap + 1)));
     HIPCHK(hipMalloc(&c->cellStart, sizeof(int) * (cap + 2)));
     HIPCHK(hipMalloc(&c->tileSums, sizeof(int) * (tiles + 1)));
     HIPCHK(hipMemset(c->counts, 0, sizeof(int) * (cap + 1)));
     HIPCHK(hipMemset(c->cellStart, 0, sizeof(int) * (cap + 2)));
+    HIPCHK(hipMalloc(&c->binflag, sizeof(int) * (cap + 1)));
+    HIPCHK(hipMemset(c->binflag, 0, sizeof(int) * (cap + 1)));
     c->cap_cells = cap;
     return 0;
 }
@@ -845,7 +990,9 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
               hipMalloc(&c->permKey, sizeof(int) * n) == hipSuccess &&
               hipMalloc(&c->cxy_old, sizeof(int2) * n) == hipSuccess &&
               hipMalloc(&c->cxy_s, sizeof(int2) * n) == hipSuccess &&
-              hipMalloc(&c->red, sizeof(double) * 8 * kMaxRedBlocks) == hipSuccess &&
+
+              hipMalloc(&c->part_b, sizeof(double) * kBoundsQ * kRedBlocks) == hipSuccess &&
+              hipMalloc(&c->part_d, sizeof(double) * kDtQ * kRedBlocks) == hipSuccess &&
               hipMalloc(&c->slot_of, sizeof(int) * n) == hipSuccess;
     if (ok) {
         const bool f32 = params->precision == SPH_PREC_FP32C;
@@ -897,6 +1044,7 @@ int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params) {
     int rc = check_params(params);
     if (rc) return rc;
     ctx->P = *params;
+    ctx->dt_fresh = false;
     Ctrl h;
     HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
     ctrl_from_params(h, ctx->P);
@@ -914,6 +1062,7 @@ int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock,
         return fail(SPH_EINVAL, "upload needs x, y, vx, vy, rho, m, h and label");
     hipStream_t s = S(stream);
     ctx->cur = 0;
+    ctx->dt_fresh = false;
     k_upload<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->buf[0], *st, ctx->n);
     HIPCHK(hipGetLastError());
     Ctrl h;
@@ -954,7 +1103,9 @@ static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipSt
     return 0;
 }
 
-static int nn_update(sph_ctx *c, hipStream_t s) {
+// pred != 0: the PEC predictor runs inside the update (bounds and keys see
+// the predicted positions, the reorder writes the predicted state).
+static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0) {
     const int n = c->n;
     State a = c->buf[c->cur];
     State b = c->buf[1 - c->cur];
@@ -962,20 +1113,21 @@ static int nn_update(sph_ctx *c, hipStream_t s) {
         int rc = ev_record(c, c->ev_nn, c->n_nn, s);
         if (rc) return rc;
     }
-    int rb = redblk(n);
-    k_bounds_partial<<<rb, kBlock, 0, s>>>(c->ctrl, a.x, a.y, a.h, n, c->red);
-    k_bounds_final<<<1, kBlock, 0, s>>>(c->ctrl, c->red, rb, c->P.scale, c->cap_cells);
-    k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a.x, a.y, n, c->counts, c->bin, c->rank, c->cxy_old);
+    const int R = redblk(n);
+    k_bounds<<<R, kBlock, 0, s>>>(c->ctrl, a, n, pred, c->part_b);
+    k_grid<<<1, 1024, 0, s>>>(c->ctrl, c->part_b, R, c->P.scale, c->cap_cells);
+    k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, n, pred, c->counts, c->bin, c->rank, c->cxy_old, c->binflag);
     long long tiles_cap = (c->cap_cells + 1 + kScanTile - 1) / kScanTile;
     int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
     k_scan_reduce<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums);
     k_scan_tiles<<<1, 1024, 0, s>>>(c->ctrl, c->tileSums);
     k_scan_apply<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums, c->cellStart);
     k_scatter<<<nblk(n), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->bin, c->rank, a.id, c->perm, c->permKey, n);
-    k_cellsort<<<redblk(c->cap_cells + 1), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->perm, c->permKey);
+    // bin/rank are free once scattered: they stage the rare bins of more than 64
+    k_cellsort<<<redblk(c->cap_cells + 1), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->perm, c->permKey,
+                                                           c->binflag, c->bin, c->rank);
     k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_old, c->cxy_s, make_phys(c->P), n,
-                                         c->P.precision == SPH_PREC_FP32C, c->recA, c->recB);
-    k_mark_nn<<<1, 1, 0, s>>>(c->ctrl, 1);
+                                         c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict);
     HIPCHK(hipGetLastError());
     if (c->timing) {
         int rc = ev_record(c, c->ev_nn, c->n_nn, s);
@@ -1006,9 +1158,10 @@ static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr) {
 
 static int dt_step(sph_ctx *c, hipStream_t s) {
     State st = c->buf[c->cur];
-    int rb = redblk(c->n);
-    k_dt_partial<<<rb, kBlock, 0, s>>>(st.label, st.h, st.ax, st.ay, c->n, c->red);
-    k_dt_final<<<1, kBlock, 0, s>>>(c->ctrl, c->red, rb, c->P.gamma_c, c->P.gamma_f, c->dt_hist);
+    if (!c->dt_fresh) {
+        k_dt_partial<<<redblk(c->n), kBlock, 0, s>>>(st.label, st.h, st.ax, st.ay, c->n, c->part_d);
+    }
+    k_dt_final<<<1, 1024, 0, s>>>(c->ctrl, c->P.gamma_c, c->P.gamma_f, c->dt_hist, c->part_d, redblk(c->n));
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1021,18 +1174,22 @@ static int predict(sph_ctx *c, hipStream_t s) {
 }
 
 static int correct(sph_ctx *c, hipStream_t s) {
-    k_correct<<<nblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, c->P.integrator, c->P.strict);
+    k_correct<<<redblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, c->P.integrator, c->P.strict,
+                                              c->part_d);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
+// One Solver.run loop body (src/Solver.py:366-410) with the predictor fused
+// into the neighbour update and the next dt reduction into the corrector.
 static int one_step(sph_ctx *c, hipStream_t s) {
     int rc;
     if ((rc = dt_step(c, s))) return rc;
-    if ((rc = predict(c, s))) return rc;
-    if ((rc = nn_update(c, s))) return rc;
+    if ((rc = nn_update(c, s, c->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
     if ((rc = compute(c, s))) return rc;
-    return correct(c, s);
+    if ((rc = correct(c, s))) return rc;
+    c->dt_fresh = true;
+    return 0;
 }
 
 int sph_nn_update(sph_ctx_t *ctx, void *stream) {
@@ -1042,6 +1199,7 @@ int sph_nn_update(sph_ctx_t *ctx, void *stream) {
 
 int sph_compute(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    ctx->dt_fresh = false;
     return compute(ctx, S(stream));
 }
 
@@ -1057,7 +1215,9 @@ int sph_predict(sph_ctx_t *ctx, void *stream) {
 
 int sph_correct(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
-    return correct(ctx, S(stream));
+    int rc = correct(ctx, S(stream));
+    if (!rc) ctx->dt_fresh = true;
+    return rc;
 }
 
 int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream) {
@@ -1080,6 +1240,11 @@ int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream) {
     HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
     HIPCHK(hipEventRecord(e0, s));
     HIPCHK(hipStreamWaitEvent(g, e0, 0));
+    if (!ctx->dt_fresh && nsteps > 0) {  // the captured step expects the corrector's dt partials
+        int rc = one_step(ctx, g);
+        if (rc) return rc;
+        --nsteps;
+    }
     for (int64_t k = 0; k < nsteps; ++k) {
         int start = ctx->cur;
         if (!ctx->gexec[start]) {
@@ -1207,6 +1372,7 @@ int sph_compute_counts(sph_ctx_t *ctx, int32_t *counts, void *stream) {
     int rc = sph_nn_grid(ctx, nullptr, nullptr, stream);
     if (rc) return rc;
     // ctx->bin is free between updates: use it as the cell-order count buffer
+    ctx->dt_fresh = false;
     if ((rc = compute(ctx, s, ctx->bin))) return rc;
     k_unpermute_int<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->bin, ctx->buf[ctx->cur].id, counts, ctx->n);
     HIPCHK(hipGetLastError());

@@ -26,11 +26,12 @@ constexpr int kFluid = 0;
 constexpr int kBlock = 256;        // threads per block of streaming kernels
 constexpr int kScanItems = 8;      // bins per thread in the cell scan
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr int kMaxRedBlocks = 1024;
+constexpr int kRedBlocks = 2048;   // max workgroups of a grid-wide reduction
 
 struct Grid {
     double xmin, xmax, ymin, ymax, cs;
     long long ncx, ncy, ncells;
+    int uniform_h;  // min h == max h over all particles
 };
 
 struct Ctrl {
@@ -44,6 +45,15 @@ struct Ctrl {
     int advance_t;       // 0 while settling (t does not advance)
     int nn_valid;        // cell order and offsets match the positions
 };
+
+// Grid-wide reductions (bounds, dt) run over at most kRedBlocks workgroups in
+// grid-stride loops; each workgroup stores its partial result with plain
+// stores into part[q * R + blockIdx.x] (q = quantity, R = workgroups of that
+// launch) and a single-workgroup kernel folds the R partials. Same-address
+// device-scope atomics from every workgroup serialise across the XCDs and
+// cost ~10 ns each, far more than the second launch.
+constexpr int kBoundsQ = 7;  // min/max x, min/max y, min/max h, non-finite count
+constexpr int kDtQ = 4;      // min h, max |a|^2, fluid count, non-finite count
 
 // One cell-ordered buffer set.
 struct State {
