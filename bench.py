@@ -93,6 +93,31 @@ def run_cpu(sc, budget_s=12.0, max_steps=20, nthreads=1):
     return {'steps': k, 'elapsed_s': el, 'value': sc.n_fluid * k / el}
 
 
+def profile_pair(sc, launches):
+    """Counter-collection driver: `launches` pair passes per precision on the
+    1M workload (after 3 steps), then a plain 256 MiB device copy whose
+    FETCH_SIZE / WRITE_SIZE calibrate the counters to bytes."""
+    import torch
+    from hipsph.device import Engine, EngineParams, soa_from_particles
+    for prec in ('fp32c', 'fp64'):
+        e = Engine(EngineParams.from_method(sc.n, sc.method(), precision=prec))
+        e.upload(soa_from_particles(sc.pA))
+        e.step(3)
+        e.nn_update()
+        for _ in range(launches):
+            e.compute()
+        torch.cuda.synchronize()
+        print(f'# profiled {launches} pair passes, {prec}', file=sys.stderr, flush=True)
+        e.close()
+    src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f'# calibration copy: {src.numel() * 8} bytes read and written per launch', file=sys.stderr, flush=True)
+
+
 def traffic_from_profiles(precision):
     p = os.path.join(ROOT, 'profiles', 'pair_traffic.json')
     if not os.path.exists(p):
@@ -114,6 +139,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=12.0)
     ap.add_argument('--small', action='store_true', help='C1 (dx 0.02) instead of the 1M workload')
+    ap.add_argument('--cpu-threads', type=int, default=16,
+                    help='threads of the second (OpenMP) CPU baseline; 0 skips it')
+    ap.add_argument('--profile-pair', type=int, default=0, metavar='K',
+                    help='only run K pair passes per precision and a calibration copy (for counter runs)')
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU (slab-decomposed) path is not built. '
@@ -123,6 +152,9 @@ def main():
     from hipsph import scenarios
     sc = scenarios.dam_break_c1() if a.small else scenarios.dam_break_c2(million=True)
     print(f'# workload {sc.name}: {sc.n} particles ({sc.n_fluid} fluid)', file=sys.stderr, flush=True)
+    if a.profile_pair:
+        profile_pair(sc, a.profile_pair)
+        return
 
     r32 = run_gpu(sc, 'fp32c', a.steps, a.warmup, a.graph_steps)
     print(f'# fp32c {r32["ms_per_step"]:.4f} ms/step, pair {r32["pair_ms"]:.4f} ms', file=sys.stderr, flush=True)
@@ -138,7 +170,7 @@ def main():
     ach64 = b64 / (r64['pair_ms'] * 1e-3) / 1e9
     pairs = r64['pairs_per_step']
 
-    cpu = None
+    cpu = cpu_omp = None
     if not a.no_cpu:
         c1 = run_cpu(sc, a.cpu_budget, nthreads=1)
         print(f'# cpu oracle 1 thread: {c1["value"]:.4g} particle-steps/s over {c1["steps"]} steps',
@@ -146,6 +178,12 @@ def main():
         cpu = {'value': c1['value'], 'unit': 'particle-steps/s', 'cores': 1, 'kind': 'port',
                'sample': f'{c1["steps"]} steps of the same {n}-particle dam break from the same initial state '
                          f'({c1["elapsed_s"]:.1f} s), serial C port of the reference step (oracle/refcpu.c)'}
+        if a.cpu_threads > 1:
+            cm = run_cpu(sc, a.cpu_budget, nthreads=a.cpu_threads)
+            print(f'# cpu oracle {a.cpu_threads} threads: {cm["value"]:.4g} particle-steps/s over {cm["steps"]} '
+                  f'steps', file=sys.stderr, flush=True)
+            cpu_omp = {'value': cm['value'], 'unit': 'particle-steps/s', 'cores': a.cpu_threads, 'kind': 'port',
+                       'sample': f'{cm["steps"]} steps ({cm["elapsed_s"]:.1f} s), OpenMP pair loop of the same port'}
 
     line = {
         'metric': METRIC,
@@ -169,6 +207,8 @@ def main():
                      'avg_launch_ms': r32['pair_ms'], 'launches': r32['timed_launches']},
         'cpu_baseline': cpu,
         'speedup_vs_cpu': (value / cpu['value']) if cpu else None,
+        'cpu_baseline_omp': cpu_omp,
+        'speedup_vs_cpu_omp': (value / cpu_omp['value']) if cpu_omp else None,
         'pairs_per_step': pairs,
         'pair_interactions_per_s': pairs * a.steps / r32['elapsed_s'],
         'stage_ms': {'pair': r32['pair_ms'], 'neighbour_update': r32['nn_ms'],

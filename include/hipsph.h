@@ -156,6 +156,14 @@ int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream);
 /* Device clock and flags: out_d = {t, dt, dt_c, dt_f}, out_i = {steps, flags}.
  * Synchronises `stream`. */
 int sph_status(sph_ctx_t *ctx, double out_d[4], int64_t out_i[2], void *stream);
+/* Settling control of Solver.run (src/Solver.py:366-410): sets the device
+ * clock to `t` and whether a step advances it (`advance_time` = 0 while the
+ * reference is settling, src/Solver.py:404-406). Synchronises `stream`. */
+int sph_set_clock(sph_ctx_t *ctx, double t, int32_t advance_time, void *stream);
+/* KineticEnergy (src/Equations/KineticEnergy.py:5-12) of the fluid of the
+ * current state: sum of 0.5 m (vx^2 + vy^2), reduced on the device (order of
+ * the sum differs from the serial loop). Synchronises `stream`. */
+int sph_kinetic_energy(sph_ctx_t *ctx, double *out, void *stream);
 /* The dt series of the last `count` steps (dt_a, dt_c, dt_f interleaved) from
  * the device ring of the most recent SPH_DT_HISTORY steps. Synchronises. */
 #define SPH_DT_HISTORY 65536

@@ -53,6 +53,7 @@ struct sph_ctx {
     int2 *cxy_old = nullptr, *cxy_s = nullptr;
     int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
     double *part_b = nullptr, *part_d = nullptr;  // per-workgroup partials: bounds, dt
+    double *part_k = nullptr;                     // kinetic-energy partials + result
     long long cap_cells = 0;
     bool dt_fresh = false;  // part_d holds the dt partials of the current state (left by the corrector)
     void *recA = nullptr, *recB = nullptr;  // packed pair-pass records (Rec<double> / Rec<float>)
@@ -122,6 +123,34 @@ struct AddOp {
     __device__ double operator()(double a, double b) const { return a + b; }
 };
 
+// Fold NQ quantities over a whole workgroup with one LDS round (the
+// single-workgroup finalisers are latency-bound: one barrier instead of one
+// per quantity). kind: 0 min, 1 max, 2 sum. The result is valid in every
+// thread. lds holds NQ * 16 doubles.
+template <int NQ>
+__device__ inline void block_fold(double (&v)[NQ], const int (&kind)[NQ], double *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double u = __shfl_xor(v[q], o, 64);
+            v[q] = kind[q] == 0 ? (u < v[q] ? u : v[q]) : kind[q] == 1 ? (u > v[q] ? u : v[q]) : v[q] + u;
+        }
+        if (lane == 0) lds[q * 16 + w] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double acc = lds[q * 16];
+        for (int k = 1; k < nw; ++k) {
+            const double u = lds[q * 16 + k];
+            acc = kind[q] == 0 ? (u < acc ? u : acc) : kind[q] == 1 ? (u > acc ? u : acc) : acc + u;
+        }
+        v[q] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // time step: src/Equations/TimeStep.py:58-90 computeVars over the fluid
 // ---------------------------------------------------------------------------
@@ -167,21 +196,21 @@ __global__ void k_dt_partial(const int8_t *label, const double *h, const double 
 // `while self.t < self.duration` (src/Solver.py:366) and `self.t += self.dt`
 // (:409) move here, so a captured step needs no host. One workgroup.
 __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *hist, const double *part, int R) {
-    __shared__ double lds[16];
-    double mn = INFINITY, mx = -INFINITY, cnt_d = 0.0, bad_d = 0.0;
+    __shared__ double lds[kDtQ * 16];
+    const int kind[kDtQ] = {0, 1, 2, 2};
+    double v[kDtQ] = {INFINITY, -INFINITY, 0.0, 0.0};
     for (int k = threadIdx.x; k < R; k += blockDim.x) {
-        const double a = part[0 * R + k], b = part[1 * R + k];
-        mn = a < mn ? a : mn;
-        mx = b > mx ? b : mx;
-        cnt_d += part[2 * R + k];
-        bad_d += part[3 * R + k];
+        double u[kDtQ];
+#pragma unroll
+        for (int q = 0; q < kDtQ; ++q) u[q] = part[q * R + k];
+        v[0] = u[0] < v[0] ? u[0] : v[0];
+        v[1] = u[1] > v[1] ? u[1] : v[1];
+        v[2] += u[2];
+        v[3] += u[3];
     }
-    mn = block_reduce(mn, MinOp(), lds);
-    mx = block_reduce(mx, MaxOp(), lds);
-    cnt_d = block_reduce(cnt_d, AddOp(), lds);
-    bad_d = block_reduce(bad_d, AddOp(), lds);
+    block_fold<kDtQ>(v, kind, lds);
     if (threadIdx.x != 0) return;
-    const double cnt = cnt_d, bad = bad_d;
+    const double mn = v[0], mx = v[1], cnt = v[2], bad = v[3];
     const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID;
     if ((ctrl->flags & halt) || !(ctrl->t < ctrl->duration)) {
         ctrl->active = 0;
@@ -215,6 +244,28 @@ __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *h
     if (ctrl->advance_t) ctrl->t = ctrl->t + dt;
     ctrl->step = ctrl->step + 1;
     ctrl->active = 1;
+}
+
+// KineticEnergy (src/Equations/KineticEnergy.py:5-12) over the fluid:
+// per-workgroup partials, then one workgroup folds them into part[R].
+__global__ void k_ke_partial(const State s, int n, double *part) {
+    __shared__ double lds[8];
+    double k = 0.0;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        if (s.label[j] == kFluid) {
+            const double vx = s.vx[j], vy = s.vy[j];
+            k += 0.5 * s.m[j] * (vx * vx + vy * vy);
+        }
+    k = block_reduce(k, AddOp(), lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = k;
+}
+
+__global__ void k_ke_final(double *part, int R) {
+    __shared__ double lds[16];
+    double k = 0.0;
+    for (int b = threadIdx.x; b < R; b += blockDim.x) k += part[b];
+    k = block_reduce(k, AddOp(), lds);
+    if (threadIdx.x == 0) part[R] = k;
 }
 
 // ---------------------------------------------------------------------------
@@ -381,23 +432,22 @@ __device__ bool derive_grid(Ctrl *ctrl, double scale, long long cap, const doubl
     return true;
 }
 
-// Fold the R bounds partials and derive the grid. One workgroup.
+// Fold the R bounds partials and derive the grid. One workgroup; every load
+// is issued before the fold.
 __global__ void k_grid(Ctrl *ctrl, const double *part, int R, double scale, long long cap) {
     if (!ctrl->active) return;
-    __shared__ double lds[16];
-    __shared__ double v[kBoundsQ];
-    for (int q = 0; q < kBoundsQ; ++q) {
-        const bool is_max = q == 1 || q == 3 || q == 5;
-        double acc = q == 6 ? 0.0 : (is_max ? -INFINITY : INFINITY);
-        for (int k = threadIdx.x; k < R; k += blockDim.x) {
-            const double u = part[q * R + k];
-            acc = q == 6 ? acc + u : (is_max ? (u > acc ? u : acc) : (u < acc ? u : acc));
-        }
-        if (q == 6) acc = block_reduce(acc, AddOp(), lds);
-        else if (is_max) acc = block_reduce(acc, MaxOp(), lds);
-        else acc = block_reduce(acc, MinOp(), lds);
-        if (threadIdx.x == 0) v[q] = acc;
+    __shared__ double lds[kBoundsQ * 16];
+    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
+    double v[kBoundsQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
+    for (int k = threadIdx.x; k < R; k += blockDim.x) {
+        double u[kBoundsQ];
+#pragma unroll
+        for (int q = 0; q < kBoundsQ; ++q) u[q] = part[q * R + k];
+#pragma unroll
+        for (int q = 0; q < kBoundsQ; ++q)
+            v[q] = kind[q] == 0 ? (u[q] < v[q] ? u[q] : v[q]) : kind[q] == 1 ? (u[q] > v[q] ? u[q] : v[q]) : v[q] + u[q];
     }
+    block_fold<kBoundsQ>(v, kind, lds);
     if (threadIdx.x == 0) derive_grid(ctrl, scale, cap, v);
 }
 
@@ -884,7 +934,8 @@ static void free_ctx(sph_ctx *c) {
         if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
     }
     void *ptrs[] = {c->ctrl, c->dt_hist, c->bin, c->rank, c->perm, c->permKey, c->cxy_old, c->cxy_s,
-                    c->counts, c->cellStart, c->tileSums, c->binflag, c->part_b, c->part_d, c->recA, c->recB, c->slot_of};
+                    c->counts, c->cellStart, c->tileSums, c->binflag, c->part_b, c->part_d, c->part_k, c->recA, c->recB,
+                    c->slot_of};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : c->ev_pair) hipEventDestroy(e);
@@ -993,6 +1044,7 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
 
               hipMalloc(&c->part_b, sizeof(double) * kBoundsQ * kRedBlocks) == hipSuccess &&
               hipMalloc(&c->part_d, sizeof(double) * kDtQ * kRedBlocks) == hipSuccess &&
+              hipMalloc(&c->part_k, sizeof(double) * (kRedBlocks + 1)) == hipSuccess &&
               hipMalloc(&c->slot_of, sizeof(int) * n) == hipSuccess;
     if (ok) {
         const bool f32 = params->precision == SPH_PREC_FP32C;
@@ -1284,6 +1336,29 @@ int sph_status(sph_ctx_t *ctx, double out_d[4], int64_t out_i[2], void *stream) 
         out_i[0] = h.step;
         out_i[1] = h.flags;
     }
+    return 0;
+}
+
+int sph_set_clock(sph_ctx_t *ctx, double t, int32_t advance_time, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    Ctrl h;
+    HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    h.t = t;
+    h.advance_t = advance_time ? 1 : 0;
+    HIPCHK(hipMemcpy(ctx->ctrl, &h, sizeof h, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int sph_kinetic_energy(sph_ctx_t *ctx, double *out, void *stream) {
+    if (!ctx || !out) return fail(SPH_EINVAL, "NULL argument");
+    hipStream_t s = S(stream);
+    const int R = redblk(ctx->n);
+    k_ke_partial<<<R, kBlock, 0, s>>>(ctx->buf[ctx->cur], ctx->n, ctx->part_k);
+    k_ke_final<<<1, 1024, 0, s>>>(ctx->part_k, R);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ctx->part_k + R, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return 0;
 }
 

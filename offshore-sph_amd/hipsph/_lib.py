@@ -76,6 +76,8 @@ _PROTOS = {
     'sph_step': (C.c_int, [_P, _I64, _I32, _P]),
     'sph_status': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), _P]),
     'sph_dt_history': (C.c_int, [_P, C.POINTER(_D), _I64, _P]),
+    'sph_set_clock': (C.c_int, [_P, _D, C.c_int32, _P]),
+    'sph_kinetic_energy': (C.c_int, [_P, C.POINTER(_D), _P]),
     'sph_kernel_evaluate': (C.c_int, [_I32, _I64, _P, _P, _P, _P]),
     'sph_kernel_gradient': (C.c_int, [_I32, _I64, _P, _P, _P, _P, _P]),
     'sph_kernel_derivative': (C.c_int, [_I32, _I64, _P, _P, _P, _P]),

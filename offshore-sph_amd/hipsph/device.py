@@ -189,6 +189,16 @@ class Engine:
         call('sph_status', self._ctx, d, i, self._stream())
         return {'t': d[0], 'dt': d[1], 'dt_c': d[2], 'dt_f': d[3], 'steps': int(i[0]), 'flags': int(i[1])}
 
+    def set_clock(self, t, advance_time=True):
+        """Device time and whether steps advance it (False while settling)."""
+        call('sph_set_clock', self._ctx, float(t), int(bool(advance_time)), self._stream())
+
+    def kinetic_energy(self):
+        """KineticEnergy (src/Equations/KineticEnergy.py:5-12) of the fluid, on the device."""
+        out = C.c_double()
+        call('sph_kinetic_energy', self._ctx, C.byref(out), self._stream())
+        return out.value
+
     def dt_history(self, count):
         out = np.zeros((int(count), 3), dtype=np.float64)
         call('sph_dt_history', self._ctx, out.ctypes.data_as(C.POINTER(C.c_double)), int(count), self._stream())

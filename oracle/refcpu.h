@@ -20,6 +20,34 @@
  * left-to-right products when the exponent is a small integer (ref_ipow);
  * the numba reference runs those call sites with fastmath, so they are only
  * reproducible to a few ulp there anyway.
+ *
+ * What pins each function (tests/test_oracle_golden.py: the reference's own
+ * known answers; tests/test_oracle_invariants.py: mathematics):
+ *   cubic W, grad W       cubic KAT (test/test_kernels_cubic.py), 2D
+ *                         normalisation, W(0), grad W = dW/dr x/r
+ *   gaussian W, dW/dq, grad W   PySPH restatement fixture, normalisation of
+ *                         the 3h-truncated kernel (1 - e^-9), W(0), dW/dr
+ *   wendland W, grad W    normalisation, W(0), dW/dr
+ *   tait, co, B, height   test/test_numba_taiteos.py KAT + closed forms
+ *   continuity            test/test_numba_continuity.py KAT, uniform flow
+ *                         gives 0, two-particle closed form
+ *   momentum              test/test_numba_momentum.py KAT (pressure term),
+ *                         two-particle closed forms (pressure; Monaghan AV
+ *                         with its branch, sign and alpha-linearity),
+ *                         momentum conservation, mirror/rotation symmetry
+ *   boundary force        test/test_eq_boundary.py KAT (588.6)
+ *   timestep              test/test_eq_courant.py KAT
+ *   pec / euler           test/test_integrators_{pec,euler}.py KATs
+ *   nn update / near      test/test_linked_list.py KAT (scale 3), brute force
+ *                         over the 3x3-cell rule incl. order, cubic-support
+ *                         completeness, edge aliasing, IEEE boundary pairs
+ *   loop                  dense-pair model (no neighbour list) for all three
+ *                         kernels + the invariants above
+ *   solver step / run     the pieces above + mask test (walls fixed, fluid
+ *                         only integrated)
+ *   XSPH, summation density and dynamic h inside ref_loop / ref_solver_step:
+ *                         PARITY UNPINNED (no reference test, no check here;
+ *                         not on the device path).
  */
 #ifndef REFCPU_H
 #define REFCPU_H

@@ -60,3 +60,30 @@ def test_engine_params_to_c():
     assert p.co == m.co and p.B == m.B and p.D == m.D and p.qmax == 3.0 and p.scale == 2.0
     with pytest.raises(NotImplementedError):
         EngineParams.from_method(1, WCSPH(2.0, 0.02, 1000.0, useXSPH=True))
+
+
+def test_solver_mirror_signature_and_host_policy():
+    """Solver keeps the reference constructor (src/Solver.py:46) and fails loudly
+    for what the device path does not do; the host-stop policy is pure logic."""
+    import inspect
+    from hipsph.Solver import Solver
+    ref_params = ['method', 'integrator', 'kernel', 'duration', 'quick', 'incrementalWriteout', 'incrementalFile',
+                  'incrementalFreq', 'exportProperties', 'kE', 'maxSettle', 'timeStep', 'h', 'coupling',
+                  'couplingIntegrator', 'couplingProperties', 'customSettle', 'damping']
+    sig = inspect.signature(Solver.__init__)
+    names = [p for p in sig.parameters if p != 'self']
+    assert names[:len(ref_params)] == ref_params
+    assert all(sig.parameters[p].kind == inspect.Parameter.KEYWORD_ONLY for p in names[len(ref_params):])
+    d = sig.parameters
+    assert (d['duration'].default, d['incrementalFreq'].default, d['kE'].default, d['maxSettle'].default,
+            d['damping'].default) == (1.0, 1000, 0.8, 500, 0.05)
+    sv = Solver(None, None, None, exportProperties=[], incrementalWriteout=True, incrementalFreq=10, chunk=64)
+    # write-outs happen at t_step % incrementalFreq == 0 (src/Solver.py:485-486)
+    assert sv._next_stop(0) == 1 and sv._next_stop(1) == 10 and sv._next_stop(5) == 6
+    sv.exportProperties = ['x']
+    assert sv._next_stop(3) == 1
+    sv2 = Solver(None, None, None, exportProperties=[], incrementalWriteout=False, chunk=64)
+    assert sv2._next_stop(7) == 64
+    sv2.h = None
+    with pytest.raises(NotImplementedError):
+        sv2.setup()

@@ -331,3 +331,74 @@ def test_nn_boundary_pairs_follow_ieee_expression():
         assert (2 * k + 1 in near_i) == bool(z['ref_in'][k])
         assert (2 * k in near_j) == bool(z['ref_in'][k])  # x_j - x_i = -(x_i - x_j) exactly
     assert z['ref_in'].any() and (~z['ref_in']).any() and np.any(z['ref_in'] != z['fused_in'])
+
+
+@pytest.mark.parametrize('kernel,support,mass', [('cubic', 2.0, 1.0), ('wendland', 2.0, 1.0),
+                                                 ('gaussian', 3.0, 1.0 - math.exp(-9.0))])
+def test_kernel_normalisation_and_gradient(kernel, support, mass):
+    """Pins W and grad W of each oracle kernel by what the mathematics fixes,
+    independently of how the formula is written: the 2D normalisation
+    (integral of W over the support = 1; the Gaussian truncated at 3h keeps
+    1 - e^-9), the value at the origin (the normalisation constant:
+    cubic 10/(7 pi h^2), Wendland C4 9/(4 pi h^2), Gaussian 1/(pi h^2)), and
+    grad W = dW/dr * x / r against a central difference of W. A dropped term,
+    a wrong constant or a wrong power in either function fails one of these."""
+    h = 0.37
+    n = 200001
+    r = np.linspace(0.0, support * h, n)
+    hv = np.full_like(r, h)
+    w = oracle.kernel_evaluate(kernel, r, hv)
+    integral = np.trapezoid(w * 2 * math.pi * r, r)
+    assert integral == pytest.approx(mass, rel=2e-9)
+    w0 = {'cubic': 10 / (7 * math.pi), 'wendland': 9 / (4 * math.pi), 'gaussian': 1 / math.pi}[kernel] / h ** 2
+    assert w[0] == pytest.approx(w0, rel=1e-15)
+    rr = np.linspace(0.05, support * h * 0.995, 997)
+    eps = 1e-6 * h
+    dwdr = (oracle.kernel_evaluate(kernel, rr + eps, np.full_like(rr, h)) -
+            oracle.kernel_evaluate(kernel, rr - eps, np.full_like(rr, h))) / (2 * eps)
+    x = 0.6 * rr  # a component of the separation
+    g = oracle.kernel_gradient(kernel, x, rr, np.full_like(rr, h))
+    np.testing.assert_allclose(g, dwdr * x / rr, rtol=2e-6, atol=1e-9 * np.max(np.abs(dwdr)))
+
+
+def _av_pair(u, alpha, beta=0.0):
+    h, d, m0 = 0.026, 0.6 * 0.026, 0.4
+    pA = _particles(np.array([0.0, d]), np.array([0.0, 0.0]), h, vx=np.array([u, -u]),
+                    rho=np.array([1002.0, 1001.0]), m=m0)
+    m = _M()
+    m.alpha, m.beta = alpha, beta
+    pA['c'] = m.co  # WCSPH.compute_speed_of_sound (src/Methods/WCSPH.py:110-129)
+    oracle.loop(pA, m, 'cubic')
+    return pA, m, h, d, m0
+
+
+def test_two_particles_artificial_viscosity_closed_form():
+    """Monaghan (1992) artificial viscosity on two approaching fluid particles,
+    with the reference's choices (src/Equations/Momentum.py:38-53): the mean
+    sound speed is 0.5 c_i (the neighbour's c is never filled, Q5) and the
+    smoothing length is re-averaged, h_bar = 0.5 (h_i + h_ij) = h here.
+    Also: receding particles get no viscous term, the term removes relative
+    kinetic energy, and at beta = 0 it is linear in alpha."""
+    u, alpha = 0.8, 0.3
+    pA, m, h, d, m0 = _av_pair(u, alpha)
+    p = ((pA['rho'] / m.rho0) ** 7 - 1) * m.B
+    q = d / h
+    xij = -d                      # x_0 - x_1
+    vij = 2 * u                   # v_0 - v_1 (approaching: v.x < 0)
+    dwdx = 10 / (7 * math.pi * h * h) * (-3 * q * (1 - 0.75 * q)) / (h * d) * xij
+    mu = h * (vij * xij) / (d * d + 0.01 * h * h)
+    rho_bar = 0.5 * (pA['rho'][0] + pA['rho'][1])
+    pi_ij = mu * (0.0 * mu - alpha * 0.5 * m.co) / rho_bar
+    a0 = -m0 * (p[0] / pA['rho'][0] ** 2 + p[1] / pA['rho'][1] ** 2 + pi_ij) * dwdx
+    assert pA['ax'][0] == pytest.approx(a0, rel=1e-12)
+    assert pA['drho'][0] == pytest.approx(m0 * vij * dwdx, rel=1e-12)
+    # branch: receding pair -> pressure term only
+    rec, *_ = _av_pair(-u, alpha)
+    a_rec = -m0 * (p[0] / pA['rho'][0] ** 2 + p[1] / pA['rho'][1] ** 2) * dwdx
+    assert rec['ax'][0] == pytest.approx(a_rec, rel=1e-12)
+    # dissipation: the viscous part decelerates the approach (a_0 < pressure-only value for u > 0)
+    base, *_ = _av_pair(u, 0.0)
+    assert pA['ax'][0] - base['ax'][0] < 0 and pA['ax'][1] - base['ax'][1] > 0
+    # linear in alpha at beta = 0
+    dbl, *_ = _av_pair(u, 2 * alpha)
+    assert (dbl['ax'][0] - base['ax'][0]) == pytest.approx(2 * (pA['ax'][0] - base['ax'][0]), rel=1e-10)
