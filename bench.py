@@ -118,6 +118,136 @@ def profile_pair(sc, launches):
     print(f'# calibration copy: {src.numel() * 8} bytes read and written per launch', file=sys.stderr, flush=True)
 
 
+def precision_report(sc, checkpoints=(100, 500, 1000)):
+    """FP32C pair pass (FP64 integrator) against the FP64 pair pass, both on
+    the device from the same initial state: max and RMS over the fluid of
+    |drho|/rho, |dx|/dx and |dv|/c0 at each checkpoint step."""
+    import numpy as np
+    from hipsph.device import Engine, EngineParams, soa_from_particles
+    m = sc.method()
+    eng = {}
+    for prec in ('fp64', 'fp32c'):
+        eng[prec] = Engine(EngineParams.from_method(sc.n, m, precision=prec))
+        eng[prec].upload(soa_from_particles(sc.pA))
+    f = sc.pA['label'] == 0
+    out, done = [], 0
+    for k in checkpoints:
+        for e in eng.values():
+            e.step(k - done, graph=True)
+        done = k
+        a, b = eng['fp64'].download(), eng['fp32c'].download()
+        drho = np.abs(b['rho'][f] - a['rho'][f]) / a['rho'][f]
+        dxn = np.hypot(b['x'][f] - a['x'][f], b['y'][f] - a['y'][f]) / sc.dx
+        dvn = np.hypot(b['vx'][f] - a['vx'][f], b['vy'][f] - a['vy'][f]) / m.co
+        st = eng['fp64'].status()
+        out.append({'step': k, 't': st['t'],
+                    'rho_rel_max': float(drho.max()), 'rho_rel_rms': float(np.sqrt(np.mean(drho ** 2))),
+                    'x_over_dx_max': float(dxn.max()), 'x_over_dx_rms': float(np.sqrt(np.mean(dxn ** 2))),
+                    'v_over_c0_max': float(dvn.max()), 'v_over_c0_rms': float(np.sqrt(np.mean(dvn ** 2)))})
+        print(f'# precision at step {k}: rho {out[-1]["rho_rel_max"]:.3g} max, x/dx {out[-1]["x_over_dx_max"]:.3g}'
+              f' max, v/c0 {out[-1]["v_over_c0_max"]:.3g} max', file=sys.stderr, flush=True)
+    for e in eng.values():
+        e.close()
+    return out
+
+
+def adjoint_timing(repeats=3, nsteps=200, every=20):
+    """The differentiable dam break (0.5 m x 1.0 m column at dx 0.005: 20,000
+    fluid particles, 200 PEC steps, checkpoints every 20): wall time of the
+    forward alone and of forward (with checkpoints) + adjoint, which
+    recomputes each segment while recording and reverses it. Loss: mean
+    squared distance of the final positions to a run at alpha 0.03, c0 x 1.1."""
+    import torch
+    from hipsph import scenarios
+    from hipsph.adjoint import SPHRun
+    sc = scenarios.dam_break_c5()
+    m = sc.method()
+    run = SPHRun(sc.pA, m, nsteps, checkpoint_every=every)
+    v0 = torch.zeros((sc.n_fluid, 2), dtype=torch.float64, device=run.device)
+    target = run.forward(0.03, 1.1 * m.co, v0, keep=False).clone()
+
+    def fwd_bwd():
+        xt = run.forward(m.alpha, m.co, v0, keep=True).clone().requires_grad_(True)
+        loss = torch.mean(torch.sum((xt - target) ** 2, 1))
+        (g,) = torch.autograd.grad(loss, [xt])
+        return run.backward(g)
+
+    fwd_bwd()
+    t_f, t_fb = [], []
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.forward(m.alpha, m.co, v0, keep=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ga, gc, gv = fwd_bwd()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_f.append(t1 - t0)
+        t_fb.append(t2 - t1)
+    run.close()
+    f, fb = min(t_f), min(t_fb)
+    print(f'# adjoint: forward {f * 1e3:.2f} ms, forward+adjoint {fb * 1e3:.2f} ms ({fb / f:.2f}x)', file=sys.stderr,
+          flush=True)
+    return {'workload': sc.name, 'n_particles': sc.n, 'n_fluid': sc.n_fluid, 'steps': nsteps,
+            'checkpoint_every': every, 'forward_ms': f * 1e3, 'forward_adjoint_ms': fb * 1e3, 'ratio': fb / f,
+            'force_limited_steps': run.force_steps // (repeats + 1),
+            'dL_dalpha': float(ga), 'dL_dc0': float(gc), 'dL_dv0_norm': float(torch.linalg.norm(gv))}
+
+
+STRESS_BYTES = {  # algorithmic bytes per launch (N all, N_f fluid, C cells)
+    ('fused', 'fp32c'): (21, 12, 8), ('fused', 'fp64'): (41, 24, 8),
+    ('continuity', 'fp32c'): (17, 4, 8), ('continuity', 'fp64'): (33, 8, 8),
+    ('momentum', 'fp32c'): (21, 8, 8), ('momentum', 'fp64'): (41, 16, 8),
+}
+
+
+def stress(reps):
+    """Pair-kernel stress: 4,000,000 jittered particles (2000 x 2000, dx 1e-3)
+    in a closed box; per kernel and precision, the neighbour update and the
+    continuity, momentum and fused pair passes timed apart (HIP events)."""
+    import numpy as np
+    from hipsph import scenarios
+    from hipsph.device import Engine, EngineParams, soa_from_particles
+    sc = scenarios.stress_lattice()
+    n, nf = sc.n, sc.n_fluid
+    print(f'# stress workload {sc.name}: {n} particles ({nf} fluid)', file=sys.stderr, flush=True)
+    rows = []
+    for kernel in ('cubic', 'gaussian'):
+        for prec in ('fp32c', 'fp64'):
+            e = Engine(EngineParams.from_method(n, sc.method(), kernel=kernel, precision=prec))
+            e.upload(soa_from_particles(sc.pA))
+            e.nn_update()
+            pairs = int(np.sum(np.maximum(e.compute_counts(), 0)))
+            cells = e.grid()['n_cells']
+            row = {'kernel': kernel, 'precision': prec, 'pairs': pairs, 'cells': cells}
+            e.timing_enable(True)
+            for _ in range(reps):
+                e.nn_update()
+            row['neighbour_update_ms'] = e.timing_read()['nn_ms']
+            for stage, fn in (('continuity', e.continuity), ('momentum', e.momentum), ('fused', e.compute)):
+                fn()
+                e.timing_read()
+                for _ in range(reps):
+                    fn()
+                ms = e.timing_read()['pair_ms']
+                a, b, c = STRESS_BYTES[(stage, prec)]
+                byts = a * n + b * nf + c * cells
+                row[f'{stage}_ms'] = ms
+                row[f'{stage}_hbm_frac'] = byts / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
+            row['fused_pair_interactions_per_s'] = pairs / (row['fused_ms'] * 1e-3)
+            e.timing_enable(False)
+            e.close()
+            rows.append(row)
+            print(f'# stress {kernel} {prec}: nn {row["neighbour_update_ms"]:.3f} ms, continuity '
+                  f'{row["continuity_ms"]:.3f}, momentum {row["momentum_ms"]:.3f}, fused {row["fused_ms"]:.3f} ms',
+                  file=sys.stderr, flush=True)
+    print(json.dumps({'metric': 'pair-kernel stress stage times (4M particles)', 'unit': 'ms', 'n_gpus': 1,
+                      'higher_is_better': False, 'reps': reps, 'data': 'synthetic',
+                      'config': {'workload': sc.name, 'n_particles': n, 'n_fluid': nf, 'h': sc.h},
+                      'stages': rows}), flush=True)
+
+
 def traffic_from_profiles(precision):
     p = os.path.join(ROOT, 'profiles', 'pair_traffic.json')
     if not os.path.exists(p):
@@ -143,12 +273,24 @@ def main():
                     help='threads of the second (OpenMP) CPU baseline; 0 skips it')
     ap.add_argument('--profile-pair', type=int, default=0, metavar='K',
                     help='only run K pair passes per precision and a calibration copy (for counter runs)')
+    ap.add_argument('--stress', type=int, default=0, metavar='REPS',
+                    help='only run the 4M-particle pair-kernel stress timings (REPS launches per stage)')
+    ap.add_argument('--no-precision', action='store_true', help='skip the FP32C-vs-FP64 error report')
+    ap.add_argument('--no-adjoint', action='store_true', help='skip the forward+adjoint timing')
+    ap.add_argument('--adjoint', action='store_true', help='only run the forward+adjoint timing')
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU (slab-decomposed) path is not built. '
               'Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
 
+    if a.stress:
+        stress(a.stress)
+        return
+    if a.adjoint:
+        print(json.dumps({'metric': 'forward+adjoint / forward wall time', 'n_gpus': 1, 'higher_is_better': False,
+                          'data': 'synthetic', 'adjoint': adjoint_timing()}), flush=True)
+        return
     from hipsph import scenarios
     sc = scenarios.dam_break_c1() if a.small else scenarios.dam_break_c2(million=True)
     print(f'# workload {sc.name}: {sc.n} particles ({sc.n_fluid} fluid)', file=sys.stderr, flush=True)
@@ -169,6 +311,9 @@ def main():
     ach32 = b32 / (r32['pair_ms'] * 1e-3) / 1e9
     ach64 = b64 / (r64['pair_ms'] * 1e-3) / 1e9
     pairs = r64['pairs_per_step']
+
+    prec_report = None if a.no_precision else precision_report(sc)
+    adj = None if a.no_adjoint else adjoint_timing()
 
     cpu = cpu_omp = None
     if not a.no_cpu:
@@ -221,7 +366,9 @@ def main():
                               'kernel': 'k_pair_fp64<cubic>', 'algorithmic_bytes_per_launch': b64,
                               'avg_launch_ms': r64['pair_ms']},
                  'stage_ms': {'pair': r64['pair_ms'], 'neighbour_update': r64['nn_ms']}},
-        'fwd_adjoint_over_fwd': None,
+        'precision': prec_report,
+        'fwd_adjoint_over_fwd': adj['ratio'] if adj else None,
+        'adjoint': adj,
         'sim_time_s': r32['status']['t'],
     }
     print(json.dumps(line), flush=True)

@@ -130,6 +130,17 @@ int sph_nn_dump(sph_ctx_t *ctx, int64_t *keys, int64_t *off, int32_t *idx, doubl
  * Requires a prior sph_nn_update. */
 int sph_compute(sph_ctx_t *ctx, void *stream);
 
+/* The same pass split by equation, for timing the stages apart (and for
+ * callers that want one of them): sph_continuity writes drho only
+ * (src/Equations/Continuity.py:4-16 via WCSPH.compute_density_change,
+ * src/Methods/WCSPH.py:191-203); sph_momentum writes ax, ay only (momentum
+ * + AV + gravity + Lennard-Jones wall force, src/Equations/Momentum.py:5-57,
+ * BoundaryForce.py:6-42, WCSPH.py:151-169). Each evaluates the same
+ * expressions in the same order as sph_compute. Requires a prior
+ * sph_nn_update. */
+int sph_continuity(sph_ctx_t *ctx, void *stream);
+int sph_momentum(sph_ctx_t *ctx, void *stream);
+
 /* For parity: one pair pass (as sph_compute) that also reports, per particle
  * in reference order, the number of neighbours the pass itself found
  * (len(near(i)) of src/Tools/SolverTools.py:163; -1 for non-fluid).
@@ -168,6 +179,45 @@ int sph_kinetic_energy(sph_ctx_t *ctx, double *out, void *stream);
  * the device ring of the most recent SPH_DT_HISTORY steps. Synchronises. */
 #define SPH_DT_HISTORY 65536
 int sph_dt_history(sph_ctx_t *ctx, double *out3, int64_t count, void *stream);
+
+/* ---- reverse mode (no counterpart in the reference) ----
+ * For gradients of a loss on a later state with respect to the initial
+ * state, the artificial-viscosity alpha and the sound speed c0 (the caller
+ * keeps B = rho0 c0^2 / gamma): a tape holds `nslots` recorded steps;
+ * sph_step_record performs one ordinary step (as sph_step) and records it in
+ * `slot`; sph_adjoint_step maps the adjoint of that step's outputs to the
+ * adjoint of its inputs, in place, and adds the step's contributions to
+ * *g_alpha and *g_co. Steps are reversed in the opposite order of recording;
+ * checkpointing (re-recording a segment from a saved state) is up to the
+ * caller. All arrays are device double[n] in reference index order; entries
+ * of non-fluid particles are ignored and set to 0. PEC (strict = 0) and Euler.
+ * dt is differentiated on both branches of min(courant, force)
+ * (src/Equations/TimeStep.py:40-56): the Courant value through c0, the force
+ * value through the carried acceleration of the particle attaining max |a|^2
+ * (recorded; smallest index on ties). sph_tape_force_steps counts the
+ * reversed steps whose dt came from the force criterion. */
+typedef struct sph_tape sph_tape_t;
+typedef struct {
+    double *x, *y, *vx, *vy, *rho; /* adjoint of the state: in after the step, out before it */
+    double *ax, *ay, *drho;        /* adjoint of the rates carried into the next step's predictor */
+    double *g_alpha, *g_co;        /* device scalars, accumulated */
+} sph_adjoint_t;
+int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out);
+int sph_tape_destroy(sph_tape_t *tape);
+int sph_tape_force_steps(sph_tape_t *tape, int64_t *out);
+int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream);
+int sph_adjoint_step(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, const sph_adjoint_t *adj, void *stream);
+/* One checkpoint segment in a single call: record `nsteps` steps from the
+ * current state into slots 0..nsteps-1, then reverse them (as
+ * sph_step_record and sph_adjoint_step, in that order). Steps after the first
+ * are replayed from hipGraphs cached in the tape (per slot and buffer parity
+ * for the recording; per slot for the reversal, for the adjoint buffers of
+ * the last call). */
+int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const sph_adjoint_t *adj, void *stream);
+/* A saved state back into the context without a host round trip (for
+ * checkpoint restores): as sph_state_upload without resetting the step count,
+ * with the clock set to t and advancing, flags cleared. */
+int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *stream);
 
 /* ---- element-wise plugin functions on device arrays ---- */
 /* Kernel.evaluate / Kernel.gradient (src/Kernels/Kernel.py:6-12) for

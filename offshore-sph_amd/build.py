@@ -8,6 +8,7 @@ gfx950 and links them into offshore-sph_amd/hipsph/libhipsph.so:
   sph_pair_fp64.hip -ffp-contract=off  (double pair pass, reference operation order)
   sph_pair_fp32.hip -ffp-contract=fast-honor-pragmas (float pair pass; the exact
                     neighbour re-test and the TwoSum steps switch contraction off)
+  sph_adjoint.hip   -ffp-contract=fast (tape and reverse step; not bit-matched)
 Packed FP32 instructions are disabled for every device compile
 (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
@@ -33,6 +34,7 @@ UNITS = [
     ('sph_engine.hip', ['-ffp-contract=off']),
     ('sph_pair_fp64.hip', ['-ffp-contract=off']),
     ('sph_pair_fp32.hip', ['-ffp-contract=fast-honor-pragmas']),
+    ('sph_adjoint.hip', ['-ffp-contract=fast']),
 ]
 
 
@@ -52,7 +54,7 @@ def _compile(src, flags):
     return obj, r.stderr
 
 
-def build(jobs=3, verbose=False):
+def build(jobs=4, verbose=False):
     os.makedirs(BUILD_DIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda u: _compile(*u), UNITS))

@@ -1,0 +1,105 @@
+// sph_ctx.h — the context and tape objects behind the C ABI, shared by the
+// engine and the adjoint translation units.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "sph_internal.h"
+
+// One simulation on one GPU (opaque `sph_ctx_t` of the C ABI).
+struct sph_ctx {
+    sph_params_t P;
+    int n = 0;
+    int dev = 0;
+    hipsph::State buf[2];
+    void *arena[2] = {nullptr, nullptr};
+    int cur = 0;
+    hipsph::Ctrl *ctrl = nullptr;
+    double *dt_hist = nullptr;
+    int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
+    int2 *cxy_old = nullptr, *cxy_s = nullptr;
+    int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
+    double *part_b = nullptr, *part_d = nullptr;  // per-workgroup partials: bounds, dt
+    double *part_k = nullptr;                     // kinetic-energy partials + result
+    long long cap_cells = 0;
+    bool dt_fresh = false;  // part_d holds the dt partials of the current state (left by the corrector)
+    void *recA = nullptr, *recB = nullptr;  // packed pair-pass records (Rec<double> / Rec<float>)
+    int *slot_of = nullptr;
+    hipStream_t own = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int pair_threads = 256;
+    // timing of the pair pass and of the neighbour update
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pair, ev_nn;
+    size_t n_pair = 0, n_nn = 0;
+};
+
+namespace hipsph {
+
+// Per recorded step: what the reverse pass reads besides the arrays.
+struct TapeHdr {
+    Grid grid;
+    double dt, dt_c, dt_f;
+    double amax2;  // max |a_prev|^2 over the fluid (the force criterion's argument)
+    int amax_id;   // a fluid particle attaining it (smallest id)
+    int active;
+};
+
+// One recorded step. Per-particle values are stored in reference index
+// order (id); the walk structures in the cell order of the step's pair pass.
+struct TapeSlot {
+    double *xh, *yh, *vxh, *vyh, *rhoh;  // pair inputs: the predicted state
+    double *ax, *ay, *drho;              // pair outputs
+    double *vx0, *vy0;                   // velocity at the start of the step
+    double *axp, *ayp, *drhop;           // rates carried into the step (previous pair pass)
+    int *id;                             // cell slot -> id
+    int2 *cxy;                           // cell of each slot
+    int *cellStart;                      // cell offsets [0 .. ncells + 1]
+};
+
+}  // namespace hipsph
+
+// A step tape for the adjoint (opaque `sph_tape_t` of the C ABI).
+struct sph_tape {
+    sph_ctx *ctx = nullptr;
+    int nslots = 0;
+    long long cells = 0;  // cellStart entries per slot
+    void *arena = nullptr;
+    hipsph::TapeHdr *hdr = nullptr;  // device, one per slot
+    std::vector<hipsph::TapeSlot> slot;
+    // static per-particle data in id order
+    double *m = nullptr, *h = nullptr;
+    int8_t *label = nullptr;
+    // reverse-pass scratch in id order
+    double *lamx = nullptr, *lamy = nullptr, *kap = nullptr, *vbx = nullptr, *vby = nullptr;
+    double *gx = nullptr, *gy = nullptr, *gvx = nullptr, *gvy = nullptr, *grho = nullptr;
+    double *part = nullptr;  // reduction partials
+    double *amax_v = nullptr;  // argmax partials (value, id) per workgroup
+    int *amax_i = nullptr;
+    int *force_steps = nullptr;  // device counter: reversed steps whose dt came from the force criterion
+    // captured steps of sph_adjoint_segment: recording per (slot, buffer
+    // parity), reversal per slot for the adjoint buffers in adj_key
+    std::vector<hipGraphExec_t> rec_exec, adj_exec;
+    sph_adjoint_t adj_key{};
+};
+
+namespace hipsph {
+
+// Error text for sph_last_error(); returns `code`.
+int fail(int code, const char *fmt, ...);
+Phys make_phys(const sph_params_t &P);
+
+// Tape hooks called by sph_step_record (implemented with the adjoint).
+int tape_record_begin(sph_ctx *c, sph_tape *t, int slot, hipStream_t s);
+int tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s);
+// The reverse of one recorded step (the kernels of sph_adjoint_step).
+int adjoint_launch(sph_ctx *c, sph_tape *t, int slot, const sph_adjoint_t *adj, hipStream_t s);
+void tape_drop_graphs(sph_tape *t);
+
+}  // namespace hipsph
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) return hipsph::fail(SPH_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)

@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "sph_internal.h"
+#include "sph_ctx.h"
 
 using namespace hipsph;
 
@@ -21,7 +21,8 @@ using namespace hipsph;
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...) {
+namespace hipsph {
+int fail(int code, const char *fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -30,44 +31,9 @@ static int fail(int code, const char *fmt, ...) {
     g_err = buf;
     return code;
 }
+}  // namespace hipsph
 
-#define HIPCHK(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess) return fail(SPH_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-// ---------------------------------------------------------------------------
-// context
-// ---------------------------------------------------------------------------
-struct sph_ctx {
-    sph_params_t P;
-    int n = 0;
-    int dev = 0;
-    State buf[2];
-    void *arena[2] = {nullptr, nullptr};
-    int cur = 0;
-    Ctrl *ctrl = nullptr;
-    double *dt_hist = nullptr;
-    int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
-    int2 *cxy_old = nullptr, *cxy_s = nullptr;
-    int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
-    double *part_b = nullptr, *part_d = nullptr;  // per-workgroup partials: bounds, dt
-    double *part_k = nullptr;                     // kinetic-energy partials + result
-    long long cap_cells = 0;
-    bool dt_fresh = false;  // part_d holds the dt partials of the current state (left by the corrector)
-    void *recA = nullptr, *recB = nullptr;  // packed pair-pass records (Rec<double> / Rec<float>)
-    int *slot_of = nullptr;
-    hipStream_t own = nullptr;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    int pair_threads = 256;
-    // timing of the pair pass and of the neighbour update
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pair, ev_nn;
-    size_t n_pair = 0, n_nn = 0;
-};
-
-static Phys make_phys(const sph_params_t &P) {
+Phys hipsph::make_phys(const sph_params_t &P) {
     Phys ph;
     ph.rho0 = P.rho0;
     ph.gamma = P.gamma;
@@ -767,6 +733,16 @@ __global__ void k_upload(State d, sph_state_t in, int n) {
     d.id[i] = i;
 }
 
+// Clock and flags of a restored state (device side, so a restore needs no
+// host round trip): time t, advancing, running, grid to be rebuilt.
+__global__ void k_restore_ctrl(Ctrl *ctrl, double t) {
+    ctrl->t = t;
+    ctrl->flags = 0;
+    ctrl->active = 1;
+    ctrl->nn_valid = 0;
+    ctrl->advance_t = 1;
+}
+
 __global__ void k_download(State s, sph_state_t out, Phys ph, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -1138,6 +1114,18 @@ int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock,
     return 0;
 }
 
+int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *stream) {
+    if (!ctx || !st || !st->x || !st->y || !st->vx || !st->vy || !st->rho || !st->m || !st->h || !st->label)
+        return fail(SPH_EINVAL, "restore needs x, y, vx, vy, rho, m, h and label");
+    hipStream_t s = S(stream);
+    ctx->cur = 0;
+    ctx->dt_fresh = false;
+    k_upload<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->buf[0], *st, ctx->n);
+    k_restore_ctrl<<<1, 1, 0, s>>>(ctx->ctrl, t);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int sph_state_download(sph_ctx_t *ctx, const sph_state_t *st, void *stream) {
     if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
     k_download<<<nblk(ctx->n), kBlock, 0, S(stream)>>>(ctx->buf[ctx->cur], *st, make_phys(ctx->P), ctx->n);
@@ -1189,10 +1177,10 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0) {
     return 0;
 }
 
-static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr) {
+static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr, int mode = 0) {
     State st = c->buf[c->cur];
     PairIn in{c->recA, c->recB, st.h, c->cxy_s, c->cellStart};
-    PairOut out{st.ax, st.ay, st.drho, count};
+    PairOut out{st.ax, st.ay, st.drho, count, mode};
     if (c->timing) {
         int rc = ev_record(c, c->ev_pair, c->n_pair, s);
         if (rc) return rc;
@@ -1244,6 +1232,93 @@ static int one_step(sph_ctx *c, hipStream_t s) {
     return 0;
 }
 
+int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream) {
+    if (!ctx || !tape) return fail(SPH_EINVAL, "NULL argument");
+    if (tape->ctx != ctx) return fail(SPH_EINVAL, "tape belongs to another context");
+    if (slot < 0 || slot >= tape->nslots) return fail(SPH_EINVAL, "slot %d out of range", slot);
+    hipStream_t s = S(stream);
+    int rc;
+    if ((rc = dt_step(ctx, s))) return rc;
+    if ((rc = tape_record_begin(ctx, tape, slot, s))) return rc;
+    if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
+    if ((rc = compute(ctx, s))) return rc;
+    if ((rc = tape_record_pair(ctx, tape, slot, s))) return rc;
+    if ((rc = correct(ctx, s))) return rc;
+    ctx->dt_fresh = true;
+    return 0;
+}
+
+// One recorded step (as sph_step_record, without the argument checks).
+static int record_step(sph_ctx *ctx, sph_tape *tape, int slot, hipStream_t s) {
+    int rc;
+    if ((rc = dt_step(ctx, s))) return rc;
+    if ((rc = tape_record_begin(ctx, tape, slot, s))) return rc;
+    if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
+    if ((rc = compute(ctx, s))) return rc;
+    if ((rc = tape_record_pair(ctx, tape, slot, s))) return rc;
+    if ((rc = correct(ctx, s))) return rc;
+    ctx->dt_fresh = true;
+    return 0;
+}
+
+// Capture `body` on stream g into *exec (once) and launch it.
+extern "C++" template <typename F>
+static int graph_launch(hipGraphExec_t *exec, hipStream_t g, F body) {
+    if (!*exec) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+        int rc = body();
+        hipError_t ce = hipStreamEndCapture(g, &graph);
+        if (rc) return rc;
+        if (ce != hipSuccess) return fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
+        ce = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ce != hipSuccess) return fail(SPH_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+    }
+    HIPCHK(hipGraphLaunch(*exec, g));
+    return 0;
+}
+
+int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const sph_adjoint_t *adj, void *stream) {
+    if (!ctx || !tape || !adj) return fail(SPH_EINVAL, "NULL argument");
+    if (tape->ctx != ctx) return fail(SPH_EINVAL, "tape belongs to another context");
+    if (nsteps < 1 || nsteps > tape->nslots) return fail(SPH_EINVAL, "nsteps %d out of 1..%d", nsteps, tape->nslots);
+    if (!adj->x || !adj->y || !adj->vx || !adj->vy || !adj->rho || !adj->ax || !adj->ay || !adj->drho ||
+        !adj->g_alpha || !adj->g_co)
+        return fail(SPH_EINVAL, "every sph_adjoint_t pointer is required");
+    if (ctx->P.strict) return fail(SPH_EUNSUPPORTED, "the adjoint of the strict density clamp is not implemented");
+    if (std::memcmp(&tape->adj_key, adj, sizeof *adj) != 0) {  // other buffers: the reverse graphs are stale
+        for (auto &g : tape->adj_exec)
+            if (g) hipGraphExecDestroy(g), g = nullptr;
+        tape->adj_key = *adj;
+    }
+    hipStream_t s = S(stream), g = ctx->own;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(hipStreamWaitEvent(g, e0, 0));
+    int rc = 0;
+    for (int k = 0; k < nsteps && !rc; ++k) {
+        if (!ctx->dt_fresh) {  // a captured step expects the corrector's dt partials
+            rc = record_step(ctx, tape, k, g);
+            continue;
+        }
+        const int start = ctx->cur;
+        rc = graph_launch(&tape->rec_exec[2 * (size_t)k + start], g, [&] { return record_step(ctx, tape, k, g); });
+        ctx->cur = 1 - start;  // a capture flips it without running; either way the step ends on the other buffer
+    }
+    for (int k = nsteps - 1; k >= 0 && !rc; --k)
+        rc = graph_launch(&tape->adj_exec[k], g, [&] { return adjoint_launch(ctx, tape, k, adj, g); });
+    if (!rc) {
+        hipEventRecord(e1, g);
+        hipStreamWaitEvent(s, e1, 0);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
 int sph_nn_update(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
     return nn_update(ctx, S(stream));
@@ -1253,6 +1328,18 @@ int sph_compute(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
     ctx->dt_fresh = false;
     return compute(ctx, S(stream));
+}
+
+int sph_continuity(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    ctx->dt_fresh = false;
+    return compute(ctx, S(stream), nullptr, 1);
+}
+
+int sph_momentum(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    ctx->dt_fresh = false;
+    return compute(ctx, S(stream), nullptr, 2);
 }
 
 int sph_dt(sph_ctx_t *ctx, void *stream) {

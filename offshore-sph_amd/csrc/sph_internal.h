@@ -92,6 +92,7 @@ struct PairIn {
 struct PairOut {
     double *ax, *ay, *drho;
     int *count;  // optional: neighbour count per particle (cell order), -1 for walls
+    int mode;    // 0 fused, 1 continuity only (drho), 2 momentum only (ax, ay)
 };
 
 struct Phys {

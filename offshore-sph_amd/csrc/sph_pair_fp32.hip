@@ -30,7 +30,7 @@ hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, 
     P.drho = out.drho;
     P.count = out.count;
     P.n = n;
-    return launch_pair<float, false>(ctrl, P, ph, threads, s);
+    return launch_pair<float, false>(ctrl, P, ph, threads, out.mode, s);
 }
 
 }  // namespace hipsph

@@ -30,7 +30,7 @@ hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, i
     P.drho = out.drho;
     P.count = out.count;
     P.n = n;
-    return launch_pair<double, true>(ctrl, P, ph, threads, s);
+    return launch_pair<double, true>(ctrl, P, ph, threads, out.mode, s);
 }
 
 }  // namespace hipsph

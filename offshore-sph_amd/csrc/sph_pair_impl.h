@@ -3,8 +3,9 @@
 // Included by sph_pair_fp64.hip (R = double, reference order,
 // -ffp-contract=off) and sph_pair_fp32.hip (R = float, -ffp-contract=fast-honor-pragmas).
 //
-// The pass is VALU-bound (rocprof: ~93% of the VALU issue slots busy), so the
-// design minimises instructions per candidate:
+// The pass is VALU-bound (counters: ~2/3 of the VALU issue slots busy, the
+// rest waiting on loads; measured HBM traffic ~0.5 TB/s), so the design
+// minimises instructions per candidate:
 //   * One thread per particle of the cell-ordered state; fluid particles are
 //     targets. Consecutive threads hold neighbouring particles, so the
 //     candidates of a wavefront overlap and are served by L1/L2.
@@ -22,6 +23,10 @@
 //     runs the costly pair arithmetic over the list only, so lanes of a wave
 //     stay busy instead of idling through candidates that add nothing. The list
 //     is flushed when full, which keeps the order of the sums.
+//
+// MODE selects the equations: kFused (continuity + momentum + wall force +
+// gravity), kContinuity (drho only) or kMomentum (ax, ay only). The split
+// passes evaluate the same expressions in the same order as the fused one.
 #pragma once
 #include "sph_internal.h"
 
@@ -31,6 +36,7 @@ namespace {
 constexpr int kPairThreads = 256;
 constexpr int kListLen = 32;  // list entries per thread before a flush
 constexpr double kPI = 3.141592653589793;
+constexpr int kFused = 0, kContinuity = 1, kMomentum = 2;
 
 __device__ inline double rec_vx(const Rec<double>::A &, const Rec<double>::B &b) { return b.vx; }
 __device__ inline double rec_vy(const Rec<double>::A &, const Rec<double>::B &b) { return b.vy; }
@@ -156,7 +162,7 @@ struct Sums {
 };
 
 // Pass 2: the pair arithmetic for candidate u.
-template <typename R, int KERNEL>
+template <typename R, int KERNEL, int MODE>
 __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int u, Sums<R> &S) {
     const typename Rec<R>::A a = P.A[u];
     if constexpr (sizeof(R) == 8) {
@@ -173,8 +179,11 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
             const typename Rec<R>::B b = P.B[u];
             const double vx = (double)T.vx - (double)rec_vx(a, b), vy = (double)T.vy - (double)rec_vy(a, b);
             const double mj = (double)a.w;
-            const double dot = vx * dwx + vy * dwy;
-            S.arho.add((R)(mj * dot));
+            if constexpr (MODE != kMomentum) {
+                const double dot = vx * dwx + vy * dwy;
+                S.arho.add((R)(mj * dot));
+            }
+            if constexpr (MODE == kContinuity) return;
             const double dv = vx * dx + vy * dy;
             double PI_ij = 0.0;
             if (dv < 0) {
@@ -188,7 +197,7 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
             const double factor = (double)T.slf + (double)b.prs + PI_ij;
             S.a0.add((R)(-mj * factor * dwx));
             S.a1.add((R)(-mj * factor * dwy));
-        } else if (!(r > ph.r0) && r > 1e-12) {
+        } else if (MODE != kContinuity && !(r > ph.r0) && r > 1e-12) {
             const double frac = ph.r0 / r;
             const double tmp = ipow(frac, ph.p1) - ipow(frac, ph.p2);
             const double fac = ph.D * tmp;
@@ -210,7 +219,8 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
             const float dwx = norm * dx, dwy = norm * dy;
             const float vx = (float)T.vx - (float)rec_vx(a, b), vy = (float)T.vy - (float)rec_vy(a, b);
             const float mj = (float)a.w;
-            S.arho.add(mj * (vx * dwx + vy * dwy));
+            if constexpr (MODE != kMomentum) S.arho.add(mj * (vx * dwx + vy * dwy));
+            if constexpr (MODE == kContinuity) return;
             const float dv = vx * dx + vy * dy;
             float PI_ij = 0.0f;
             if (dv < 0.f) {
@@ -226,7 +236,7 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
             const float fm = -mj * ((float)T.slf + (float)b.prs + PI_ij);
             S.a0.add(fm * dwx);
             S.a1.add(fm * dwy);
-        } else if (!(r > (float)ph.r0) && r > 1e-12f) {
+        } else if (MODE != kContinuity && !(r > (float)ph.r0) && r > 1e-12f) {
             const float frac = (float)ph.r0 * rinv;
             const float tmp = ipow(frac, (float)ph.p1) - ipow(frac, (float)ph.p2);
             const float fac = (float)ph.D * tmp * (rinv * rinv);
@@ -236,16 +246,16 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
     }
 }
 
-template <typename R, int KERNEL>
+template <typename R, int KERNEL, int MODE>
 __device__ inline void flush(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, const unsigned *list, int nlist,
                              Sums<R> &S) {
-    for (int j = 0; j < nlist; ++j) contribute<R, KERNEL>(P, T, ph, (int)list[j * blockDim.x], S);
+    for (int j = 0; j < nlist; ++j) contribute<R, KERNEL, MODE>(P, T, ph, (int)list[j * blockDim.x], S);
 }
 
 // Pass 1 over the candidates [s, e) of one cell (descending for the reference
 // order): counts neighbours and appends contributors to this thread's list,
 // flushing it through pass 2 when full.
-template <typename R, int KERNEL, bool DESC>
+template <typename R, int KERNEL, bool DESC, int MODE>
 __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int s, int e, unsigned *list,
                                 int &nlist, Sums<R> &S) {
     constexpr double kEps = sizeof(R) == 8 ? 1e-12 : 1e-5;
@@ -270,10 +280,25 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
             in = exact_q_test(T.X, T.Y, T.H, a.x, a.y, hj, ph.qmax);
         }
         cnt += in;
-        if (in && r2 > R(0) && !(r2 > sup2)) {  // r = 0 pairs contribute exactly nothing
+        // r = 0 pairs contribute exactly nothing; walls add nothing to continuity
+        if (in && r2 > R(0) && !(r2 > sup2) && (MODE != kContinuity || a.w >= R(0))) {
             list[nlist * blockDim.x] = (unsigned)u;
-            if (++nlist == kListLen) {
-                flush<R, KERNEL>(P, T, ph, list, nlist, S);
+            ++nlist;
+            if constexpr (KERNEL != SPH_KERNEL_GAUSSIAN) {
+                // 2h support: ~21 contributors, the list rarely fills
+                if (nlist == kListLen) {
+                    flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
+                    nlist = 0;
+                }
+            }
+        }
+        if constexpr (KERNEL == SPH_KERNEL_GAUSSIAN) {
+            // 3h support: ~42 contributors, every list fills at least once.
+            // When one lane's list is full, every active lane flushes
+            // together; per-lane flushes at different candidates would run
+            // the flush loop once per lane. A lane's sums keep their order.
+            if (__ballot(nlist == kListLen) != 0) {
+                flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
                 nlist = 0;
             }
         }
@@ -281,7 +306,7 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
     return cnt;
 }
 
-template <typename R, int KERNEL, bool EXACT>
+template <typename R, int KERNEL, bool EXACT, int MODE>
 __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArgs<R> P, Phys ph) {
     __shared__ unsigned s_list[kListLen * kPairThreads];
     if (!ctrl->active) return;
@@ -319,32 +344,45 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArg
             const int cy = c.y + db;
             if (!(ncy > cy && cy > -1)) continue;
             const int k = cx + ncx * cy;
-            cnt += scan_cell<R, KERNEL, EXACT>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list, nlist, S);
+            cnt += scan_cell<R, KERNEL, EXACT, MODE>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list, nlist,
+                                                      S);
         }
     }
-    flush<R, KERNEL>(P, T, ph, list, nlist, S);
+    flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
     if (P.count) P.count[t] = cnt;
     if (cnt == 0) return;  // SolverTools.py:166-169: no neighbours, keep the previous values
-    P.drho[t] = S.arho.get();
-    const double a0 = S.a0.get(), a1 = S.a1.get(), f0 = S.f0.get(), f1 = S.f1.get();
-    P.ax[t] = a0 + f0;
-    P.ay[t] = (a1 - ph.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+    if constexpr (MODE != kMomentum) P.drho[t] = S.arho.get();
+    if constexpr (MODE != kContinuity) {
+        const double a0 = S.a0.get(), a1 = S.a1.get(), f0 = S.f0.get(), f1 = S.f1.get();
+        P.ax[t] = a0 + f0;
+        P.ay[t] = (a1 - ph.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+    }
+}
+
+template <typename R, bool EXACT, int MODE>
+void launch_pair_mode(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, int nb, int threads, hipStream_t s) {
+    switch (ph.kernel) {
+        case SPH_KERNEL_GAUSSIAN:
+            k_pair<R, SPH_KERNEL_GAUSSIAN, EXACT, MODE><<<nb, threads, 0, s>>>(ctrl, P, ph);
+            break;
+        case SPH_KERNEL_WENDLAND:
+            k_pair<R, SPH_KERNEL_WENDLAND, EXACT, MODE><<<nb, threads, 0, s>>>(ctrl, P, ph);
+            break;
+        default:
+            k_pair<R, SPH_KERNEL_CUBIC, EXACT, MODE><<<nb, threads, 0, s>>>(ctrl, P, ph);
+    }
 }
 
 template <typename R, bool EXACT>
-hipError_t launch_pair(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, int threads, hipStream_t s) {
+hipError_t launch_pair(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, int threads, int mode, hipStream_t s) {
     if (threads != 64 && threads != 128 && threads != 256) threads = kPairThreads;
     const int nb = xcd_blocks(P.n, threads);
-    switch (ph.kernel) {
-        case SPH_KERNEL_GAUSSIAN:
-            k_pair<R, SPH_KERNEL_GAUSSIAN, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph);
-            break;
-        case SPH_KERNEL_WENDLAND:
-            k_pair<R, SPH_KERNEL_WENDLAND, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph);
-            break;
-        default:
-            k_pair<R, SPH_KERNEL_CUBIC, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph);
-    }
+    if (mode == kContinuity)
+        launch_pair_mode<R, EXACT, kContinuity>(ctrl, P, ph, nb, threads, s);
+    else if (mode == kMomentum)
+        launch_pair_mode<R, EXACT, kMomentum>(ctrl, P, ph, nb, threads, s);
+    else
+        launch_pair_mode<R, EXACT, kFused>(ctrl, P, ph, nb, threads, s);
     return hipGetLastError();
 }
 

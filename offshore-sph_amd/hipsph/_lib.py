@@ -54,6 +54,10 @@ class sph_state_t(C.Structure):
     _fields_ = [(f, C.c_void_p) for f in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p', 'label')]
 
 
+class sph_adjoint_t(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'g_alpha', 'g_co')]
+
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _I32 = C.c_int32
@@ -77,6 +81,15 @@ _PROTOS = {
     'sph_status': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), _P]),
     'sph_dt_history': (C.c_int, [_P, C.POINTER(_D), _I64, _P]),
     'sph_set_clock': (C.c_int, [_P, _D, C.c_int32, _P]),
+    'sph_continuity': (C.c_int, [_P, _P]),
+    'sph_tape_create': (C.c_int, [_P, _I32, C.POINTER(C.c_void_p)]),
+    'sph_tape_destroy': (C.c_int, [_P]),
+    'sph_tape_force_steps': (C.c_int, [_P, C.POINTER(_I64)]),
+    'sph_step_record': (C.c_int, [_P, _P, _I32, _P]),
+    'sph_adjoint_step': (C.c_int, [_P, _P, _I32, C.POINTER(sph_adjoint_t), _P]),
+    'sph_adjoint_segment': (C.c_int, [_P, _P, _I32, C.POINTER(sph_adjoint_t), _P]),
+    'sph_state_restore': (C.c_int, [_P, C.POINTER(sph_state_t), _D, _P]),
+    'sph_momentum': (C.c_int, [_P, _P]),
     'sph_kinetic_energy': (C.c_int, [_P, C.POINTER(_D), _P]),
     'sph_kernel_evaluate': (C.c_int, [_I32, _I64, _P, _P, _P, _P]),
     'sph_kernel_gradient': (C.c_int, [_I32, _I64, _P, _P, _P, _P, _P]),

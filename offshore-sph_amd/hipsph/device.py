@@ -148,6 +148,16 @@ class Engine:
             raise ValueError('fluid particles need a finite, non-negative mass')
         call('sph_state_upload', self._ctx, C.byref(self._state(fields)), int(bool(reset_clock)), self._stream())
 
+    def restore(self, arrays, t):
+        """A saved state (device tensors or arrays, as from download) back in,
+        clock at t, without a host round trip."""
+        fields = set()
+        for k, v in arrays.items():
+            if k in self.io and k != 'p':
+                self.io[k].copy_(v.reshape(-1) if hasattr(v, 'reshape') else v)
+                fields.add(k)
+        call('sph_state_restore', self._ctx, C.byref(self._state(fields)), float(t), self._stream())
+
     def download(self, to_numpy=True):
         """State in reference index order (plus p = Tait + Pb)."""
         import torch
@@ -169,6 +179,14 @@ class Engine:
 
     def compute(self):
         call('sph_compute', self._ctx, self._stream())
+
+    def continuity(self):
+        """Continuity only (drho); ax, ay untouched."""
+        call('sph_continuity', self._ctx, self._stream())
+
+    def momentum(self):
+        """Momentum + AV + gravity + wall force only (ax, ay); drho untouched."""
+        call('sph_momentum', self._ctx, self._stream())
 
     def compute_counts(self):
         """One pair pass; returns the neighbour count it found per particle (-1 for walls)."""

@@ -100,6 +100,14 @@ def dam_break_c2(million=True):
     return dam_break(1.0, 2.0, 0.002, name='dam_break_500k_dx0.002')
 
 
+def dam_break_c5(small=False):
+    """Differentiable dam break: 0.5 x 1.0 column at dx = 0.005 (20,000 fluid) in a
+    2.0 x 1.5 tank; small: 0.4 x 0.5 at dx = 0.01 (2,000 fluid), same tank."""
+    if small:
+        return dam_break(0.4, 0.5, 0.01, tank_w=2.0, tank_h=1.5, name='dam_break_2k_dx0.01')
+    return dam_break(0.5, 1.0, 0.005, tank_w=2.0, tank_h=1.5, name='dam_break_20k_dx0.005')
+
+
 def stress_lattice(nside=2000, dx=1e-3, H=2.0, layers=3, rho0=1000.0, hfac=1.3, seed=SEED):
     """Pair-kernel stress: nside^2 jittered fluid particles in a closed box.
 

@@ -39,7 +39,7 @@ def test_binding_covers_header():
 
 
 def test_struct_layout_matches_header(tmp_path):
-    """Every field offset of sph_params_t / sph_state_t as gcc lays out the header."""
+    """Every field offset of sph_params_t / sph_state_t / sph_adjoint_t as gcc lays out the header."""
     import shutil
     import subprocess
     import pytest
@@ -47,7 +47,7 @@ def test_struct_layout_matches_header(tmp_path):
     if not shutil.which('gcc'):
         pytest.skip('gcc not available')
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hipsph.h"', 'int main(void) {']
-    for st in (_lib.sph_params_t, _lib.sph_state_t):
+    for st in (_lib.sph_params_t, _lib.sph_state_t, _lib.sph_adjoint_t):
         cname = st.__name__
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in st._fields_:
@@ -59,7 +59,7 @@ def test_struct_layout_matches_header(tmp_path):
     subprocess.run(['gcc', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
     got = dict(line.rsplit(' ', 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                                               check=True).stdout.split('\n') if line)
-    for st in (_lib.sph_params_t, _lib.sph_state_t):
+    for st in (_lib.sph_params_t, _lib.sph_state_t, _lib.sph_adjoint_t):
         cname = st.__name__
         assert int(got[cname]) == ctypes.sizeof(st)
         for f, _ in st._fields_:

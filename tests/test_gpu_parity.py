@@ -201,6 +201,44 @@ def test_compute_fp32c_vs_oracle(kernel):
     e.close()
 
 
+@pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian'])
+def test_split_passes_match_fused_and_oracle(kernel, precision):
+    """sph_continuity writes drho only and sph_momentum ax, ay only; each
+    equals the fused pass (same expressions, same order) and so the oracle."""
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 5)
+    m = sc.method()
+    say('split passes', kernel, precision)
+    fused = make_engine(pA, m, kernel=kernel, precision=precision)
+    fused.nn_update()
+    fused.compute()
+    want = fused.download()
+    fused.close()
+    e = make_engine(pA, m, kernel=kernel, precision=precision)
+    e.nn_update()
+    e.continuity()
+    c = e.download()
+    e.momentum()
+    mo = e.download()
+    e.close()
+    f = pA['label'] == 0
+    np.testing.assert_array_equal(c['ax'], pA['ax'])   # continuity leaves ax, ay alone
+    np.testing.assert_array_equal(c['ay'], pA['ay'])
+    np.testing.assert_array_equal(mo['drho'], c['drho'])  # momentum leaves drho alone
+    if precision == 'fp64':
+        np.testing.assert_array_equal(c['drho'][f], want['drho'][f])
+        np.testing.assert_array_equal(mo['ax'][f], want['ax'][f])
+        np.testing.assert_array_equal(mo['ay'][f], want['ay'][f])
+    else:
+        for k, got in (('drho', c['drho']), ('ax', mo['ax']), ('ay', mo['ay'])):
+            assert rel_err(got, want[k], f) < 1e-6, k
+    ref, _ = oracle_loop(pA, m, kernel)
+    tol = 0 if (precision == 'fp64' and kernel == 'cubic') else (1e-12 if precision == 'fp64' else 2e-5)
+    assert rel_err(c['drho'], ref['drho'], f) <= tol
+    assert rel_err(mo['ax'], ref['ax'], f) <= tol
+
+
 def test_compute_stress_lattice_fp64():
     sc = scenarios.stress_lattice(nside=200)
     m = sc.method()

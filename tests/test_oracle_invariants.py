@@ -402,3 +402,28 @@ def test_two_particles_artificial_viscosity_closed_form():
     # linear in alpha at beta = 0
     dbl, *_ = _av_pair(u, 2 * alpha)
     assert (dbl['ax'][0] - base['ax'][0]) == pytest.approx(2 * (pA['ax'][0] - base['ax'][0]), rel=1e-10)
+
+
+def test_torch_dense_model_matches_oracle_trajectory():
+    """The FP64 dense-pair torch model (the reference gradient of the adjoint
+    tests) follows the oracle's trajectory: PEC, dynamic dt, walls, AV."""
+    import torch
+    from torch_ref import DenseWCSPH
+    from hipsph import scenarios
+    sc = scenarios.dam_break(0.2, 0.2, 0.02, tank_w=0.5, tank_h=0.4, name='tiny')
+    pA = oracle.as_particles(sc.pA.copy())
+    rng = np.random.default_rng(11)
+    f = pA['label'] == 0
+    pA['vx'][f] = rng.normal(0, 0.3, f.sum())
+    pA['vy'][f] = rng.normal(0, 0.3, f.sum())
+    m = sc.method()
+    model = DenseWCSPH(pA, m, sc.h)
+    K = 12
+    x, y, vx, vy, rho = model.run(torch.tensor(m.alpha, dtype=torch.float64),
+                                  torch.tensor(m.co, dtype=torch.float64),
+                                  torch.as_tensor(pA['vx'][f].copy()), torch.as_tensor(pA['vy'][f].copy()), K)
+    o = oracle.Solver(pA, m, 'cubic', 'pec', h=sc.h)
+    o.run(K)
+    for got, k in ((x, 'x'), (y, 'y'), (vx, 'vx'), (vy, 'vy'), (rho, 'rho')):
+        want = o.pA[k][f]
+        assert np.max(np.abs(got.numpy() - want)) <= 1e-11 * max(1.0, np.max(np.abs(want))), k
