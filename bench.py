@@ -195,6 +195,71 @@ def adjoint_timing(repeats=3, nsteps=200, every=20):
             'dL_dalpha': float(ga), 'dL_dc0': float(gc), 'dL_dv0_norm': float(torch.linalg.norm(gv))}
 
 
+def ensemble(steps=500, count=256, seq_steps=10, seq_instances=64):
+    """The batched ensemble: `count` dam breaks (width U(0.5, 1.5), height
+    U(0.5, 2.5), dx from {0.005, 0.01, 0.02}, alpha U(0.005, 0.05), seeded per
+    instance), FP32C pairs and FP64 integrator, `steps` steps each in one batch
+    (one launch per stage, hipGraph replay, pair workgroup size autotuned).
+    Beside it: the same instances as separate engines one after another
+    (graph replay each), timed on the first `seq_instances` for `seq_steps`."""
+    import torch
+    import numpy as np
+    from hipsph import scenarios
+    from hipsph.batch import BatchEngine, autotune, params_for
+    from hipsph.device import Engine, soa_from_particles
+    scs, alphas = scenarios.ensemble_c4(count)
+    nf = sum(s.n_fluid for s in scs)
+    n = sum(s.n for s in scs)
+    print(f'# ensemble: {count} instances, {n} particles ({nf} fluid)', file=sys.stderr, flush=True)
+    b = BatchEngine([params_for(s, a, precision='fp32c') for s, a in zip(scs, alphas)])
+    b.upload([soa_from_particles(s.pA) for s in scs])
+    best, table = autotune(b)
+    b.upload([soa_from_particles(s.pA) for s in scs])
+    b.step(2, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps, graph=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = b.status()
+    b.timing_enable(True)
+    b.step(5, graph=False)
+    pair_ms = b.timing_read()['pair_ms']
+    b.timing_enable(False)
+    b.close()
+    steps_done = [s['steps'] for s in st]
+    halted = sum(1 for s in st if s['flags'] & 3)
+    # sequential baseline on a subset
+    sub = list(zip(scs, alphas))[:seq_instances]
+    seq = 0.0
+    for s, a in sub:
+        m = s.method()
+        m.alpha = a
+        from hipsph.device import EngineParams
+        e = Engine(EngineParams.from_method(s.n, m, precision='fp32c'))
+        e.upload(soa_from_particles(s.pA))
+        e.step(3, graph=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        e.step(seq_steps, graph=True)
+        torch.cuda.synchronize()
+        seq += (time.perf_counter() - t1) / seq_steps
+        e.close()
+    sub_nf = sum(s.n_fluid for s, _ in sub)
+    seq_full_ms = seq * 1e3 * nf / sub_nf  # scaled by fluid count to the whole ensemble
+    res = {'instances': count, 'n_particles': n, 'n_fluid': nf, 'steps_each': steps,
+           'min_steps': int(min(steps_done)), 'halted_instances': halted,
+           'ms_per_step': el / steps * 1e3,
+           'particle_steps_per_s': nf * steps / el, 'pair_ms': pair_ms,
+           'pair_threads': best, 'autotune': table,
+           'sequential_engines_ms_per_step_est': seq_full_ms,
+           'sequential_sample': f'{seq_instances} instances x {seq_steps} steps, scaled by fluid count',
+           'batch_speedup_vs_sequential': seq_full_ms / (el / steps * 1e3)}
+    print(f'# ensemble: {res["ms_per_step"]:.3f} ms/step ({res["particle_steps_per_s"]:.3g} particle-steps/s), '
+          f'pair {pair_ms:.3f} ms, sequential engines ~{seq_full_ms:.2f} ms/step', file=sys.stderr, flush=True)
+    return res
+
+
 STRESS_BYTES = {  # algorithmic bytes per launch (N all, N_f fluid, C cells)
     ('fused', 'fp32c'): (21, 12, 8), ('fused', 'fp64'): (41, 24, 8),
     ('continuity', 'fp32c'): (17, 4, 8), ('continuity', 'fp64'): (33, 8, 8),
@@ -278,6 +343,7 @@ def main():
     ap.add_argument('--no-precision', action='store_true', help='skip the FP32C-vs-FP64 error report')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward+adjoint timing')
     ap.add_argument('--adjoint', action='store_true', help='only run the forward+adjoint timing')
+    ap.add_argument('--ensemble', action='store_true', help='only run the 256-instance batched ensemble')
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU (slab-decomposed) path is not built. '
@@ -286,6 +352,11 @@ def main():
 
     if a.stress:
         stress(a.stress)
+        return
+    if a.ensemble:
+        print(json.dumps({'metric': 'batched ensemble particle-steps/s (256 dam breaks)', 'n_gpus': 1,
+                          'higher_is_better': True, 'data': 'synthetic', 'dtype': 'f32',
+                          'ensemble': ensemble()}), flush=True)
         return
     if a.adjoint:
         print(json.dumps({'metric': 'forward+adjoint / forward wall time', 'n_gpus': 1, 'higher_is_better': False,

@@ -219,6 +219,29 @@ int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const 
  * with the clock set to t and advancing, flags cleared. */
 int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *stream);
 
+/* ---- batched ensemble (no counterpart in the reference) ----
+ * K independent simulations of uneven sizes advanced together: one launch per
+ * stage covers every instance (segmented cell grid, per-instance clock, dt,
+ * grid and constants), and a step replays as one hipGraph. Each instance
+ * evolves exactly as sph_step would advance it alone. params[k] describes
+ * instance k (n, constants, duration, cell_capacity); kernel, precision and
+ * integrator must be the same for all. State arrays are the instances'
+ * states concatenated in instance order (instance k at offset sum of n_j for
+ * j < k), in each instance's reference index order. */
+typedef struct sph_batch sph_batch_t;
+int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **out);
+int sph_batch_destroy(sph_batch_t *batch);
+int sph_batch_upload(sph_batch_t *batch, const sph_state_t *st, void *stream);
+int sph_batch_download(sph_batch_t *batch, const sph_state_t *st, void *stream);
+int sph_batch_step(sph_batch_t *batch, int64_t nsteps, int32_t use_graph, void *stream);
+/* Per instance k: out_d[4k..4k+3] = {t, dt, dt_c, dt_f}, out_i[2k..2k+1] =
+ * {steps, flags}. Synchronises `stream`. */
+int sph_batch_status(sph_batch_t *batch, double *out_d, int64_t *out_i, void *stream);
+/* Pair-pass workgroup size (64, 128, 256), as sph_ctx_set_tuning. */
+int sph_batch_set_tuning(sph_batch_t *batch, int32_t pair_threads);
+int sph_batch_timing_enable(sph_batch_t *batch, int32_t on);
+int sph_batch_timing_read(sph_batch_t *batch, double *avg_ms_pair, int64_t *count, int32_t reset);
+
 /* ---- element-wise plugin functions on device arrays ---- */
 /* Kernel.evaluate / Kernel.gradient (src/Kernels/Kernel.py:6-12) for
  * CubicSpline (src/Kernels/CubicSpline.py:10-70), Gaussian

@@ -9,6 +9,7 @@ gfx950 and links them into offshore-sph_amd/hipsph/libhipsph.so:
   sph_pair_fp32.hip -ffp-contract=fast-honor-pragmas (float pair pass; the exact
                     neighbour re-test and the TwoSum steps switch contraction off)
   sph_adjoint.hip   -ffp-contract=fast (tape and reverse step; not bit-matched)
+  sph_batch.hip     -ffp-contract=off  (batched ensemble: the engine's arithmetic per instance)
 Packed FP32 instructions are disabled for every device compile
 (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
@@ -35,6 +36,7 @@ UNITS = [
     ('sph_pair_fp64.hip', ['-ffp-contract=off']),
     ('sph_pair_fp32.hip', ['-ffp-contract=fast-honor-pragmas']),
     ('sph_adjoint.hip', ['-ffp-contract=fast']),
+    ('sph_batch.hip', ['-ffp-contract=off']),
 ]
 
 

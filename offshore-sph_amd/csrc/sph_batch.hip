@@ -1,0 +1,873 @@
+// sph_batch.hip — many independent WCSPH simulations advanced together.
+//
+// A batch holds K instances of uneven particle counts (an ensemble of dam
+// breaks with different geometry, resolution and constants). All instances
+// share one concatenated, cell-ordered state and one segmented cell grid, and
+// every stage of the step is ONE launch covering all instances:
+//   * instance k owns particle slots [off_k, off_k + n_k) for good (its
+//     particles only ever move among its own slots) and the bins
+//     [cellBase_k, cellBase_k + ncells_k] of the grid, rebased every step by
+//     a scan over the instances of (ncells_k + 1);
+//   * reductions (bounds, dt) run over fixed chunks of one instance each and a
+//     workgroup per instance folds them;
+//   * keys, the prefix scan, scatter, bin sort, reorder and the pair pass are
+//     the single-simulation kernels with a per-particle instance lookup
+//     (the scan/scatter/sort kernels are literally shared, over the union of
+//     the instances' bins);
+//   * every instance has its own control block (BInst): clock, dt, grid,
+//     halt flags and constants (alpha, c0, B, r0, D ...).
+// A step is captured in a hipGraph like the single simulation's. The
+// arithmetic per particle is the single engine's, in the same order, so an
+// instance evolves exactly as it would alone.
+//
+// Built with -ffp-contract=off (as sph_engine.hip).
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "sph_ctx.h"
+#include "sph_common.h"
+
+using namespace hipsph;
+
+namespace {
+
+constexpr int kChunk = 2048;  // particles per reduction chunk (one instance each)
+
+struct BState {  // the concatenated cell-ordered state (as State)
+    State s;
+};
+
+// ---------------------------------------------------------------------------
+// per-instance dt (src/Solver.py:210-230, src/Equations/TimeStep.py:40-90)
+// ---------------------------------------------------------------------------
+struct DtAcc {
+    double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
+    __device__ void add(double h, double ax, double ay) {
+        double a2 = ax * ax + ay * ay;
+        if (h < mn) mn = h;
+        if (a2 > mx) mx = a2;
+        if (!(a2 == a2)) bad += 1.0;
+        cnt += 1.0;
+    }
+    __device__ void store(double *part, int NC, int c, double *lds) {
+        mn = block_reduce(mn, MinOp(), lds);
+        mx = block_reduce(mx, MaxOp(), lds);
+        cnt = block_reduce(cnt, AddOp(), lds);
+        bad = block_reduce(bad, AddOp(), lds);
+        if (threadIdx.x == 0) {
+            part[0 * NC + c] = mn;
+            part[1 * NC + c] = mx;
+            part[2 * NC + c] = cnt;
+            part[3 * NC + c] = bad;
+        }
+    }
+};
+
+// chunk c covers [cs[c], cs[c + 1]) inside one instance
+__global__ void kb_dt_partial(const int *cs, State s, double *part) {
+    __shared__ double lds[8];
+    const int c = blockIdx.x, NC = gridDim.x;
+    DtAcc v;
+    for (int t = cs[c] + threadIdx.x; t < cs[c + 1]; t += blockDim.x)
+        if (s.label[t] == kFluid) v.add(s.h[t], s.ax[t], s.ay[t]);
+    v.store(part, NC, c, lds);
+}
+
+// One workgroup per instance: fold its chunks, open or close its step.
+__global__ void kb_dt_final(BInst *inst, const int *chunk0, const double *part, int NC) {
+    __shared__ double lds[4 * 16];
+    BInst &I = inst[blockIdx.x];
+    const int c0 = chunk0[blockIdx.x], c1 = chunk0[blockIdx.x + 1];
+    const int kind[4] = {0, 1, 2, 2};
+    double v[4] = {INFINITY, -INFINITY, 0.0, 0.0};
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const double a = part[0 * NC + c], b = part[1 * NC + c];
+        v[0] = a < v[0] ? a : v[0];
+        v[1] = b > v[1] ? b : v[1];
+        v[2] += part[2 * NC + c];
+        v[3] += part[3 * NC + c];
+    }
+    block_fold<4>(v, kind, lds);
+    if (threadIdx.x != 0) return;
+    const double mn = v[0], mx = v[1], cnt = v[2], bad = v[3];
+    const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID;
+    if ((I.flags & halt) || !(I.t < I.duration)) {
+        I.active = 0;
+        return;
+    }
+    if (cnt == 0) {
+        I.flags |= SPH_FLAG_NO_FLUID;
+        I.active = 0;
+        return;
+    }
+    if (bad != 0) I.flags |= SPH_FLAG_NONFINITE;
+    double dt, c, f;
+    if (I.fixed_dt > 0.0) {
+        dt = I.fixed_dt;
+        c = 0.0;
+        f = 0.0;
+    } else {
+        c = I.gamma_c * mn / I.co;
+        f = (mx < 1e-12) ? 1e10 : I.gamma_f * sqrt(mn / mx);
+        dt = (f < c) ? f : c;
+        if (dt < 1e-6) I.flags |= SPH_FLAG_DT_SMALL;
+    }
+    I.dt = dt;
+    I.dt_c = c;
+    I.dt_f = f;
+    if (I.advance_t) I.t = I.t + dt;
+    I.step = I.step + 1;
+    I.active = 1;
+}
+
+// ---------------------------------------------------------------------------
+// neighbour update over the segmented grid (src/Tools/NNLinkedList.py)
+// ---------------------------------------------------------------------------
+__device__ inline void bupd_pos(const State &a, int i, bool pred, double dt, double &x, double &y) {
+    x = a.x[i];
+    y = a.y[i];
+    if (pred && a.label[i] == kFluid) {
+        x = pec_half_pos(x, a.vx[i], dt);
+        y = pec_half_pos(y, a.vy[i], dt);
+    }
+}
+
+__global__ void kb_bounds(const int *cs, const int *chunkInst, const BInst *inst, State a, int pec, double *part) {
+    __shared__ double lds[8];
+    const int c = blockIdx.x, NC = gridDim.x;
+    const BInst &I = inst[chunkInst[c]];
+    const bool pred = pec && I.active;
+    double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, hmx = -INFINITY;
+    double bad = 0.0;
+    for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x) {
+        double xi, yi;
+        bupd_pos(a, i, pred, I.dt, xi, yi);
+        const double hi = a.h[i];
+        if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;
+        xmn = xi < xmn ? xi : xmn;
+        xmx = xi > xmx ? xi : xmx;
+        ymn = yi < ymn ? yi : ymn;
+        ymx = yi > ymx ? yi : ymx;
+        hmn = hi < hmn ? hi : hmn;
+        hmx = hi > hmx ? hi : hmx;
+    }
+    xmn = block_reduce(xmn, MinOp(), lds);
+    xmx = block_reduce(xmx, MaxOp(), lds);
+    ymn = block_reduce(ymn, MinOp(), lds);
+    ymx = block_reduce(ymx, MaxOp(), lds);
+    hmn = block_reduce(hmn, MinOp(), lds);
+    hmx = block_reduce(hmx, MaxOp(), lds);
+    bad = block_reduce(bad, AddOp(), lds);
+    if (threadIdx.x == 0) {
+        part[0 * NC + c] = xmn;
+        part[1 * NC + c] = xmx;
+        part[2 * NC + c] = ymn;
+        part[3 * NC + c] = ymx;
+        part[4 * NC + c] = hmn;
+        part[5 * NC + c] = hmx;
+        part[6 * NC + c] = bad;
+    }
+}
+
+// One workgroup per instance: its grid (_get_cell_size :103-110,
+// _get_number_of_cells :112-127). A finished or halted instance gets
+// ncells = 0 (its particles park in its one bin).
+__global__ void kb_grid(BInst *inst, const int *chunk0, const double *part, int NC) {
+    __shared__ double lds[kBoundsQ * 16];
+    BInst &I = inst[blockIdx.x];
+    const int c0 = chunk0[blockIdx.x], c1 = chunk0[blockIdx.x + 1];
+    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
+    double v[kBoundsQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < kBoundsQ; ++q) {
+            const double u = part[q * NC + c];
+            v[q] = kind[q] == 0 ? (u < v[q] ? u : v[q]) : kind[q] == 1 ? (u > v[q] ? u : v[q]) : v[q] + u;
+        }
+    }
+    block_fold<kBoundsQ>(v, kind, lds);
+    if (threadIdx.x != 0) return;
+    Grid g = I.grid;
+    g.ncells = 0;
+    if (I.active) {
+        const double xmn = v[0], xmx = v[1], ymn = v[2], ymx = v[3], hmn = v[4], hmx = v[5], bad = v[6];
+        double cell_size = hmn * I.scale;
+        if (cell_size < 1e-6) cell_size = 1.0;
+        const double cell_size1 = 1. / cell_size;
+        const double fx = ceil(cell_size1 * (xmx - xmn));
+        const double fy = ceil(cell_size1 * (ymx - ymn));
+        if (bad != 0.0 || !(fx < 2.0e9) || !(fy < 2.0e9)) {
+            I.flags |= SPH_FLAG_NONFINITE;
+            I.active = 0;
+        } else {
+            long long ncx = (long long)fx, ncy = (long long)fy;
+            if (ncx < 1) ncx = 1;
+            if (ncy < 1) ncy = 1;
+            if (ncx * ncy > I.cap) {
+                I.flags |= SPH_FLAG_CAPACITY;
+                I.active = 0;
+            } else {
+                g.xmin = xmn;
+                g.xmax = xmx;
+                g.ymin = ymn;
+                g.ymax = ymx;
+                g.cs = cell_size;
+                g.uniform_h = hmn == hmx;
+                g.ncx = ncx;
+                g.ncy = ncy;
+                g.ncells = ncx * ncy;
+            }
+        }
+    }
+    I.grid = g;
+}
+
+// Rebase the instances' bins (exclusive scan of ncells + 1) and publish the
+// union as one grid for the shared scan / scatter / bin-sort kernels.
+__global__ void kb_bases(BInst *inst, int K, Ctrl *view, long long capBins) {
+    __shared__ long long lds[16];
+    long long carry = 0;
+    for (int b0 = 0; b0 < K; b0 += blockDim.x) {
+        const int k = b0 + threadIdx.x;
+        long long v = k < K ? inst[k].grid.ncells + 1 : 0;
+        // inclusive scan in the wave, then across waves
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        long long sc = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const long long u = __shfl_up(sc, o, 64);
+            if (lane >= o) sc += u;
+        }
+        if (lane == 63) lds[w] = sc;
+        __syncthreads();
+        long long off = 0, tot = 0;
+        for (int q = 0; q < nw; ++q) {
+            if (q < w) off += lds[q];
+            tot += lds[q];
+        }
+        __syncthreads();
+        if (k < K) inst[k].cellBase = carry + off + sc - v;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        view->active = carry <= capBins;
+        if (carry > capBins) {  // cannot happen with per-instance caps summed into capBins; kept as a guard
+            for (int k = 0; k < K; ++k) {
+                inst[k].flags |= SPH_FLAG_CAPACITY;
+                inst[k].active = 0;
+            }
+        }
+        view->grid.ncells = carry - 1;
+    }
+}
+
+// _bin :129-141 with the instance's grid; rank by runs as k_keys does.
+__global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, int pec, int *counts, int *bin,
+                        int *rank, int2 *cxy, int *binflag, const Ctrl *view) {
+    if (!view->active) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int b = -1;
+    if (t < n) {
+        const BInst &I = inst[instOf[t]];
+        const Grid g = I.grid;
+        if (I.active) {
+            double x, y;
+            bupd_pos(a, t, pec != 0, I.dt, x, y);
+            double fx = floor((x - g.xmin) / g.cs);
+            double fy = floor((y - g.ymin) / g.cs);
+            fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
+            fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
+            const long long cx = (long long)fx, cy = (long long)fy;
+            const long long key = cx + g.ncx * cy;
+            b = (int)(I.cellBase + ((key >= 0 && key < g.ncells) ? key : g.ncells));
+            cxy[t] = make_int2((int)cx, (int)cy);
+        } else {
+            b = (int)I.cellBase;  // parked: the instance's single bin
+            cxy[t] = make_int2(0, 0);
+        }
+    }
+    const int id = t < n ? a.id[t] : 0x7fffffff;
+    const int prev = __shfl_up(b, 1, 64);
+    const int prev_id = __shfl_up(id, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || b != prev || id < prev_id);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int leader = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int next = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (lane == leader && b >= 0) {
+        base = atomicAdd(&counts[b], next - leader);
+        if (base != 0) binflag[b] = 1;
+    }
+    base = __shfl(base, leader, 64);
+    if (t < n) {
+        bin[t] = b;
+        rank[t] = base + (lane - leader);
+    }
+}
+
+// Permute the state into the new order with the PEC predictor of active
+// instances fused in, and write the pair records (as k_reorder).
+__global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State b, const int *perm,
+                           const int2 *cxy_old, int2 *cxy_s, int n, int fp32, void *recA, void *recB, int pec) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const BInst &I = inst[instOf[t]];  // slots never change instance
+    const int s = perm[t];
+    double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
+    const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
+    const int8_t lab = a.label[s];
+    if (I.active && pec && lab == kFluid) {
+        const double dt = I.dt;
+        b.x0[t] = x;
+        b.y0[t] = y;
+        b.vx0[t] = vx;
+        b.vy0[t] = vy;
+        b.rho0[t] = rho;
+        x = pec_half_pos(x, vx, dt);
+        y = pec_half_pos(y, vy, dt);
+        vx = pec_half_vel(vx, ax, dt, I.damping);
+        vy = pec_half_vel(vy, ay, dt, I.damping);
+        rho = pec_half_rho(rho, drho, dt, I.strict);
+    } else {
+        b.x0[t] = a.x0[s];
+        b.y0[t] = a.y0[s];
+        b.vx0[t] = a.vx0[s];
+        b.vy0[t] = a.vy0[s];
+        b.rho0[t] = a.rho0[s];
+    }
+    b.x[t] = x;
+    b.y[t] = y;
+    b.vx[t] = vx;
+    b.vy[t] = vy;
+    b.rho[t] = rho;
+    b.m[t] = m;
+    b.h[t] = h;
+    b.label[t] = lab;
+    b.id[t] = a.id[s];
+    b.ax[t] = ax;
+    b.ay[t] = ay;
+    b.drho[t] = drho;
+    cxy_s[t] = cxy_old[s];
+    const double p = tait_p(rho, lab, I.gamma, I.B, I.rho0, I.Pb);
+    const double pr = p / (rho * rho);
+    const double w = lab == kFluid ? m : -(double)lab;
+    if (fp32) {
+        Rec<float>::A ra;
+        ra.x = x;
+        ra.y = y;
+        ra.h = (float)h;
+        ra.w = (float)w;
+        ra.vx = (float)vx;
+        ra.vy = (float)vy;
+        ((Rec<float>::A *)recA)[t] = ra;
+        ((Rec<float>::B *)recB)[t] = Rec<float>::B{(float)rho, (float)pr};
+    } else {
+        ((Rec<double>::A *)recA)[t] = Rec<double>::A{x, y, h, w};
+        ((Rec<double>::B *)recB)[t] = Rec<double>::B{vx, vy, rho, pr};
+    }
+}
+
+// PEC.correct / Euler (fluid of active instances) + the next dt partials.
+__global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *inst, State s, int euler,
+                           double *part) {
+    __shared__ double lds[8];
+    const int c = blockIdx.x, NC = gridDim.x;
+    const BInst &I = inst[chunkInst[c]];
+    const double dt = I.dt, damping = I.damping;
+    DtAcc v;
+    for (int j = cs[c] + threadIdx.x; j < cs[c + 1]; j += blockDim.x) {
+        if (s.label[j] != kFluid) continue;
+        const double ax = s.ax[j], ay = s.ay[j];
+        v.add(s.h[j], ax, ay);
+        if (!I.active) continue;
+        if (!euler) {
+            double x0 = s.x0[j], y0 = s.y0[j], vx0 = s.vx0[j], vy0 = s.vy0[j], rho0 = s.rho0[j];
+            double mid_x = x0 + 0.5 * dt * s.vx[j];
+            double mid_y = y0 + 0.5 * dt * s.vy[j];
+            double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
+            double mid_vy = (vy0 + 0.5 * dt * ay) / (1 + 0.5 * damping);
+            s.x[j] = 2 * mid_x - x0;
+            s.y[j] = 2 * mid_y - y0;
+            s.vx[j] = 2 * mid_vx - vx0;
+            s.vy[j] = 2 * mid_vy - vy0;
+            double mid_rho = rho0 + 0.5 * dt * s.drho[j];
+            double r = 2 * mid_rho - rho0;
+            if (I.strict && r < 0.0) r = 0.0;
+            s.rho[j] = r;
+        } else {
+            double vx = s.vx[j], vy = s.vy[j];
+            s.x[j] = s.x[j] + dt * vx + 0.5 * dt * dt * ax;
+            s.y[j] = s.y[j] + dt * vy + 0.5 * dt * dt * ay;
+            s.vx[j] = vx + dt * ax;
+            s.vy[j] = vy + dt * ay;
+            s.rho[j] = s.rho[j] + dt * s.drho[j];
+        }
+    }
+    v.store(part, NC, c, lds);
+}
+
+__global__ void kb_upload(State d, sph_state_t in, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d.x[i] = d.x0[i] = in.x[i];
+    d.y[i] = d.y0[i] = in.y[i];
+    d.vx[i] = d.vx0[i] = in.vx[i];
+    d.vy[i] = d.vy0[i] = in.vy[i];
+    d.rho[i] = d.rho0[i] = in.rho[i];
+    d.ax[i] = in.ax ? in.ax[i] : 0.0;
+    d.ay[i] = in.ay ? in.ay[i] : 0.0;
+    d.drho[i] = in.drho ? in.drho[i] : 0.0;
+    d.m[i] = in.m[i];
+    d.h[i] = in.h[i];
+    d.label[i] = in.label[i];
+    d.id[i] = i;
+}
+
+__global__ void kb_download(State s, sph_state_t out, const BInst *inst, const int *instOf, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = s.id[t];
+    if (out.x) out.x[i] = s.x[t];
+    if (out.y) out.y[i] = s.y[t];
+    if (out.vx) out.vx[i] = s.vx[t];
+    if (out.vy) out.vy[i] = s.vy[t];
+    if (out.rho) out.rho[i] = s.rho[t];
+    if (out.ax) out.ax[i] = s.ax[t];
+    if (out.ay) out.ay[i] = s.ay[t];
+    if (out.drho) out.drho[i] = s.drho[t];
+    if (out.m) out.m[i] = s.m[t];
+    if (out.h) out.h[i] = s.h[t];
+    if (out.label) out.label[i] = s.label[t];
+    if (out.p) {
+        const BInst &I = inst[instOf[t]];
+        out.p[i] = tait_p(s.rho[t], s.label[t], I.gamma, I.B, I.rho0, I.Pb);
+    }
+}
+
+inline int nblk_b(long long n, int bs = kBlock) {
+    long long b = (n + bs - 1) / bs;
+    return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the batch object
+// ---------------------------------------------------------------------------
+struct sph_batch {
+    int K = 0, n = 0, NC = 0;
+    int kernel = 0, precision = 0, integrator = 0;
+    std::vector<sph_params_t> P;
+    State buf[2];
+    void *arena[2] = {nullptr, nullptr};
+    int cur = 0;
+    BInst *inst = nullptr;
+    Ctrl *view = nullptr;
+    int *instOf = nullptr, *cs = nullptr, *chunkInst = nullptr, *chunk0 = nullptr;
+    int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
+    int2 *cxy_old = nullptr, *cxy_s = nullptr;
+    int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
+    double *part_b = nullptr, *part_d = nullptr;
+    void *recA = nullptr, *recB = nullptr;
+    long long capBins = 0;
+    bool dt_fresh = false;
+    hipStream_t own = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int pair_threads = 256;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    size_t n_ev = 0;
+    std::vector<int> offs;
+};
+
+namespace {
+
+void free_batch(sph_batch *b) {
+    if (!b) return;
+    for (int k = 0; k < 2; ++k) {
+        if (b->arena[k]) hipFree(b->arena[k]);
+        if (b->gexec[k]) hipGraphExecDestroy(b->gexec[k]);
+    }
+    void *ptrs[] = {b->inst, b->view, b->instOf, b->cs, b->chunkInst, b->chunk0, b->bin, b->rank, b->perm,
+                    b->permKey, b->cxy_old, b->cxy_s, b->counts, b->cellStart, b->tileSums, b->binflag,
+                    b->part_b, b->part_d, b->recA, b->recB};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto e : b->ev) hipEventDestroy(e);
+    if (b->own) hipStreamDestroy(b->own);
+    delete b;
+}
+
+int b_dt(sph_batch *b, hipStream_t s) {
+    if (!b->dt_fresh)
+        kb_dt_partial<<<b->NC, kBlock, 0, s>>>(b->cs, b->buf[b->cur], b->part_d);
+    kb_dt_final<<<b->K, kBlock, 0, s>>>(b->inst, b->chunk0, b->part_d, b->NC);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int b_nn(sph_batch *b, hipStream_t s) {
+    const int n = b->n, pec = b->integrator == SPH_INTEGRATOR_PEC;
+    State a = b->buf[b->cur], c = b->buf[1 - b->cur];
+    kb_bounds<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->inst, a, pec, b->part_b);
+    kb_grid<<<b->K, kBlock, 0, s>>>(b->inst, b->chunk0, b->part_b, b->NC);
+    kb_bases<<<1, 1024, 0, s>>>(b->inst, b->K, b->view, b->capBins);
+    kb_keys<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, n, pec, b->counts, b->bin, b->rank, b->cxy_old,
+                                         b->binflag, b->view);
+    long long tiles_cap = (b->capBins + kScanTile - 1) / kScanTile;
+    int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
+    k_scan_reduce<<<tiles, kBlock, 0, s>>>(b->view, b->counts, b->tileSums);
+    k_scan_tiles<<<1, 1024, 0, s>>>(b->view, b->tileSums);
+    k_scan_apply<<<tiles, kBlock, 0, s>>>(b->view, b->counts, b->tileSums, b->cellStart);
+    k_scatter<<<nblk_b(n), kBlock, 0, s>>>(b->view, b->cellStart, b->bin, b->rank, a.id, b->perm, b->permKey, n);
+    long long cb = (b->capBins + kBlock - 1) / kBlock;
+    k_cellsort<<<(int)(cb < kRedBlocks ? (cb < 1 ? 1 : cb) : kRedBlocks), kBlock, 0, s>>>(
+        b->view, b->cellStart, b->perm, b->permKey, b->binflag, b->bin, b->rank);
+    kb_reorder<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, c, b->perm, b->cxy_old, b->cxy_s, n,
+                                            b->precision == SPH_PREC_FP32C, b->recA, b->recB, pec);
+    HIPCHK(hipGetLastError());
+    b->cur = 1 - b->cur;
+    return 0;
+}
+
+int b_pair(sph_batch *b, hipStream_t s) {
+    State st = b->buf[b->cur];
+    PairIn in{b->recA, b->recB, st.h, b->cxy_s, b->cellStart};
+    PairOut out{st.ax, st.ay, st.drho, nullptr, 0};
+    if (b->timing) {
+        if (b->n_ev >= b->ev.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            b->ev.push_back(e);
+        }
+        HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
+    }
+    hipError_t e = b->precision == SPH_PREC_FP32C
+                       ? launch_pair_batch_fp32c(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads, s)
+                       : launch_pair_batch_fp64(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads, s);
+    if (e != hipSuccess) return fail(SPH_EHIP, "batch pair launch: %s", hipGetErrorString(e));
+    if (b->timing) {
+        if (b->n_ev >= b->ev.size()) {
+            hipEvent_t e2;
+            HIPCHK(hipEventCreate(&e2));
+            b->ev.push_back(e2);
+        }
+        HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
+    }
+    return 0;
+}
+
+int b_step(sph_batch *b, hipStream_t s) {
+    int rc;
+    if ((rc = b_dt(b, s))) return rc;
+    if ((rc = b_nn(b, s))) return rc;
+    if ((rc = b_pair(b, s))) return rc;
+    kb_correct<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->inst, b->buf[b->cur],
+                                        b->integrator == SPH_INTEGRATOR_EULER, b->part_d);
+    HIPCHK(hipGetLastError());
+    b->dt_fresh = true;
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **out) {
+    if (!params || !out) return fail(SPH_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (ninst < 1 || ninst > (1 << 20)) return fail(SPH_EINVAL, "ninst=%d out of range", ninst);
+    long long ntot = 0, capTot = 0;
+    for (int k = 0; k < ninst; ++k) {
+        const sph_params_t &P = params[k];
+        if (P.n <= 0) return fail(SPH_EINVAL, "instance %d: n=%lld", k, (long long)P.n);
+        if (P.kernel != params[0].kernel || P.precision != params[0].precision ||
+            P.integrator != params[0].integrator)
+            return fail(SPH_EINVAL, "instance %d: kernel, precision and integrator must agree across the batch", k);
+        if (P.kernel < 0 || P.kernel > 2) return fail(SPH_EINVAL, "unknown kernel id %d", P.kernel);
+        if (P.precision != SPH_PREC_FP64 && P.precision != SPH_PREC_FP32C)
+            return fail(SPH_EINVAL, "unknown precision %d", P.precision);
+        if (P.integrator != SPH_INTEGRATOR_PEC && P.integrator != SPH_INTEGRATOR_EULER)
+            return fail(SPH_EINVAL, "unknown integrator %d", P.integrator);
+        if (P.use_xsph || P.use_summation_density)
+            return fail(SPH_EUNSUPPORTED, "XSPH and summation density are not on the device path");
+        if (!(P.scale > 0) || !(P.qmax > 0) || !(P.rho0 > 0))
+            return fail(SPH_EINVAL, "instance %d: scale, qmax and rho0 must be positive", k);
+        ntot += P.n;
+        const long long cap = P.cell_capacity > 0 ? P.cell_capacity : P.n * 8 + 4096;
+        capTot += cap + 1;
+    }
+    if (ntot > (1LL << 30)) return fail(SPH_EINVAL, "%lld particles in the batch (max 2^30)", ntot);
+    if (capTot > (1LL << 31) - 16) return fail(SPH_EINVAL, "too many cells in the batch");
+    sph_batch *b = new (std::nothrow) sph_batch;
+    if (!b) return fail(SPH_ENOMEM, "host allocation failed");
+    b->K = ninst;
+    b->n = (int)ntot;
+    b->kernel = params[0].kernel;
+    b->precision = params[0].precision;
+    b->integrator = params[0].integrator;
+    b->P.assign(params, params + ninst);
+    b->capBins = capTot;
+    const size_t n = (size_t)ntot;
+    // chunk table
+    std::vector<int> cs, chunkInst, chunk0;
+    std::vector<BInst> hI(ninst);
+    long long off = 0;
+    for (int k = 0; k < ninst; ++k) {
+        const sph_params_t &P = params[k];
+        chunk0.push_back((int)cs.size());
+        for (long long c = 0; c < P.n; c += kChunk) {
+            cs.push_back((int)(off + c));
+            chunkInst.push_back(k);
+        }
+        BInst &I = hI[k];
+        std::memset(&I, 0, sizeof I);
+        I.off = (int)off;
+        I.n = (int)P.n;
+        I.active = 1;
+        I.advance_t = 1;
+        I.cap = P.cell_capacity > 0 ? P.cell_capacity : P.n * 8 + 4096;
+        I.duration = P.duration;
+        I.fixed_dt = P.fixed_dt;
+        I.damping = P.damping;
+        I.rho0 = P.rho0;
+        I.gamma = P.gamma;
+        I.co = P.co;
+        I.B = P.B;
+        I.Pb = P.Pb;
+        I.alpha = P.alpha;
+        I.beta = P.beta;
+        I.r0 = P.r0;
+        I.D = P.D;
+        I.p1 = P.p1;
+        I.p2 = P.p2;
+        I.g = P.gravity;
+        I.qmax = P.qmax;
+        I.scale = P.scale;
+        I.gamma_c = P.gamma_c;
+        I.gamma_f = P.gamma_f;
+        I.strict = P.strict;
+        I.kernel = P.kernel;
+        b->offs.push_back((int)off);
+        off += P.n;
+    }
+    b->offs.push_back((int)off);
+    cs.push_back((int)off);
+    chunk0.push_back((int)chunkInst.size());
+    b->NC = (int)chunkInst.size();
+    std::vector<int> instOf(n);
+    for (int k = 0; k < ninst; ++k)
+        for (int i = b->offs[k]; i < b->offs[k + 1]; ++i) instOf[i] = k;
+    const size_t nd = 15;
+    const size_t bytes = n * (nd * sizeof(double) + sizeof(int) + 8) + 1024;
+    bool ok = true;
+    for (int k = 0; k < 2 && ok; ++k) {
+        ok = hipMalloc(&b->arena[k], bytes) == hipSuccess;
+        if (!ok) break;
+        char *p = (char *)b->arena[k];
+        double **fields[] = {&b->buf[k].x,  &b->buf[k].y,   &b->buf[k].vx,   &b->buf[k].vy, &b->buf[k].rho,
+                             &b->buf[k].ax, &b->buf[k].ay,  &b->buf[k].drho, &b->buf[k].x0, &b->buf[k].y0,
+                             &b->buf[k].vx0, &b->buf[k].vy0, &b->buf[k].rho0, &b->buf[k].m, &b->buf[k].h};
+        for (double **f : fields) {
+            *f = (double *)p;
+            p += n * sizeof(double);
+        }
+        b->buf[k].id = (int *)p;
+        p += n * sizeof(int);
+        b->buf[k].label = (int8_t *)p;
+    }
+    const long long tiles = (capTot + kScanTile - 1) / kScanTile;
+    const size_t ra = b->precision == SPH_PREC_FP32C ? sizeof(Rec<float>::A) : sizeof(Rec<double>::A);
+    const size_t rb = b->precision == SPH_PREC_FP32C ? sizeof(Rec<float>::B) : sizeof(Rec<double>::B);
+    ok = ok && hipMalloc(&b->inst, sizeof(BInst) * ninst) == hipSuccess &&
+         hipMalloc(&b->view, sizeof(Ctrl)) == hipSuccess && hipMalloc(&b->instOf, sizeof(int) * n) == hipSuccess &&
+         hipMalloc(&b->cs, sizeof(int) * cs.size()) == hipSuccess &&
+         hipMalloc(&b->chunkInst, sizeof(int) * chunkInst.size()) == hipSuccess &&
+         hipMalloc(&b->chunk0, sizeof(int) * chunk0.size()) == hipSuccess &&
+         hipMalloc(&b->bin, sizeof(int) * n) == hipSuccess && hipMalloc(&b->rank, sizeof(int) * n) == hipSuccess &&
+         hipMalloc(&b->perm, sizeof(int) * n) == hipSuccess && hipMalloc(&b->permKey, sizeof(int) * n) == hipSuccess &&
+         hipMalloc(&b->cxy_old, sizeof(int2) * n) == hipSuccess && hipMalloc(&b->cxy_s, sizeof(int2) * n) == hipSuccess &&
+         hipMalloc(&b->counts, sizeof(int) * (capTot + 1)) == hipSuccess &&
+         hipMalloc(&b->cellStart, sizeof(int) * (capTot + 2)) == hipSuccess &&
+         hipMalloc(&b->tileSums, sizeof(int) * (tiles + 1)) == hipSuccess &&
+         hipMalloc(&b->binflag, sizeof(int) * (capTot + 1)) == hipSuccess &&
+         hipMalloc(&b->part_b, sizeof(double) * kBoundsQ * b->NC) == hipSuccess &&
+         hipMalloc(&b->part_d, sizeof(double) * kDtQ * b->NC) == hipSuccess &&
+         hipMalloc(&b->recA, ra * n) == hipSuccess && hipMalloc(&b->recB, rb * n) == hipSuccess;
+    if (!ok) {
+        free_batch(b);
+        return fail(SPH_ENOMEM, "device allocation for a batch of %lld particles failed", ntot);
+    }
+    Ctrl v;
+    std::memset(&v, 0, sizeof v);
+    v.active = 1;
+    ok = hipMemcpy(b->inst, hI.data(), sizeof(BInst) * ninst, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->view, &v, sizeof v, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->instOf, instOf.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->cs, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->chunkInst, chunkInst.data(), sizeof(int) * chunkInst.size(), hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(b->chunk0, chunk0.data(), sizeof(int) * chunk0.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(b->counts, 0, sizeof(int) * (capTot + 1)) == hipSuccess &&
+         hipMemset(b->cellStart, 0, sizeof(int) * (capTot + 2)) == hipSuccess &&
+         hipMemset(b->binflag, 0, sizeof(int) * (capTot + 1)) == hipSuccess &&
+         hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        free_batch(b);
+        return fail(SPH_EHIP, "batch initialisation failed");
+    }
+    *out = b;
+    return 0;
+}
+
+int sph_batch_destroy(sph_batch_t *b) {
+    if (!b) return 0;
+    hipDeviceSynchronize();
+    free_batch(b);
+    return 0;
+}
+
+int sph_batch_upload(sph_batch_t *b, const sph_state_t *st, void *stream) {
+    if (!b || !st || !st->x || !st->y || !st->vx || !st->vy || !st->rho || !st->m || !st->h || !st->label)
+        return fail(SPH_EINVAL, "upload needs x, y, vx, vy, rho, m, h and label");
+    hipStream_t s = (hipStream_t)stream;
+    b->cur = 0;
+    b->dt_fresh = false;
+    kb_upload<<<nblk_b(b->n), kBlock, 0, s>>>(b->buf[0], *st, b->n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<BInst> hI(b->K);
+    HIPCHK(hipMemcpy(hI.data(), b->inst, sizeof(BInst) * b->K, hipMemcpyDeviceToHost));
+    for (auto &I : hI) {
+        I.t = 0.0;
+        I.step = 0;
+        I.dt = I.dt_c = I.dt_f = 0.0;
+        I.flags = 0;
+        I.active = 1;
+    }
+    HIPCHK(hipMemcpy(b->inst, hI.data(), sizeof(BInst) * b->K, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int sph_batch_download(sph_batch_t *b, const sph_state_t *st, void *stream) {
+    if (!b || !st) return fail(SPH_EINVAL, "NULL argument");
+    kb_download<<<nblk_b(b->n), kBlock, 0, (hipStream_t)stream>>>(b->buf[b->cur], *st, b->inst, b->instOf, b->n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sph_batch_step(sph_batch_t *b, int64_t nsteps, int32_t use_graph, void *stream) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    if (nsteps < 0) return fail(SPH_EINVAL, "nsteps < 0");
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_graph || b->timing) {
+        for (int64_t k = 0; k < nsteps; ++k) {
+            int rc = b_step(b, s);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    hipStream_t g = b->own;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(hipStreamWaitEvent(g, e0, 0));
+    int rc = 0;
+    if (!b->dt_fresh && nsteps > 0) {
+        rc = b_step(b, g);
+        --nsteps;
+    }
+    for (int64_t k = 0; k < nsteps && !rc; ++k) {
+        const int start = b->cur;
+        if (!b->gexec[start]) {
+            hipGraph_t graph;
+            HIPCHK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+            rc = b_step(b, g);
+            hipError_t ce = hipStreamEndCapture(g, &graph);
+            if (rc) break;
+            if (ce != hipSuccess) {
+                rc = fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
+                break;
+            }
+            ce = hipGraphInstantiate(&b->gexec[start], graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ce != hipSuccess) {
+                rc = fail(SPH_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+                break;
+            }
+        }
+        HIPCHK(hipGraphLaunch(b->gexec[start], g));
+        b->cur = 1 - start;
+    }
+    hipEventRecord(e1, g);
+    hipStreamWaitEvent(s, e1, 0);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int sph_batch_status(sph_batch_t *b, double *out_d, int64_t *out_i, void *stream) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    std::vector<BInst> hI(b->K);
+    HIPCHK(hipMemcpy(hI.data(), b->inst, sizeof(BInst) * b->K, hipMemcpyDeviceToHost));
+    for (int k = 0; k < b->K; ++k) {
+        if (out_d) {
+            out_d[4 * k + 0] = hI[k].t;
+            out_d[4 * k + 1] = hI[k].dt;
+            out_d[4 * k + 2] = hI[k].dt_c;
+            out_d[4 * k + 3] = hI[k].dt_f;
+        }
+        if (out_i) {
+            out_i[2 * k + 0] = hI[k].step;
+            out_i[2 * k + 1] = hI[k].flags;
+        }
+    }
+    return 0;
+}
+
+int sph_batch_set_tuning(sph_batch_t *b, int32_t pair_threads) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    if (pair_threads != 64 && pair_threads != 128 && pair_threads != 256)
+        return fail(SPH_EINVAL, "pair_threads must be 64, 128 or 256");
+    if (pair_threads != b->pair_threads) {
+        b->pair_threads = pair_threads;
+        for (auto &g : b->gexec)
+            if (g) hipGraphExecDestroy(g), g = nullptr;
+    }
+    return 0;
+}
+
+int sph_batch_timing_enable(sph_batch_t *b, int32_t on) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    b->timing = on != 0;
+    b->n_ev = 0;
+    return 0;
+}
+
+int sph_batch_timing_read(sph_batch_t *b, double *avg_ms_pair, int64_t *count, int32_t reset) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    HIPCHK(hipDeviceSynchronize());
+    double tot = 0.0;
+    int64_t k = 0;
+    for (size_t i = 0; i + 1 < b->n_ev; i += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
+        tot += ms;
+        ++k;
+    }
+    if (avg_ms_pair) *avg_ms_pair = k ? tot / k : 0.0;
+    if (count) *count = k;
+    if (reset) b->n_ev = 0;
+    return 0;
+}
+
+}  // extern "C"

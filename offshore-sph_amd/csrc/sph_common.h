@@ -1,0 +1,260 @@
+// sph_common.h — device code shared by the single-simulation engine and the
+// batched ensemble: workgroup reductions, the PEC half-step expressions, the
+// Tait pressure, and the cell-sort pipeline (prefix scan of bin counts,
+// scatter, per-bin rank sort). Included by more than one translation unit:
+// everything is in an anonymous namespace, so each gets its own copy.
+#pragma once
+#include "sph_internal.h"
+
+namespace hipsph {
+namespace {
+
+// ---------------------------------------------------------------------------
+// reductions (wavefront shuffles, then LDS across the 4 waves of a block)
+// ---------------------------------------------------------------------------
+template <typename T, typename Op>
+__device__ inline T wave_reduce(T v, Op op) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <typename T, typename Op>
+__device__ inline T block_reduce(T v, Op op, T *lds) {
+    v = wave_reduce(v, op);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = (lane < (int)(blockDim.x >> 6)) ? lds[lane] : Op::identity();
+        v = wave_reduce(v, op);
+    }
+    __syncthreads();
+    return v;  // valid in wave 0
+}
+
+struct MinOp {
+    __device__ static double identity() { return INFINITY; }
+    __device__ double operator()(double a, double b) const { return b < a ? b : a; }
+};
+struct MaxOp {
+    __device__ static double identity() { return -INFINITY; }
+    __device__ double operator()(double a, double b) const { return b > a ? b : a; }
+};
+struct AddOp {
+    __device__ static double identity() { return 0.0; }
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+
+// Fold NQ quantities over a whole workgroup with one LDS round (the
+// single-workgroup finalisers are latency-bound: one barrier instead of one
+// per quantity). kind: 0 min, 1 max, 2 sum. The result is valid in every
+// thread. lds holds NQ * 16 doubles.
+template <int NQ>
+__device__ inline void block_fold(double (&v)[NQ], const int (&kind)[NQ], double *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double u = __shfl_xor(v[q], o, 64);
+            v[q] = kind[q] == 0 ? (u < v[q] ? u : v[q]) : kind[q] == 1 ? (u > v[q] ? u : v[q]) : v[q] + u;
+        }
+        if (lane == 0) lds[q * 16 + w] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double acc = lds[q * 16];
+        for (int k = 1; k < nw; ++k) {
+            const double u = lds[q * 16 + k];
+            acc = kind[q] == 0 ? (u < acc ? u : acc) : kind[q] == 1 ? (u > acc ? u : acc) : acc + u;
+        }
+        v[q] = acc;
+    }
+}
+
+// src/Integrators/PEC.py:32-59 (useXSPH = False branch), one particle. The
+// fused step evaluates the same expressions inside the neighbour update.
+__device__ inline double pec_half_pos(double x, double v, double dt) { return x + 0.5 * dt * v; }
+__device__ inline double pec_half_vel(double v, double a, double dt, double damping) {
+    return (v + 0.5 * dt * a) / (1 + 0.5 * damping);
+}
+__device__ inline double pec_half_rho(double rho, double drho, double dt, int strict) {
+    double r = rho + 0.5 * dt * drho;
+    if (strict && r < 0.0) r = 0.0;
+    return r;
+}
+
+// Exclusive prefix scan of counts[0 .. ncells] into cellStart, in tiles of
+// kScanTile bins: tile sums, a scan of the tile sums, then the tiles.
+__device__ inline int block_scan_excl(int v, int *lds, int *total) {
+    // inclusive scan inside each wave
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int s = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(s, o, 64);
+        if (lane >= o) s += u;
+    }
+    if (lane == 63) lds[w] = s;
+    __syncthreads();
+    int nw = blockDim.x >> 6;
+    int off = 0, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+        int ws = lds[k];
+        if (k < w) off += ws;
+        tot += ws;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + s - v;
+}
+
+__global__ void k_scan_reduce(const Ctrl *ctrl, const int *counts, int *tileSums) {
+    if (!ctrl->active) return;
+    long long nbins = ctrl->grid.ncells + 1;
+    long long ntiles = (nbins + kScanTile - 1) / kScanTile;
+    __shared__ double lds[8];
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        long long base = tile * kScanTile;
+        double s = 0.0;
+        for (int k = 0; k < kScanItems; ++k) {
+            long long b = base + (long long)threadIdx.x * kScanItems + k;
+            if (b < nbins) s += (double)counts[b];
+        }
+        s = block_reduce(s, AddOp(), lds);
+        if (threadIdx.x == 0) tileSums[tile] = (int)s;
+    }
+}
+
+__global__ void k_scan_tiles(const Ctrl *ctrl, int *tileSums) {
+    if (!ctrl->active) return;
+    __shared__ int lds[16];
+    long long nbins = ctrl->grid.ncells + 1;
+    int ntiles = (int)((nbins + kScanTile - 1) / kScanTile);
+    int carry = 0;
+    for (int base = 0; base < ntiles; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int v = i < ntiles ? tileSums[i] : 0;
+        int tot;
+        int ex = block_scan_excl(v, lds, &tot);
+        if (i < ntiles) tileSums[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_apply(const Ctrl *ctrl, int *counts, const int *tileSums, int *cellStart) {
+    if (!ctrl->active) return;
+    long long nbins = ctrl->grid.ncells + 1;
+    long long ntiles = (nbins + kScanTile - 1) / kScanTile;
+    __shared__ int lds[16];
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        long long base = tile * kScanTile;
+        int v[kScanItems];
+        int sum = 0;
+        long long b0 = base + (long long)threadIdx.x * kScanItems;
+        for (int k = 0; k < kScanItems; ++k) {
+            long long b = b0 + k;
+            v[k] = b < nbins ? counts[b] : 0;
+            sum += v[k];
+        }
+        int tot;
+        int ex = block_scan_excl(sum, lds, &tot) + tileSums[tile];
+        for (int k = 0; k < kScanItems; ++k) {
+            long long b = b0 + k;
+            if (b < nbins) {
+                cellStart[b] = ex;
+                counts[b] = 0;  // ready for the next update (no memset node in a graph)
+            }
+            ex += v[k];
+        }
+        if (tile == ntiles - 1 && threadIdx.x == blockDim.x - 1) cellStart[nbins] = ex;
+    }
+}
+
+__global__ void k_scatter(const Ctrl *ctrl, const int *cellStart, const int *bin, const int *rank, const int *id,
+                          int *perm, int *permKey, int n) {
+    if (!ctrl->active) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int pos = cellStart[bin[t]] + rank[t];
+    perm[pos] = t;
+    permKey[pos] = id[t];
+}
+
+// Sort the slots [s, e) of one bin by key (the reference index; keys are
+// distinct) with one wavefront: each element's destination is the number of
+// smaller keys in the bin. Bins of up to 64 sort in registers (one load and
+// one store round); larger ones rank through global memory and stage the
+// result in tmpKey/tmpVal before copying it back.
+__device__ void wave_sort_bin(int s, int e, int *perm, int *permKey, int *tmpKey, int *tmpVal) {
+    const int lane = threadIdx.x & 63;
+    const int len = e - s;
+    if (len <= 64) {
+        const int key = lane < len ? permKey[s + lane] : 0x7fffffff;
+        const int val = lane < len ? perm[s + lane] : 0;
+        int r = 0;
+        for (int j = 0; j < len; ++j) r += __shfl(key, j, 64) < key;
+        if (lane < len) {
+            permKey[s + r] = key;
+            perm[s + r] = val;
+        }
+        return;
+    }
+    for (int i = s + lane; i < e; i += 64) {
+        const int key = permKey[i];
+        int r = 0;
+        for (int j = s; j < e; ++j) r += permKey[j] < key;
+        tmpKey[s + r] = key;
+        tmpVal[s + r] = perm[i];
+    }
+    __threadfence_block();
+    for (int i = s + lane; i < e; i += 64) {
+        permKey[i] = tmpKey[i];
+        perm[i] = tmpVal[i];
+    }
+}
+
+// Order each bin by the reference index, which is what the reference's
+// head-inserted lists are ordered by (descending there; the pair pass walks
+// the bin backwards). Only bins flagged by k_keys (filled by more than one
+// run) can be out of order. Each wavefront reads the flags of 64 consecutive
+// bins, clears them and sorts the flagged ones one after another.
+__global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, int *permKey, int *binflag,
+                           int *tmpKey, int *tmpVal) {
+    if (!ctrl->active) return;
+    const long long nbins = ctrl->grid.ncells + 1;
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long b0 = wave * 64; b0 < nbins; b0 += nwaves * 64) {
+        const long long bl = b0 + lane;
+        int f = 0;
+        if (bl < nbins) {
+            f = binflag[bl];
+            if (f) binflag[bl] = 0;
+        }
+        unsigned long long m = __ballot(f != 0);
+        while (m) {
+            const int k = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const long long b = b0 + k;
+            wave_sort_bin(cellStart[b], cellStart[b + 1], perm, permKey, tmpKey, tmpVal);
+        }
+    }
+}
+
+// Tait pressure + Pb (src/Equations/TaitEOS.py:6-31; WCSPH.py:131-149).
+__device__ inline double tait_p(double rho, int8_t label, double gamma, double B, double rho0, double Pb) {
+    double p = 0.0;
+    if (label == kFluid) {
+        double ratio = ipow(rho / rho0, gamma);
+        p = (ratio - 1.0) * B;
+    }
+    return p + Pb;
+}
+
+
+}  // namespace
+}  // namespace hipsph

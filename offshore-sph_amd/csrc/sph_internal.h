@@ -105,6 +105,52 @@ struct Phys {
 hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
 hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
 
+// One simulation of a batched ensemble (device array, one per instance).
+// Instance k owns the particle slots [off, off + n) of the concatenated
+// cell-ordered state and the bins [cellBase, cellBase + grid.ncells] of the
+// segmented cell grid (the last one collects keys outside its grid; an
+// inactive instance has ncells = 0 and parks every particle there).
+struct BInst {
+    double t, dt, dt_c, dt_f;
+    long long step;
+    int active;
+    unsigned flags;
+    Grid grid;
+    long long cellBase;
+    int off, n;
+    int advance_t;
+    int nfluid;
+    long long cap;  // most cells this instance's grid may have
+    double duration, fixed_dt, damping;
+    double rho0, gamma, co, B, Pb, alpha, beta, r0, D, p1, p2, g, qmax, scale, gamma_c, gamma_f;
+    int strict, kernel;
+};
+
+__host__ __device__ inline Phys inst_phys(const BInst &I) {
+    Phys ph;
+    ph.rho0 = I.rho0;
+    ph.gamma = I.gamma;
+    ph.B = I.B;
+    ph.Pb = I.Pb;
+    ph.co = I.co;
+    ph.alpha = I.alpha;
+    ph.beta = I.beta;
+    ph.r0 = I.r0;
+    ph.D = I.D;
+    ph.p1 = I.p1;
+    ph.p2 = I.p2;
+    ph.g = I.g;
+    ph.qmax = I.qmax;
+    ph.kernel = I.kernel;
+    return ph;
+}
+
+// The pair pass over a batch: instOf[t] is the instance of slot t.
+hipError_t launch_pair_batch_fp64(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
+                                  int threads, hipStream_t s);
+hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
+                                   int threads, hipStream_t s);
+
 // Number of blocks for a grid over n items, padded to a multiple of the XCD
 // count so that the block remap below is a bijection.
 constexpr int kXcds = 8;

@@ -306,20 +306,17 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
     return cnt;
 }
 
+// The pass for target slot t: the 3x3 cells of its grid (ncx x ncy cells
+// starting at bin `base`), sums, outputs.
 template <typename R, int KERNEL, bool EXACT, int MODE>
-__global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArgs<R> P, Phys ph) {
-    __shared__ unsigned s_list[kListLen * kPairThreads];
-    if (!ctrl->active) return;
-    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= P.n) return;
+__device__ inline void pair_target(const PairArgs<R> &P, const Phys &ph, int ncx, int ncy, long long base, int t,
+                                   unsigned *s_list) {
     const typename Rec<R>::A at = P.A[t];
     if (!(at.w >= R(0))) {  // walls are not targets
         if (P.count) P.count[t] = -1;
         return;
     }
     const typename Rec<R>::B bt = P.B[t];
-    const Grid g = ctrl->grid;
-    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
     Target<R> T;
     T.X = at.x;
     T.Y = at.y;
@@ -343,7 +340,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArg
         for (int db = -1; db <= 1; ++db) {
             const int cy = c.y + db;
             if (!(ncy > cy && cy > -1)) continue;
-            const int k = cx + ncx * cy;
+            const long long k = base + cx + (long long)ncx * cy;
             cnt += scan_cell<R, KERNEL, EXACT, MODE>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list, nlist,
                                                       S);
         }
@@ -359,6 +356,28 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArg
     }
 }
 
+template <typename R, int KERNEL, bool EXACT, int MODE>
+__global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArgs<R> P, Phys ph) {
+    __shared__ unsigned s_list[kListLen * kPairThreads];
+    if (!ctrl->active) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const Grid g = ctrl->grid;
+    pair_target<R, KERNEL, EXACT, MODE>(P, ph, (int)g.ncx, (int)g.ncy, 0, t, s_list);
+}
+
+// Batched ensemble: each target walks its own instance's cells with its own
+// constants; targets of finished or halted instances do nothing.
+template <typename R, int KERNEL, bool EXACT>
+__global__ __launch_bounds__(kPairThreads) void k_pair_batch(const BInst *inst, const int *instOf, PairArgs<R> P) {
+    __shared__ unsigned s_list[kListLen * kPairThreads];
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const BInst &I = inst[instOf[t]];
+    if (!I.active) return;
+    pair_target<R, KERNEL, EXACT, kFused>(P, inst_phys(I), (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, s_list);
+}
+
 template <typename R, bool EXACT, int MODE>
 void launch_pair_mode(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, int nb, int threads, hipStream_t s) {
     switch (ph.kernel) {
@@ -371,6 +390,24 @@ void launch_pair_mode(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, in
         default:
             k_pair<R, SPH_KERNEL_CUBIC, EXACT, MODE><<<nb, threads, 0, s>>>(ctrl, P, ph);
     }
+}
+
+template <typename R, bool EXACT>
+hipError_t launch_pair_batch(const BInst *inst, const int *instOf, const PairArgs<R> &P, int kernel, int threads,
+                             hipStream_t s) {
+    if (threads != 64 && threads != 128 && threads != 256) threads = kPairThreads;
+    const int nb = xcd_blocks(P.n, threads);
+    switch (kernel) {
+        case SPH_KERNEL_GAUSSIAN:
+            k_pair_batch<R, SPH_KERNEL_GAUSSIAN, EXACT><<<nb, threads, 0, s>>>(inst, instOf, P);
+            break;
+        case SPH_KERNEL_WENDLAND:
+            k_pair_batch<R, SPH_KERNEL_WENDLAND, EXACT><<<nb, threads, 0, s>>>(inst, instOf, P);
+            break;
+        default:
+            k_pair_batch<R, SPH_KERNEL_CUBIC, EXACT><<<nb, threads, 0, s>>>(inst, instOf, P);
+    }
+    return hipGetLastError();
 }
 
 template <typename R, bool EXACT>

@@ -89,6 +89,15 @@ _PROTOS = {
     'sph_adjoint_step': (C.c_int, [_P, _P, _I32, C.POINTER(sph_adjoint_t), _P]),
     'sph_adjoint_segment': (C.c_int, [_P, _P, _I32, C.POINTER(sph_adjoint_t), _P]),
     'sph_state_restore': (C.c_int, [_P, C.POINTER(sph_state_t), _D, _P]),
+    'sph_batch_create': (C.c_int, [C.POINTER(sph_params_t), _I32, C.POINTER(C.c_void_p)]),
+    'sph_batch_destroy': (C.c_int, [_P]),
+    'sph_batch_upload': (C.c_int, [_P, C.POINTER(sph_state_t), _P]),
+    'sph_batch_download': (C.c_int, [_P, C.POINTER(sph_state_t), _P]),
+    'sph_batch_step': (C.c_int, [_P, _I64, _I32, _P]),
+    'sph_batch_status': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), _P]),
+    'sph_batch_set_tuning': (C.c_int, [_P, _I32]),
+    'sph_batch_timing_enable': (C.c_int, [_P, _I32]),
+    'sph_batch_timing_read': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), _I32]),
     'sph_momentum': (C.c_int, [_P, _P]),
     'sph_kinetic_energy': (C.c_int, [_P, C.POINTER(_D), _P]),
     'sph_kernel_evaluate': (C.c_int, [_I32, _I64, _P, _P, _P, _P]),

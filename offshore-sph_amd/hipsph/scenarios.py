@@ -108,6 +108,22 @@ def dam_break_c5(small=False):
     return dam_break(0.5, 1.0, 0.005, tank_w=2.0, tank_h=1.5, name='dam_break_20k_dx0.005')
 
 
+def ensemble_c4(count=256, seed=SEED, dxs=(0.005, 0.01, 0.02)):
+    """Batched ensemble: dam breaks with column width U(0.5, 1.5) m, height
+    U(0.5, 2.5) m and dx drawn from dxs in a 4.0 x 3.0 tank; instance k uses
+    PCG64(seed + k), which also draws its alpha ~ U(0.005, 0.05). Returns
+    (scenarios, alphas)."""
+    out, alphas = [], []
+    for k in range(count):
+        rng = np.random.Generator(np.random.PCG64(seed + k))
+        w = float(rng.uniform(0.5, 1.5))
+        hgt = float(rng.uniform(0.5, 2.5))
+        dx = float(dxs[int(rng.integers(len(dxs)))])
+        alphas.append(float(rng.uniform(0.005, 0.05)))
+        out.append(dam_break(w, hgt, dx, name=f'ensemble_{k}'))
+    return out, alphas
+
+
 def stress_lattice(nside=2000, dx=1e-3, H=2.0, layers=3, rho0=1000.0, hfac=1.3, seed=SEED):
     """Pair-kernel stress: nside^2 jittered fluid particles in a closed box.
 

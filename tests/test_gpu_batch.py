@@ -1,0 +1,105 @@
+"""Batched ensemble (sph_batch_*): every instance evolves exactly as alone.
+
+Each check runs a batch of small dam breaks of different geometry,
+resolution, height (hence c0, B, D) and alpha, and compares every instance
+with a single-simulation engine (which is itself bit-exact against the CPU
+oracle) on the same initial state:
+  * FP64 pair pass: bit-identical state and clock after 40 steps, eager and
+    hipGraph replay;
+  * FP32C pair pass: the same pair code per particle, equal to 1e-12;
+  * an instance whose duration ends early stops while the others go on,
+    exactly as a single engine with that duration.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import FIELDS, make_engine, say
+from hipsph import scenarios
+from hipsph.batch import BatchEngine, autotune, params_for
+from hipsph.device import soa_from_particles
+
+pytestmark = pytest.mark.gpu
+
+
+def _members():
+    geo = [(0.4, 0.5, 0.02, 0.01), (0.3, 0.6, 0.02, 0.03), (0.2, 0.2, 0.01, 0.02), (0.5, 0.3, 0.025, 0.005),
+           (0.25, 0.4, 0.0125, 0.045)]
+    out = []
+    for k, (w, hgt, dx, alpha) in enumerate(geo):
+        sc = scenarios.dam_break(w, hgt, dx, tank_w=1.0, tank_h=0.8, name=f'member{k}')
+        out.append((sc, alpha))
+    return out
+
+
+def _single(sc, alpha, steps, precision, duration=np.inf):
+    m = sc.method()
+    m.alpha = alpha
+    e = make_engine(sc.pA, m, precision=precision, duration=duration)
+    e.step(steps)
+    d = e.download()
+    st = e.status()
+    e.close()
+    return d, st
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
+@pytest.mark.parametrize('graph', [False, True])
+def test_batch_instances_match_single_engines(precision, graph):
+    mem = _members()
+    K = 40
+    b = BatchEngine([params_for(sc, alpha, precision=precision) for sc, alpha in mem])
+    b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+    b.step(K, graph=graph)
+    got = b.download()
+    st = b.status()
+    b.close()
+    say('batch', precision, 'graph', graph, 'sizes', [sc.n for sc, _ in mem])
+    for k, (sc, alpha) in enumerate(mem):
+        want, wst = _single(sc, alpha, K, precision)
+        assert st[k]['steps'] == wst['steps'] == K
+        f = sc.pA['label'] == 0
+        if precision == 'fp64':
+            assert st[k]['t'] == wst['t'] and st[k]['dt'] == wst['dt']
+            for fld in FIELDS:
+                np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
+        else:
+            assert st[k]['t'] == pytest.approx(wst['t'], rel=1e-12)
+            for fld in FIELDS:
+                scale = max(1.0, float(np.max(np.abs(want[fld][f]))))
+                assert np.max(np.abs(got[k][fld] - want[fld])) <= 1e-12 * scale, (k, fld)
+
+
+def test_batch_duration_per_instance():
+    mem = _members()[:3]
+    # instance 1 stops after about 10 steps' time
+    sc1, a1 = mem[1]
+    _, probe = _single(sc1, a1, 10, 'fp64')
+    durations = [np.inf, probe['t'] * 0.999, np.inf]
+    ps = []
+    for (sc, alpha), dur in zip(mem, durations):
+        p = params_for(sc, alpha, precision='fp64')
+        p.duration = dur
+        ps.append(p)
+    b = BatchEngine(ps)
+    b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+    b.step(25, graph=True)
+    got = b.download()
+    st = b.status()
+    b.close()
+    assert st[0]['steps'] == 25 and st[2]['steps'] == 25
+    assert st[1]['steps'] == 10
+    for k, ((sc, alpha), dur) in enumerate(zip(mem, durations)):
+        want, wst = _single(sc, alpha, 25, 'fp64', duration=dur)
+        assert st[k]['steps'] == wst['steps'] and st[k]['t'] == wst['t']
+        for fld in FIELDS:
+            np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
+
+
+def test_batch_autotune_keeps_results():
+    mem = _members()
+    b = BatchEngine([params_for(sc, alpha, precision='fp32c') for sc, alpha in mem])
+    b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+    best, table = autotune(b, steps=2)
+    say('batch autotune', best, table)
+    assert best in (64, 128, 256) and len(table) == 3
+    b.close()
