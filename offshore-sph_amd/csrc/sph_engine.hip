@@ -194,7 +194,11 @@ __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
 // src/Integrators/PEC.py:61-89 and src/Integrators/Euler.py:17-26. Also
 // leaves the next step's dt partials (h and a of the fluid do not change
 // until the next pair pass) in `part`. Grid-stride over R workgroups.
-__global__ void k_correct(Ctrl *ctrl, State s, int n, int integrator, int strict, double *part) {
+// PEC's start-of-step values come from x0 ... rho0 of the same buffer, or,
+// with `perm` (a step whose reorder did not write them), from the buffer
+// before the reorder at the particle's previous slot perm[j]: the same values.
+__global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, int integrator, int strict,
+                          double *part) {
     if (!ctrl->active) return;
     __shared__ double lds[8];
     const double dt = ctrl->dt, damping = ctrl->damping;
@@ -204,7 +208,21 @@ __global__ void k_correct(Ctrl *ctrl, State s, int n, int integrator, int strict
         const double ax = s.ax[j], ay = s.ay[j];
         v.add(s.h[j], ax, ay);
         if (integrator == SPH_INTEGRATOR_PEC) {
-            double x0 = s.x0[j], y0 = s.y0[j], vx0 = s.vx0[j], vy0 = s.vy0[j], rho0 = s.rho0[j];
+            double x0, y0, vx0, vy0, rho0;
+            if (perm) {
+                const int q = perm[j];
+                x0 = o.x[q];
+                y0 = o.y[q];
+                vx0 = o.vx[q];
+                vy0 = o.vy[q];
+                rho0 = o.rho[q];
+            } else {
+                x0 = s.x0[j];
+                y0 = s.y0[j];
+                vx0 = s.vx0[j];
+                vy0 = s.vy0[j];
+                rho0 = s.rho0[j];
+            }
             double mid_x = x0 + 0.5 * dt * s.vx[j];
             double mid_y = y0 + 0.5 * dt * s.vy[j];
             double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
@@ -403,8 +421,15 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // pair-pass records: p / rho^2 per particle (the momentum equation's slf and
 // othr terms, src/Equations/Momentum.py:28,36) and w = m for fluid, -label
 // otherwise.
+// lean (a PEC step whose fused pair pass and corrector follow at once): for
+// the fluid, x0 ... rho0 are not written (the corrector reads them from this
+// kernel's source buffer), nor the half-step x, y, rho (the records carry
+// them to the pair pass; the corrector overwrites them), nor the rates of a
+// particle inside the grid (its own cell is walked, so it has at least one
+// neighbour, q = 0, and the pair pass writes all three). A particle outside
+// the grid may find none and keep its previous rates (SolverTools.py:166-169).
 __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const int2 *cxy_old, int2 *cxy_s, Phys ph,
-                          int n, int fp32, void *recA, void *recB, int pred, int strict) {
+                          int n, int fp32, void *recA, void *recB, int pred, int strict, int lean) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
     const int act = ctrl->active;
@@ -415,14 +440,18 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const i
     double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
     const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
     int8_t lab = a.label[s];
+    const bool skip = lean && act && pred && lab == kFluid;
+    const int2 cc = act ? cxy_old[s] : make_int2(0, 0);
     if (act && pred && lab == kFluid) {
         // PEC.predict (src/Integrators/PEC.py:32-59) fused into the permutation
         const double dt = ctrl->dt, damping = ctrl->damping;
-        b.x0[t] = x;
-        b.y0[t] = y;
-        b.vx0[t] = vx;
-        b.vy0[t] = vy;
-        b.rho0[t] = rho;
+        if (!skip) {
+            b.x0[t] = x;
+            b.y0[t] = y;
+            b.vx0[t] = vx;
+            b.vy0[t] = vy;
+            b.rho0[t] = rho;
+        }
         x = pec_half_pos(x, vx, dt);
         y = pec_half_pos(y, vy, dt);
         vx = pec_half_vel(vx, ax, dt, damping);
@@ -435,20 +464,25 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const i
         b.vy0[t] = a.vy0[s];
         b.rho0[t] = a.rho0[s];
     }
-    b.x[t] = x;
-    b.y[t] = y;
+    if (!skip) {
+        b.x[t] = x;
+        b.y[t] = y;
+        b.rho[t] = rho;
+    }
     b.vx[t] = vx;
     b.vy[t] = vy;
-    b.rho[t] = rho;
     b.m[t] = m;
     b.h[t] = h;
     b.label[t] = lab;
     b.id[t] = a.id[s];
-    b.ax[t] = ax;
-    b.ay[t] = ay;
-    b.drho[t] = drho;
+    const Grid &g = ctrl->grid;
+    if (!skip || !(cc.x >= 0 && cc.x < g.ncx && cc.y >= 0 && cc.y < g.ncy && h > 0.0 && ph.qmax >= 0.0)) {
+        b.ax[t] = ax;
+        b.ay[t] = ay;
+        b.drho[t] = drho;
+    }
     if (!act) return;
-    cxy_s[t] = cxy_old[s];
+    cxy_s[t] = cc;
     const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
     const double pr = p / (rho * rho);
     const double w = lab == kFluid ? m : -(double)lab;
@@ -900,7 +934,7 @@ static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipSt
 
 // pred != 0: the PEC predictor runs inside the update (bounds and keys see
 // the predicted positions, the reorder writes the predicted state).
-static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0) {
+static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
     const int n = c->n;
     State a = c->buf[c->cur];
     State b = c->buf[1 - c->cur];
@@ -922,7 +956,8 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0) {
     k_cellsort<<<redblk(c->cap_cells + 1), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->perm, c->permKey,
                                                            c->binflag, c->bin, c->rank);
     k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_old, c->cxy_s, make_phys(c->P), n,
-                                         c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict);
+                                         c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict,
+                                         lean);
     HIPCHK(hipGetLastError());
     if (c->timing) {
         int rc = ev_record(c, c->ev_nn, c->n_nn, s);
@@ -968,8 +1003,11 @@ static int predict(sph_ctx *c, hipStream_t s) {
     return 0;
 }
 
-static int correct(sph_ctx *c, hipStream_t s) {
-    k_correct<<<redblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, c->P.integrator, c->P.strict,
+// lean: the step's reorder ran lean (start-of-step values from the previous
+// buffer through perm).
+static int correct(sph_ctx *c, hipStream_t s, bool lean = false) {
+    k_correct<<<redblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->buf[1 - c->cur],
+                                              lean ? c->perm : nullptr, c->n, c->P.integrator, c->P.strict,
                                               c->part_d);
     HIPCHK(hipGetLastError());
     return 0;
@@ -979,10 +1017,11 @@ static int correct(sph_ctx *c, hipStream_t s) {
 // into the neighbour update and the next dt reduction into the corrector.
 static int one_step(sph_ctx *c, hipStream_t s) {
     int rc;
+    const bool pec = c->P.integrator == SPH_INTEGRATOR_PEC;
     if ((rc = dt_step(c, s))) return rc;
-    if ((rc = nn_update(c, s, c->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
+    if ((rc = nn_update(c, s, pec, pec))) return rc;
     if ((rc = compute(c, s))) return rc;
-    if ((rc = correct(c, s))) return rc;
+    if ((rc = correct(c, s, pec))) return rc;
     c->dt_fresh = true;
     return 0;
 }
