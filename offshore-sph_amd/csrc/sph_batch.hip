@@ -310,7 +310,11 @@ __global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, in
 }
 
 // Permute the state into the new order with the PEC predictor of active
-// instances fused in, and write the pair records (as k_reorder).
+// instances fused in, and write the pair records (as k_reorder). With PEC the
+// reorder is lean as k_reorder's for the fluid of active instances: no
+// x0 ... rho0 (kb_correct gathers them from the source buffer), no half-step
+// x, y, rho (overwritten by kb_correct), no rates of a particle inside its
+// instance's grid (written by the pair pass).
 __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State b, const int *perm,
                            const int2 *cxy_old, int2 *cxy_s, int n, int fp32, void *recA, void *recB, int pec) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,38 +324,35 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
     double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
     const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
     const int8_t lab = a.label[s];
-    if (I.active && pec && lab == kFluid) {
+    const int2 cc = cxy_old[s];
+    const bool skip = I.active && pec && lab == kFluid;
+    if (skip) {
         const double dt = I.dt;
-        b.x0[t] = x;
-        b.y0[t] = y;
-        b.vx0[t] = vx;
-        b.vy0[t] = vy;
-        b.rho0[t] = rho;
         x = pec_half_pos(x, vx, dt);
         y = pec_half_pos(y, vy, dt);
         vx = pec_half_vel(vx, ax, dt, I.damping);
         vy = pec_half_vel(vy, ay, dt, I.damping);
         rho = pec_half_rho(rho, drho, dt, I.strict);
-    } else {
-        b.x0[t] = a.x0[s];
-        b.y0[t] = a.y0[s];
-        b.vx0[t] = a.vx0[s];
-        b.vy0[t] = a.vy0[s];
-        b.rho0[t] = a.rho0[s];
     }
-    b.x[t] = x;
-    b.y[t] = y;
+    // x0 ... rho0 are not kept: nothing in a batch step reads them
+    if (!skip) {
+        b.x[t] = x;
+        b.y[t] = y;
+        b.rho[t] = rho;
+    }
     b.vx[t] = vx;
     b.vy[t] = vy;
-    b.rho[t] = rho;
     b.m[t] = m;
     b.h[t] = h;
     b.label[t] = lab;
     b.id[t] = a.id[s];
-    b.ax[t] = ax;
-    b.ay[t] = ay;
-    b.drho[t] = drho;
-    cxy_s[t] = cxy_old[s];
+    const Grid &g = I.grid;
+    if (!skip || !(cc.x >= 0 && cc.x < g.ncx && cc.y >= 0 && cc.y < g.ncy && h > 0.0 && I.qmax >= 0.0)) {
+        b.ax[t] = ax;
+        b.ay[t] = ay;
+        b.drho[t] = drho;
+    }
+    cxy_s[t] = cc;
     const double p = tait_p(rho, lab, I.gamma, I.B, I.rho0, I.Pb);
     const double pr = p / (rho * rho);
     const double w = lab == kFluid ? m : -(double)lab;
@@ -372,8 +373,10 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
 }
 
 // PEC.correct / Euler (fluid of active instances) + the next dt partials.
-__global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *inst, State s, int euler,
-                           double *part) {
+// PEC's start-of-step values come from the buffer before the reorder, at the
+// particle's previous slot perm[j] (the lean reorder did not copy them).
+__global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *inst, State s, State o,
+                           const int *perm, int euler, double *part) {
     __shared__ double lds[8];
     const int c = blockIdx.x, NC = gridDim.x;
     const BInst &I = inst[chunkInst[c]];
@@ -385,7 +388,8 @@ __global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *ins
         v.add(s.h[j], ax, ay);
         if (!I.active) continue;
         if (!euler) {
-            double x0 = s.x0[j], y0 = s.y0[j], vx0 = s.vx0[j], vy0 = s.vy0[j], rho0 = s.rho0[j];
+            const int q = perm[j];
+            const double x0 = o.x[q], y0 = o.y[q], vx0 = o.vx[q], vy0 = o.vy[q], rho0 = o.rho[q];
             double mid_x = x0 + 0.5 * dt * s.vx[j];
             double mid_y = y0 + 0.5 * dt * s.vy[j];
             double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
@@ -566,7 +570,7 @@ int b_step(sph_batch *b, hipStream_t s) {
     if ((rc = b_dt(b, s))) return rc;
     if ((rc = b_nn(b, s))) return rc;
     if ((rc = b_pair(b, s))) return rc;
-    kb_correct<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->inst, b->buf[b->cur],
+    kb_correct<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->inst, b->buf[b->cur], b->buf[1 - b->cur], b->perm,
                                         b->integrator == SPH_INTEGRATOR_EULER, b->part_d);
     HIPCHK(hipGetLastError());
     b->dt_fresh = true;
