@@ -51,23 +51,17 @@ struct DtAcc {
         if (!(a2 == a2)) bad += 1.0;
         cnt += 1.0;
     }
+    // lds holds kDtQ * 16 doubles
     __device__ void store(double *part, int NC, int c, double *lds) {
-        mn = block_reduce(mn, MinOp(), lds);
-        mx = block_reduce(mx, MaxOp(), lds);
-        cnt = block_reduce(cnt, AddOp(), lds);
-        bad = block_reduce(bad, AddOp(), lds);
-        if (threadIdx.x == 0) {
-            part[0 * NC + c] = mn;
-            part[1 * NC + c] = mx;
-            part[2 * NC + c] = cnt;
-            part[3 * NC + c] = bad;
-        }
+        const int kind[kDtQ] = {0, 1, 2, 2};
+        double v[kDtQ] = {mn, mx, cnt, bad};
+        block_partials<kDtQ>(v, kind, lds, part, NC, c);
     }
 };
 
 // chunk c covers [cs[c], cs[c + 1]) inside one instance
 __global__ void kb_dt_partial(const int *cs, State s, double *part) {
-    __shared__ double lds[8];
+    __shared__ double lds[kDtQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
     DtAcc v;
     for (int t = cs[c] + threadIdx.x; t < cs[c + 1]; t += blockDim.x)
@@ -135,7 +129,7 @@ __device__ inline void bupd_pos(const State &a, int i, bool pred, double dt, dou
 }
 
 __global__ void kb_bounds(const int *cs, const int *chunkInst, const BInst *inst, State a, int pec, double *part) {
-    __shared__ double lds[8];
+    __shared__ double lds[kBoundsQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
     const BInst &I = inst[chunkInst[c]];
     const bool pred = pec && I.active;
@@ -153,22 +147,9 @@ __global__ void kb_bounds(const int *cs, const int *chunkInst, const BInst *inst
         hmn = hi < hmn ? hi : hmn;
         hmx = hi > hmx ? hi : hmx;
     }
-    xmn = block_reduce(xmn, MinOp(), lds);
-    xmx = block_reduce(xmx, MaxOp(), lds);
-    ymn = block_reduce(ymn, MinOp(), lds);
-    ymx = block_reduce(ymx, MaxOp(), lds);
-    hmn = block_reduce(hmn, MinOp(), lds);
-    hmx = block_reduce(hmx, MaxOp(), lds);
-    bad = block_reduce(bad, AddOp(), lds);
-    if (threadIdx.x == 0) {
-        part[0 * NC + c] = xmn;
-        part[1 * NC + c] = xmx;
-        part[2 * NC + c] = ymn;
-        part[3 * NC + c] = ymx;
-        part[4 * NC + c] = hmn;
-        part[5 * NC + c] = hmx;
-        part[6 * NC + c] = bad;
-    }
+    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
+    double v[kBoundsQ] = {xmn, xmx, ymn, ymx, hmn, hmx, bad};
+    block_partials<kBoundsQ>(v, kind, lds, part, NC, c);
 }
 
 // One workgroup per instance: its grid (_get_cell_size :103-110,
@@ -313,8 +294,9 @@ __global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, in
 // instances fused in, and write the pair records (as k_reorder). With PEC the
 // reorder is lean as k_reorder's for the fluid of active instances: no
 // x0 ... rho0 (kb_correct gathers them from the source buffer), no half-step
-// x, y, rho (overwritten by kb_correct), no rates of a particle inside its
-// instance's grid (written by the pair pass).
+// x, y, vx, vy, rho (kb_correct forms the half-step velocity again and
+// overwrites the rest), no rates of a particle inside its instance's grid
+// (written by the pair pass).
 __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State b, const int *perm,
                            const int2 *cxy_old, int2 *cxy_s, int n, int fp32, void *recA, void *recB, int pec) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -338,10 +320,10 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
     if (!skip) {
         b.x[t] = x;
         b.y[t] = y;
+        b.vx[t] = vx;
+        b.vy[t] = vy;
         b.rho[t] = rho;
     }
-    b.vx[t] = vx;
-    b.vy[t] = vy;
     b.m[t] = m;
     b.h[t] = h;
     b.label[t] = lab;
@@ -377,7 +359,7 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
 // particle's previous slot perm[j] (the lean reorder did not copy them).
 __global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *inst, State s, State o,
                            const int *perm, int euler, double *part) {
-    __shared__ double lds[8];
+    __shared__ double lds[kDtQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
     const BInst &I = inst[chunkInst[c]];
     const double dt = I.dt, damping = I.damping;
@@ -390,8 +372,10 @@ __global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *ins
         if (!euler) {
             const int q = perm[j];
             const double x0 = o.x[q], y0 = o.y[q], vx0 = o.vx[q], vy0 = o.vy[q], rho0 = o.rho[q];
-            double mid_x = x0 + 0.5 * dt * s.vx[j];
-            double mid_y = y0 + 0.5 * dt * s.vy[j];
+            const double vxh = pec_half_vel(vx0, o.ax[q], dt, damping);  // as kb_reorder formed it
+            const double vyh = pec_half_vel(vy0, o.ay[q], dt, damping);
+            double mid_x = x0 + 0.5 * dt * vxh;
+            double mid_y = y0 + 0.5 * dt * vyh;
             double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
             double mid_vy = (vy0 + 0.5 * dt * ay) / (1 + 0.5 * damping);
             s.x[j] = 2 * mid_x - x0;
@@ -525,7 +509,6 @@ int b_nn(sph_batch *b, hipStream_t s) {
     long long tiles_cap = (b->capBins + kScanTile - 1) / kScanTile;
     int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
     k_scan_reduce<<<tiles, kBlock, 0, s>>>(b->view, b->counts, b->tileSums);
-    k_scan_tiles<<<1, 1024, 0, s>>>(b->view, b->tileSums);
     k_scan_apply<<<tiles, kBlock, 0, s>>>(b->view, b->counts, b->tileSums, b->cellStart);
     k_scatter<<<nblk_b(n), kBlock, 0, s>>>(b->view, b->cellStart, b->bin, b->rank, a.id, b->perm, b->permKey, n);
     long long cb = (b->capBins + kBlock - 1) / kBlock;

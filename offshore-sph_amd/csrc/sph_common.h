@@ -74,6 +74,18 @@ __device__ inline void block_fold(double (&v)[NQ], const int (&kind)[NQ], double
     }
 }
 
+// Per-workgroup partials of NQ quantities in one LDS round (block_fold):
+// thread 0 stores part[q * R + k]. lds holds NQ * 16 doubles.
+template <int NQ>
+__device__ inline void block_partials(double (&v)[NQ], const int (&kind)[NQ], double *lds, double *part, int R,
+                                      int k) {
+    block_fold<NQ>(v, kind, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) part[q * R + k] = v[q];
+    }
+}
+
 // src/Integrators/PEC.py:32-59 (useXSPH = False branch), one particle. The
 // fused step evaluates the same expressions inside the neighbour update.
 __device__ inline double pec_half_pos(double x, double v, double dt) { return x + 0.5 * dt * v; }
@@ -87,7 +99,9 @@ __device__ inline double pec_half_rho(double rho, double drho, double dt, int st
 }
 
 // Exclusive prefix scan of counts[0 .. ncells] into cellStart, in tiles of
-// kScanTile bins: tile sums, a scan of the tile sums, then the tiles.
+// kScanTile bins: the tile sums, then each tile's scan offset by the sum of
+// the tile sums before it (a few hundred integers, read by that tile's
+// workgroup: no separate pass over the tile sums).
 __device__ inline int block_scan_excl(int v, int *lds, int *total) {
     // inclusive scan inside each wave
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -128,22 +142,6 @@ __global__ void k_scan_reduce(const Ctrl *ctrl, const int *counts, int *tileSums
     }
 }
 
-__global__ void k_scan_tiles(const Ctrl *ctrl, int *tileSums) {
-    if (!ctrl->active) return;
-    __shared__ int lds[16];
-    long long nbins = ctrl->grid.ncells + 1;
-    int ntiles = (int)((nbins + kScanTile - 1) / kScanTile);
-    int carry = 0;
-    for (int base = 0; base < ntiles; base += blockDim.x) {
-        int i = base + threadIdx.x;
-        int v = i < ntiles ? tileSums[i] : 0;
-        int tot;
-        int ex = block_scan_excl(v, lds, &tot);
-        if (i < ntiles) tileSums[i] = carry + ex;
-        carry += tot;
-    }
-}
-
 __global__ void k_scan_apply(const Ctrl *ctrl, int *counts, const int *tileSums, int *cellStart) {
     if (!ctrl->active) return;
     long long nbins = ctrl->grid.ncells + 1;
@@ -159,8 +157,11 @@ __global__ void k_scan_apply(const Ctrl *ctrl, int *counts, const int *tileSums,
             v[k] = b < nbins ? counts[b] : 0;
             sum += v[k];
         }
+        int pre = 0;
+        for (long long j = threadIdx.x; j < tile; j += blockDim.x) pre += tileSums[j];
         int tot;
-        int ex = block_scan_excl(sum, lds, &tot) + tileSums[tile];
+        block_scan_excl(pre, lds, &pre);
+        int ex = block_scan_excl(sum, lds, &tot) + pre;
         for (int k = 0; k < kScanItems; ++k) {
             long long b = b0 + k;
             if (b < nbins) {

@@ -65,19 +65,12 @@ struct DtVars {
         if (!(a2 == a2)) bad += 1.0;
         cnt += 1.0;
     }
-    // block reduction, then this workgroup's partials (plain stores)
+    // this workgroup's partials (one LDS round, plain stores); lds holds
+    // kDtQ * 16 doubles
     __device__ void publish(double *part, double *lds) {
-        mn = block_reduce(mn, MinOp(), lds);
-        mx = block_reduce(mx, MaxOp(), lds);
-        cnt = block_reduce(cnt, AddOp(), lds);
-        bad = block_reduce(bad, AddOp(), lds);
-        if (threadIdx.x == 0) {
-            const int R = gridDim.x, k = blockIdx.x;
-            part[0 * R + k] = mn;
-            part[1 * R + k] = mx;
-            part[2 * R + k] = cnt;
-            part[3 * R + k] = bad;
-        }
+        const int kind[kDtQ] = {0, 1, 2, 2};
+        double v[kDtQ] = {mn, mx, cnt, bad};
+        block_partials<kDtQ>(v, kind, lds, part, gridDim.x, blockIdx.x);
     }
 };
 
@@ -85,7 +78,7 @@ struct DtVars {
 // has left the reduction of this state behind). Grid-stride over R workgroups.
 __global__ void k_dt_partial(const int8_t *label, const double *h, const double *ax, const double *ay, int n,
                              double *part) {
-    __shared__ double lds[8];
+    __shared__ double lds[kDtQ * 16];
     DtVars v;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         if (label[i] == kFluid) v.add(h[i], ax[i], ay[i]);
@@ -197,10 +190,12 @@ __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
 // PEC's start-of-step values come from x0 ... rho0 of the same buffer, or,
 // with `perm` (a step whose reorder did not write them), from the buffer
 // before the reorder at the particle's previous slot perm[j]: the same values.
+// The half-step velocity likewise comes from vx, vy or is formed again from
+// the start-of-step velocity and the previous rates there.
 __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, int integrator, int strict,
                           double *part) {
     if (!ctrl->active) return;
-    __shared__ double lds[8];
+    __shared__ double lds[kDtQ * 16];
     const double dt = ctrl->dt, damping = ctrl->damping;
     DtVars v;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
@@ -208,23 +203,29 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
         const double ax = s.ax[j], ay = s.ay[j];
         v.add(s.h[j], ax, ay);
         if (integrator == SPH_INTEGRATOR_PEC) {
-            double x0, y0, vx0, vy0, rho0;
+            double x0, y0, vx0, vy0, rho0, vxh, vyh;
             if (perm) {
+                // the half-step velocity again from the same inputs (the
+                // predictor's expression, as in k_reorder)
                 const int q = perm[j];
                 x0 = o.x[q];
                 y0 = o.y[q];
                 vx0 = o.vx[q];
                 vy0 = o.vy[q];
                 rho0 = o.rho[q];
+                vxh = pec_half_vel(vx0, o.ax[q], dt, damping);
+                vyh = pec_half_vel(vy0, o.ay[q], dt, damping);
             } else {
                 x0 = s.x0[j];
                 y0 = s.y0[j];
                 vx0 = s.vx0[j];
                 vy0 = s.vy0[j];
                 rho0 = s.rho0[j];
+                vxh = s.vx[j];
+                vyh = s.vy[j];
             }
-            double mid_x = x0 + 0.5 * dt * s.vx[j];
-            double mid_y = y0 + 0.5 * dt * s.vy[j];
+            double mid_x = x0 + 0.5 * dt * vxh;
+            double mid_y = y0 + 0.5 * dt * vyh;
             double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
             double mid_vy = (vy0 + 0.5 * dt * ay) / (1 + 0.5 * damping);
             s.x[j] = 2 * mid_x - x0;
@@ -266,7 +267,7 @@ __device__ inline void upd_pos(const State &a, int i, int pred, double dt, doubl
 // workgroups.
 __global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *part) {
     if (!ctrl->active) return;
-    __shared__ double lds[8];
+    __shared__ double lds[kBoundsQ * 16];
     double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, hmx = -INFINITY;
     double bad = 0.0;
     const double dt = ctrl->dt;
@@ -282,23 +283,9 @@ __global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *par
         hmn = hi < hmn ? hi : hmn;
         hmx = hi > hmx ? hi : hmx;
     }
-    xmn = block_reduce(xmn, MinOp(), lds);
-    xmx = block_reduce(xmx, MaxOp(), lds);
-    ymn = block_reduce(ymn, MinOp(), lds);
-    ymx = block_reduce(ymx, MaxOp(), lds);
-    hmn = block_reduce(hmn, MinOp(), lds);
-    hmx = block_reduce(hmx, MaxOp(), lds);
-    bad = block_reduce(bad, AddOp(), lds);
-    if (threadIdx.x == 0) {
-        const int R = gridDim.x, k = blockIdx.x;
-        part[0 * R + k] = xmn;
-        part[1 * R + k] = xmx;
-        part[2 * R + k] = ymn;
-        part[3 * R + k] = ymx;
-        part[4 * R + k] = hmn;
-        part[5 * R + k] = hmx;
-        part[6 * R + k] = bad;
-    }
+    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
+    double v[kBoundsQ] = {xmn, xmx, ymn, ymx, hmn, hmx, bad};
+    block_partials<kBoundsQ>(v, kind, lds, part, gridDim.x, blockIdx.x);
 }
 
 // _get_cell_size :103-110 and _get_number_of_cells :112-127 from the folded
@@ -423,8 +410,9 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // otherwise.
 // lean (a PEC step whose fused pair pass and corrector follow at once): for
 // the fluid, x0 ... rho0 are not written (the corrector reads them from this
-// kernel's source buffer), nor the half-step x, y, rho (the records carry
-// them to the pair pass; the corrector overwrites them), nor the rates of a
+// kernel's source buffer), nor the half-step x, y, vx, vy, rho (the records
+// carry them to the pair pass; the corrector forms the half-step velocity
+// again and overwrites the rest), nor the rates of a
 // particle inside the grid (its own cell is walked, so it has at least one
 // neighbour, q = 0, and the pair pass writes all three). A particle outside
 // the grid may find none and keep its previous rates (SolverTools.py:166-169).
@@ -467,10 +455,10 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const i
     if (!skip) {
         b.x[t] = x;
         b.y[t] = y;
+        b.vx[t] = vx;
+        b.vy[t] = vy;
         b.rho[t] = rho;
     }
-    b.vx[t] = vx;
-    b.vy[t] = vy;
     b.m[t] = m;
     b.h[t] = h;
     b.label[t] = lab;
@@ -949,7 +937,6 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
     long long tiles_cap = (c->cap_cells + 1 + kScanTile - 1) / kScanTile;
     int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
     k_scan_reduce<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums);
-    k_scan_tiles<<<1, 1024, 0, s>>>(c->ctrl, c->tileSums);
     k_scan_apply<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums, c->cellStart);
     k_scatter<<<nblk(n), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->bin, c->rank, a.id, c->perm, c->permKey, n);
     // bin/rank are free once scattered: they stage the rare bins of more than 64
