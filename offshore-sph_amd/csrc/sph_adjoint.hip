@@ -243,46 +243,6 @@ __device__ inline void amax_merge(double &v, int &i, double v2, int i2) {
     }
 }
 
-// Start-of-step snapshot (velocity and carried rates, scattered to id order)
-// fused with the argmax partials. Grid-stride over R workgroups.
-__global__ void k_tape_begin_amax(const Ctrl *ctrl, State s, int n, TapeSlot T, TapeHdr *hdr, double *pv, int *pi) {
-#pragma clang fp contract(off)
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        hdr->dt = ctrl->dt;
-        hdr->dt_c = ctrl->dt_c;
-        hdr->dt_f = ctrl->dt_f;
-    }
-    double v = -1.0;
-    int id = 0x7fffffff;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const int i = s.id[t];
-        const double ax = s.ax[t], ay = s.ay[t];
-        T.vx0[i] = s.vx[t];
-        T.vy0[i] = s.vy[t];
-        T.axp[i] = ax;
-        T.ayp[i] = ay;
-        T.drhop[i] = s.drho[t];
-        if (s.label[t] != 0) continue;
-        const double a2 = ax * ax + ay * ay;
-        amax_merge(v, id, a2, i);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) amax_merge(v, id, __shfl_xor(v, o, 64), __shfl_xor(id, o, 64));
-    __shared__ double lv[16];
-    __shared__ int li[16];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (lane == 0) {
-        lv[w] = v;
-        li[w] = id;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < nw; ++k) amax_merge(v, id, lv[k], li[k]);
-        pv[blockIdx.x] = v;
-        pi[blockIdx.x] = id;
-    }
-}
-
 __global__ void k_tape_amax_final(const double *pv, const int *pi, int R, TapeHdr *hdr) {
     double v = -1.0;
     int id = 0x7fffffff;
@@ -304,32 +264,70 @@ __global__ void k_tape_amax_final(const double *pv, const int *pi, int R, TapeHd
     }
 }
 
-// After the pair pass: its inputs, outputs and walk structures (cell offsets
-// in a grid-stride loop over the grid's ncells + 2 entries).
-__global__ void k_tape_pair(const Ctrl *ctrl, State s, const int2 *cxy, const int *cellStart, int n, TapeSlot T,
-                            TapeHdr *hdr, long long cap) {
+// After the pair pass, everything the reverse step needs, scattered to id
+// order: the step's start values (velocity and the carried rates, read from
+// the buffer before the reorder at the slot's previous position perm[t]),
+// the pair pass's inputs and outputs, its walk structures (cell offsets in a
+// grid-stride loop over the grid's ncells + 2 entries), dt, and per-workgroup
+// partials of the argmax of |a_prev|^2 over the fluid with the engine's
+// expression (contraction off). An inactive step is recorded as such only.
+__global__ void k_tape_pair(const Ctrl *ctrl, State s, State o, const int *perm, const int2 *cxy,
+                            const int *cellStart, int n, TapeSlot T, TapeHdr *hdr, long long cap, double *pv,
+                            int *pi) {
+#pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int act = ctrl->active;
     if (t == 0) {
         hdr->grid = ctrl->grid;
-        hdr->active = ctrl->active;
+        hdr->active = act;
+        hdr->dt = ctrl->dt;
+        hdr->dt_c = ctrl->dt_c;
+        hdr->dt_f = ctrl->dt_f;
     }
-    const long long m = ctrl->grid.ncells + 2;
-    const long long lim = m < cap ? m : cap;
-    for (long long k = t; k < lim; k += (long long)gridDim.x * blockDim.x) T.cellStart[k] = cellStart[k];
-    if (t >= n) return;
-    const int i = s.id[t];
-    T.xh[i] = s.x[t];
-    T.yh[i] = s.y[t];
-    T.vxh[i] = s.vx[t];
-    T.vyh[i] = s.vy[t];
-    T.rhoh[i] = s.rho[t];
-    T.ax[i] = s.ax[t];
-    T.ay[i] = s.ay[t];
-    T.drho[i] = s.drho[t];
-    T.id[t] = i;
-    T.cxy[t] = cxy[t];
+    double v = -1.0;
+    int id = 0x7fffffff;
+    if (act) {
+        const long long m = ctrl->grid.ncells + 2;
+        const long long lim = m < cap ? m : cap;
+        for (long long k = t; k < lim; k += (long long)gridDim.x * blockDim.x) T.cellStart[k] = cellStart[k];
+        if (t < n) {
+            const int i = s.id[t];
+            const int q = perm[t];
+            const double ax = o.ax[q], ay = o.ay[q];
+            T.vx0[i] = o.vx[q];
+            T.vy0[i] = o.vy[q];
+            T.axp[i] = ax;
+            T.ayp[i] = ay;
+            T.drhop[i] = o.drho[q];
+            T.xh[i] = s.x[t];
+            T.yh[i] = s.y[t];
+            T.vxh[i] = s.vx[t];
+            T.vyh[i] = s.vy[t];
+            T.rhoh[i] = s.rho[t];
+            T.ax[i] = s.ax[t];
+            T.ay[i] = s.ay[t];
+            T.drho[i] = s.drho[t];
+            T.id[t] = i;
+            T.cxy[t] = cxy[t];
+            if (s.label[t] == 0) amax_merge(v, id, ax * ax + ay * ay, i);
+        }
+    }
+#pragma unroll
+    for (int o2 = 32; o2 > 0; o2 >>= 1) amax_merge(v, id, __shfl_xor(v, o2, 64), __shfl_xor(id, o2, 64));
+    __shared__ double lv[16];
+    __shared__ int li[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+        lv[w] = v;
+        li[w] = id;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < nw; ++k) amax_merge(v, id, lv[k], li[k]);
+        pv[blockIdx.x] = v;
+        pi[blockIdx.x] = id;
+    }
 }
-
 
 // ---------------------------------------------------------------------------
 // reverse step
@@ -558,18 +556,12 @@ inline int redblk_(long long n) {
 // ---------------------------------------------------------------------------
 // hooks for sph_step_record
 // ---------------------------------------------------------------------------
-int hipsph::tape_record_begin(sph_ctx *c, sph_tape *t, int slot, hipStream_t s) {
-    const int R = redblk_(c->n);
-    k_tape_begin_amax<<<R, kAdjBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, t->slot[slot], t->hdr + slot,
-                                              t->amax_v, t->amax_i);
-    k_tape_amax_final<<<1, 1024, 0, s>>>(t->amax_v, t->amax_i, R, t->hdr + slot);
-    HIPCHK(hipGetLastError());
-    return 0;
-}
-
 int hipsph::tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s) {
-    k_tape_pair<<<nblk_(c->n, kAdjBlock), kAdjBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->cxy_s, c->cellStart, c->n,
-                                                             t->slot[slot], t->hdr + slot, t->cells);
+    const int R = nblk_(c->n, kAdjBlock);
+    k_tape_pair<<<R, kAdjBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->buf[1 - c->cur], c->perm, c->cxy_s,
+                                        c->cellStart, c->n, t->slot[slot], t->hdr + slot, t->cells, t->amax_v,
+                                        t->amax_i);
+    k_tape_amax_final<<<1, 1024, 0, s>>>(t->amax_v, t->amax_i, R, t->hdr + slot);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -595,7 +587,7 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
                        16 * 256;
     const size_t npart = (size_t)kAdjQ * (2 * kRedBlocks + nblk_((long long)ctx->n * kAdjLanes, kAdjBlock));
     const size_t fixed = n * (2 * sizeof(double) + 1) + n * 10 * sizeof(double) + npart * sizeof(double) +
-                         sizeof(TapeHdr) * nslots + kRedBlocks * (sizeof(double) + sizeof(int)) + 20 * 256 + 4096;
+                         sizeof(TapeHdr) * nslots + nblk_(ctx->n, kAdjBlock) * (sizeof(double) + sizeof(int)) + 20 * 256 + 4096;
     if (hipMalloc(&t->arena, per * nslots + fixed) != hipSuccess) {
         delete t;
         return fail(SPH_ENOMEM, "tape allocation of %zu bytes failed", per * nslots + fixed);
@@ -610,8 +602,8 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     t->hdr = (TapeHdr *)take(sizeof(TapeHdr) * nslots);
     t->part = (double *)take(npart * sizeof(double));
     t->force_steps = (int *)take(sizeof(int));
-    t->amax_v = (double *)take(kRedBlocks * sizeof(double));
-    t->amax_i = (int *)take(kRedBlocks * sizeof(int));
+    t->amax_v = (double *)take(nblk_(ctx->n, kAdjBlock) * sizeof(double));
+    t->amax_i = (int *)take(nblk_(ctx->n, kAdjBlock) * sizeof(int));
     t->m = (double *)take(n * sizeof(double));
     t->h = (double *)take(n * sizeof(double));
     t->label = (int8_t *)take(n);

@@ -122,9 +122,12 @@ __global__ void kb_dt_final(BInst *inst, const int *chunk0, const double *part, 
 __device__ inline void bupd_pos(const State &a, int i, bool pred, double dt, double &x, double &y) {
     x = a.x[i];
     y = a.y[i];
-    if (pred && a.label[i] == kFluid) {
-        x = pec_half_pos(x, a.vx[i], dt);
-        y = pec_half_pos(y, a.vy[i], dt);
+    if (pred) {
+        // the velocity is loaded with the label, not after it, and selected
+        const double vx = a.vx[i], vy = a.vy[i];
+        const bool f = a.label[i] == kFluid;
+        x = f ? pec_half_pos(x, vx, dt) : x;
+        y = f ? pec_half_pos(y, vy, dt) : y;
     }
 }
 
@@ -246,7 +249,7 @@ __global__ void kb_bases(BInst *inst, int K, Ctrl *view, long long capBins) {
 
 // _bin :129-141 with the instance's grid; rank by runs as k_keys does.
 __global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, int pec, int *counts, int *bin,
-                        int *rank, int2 *cxy, int *binflag, const Ctrl *view) {
+                        int *rank, int *binflag, const Ctrl *view) {
     if (!view->active) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -257,17 +260,11 @@ __global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, in
         if (I.active) {
             double x, y;
             bupd_pos(a, t, pec != 0, I.dt, x, y);
-            double fx = floor((x - g.xmin) / g.cs);
-            double fy = floor((y - g.ymin) / g.cs);
-            fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
-            fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
-            const long long cx = (long long)fx, cy = (long long)fy;
-            const long long key = cx + g.ncx * cy;
+            const int2 c = cell_of(x, y, g);
+            const long long key = (long long)c.x + g.ncx * (long long)c.y;
             b = (int)(I.cellBase + ((key >= 0 && key < g.ncells) ? key : g.ncells));
-            cxy[t] = make_int2((int)cx, (int)cy);
         } else {
             b = (int)I.cellBase;  // parked: the instance's single bin
-            cxy[t] = make_int2(0, 0);
         }
     }
     const int id = t < n ? a.id[t] : 0x7fffffff;
@@ -297,8 +294,8 @@ __global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, in
 // x, y, vx, vy, rho (kb_correct forms the half-step velocity again and
 // overwrites the rest), no rates of a particle inside its instance's grid
 // (written by the pair pass).
-__global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State b, const int *perm,
-                           const int2 *cxy_old, int2 *cxy_s, int n, int fp32, void *recA, void *recB, int pec) {
+__global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State b, const int *perm, int2 *cxy_s,
+                           int n, int fp32, void *recA, void *recB, int pec) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const BInst &I = inst[instOf[t]];  // slots never change instance
@@ -306,7 +303,6 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
     double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
     const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
     const int8_t lab = a.label[s];
-    const int2 cc = cxy_old[s];
     const bool skip = I.active && pec && lab == kFluid;
     if (skip) {
         const double dt = I.dt;
@@ -317,6 +313,9 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
         rho = pec_half_rho(rho, drho, dt, I.strict);
     }
     // x0 ... rho0 are not kept: nothing in a batch step reads them
+    // the raw cell the keys used (parked particles of an inactive instance: 0, 0)
+    const Grid &g = I.grid;
+    const int2 cc = I.active ? cell_of(x, y, g) : make_int2(0, 0);
     if (!skip) {
         b.x[t] = x;
         b.y[t] = y;
@@ -328,7 +327,6 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
     b.h[t] = h;
     b.label[t] = lab;
     b.id[t] = a.id[s];
-    const Grid &g = I.grid;
     if (!skip || !(cc.x >= 0 && cc.x < g.ncx && cc.y >= 0 && cc.y < g.ncy && h > 0.0 && I.qmax >= 0.0)) {
         b.ax[t] = ax;
         b.ay[t] = ay;
@@ -457,7 +455,7 @@ struct sph_batch {
     Ctrl *view = nullptr;
     int *instOf = nullptr, *cs = nullptr, *chunkInst = nullptr, *chunk0 = nullptr;
     int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
-    int2 *cxy_old = nullptr, *cxy_s = nullptr;
+    int2 *cxy_s = nullptr;
     int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
     double *part_b = nullptr, *part_d = nullptr;
     void *recA = nullptr, *recB = nullptr;
@@ -481,7 +479,7 @@ void free_batch(sph_batch *b) {
         if (b->gexec[k]) hipGraphExecDestroy(b->gexec[k]);
     }
     void *ptrs[] = {b->inst, b->view, b->instOf, b->cs, b->chunkInst, b->chunk0, b->bin, b->rank, b->perm,
-                    b->permKey, b->cxy_old, b->cxy_s, b->counts, b->cellStart, b->tileSums, b->binflag,
+                    b->permKey, b->cxy_s, b->counts, b->cellStart, b->tileSums, b->binflag,
                     b->part_b, b->part_d, b->recA, b->recB};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -504,8 +502,8 @@ int b_nn(sph_batch *b, hipStream_t s) {
     kb_bounds<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->inst, a, pec, b->part_b);
     kb_grid<<<b->K, kBlock, 0, s>>>(b->inst, b->chunk0, b->part_b, b->NC);
     kb_bases<<<1, 1024, 0, s>>>(b->inst, b->K, b->view, b->capBins);
-    kb_keys<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, n, pec, b->counts, b->bin, b->rank, b->cxy_old,
-                                         b->binflag, b->view);
+    kb_keys<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, n, pec, b->counts, b->bin, b->rank, b->binflag,
+                                         b->view);
     long long tiles_cap = (b->capBins + kScanTile - 1) / kScanTile;
     int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
     k_scan_reduce<<<tiles, kBlock, 0, s>>>(b->view, b->counts, b->tileSums);
@@ -514,7 +512,7 @@ int b_nn(sph_batch *b, hipStream_t s) {
     long long cb = (b->capBins + kBlock - 1) / kBlock;
     k_cellsort<<<(int)(cb < kRedBlocks ? (cb < 1 ? 1 : cb) : kRedBlocks), kBlock, 0, s>>>(
         b->view, b->cellStart, b->perm, b->permKey, b->binflag, b->bin, b->rank);
-    kb_reorder<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, c, b->perm, b->cxy_old, b->cxy_s, n,
+    kb_reorder<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, c, b->perm, b->cxy_s, n,
                                             b->precision == SPH_PREC_FP32C, b->recA, b->recB, pec);
     HIPCHK(hipGetLastError());
     b->cur = 1 - b->cur;
@@ -680,7 +678,7 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMalloc(&b->chunk0, sizeof(int) * chunk0.size()) == hipSuccess &&
          hipMalloc(&b->bin, sizeof(int) * n) == hipSuccess && hipMalloc(&b->rank, sizeof(int) * n) == hipSuccess &&
          hipMalloc(&b->perm, sizeof(int) * n) == hipSuccess && hipMalloc(&b->permKey, sizeof(int) * n) == hipSuccess &&
-         hipMalloc(&b->cxy_old, sizeof(int2) * n) == hipSuccess && hipMalloc(&b->cxy_s, sizeof(int2) * n) == hipSuccess &&
+         hipMalloc(&b->cxy_s, sizeof(int2) * n) == hipSuccess &&
          hipMalloc(&b->counts, sizeof(int) * (capTot + 1)) == hipSuccess &&
          hipMalloc(&b->cellStart, sizeof(int) * (capTot + 2)) == hipSuccess &&
          hipMalloc(&b->tileSums, sizeof(int) * (tiles + 1)) == hipSuccess &&

@@ -220,8 +220,11 @@ __device__ void wave_sort_bin(int s, int e, int *perm, int *permKey, int *tmpKey
 // Order each bin by the reference index, which is what the reference's
 // head-inserted lists are ordered by (descending there; the pair pass walks
 // the bin backwards). Only bins flagged by k_keys (filled by more than one
-// run) can be out of order. Each wavefront reads the flags of 64 consecutive
-// bins, clears them and sorts the flagged ones one after another.
+// run) can be out of order; most of them are cells whose run crossed a
+// wavefront boundary in k_keys. Each wavefront reads the flags of 64
+// consecutive bins and clears them; every lane sorts its own flagged bin.
+constexpr int kLaneSort = 16;  // bins up to this size are sorted by one lane
+
 __global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, int *permKey, int *binflag,
                            int *tmpKey, int *tmpVal) {
     if (!ctrl->active) return;
@@ -236,7 +239,34 @@ __global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, in
             f = binflag[bl];
             if (f) binflag[bl] = 0;
         }
-        unsigned long long m = __ballot(f != 0);
+        // Each lane sorts its own flagged bin of up to kLaneSort slots in
+        // registers (rank = number of smaller keys; cells hold ~3-9
+        // particles); longer bins go to the whole wavefront afterwards.
+        const bool own = f != 0;
+        int s = 0, len = 0;
+        if (own) {
+            s = cellStart[bl];
+            len = cellStart[bl + 1] - s;
+        }
+        if (own && len <= kLaneSort) {
+            int key[kLaneSort], val[kLaneSort];
+#pragma unroll
+            for (int i = 0; i < kLaneSort; ++i) {
+                key[i] = i < len ? permKey[s + i] : 0x7fffffff;
+                val[i] = i < len ? perm[s + i] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < kLaneSort; ++i) {
+                int r = 0;
+#pragma unroll
+                for (int j = 0; j < kLaneSort; ++j) r += key[j] < key[i];
+                if (i < len) {
+                    permKey[s + r] = key[i];
+                    perm[s + r] = val[i];
+                }
+            }
+        }
+        unsigned long long m = __ballot(own && len > kLaneSort);
         while (m) {
             const int k = __ffsll((long long)m) - 1;
             m &= m - 1;
@@ -244,6 +274,18 @@ __global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, in
             wave_sort_bin(cellStart[b], cellStart[b + 1], perm, permKey, tmpKey, tmpVal);
         }
     }
+}
+
+// _bin's cell of a position (src/Tools/NNLinkedList.py:146-176): floor of
+// the true quotient per axis, clamped only for the int conversion (any
+// clamped value is far outside the grid). The keys kernels and the reorders
+// both call it, so a particle's stored cell is the one it was binned by.
+__device__ inline int2 cell_of(double x, double y, const Grid &g) {
+    double fx = floor((x - g.xmin) / g.cs);
+    double fy = floor((y - g.ymin) / g.cs);
+    fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
+    fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
+    return make_int2((int)(long long)fx, (int)(long long)fy);
 }
 
 // Tait pressure + Pb (src/Equations/TaitEOS.py:6-31; WCSPH.py:131-149).

@@ -17,7 +17,7 @@ struct sph_ctx {
     hipsph::Ctrl *ctrl = nullptr;
     double *dt_hist = nullptr;
     int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
-    int2 *cxy_old = nullptr, *cxy_s = nullptr;
+    int2 *cxy_s = nullptr;  // raw cell (cx, cy) of each slot in cell order
     int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
     double *part_b = nullptr, *part_d = nullptr;  // per-workgroup partials: bounds, dt
     double *part_k = nullptr;                     // kinetic-energy partials + result
@@ -90,7 +90,6 @@ int fail(int code, const char *fmt, ...);
 Phys make_phys(const sph_params_t &P);
 
 // Tape hooks called by sph_step_record (implemented with the adjoint).
-int tape_record_begin(sph_ctx *c, sph_tape *t, int slot, hipStream_t s);
 int tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s);
 // The reverse of one recorded step (the kernels of sph_adjoint_step).
 int adjoint_launch(sph_ctx *c, sph_tape *t, int slot, const sph_adjoint_t *adj, hipStream_t s);

@@ -256,15 +256,19 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
 __device__ inline void upd_pos(const State &a, int i, int pred, double dt, double &x, double &y) {
     x = a.x[i];
     y = a.y[i];
-    if (pred && a.label[i] == kFluid) {
-        x = pec_half_pos(x, a.vx[i], dt);
-        y = pec_half_pos(y, a.vy[i], dt);
+    if (pred) {
+        // the velocity is loaded with the label, not after it, and selected
+        const double vx = a.vx[i], vy = a.vy[i];
+        const bool f = a.label[i] == kFluid;
+        x = f ? pec_half_pos(x, vx, dt) : x;
+        y = f ? pec_half_pos(y, vy, dt) : y;
     }
 }
 
 // _init :86-101 bounds and _get_cell_size :103-110 min h, over all active
 // particles: per-workgroup partials, folded by k_grid. Grid-stride over R
-// workgroups.
+// workgroups (one workgroup per 256 particles measured 40 % slower: the
+// per-workgroup fold is paid in more rounds).
 __global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *part) {
     if (!ctrl->active) return;
     __shared__ double lds[kBoundsQ * 16];
@@ -351,8 +355,7 @@ __global__ void k_grid(Ctrl *ctrl, const double *part, int R, double scale, long
 // one atomic per run of equal bins within a wavefront (the state is already
 // in the previous step's cell order, so ~7 consecutive lanes share a bin).
 // k_cellsort fixes the order inside each bin afterwards.
-__global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *bin, int *rank, int2 *cxy,
-                       int *binflag) {
+__global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *bin, int *rank, int *binflag) {
     if (!ctrl->active) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -361,20 +364,14 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
     if (t < n) {
         double x, y;
         upd_pos(a, t, pred, ctrl->dt, x, y);
-        double fx = floor((x - g.xmin) / g.cs);
-        double fy = floor((y - g.ymin) / g.cs);
-        // clamp only for the int conversion; any clamped value is far outside the grid
-        fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
-        fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
-        long long cx = (long long)fx, cy = (long long)fy;
-        long long key = cx + g.ncx * cy;
+        const int2 c = cell_of(x, y, g);
+        const long long key = (long long)c.x + g.ncx * (long long)c.y;
         if (key >= 0 && key < g.ncells) {
             b = (int)key;
         } else {
             b = (int)g.ncells;
             atomicOr(&ctrl->flags, (unsigned)SPH_FLAG_OOB_KEY);
         }
-        cxy[t] = make_int2((int)cx, (int)cy);
     }
     // Runs of equal bins in lane order, one atomic per run. A run also breaks
     // where the reference index drops (consecutive slots from two old cells),
@@ -416,8 +413,10 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // particle inside the grid (its own cell is walked, so it has at least one
 // neighbour, q = 0, and the pair pass writes all three). A particle outside
 // the grid may find none and keep its previous rates (SolverTools.py:166-169).
-__global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const int2 *cxy_old, int2 *cxy_s, Phys ph,
-                          int n, int fp32, void *recA, void *recB, int pred, int strict, int lean) {
+// The raw cell of each slot (for the pair pass's 3x3 walk) is formed again
+// from the position the keys were computed from.
+__global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fp32,
+                          void *recA, void *recB, int pred, int strict, int lean) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
     const int act = ctrl->active;
@@ -429,7 +428,6 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const i
     const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
     int8_t lab = a.label[s];
     const bool skip = lean && act && pred && lab == kFluid;
-    const int2 cc = act ? cxy_old[s] : make_int2(0, 0);
     if (act && pred && lab == kFluid) {
         // PEC.predict (src/Integrators/PEC.py:32-59) fused into the permutation
         const double dt = ctrl->dt, damping = ctrl->damping;
@@ -452,6 +450,8 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const i
         b.vy0[t] = a.vy0[s];
         b.rho0[t] = a.rho0[s];
     }
+    const Grid &g = ctrl->grid;
+    const int2 cc = act ? cell_of(x, y, g) : make_int2(0, 0);
     if (!skip) {
         b.x[t] = x;
         b.y[t] = y;
@@ -463,7 +463,6 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, const i
     b.h[t] = h;
     b.label[t] = lab;
     b.id[t] = a.id[s];
-    const Grid &g = ctrl->grid;
     if (!skip || !(cc.x >= 0 && cc.x < g.ncx && cc.y >= 0 && cc.y < g.ncy && h > 0.0 && ph.qmax >= 0.0)) {
         b.ax[t] = ax;
         b.ay[t] = ay;
@@ -686,7 +685,7 @@ static void free_ctx(sph_ctx *c) {
         if (c->arena[k]) hipFree(c->arena[k]);
         if (c->gexec[k]) hipGraphExecDestroy(c->gexec[k]);
     }
-    void *ptrs[] = {c->ctrl, c->dt_hist, c->bin, c->rank, c->perm, c->permKey, c->cxy_old, c->cxy_s,
+    void *ptrs[] = {c->ctrl, c->dt_hist, c->bin, c->rank, c->perm, c->permKey, c->cxy_s,
                     c->counts, c->cellStart, c->tileSums, c->binflag, c->part_b, c->part_d, c->part_k, c->recA, c->recB,
                     c->slot_of};
     for (void *p : ptrs)
@@ -792,7 +791,6 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
               hipMalloc(&c->bin, sizeof(int) * n) == hipSuccess && hipMalloc(&c->rank, sizeof(int) * n) == hipSuccess &&
               hipMalloc(&c->perm, sizeof(int) * n) == hipSuccess &&
               hipMalloc(&c->permKey, sizeof(int) * n) == hipSuccess &&
-              hipMalloc(&c->cxy_old, sizeof(int2) * n) == hipSuccess &&
               hipMalloc(&c->cxy_s, sizeof(int2) * n) == hipSuccess &&
 
               hipMalloc(&c->part_b, sizeof(double) * kBoundsQ * kRedBlocks) == hipSuccess &&
@@ -933,7 +931,7 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
     const int R = redblk(n);
     k_bounds<<<R, kBlock, 0, s>>>(c->ctrl, a, n, pred, c->part_b);
     k_grid<<<1, 1024, 0, s>>>(c->ctrl, c->part_b, R, c->P.scale, c->cap_cells);
-    k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, n, pred, c->counts, c->bin, c->rank, c->cxy_old, c->binflag);
+    k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, n, pred, c->counts, c->bin, c->rank, c->binflag);
     long long tiles_cap = (c->cap_cells + 1 + kScanTile - 1) / kScanTile;
     int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
     k_scan_reduce<<<tiles, kBlock, 0, s>>>(c->ctrl, c->counts, c->tileSums);
@@ -942,7 +940,7 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
     // bin/rank are free once scattered: they stage the rare bins of more than 64
     k_cellsort<<<redblk(c->cap_cells + 1), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->perm, c->permKey,
                                                            c->binflag, c->bin, c->rank);
-    k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_old, c->cxy_s, make_phys(c->P), n,
+    k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_s, make_phys(c->P), n,
                                          c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict,
                                          lean);
     HIPCHK(hipGetLastError());
@@ -1020,7 +1018,6 @@ int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream
     hipStream_t s = S(stream);
     int rc;
     if ((rc = dt_step(ctx, s))) return rc;
-    if ((rc = tape_record_begin(ctx, tape, slot, s))) return rc;
     if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
     if ((rc = compute(ctx, s))) return rc;
     if ((rc = tape_record_pair(ctx, tape, slot, s))) return rc;
@@ -1033,7 +1030,6 @@ int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream
 static int record_step(sph_ctx *ctx, sph_tape *tape, int slot, hipStream_t s) {
     int rc;
     if ((rc = dt_step(ctx, s))) return rc;
-    if ((rc = tape_record_begin(ctx, tape, slot, s))) return rc;
     if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
     if ((rc = compute(ctx, s))) return rc;
     if ((rc = tape_record_pair(ctx, tape, slot, s))) return rc;
