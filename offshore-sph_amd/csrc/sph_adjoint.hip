@@ -224,13 +224,10 @@ __device__ inline void block_partials(double (&v)[NQ], double *part, int R) {
 // ---------------------------------------------------------------------------
 // recording
 // ---------------------------------------------------------------------------
-__global__ void k_tape_static(State s, int n, double *m, double *h, int8_t *label) {
+__global__ void k_tape_static(State s, int n, int8_t *label) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int i = s.id[t];
-    m[i] = s.m[t];
-    h[i] = s.h[t];
-    label[i] = s.label[t];
+    label[s.id[t]] = s.label[t];
 }
 
 // Argmax of |a_prev|^2 over the fluid at the start of a step, with the
@@ -264,16 +261,18 @@ __global__ void k_tape_amax_final(const double *pv, const int *pi, int R, TapeHd
     }
 }
 
-// After the pair pass, everything the reverse step needs, scattered to id
-// order: the step's start values (velocity and the carried rates, read from
-// the buffer before the reorder at the slot's previous position perm[t]),
-// the pair pass's inputs and outputs, its walk structures (cell offsets in a
-// grid-stride loop over the grid's ncells + 2 entries), dt, and per-workgroup
-// partials of the argmax of |a_prev|^2 over the fluid with the engine's
-// expression (contraction off). An inactive step is recorded as such only.
+// After the pair pass, everything the reverse step needs: in id order the
+// step's start values (velocity and the carried rates, read from the buffer
+// before the reorder at the slot's previous position perm[t]) and the pair
+// pass's outputs; in the step's cell-slot order the pair pass's inputs as
+// records, with each fluid particle's p / rho^2 terms (formed once here, not
+// per pair); the walk structures (cell offsets in a grid-stride loop over the
+// grid's ncells + 2 entries); dt; and per-workgroup partials of the argmax of
+// |a_prev|^2 over the fluid with the engine's expression (contraction off).
+// An inactive step is recorded as such only.
 __global__ void k_tape_pair(const Ctrl *ctrl, State s, State o, const int *perm, const int2 *cxy,
-                            const int *cellStart, int n, TapeSlot T, TapeHdr *hdr, long long cap, double *pv,
-                            int *pi) {
+                            const int *cellStart, int n, TapeSlot T, TapeHdr *hdr, long long cap, Phys ph,
+                            double *pv, int *pi) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int act = ctrl->active;
@@ -299,17 +298,25 @@ __global__ void k_tape_pair(const Ctrl *ctrl, State s, State o, const int *perm,
             T.axp[i] = ax;
             T.ayp[i] = ay;
             T.drhop[i] = o.drho[q];
-            T.xh[i] = s.x[t];
-            T.yh[i] = s.y[t];
-            T.vxh[i] = s.vx[t];
-            T.vyh[i] = s.vy[t];
-            T.rhoh[i] = s.rho[t];
+            const double vx = s.vx[t], vy = s.vy[t], rho = s.rho[t], m = s.m[t], h = s.h[t];
+            const int8_t lab = s.label[t];
+            T.vxh[i] = vx;
+            T.vyh[i] = vy;
             T.ax[i] = s.ax[t];
             T.ay[i] = s.ay[t];
             T.drho[i] = s.drho[t];
+            T.slot[i] = t;
             T.id[t] = i;
             T.cxy[t] = cxy[t];
-            if (s.label[t] == 0) amax_merge(v, id, ax * ax + ay * ay, i);
+            T.pa[t] = make_double4(s.x[t], s.y[t], h, lab == 0 ? m : -(double)lab);
+            T.pb[t] = make_double4(vx, vy, rho, m);
+            if (lab == 0) {
+                const Side sd = make_side(h, rho, m, ph);
+                T.ps[t] = make_double4(sd.slf, sd.dslf, sd.cslf, 0.0);
+                amax_merge(v, id, ax * ax + ay * ay, i);
+            } else {
+                T.ps[t] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
         }
     }
 #pragma unroll
@@ -336,25 +343,25 @@ struct Adj {
     double *x, *y, *vx, *vy, *rho, *ax, *ay, *drho;
 };
 
-// Corrector (PEC) / Euler update, reversed. Id order, fluid only.
+// Corrector (PEC) / Euler update, reversed. Id order, fluid only; the pair
+// pass's adjoint inputs (lam, kap) go to the step's cell-slot order.
 __global__ void k_adj_correct(const TapeHdr *hdr, TapeSlot T, const int8_t *label, int n, Adj A, double damping,
-                              int euler, double *lamx, double *lamy, double *kap, double *vbx, double *vby,
-                              double *part) {
+                              int euler, double4 *lk, double *vbx, double *vby, double *part) {
     if (!hdr->active) return;
     const double dt = hdr->dt, e = 1.0 + 0.5 * damping;
     double v[kAdjQ] = {0.0, 0.0, 0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (label[i] != 0) {
-            lamx[i] = lamy[i] = kap[i] = vbx[i] = vby[i] = 0.0;
+            lk[T.slot[i]] = make_double4(0.0, 0.0, 0.0, 0.0);
+            vbx[i] = vby[i] = 0.0;
             A.x[i] = A.y[i] = A.vx[i] = A.vy[i] = A.rho[i] = A.ax[i] = A.ay[i] = A.drho[i] = 0.0;
             continue;
         }
         const double xb = A.x[i], yb = A.y[i], vxb = A.vx[i], vyb = A.vy[i], rb = A.rho[i];
         if (euler) {
             // x' = x + dt v + dt^2/2 a, v' = v + dt a, rho' = rho + dt drho (pair at the start state)
-            lamx[i] = A.ax[i] + dt * vxb + 0.5 * dt * dt * xb;
-            lamy[i] = A.ay[i] + dt * vyb + 0.5 * dt * dt * yb;
-            kap[i] = A.drho[i] + dt * rb;
+            lk[T.slot[i]] = make_double4(A.ax[i] + dt * vxb + 0.5 * dt * dt * xb, A.ay[i] + dt * vyb + 0.5 * dt * dt * yb,
+                                         A.drho[i] + dt * rb, 0.0);
             vbx[i] = 0.0;
             vby[i] = 0.0;
             A.vx[i] = vxb + dt * xb;
@@ -362,9 +369,7 @@ __global__ void k_adj_correct(const TapeHdr *hdr, TapeSlot T, const int8_t *labe
             const double vx = T.vx0[i], vy = T.vy0[i], ax = T.ax[i], ay = T.ay[i];
             v[0] += xb * (vx + dt * ax) + yb * (vy + dt * ay) + vxb * ax + vyb * ay + rb * T.drho[i];
         } else {
-            lamx[i] = A.ax[i] + dt / e * vxb;
-            lamy[i] = A.ay[i] + dt / e * vyb;
-            kap[i] = A.drho[i] + dt * rb;
+            lk[T.slot[i]] = make_double4(A.ax[i] + dt / e * vxb, A.ay[i] + dt / e * vyb, A.drho[i] + dt * rb, 0.0);
             vbx[i] = dt * xb;
             vby[i] = dt * yb;
             A.vx[i] = (2.0 / e - 1.0) * vxb;
@@ -386,25 +391,27 @@ __global__ void k_adj_correct(const TapeHdr *hdr, TapeSlot T, const int8_t *labe
 constexpr int kAdjLanes = 8;
 
 template <int KERNEL>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_pair(const TapeHdr *hdr, TapeSlot T, const double *m,
-                                                         const double *h, const int8_t *label, int n, Phys ph,
-                                                         const double *lamx, const double *lamy, const double *kap,
-                                                         double *gx, double *gy, double *gvx, double *gvy,
-                                                         double *grho, double *part) {
+__global__ __launch_bounds__(kAdjBlock) void k_adj_pair(const TapeHdr *hdr, TapeSlot T, int n, Phys ph,
+                                                         const double4 *lk, double *gx, double *gy, double *gvx,
+                                                         double *gvy, double *grho, double *part) {
     double red[kAdjQ] = {0.0, 0.0, 0.0, 0.0};
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = (int)(gt / kAdjLanes);  // uniform within a lane group
     const int l = (int)(gt % kAdjLanes);
     const bool live = hdr->active && t < n;
     int k = -1;
-    if (live) k = T.id[t];
+    double4 ak = make_double4(0.0, 0.0, 0.0, -1.0);
+    if (live) {
+        k = T.id[t];
+        ak = T.pa[t];
+    }
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // xbar, ybar, vxbar, vybar, rhobar
-    if (live && label[k] == 0) {
+    if (live && ak.w >= 0.0) {
         const Grid g = hdr->grid;
         const int ncx = (int)g.ncx, ncy = (int)g.ncy;
-        const double xk = T.xh[k], yk = T.yh[k], vxk = T.vxh[k], vyk = T.vyh[k];
-        const Side K = make_side(h[k], T.rhoh[k], m[k], ph);
-        const double lkx = lamx[k], lky = lamy[k], kk = kap[k];
+        const double4 bk = T.pb[t], sk = T.ps[t], lmk = lk[t];
+        const double xk = ak.x, yk = ak.y, hk = ak.z, vxk = bk.x, vyk = bk.y;
+        const Side K{hk, bk.z, bk.w, sk.x, sk.y, sk.z};
         double ga = 0.0, gc = 0.0;
         const int2 c = T.cxy[t];
         const double qmax = ph.qmax;
@@ -418,19 +425,21 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_pair(const TapeHdr *hdr, Tape
                 const int s0 = T.cellStart[cell], s1 = T.cellStart[cell + 1];
                 for (int u = s0 + l; u < s1; u += kAdjLanes) {
                     if (u == t) continue;
-                    const int j = T.id[u];
-                    const double rx = xk - T.xh[j], ry = yk - T.yh[j];
+                    const double4 aj = T.pa[u];
+                    const double rx = xk - aj.x, ry = yk - aj.y;
                     const double d = sqrt(rx * rx + ry * ry);
-                    if (!(d / (0.5 * (h[k] + h[j])) <= qmax)) continue;  // the forward's neighbour rule
-                    if (label[j] != 0) {
-                        wall_vjp(rx, ry, lkx, lky, ph, acc[0], acc[1]);
+                    const double hh = 0.5 * (hk + aj.z);
+                    if (!(d / hh <= qmax)) continue;  // the forward's neighbour rule
+                    if (aj.w < 0.0) {
+                        wall_vjp(rx, ry, lmk.x, lmk.y, ph, acc[0], acc[1]);
                         continue;
                     }
                     double G, Gp;
-                    if (!grad_factor<KERNEL>(d, 0.5 * (h[k] + h[j]), G, Gp)) continue;  // outside the support
-                    const Side J = make_side(h[j], T.rhoh[j], m[j], ph);
-                    pair_vjp2(rx, ry, d, G, Gp, vxk - T.vxh[j], vyk - T.vyh[j], K, J, lkx, lky, kk, lamx[j],
-                              lamy[j], kap[j], ph, acc, ga, gc);
+                    if (!grad_factor<KERNEL>(d, hh, G, Gp)) continue;  // outside the support
+                    const double4 bj = T.pb[u], sj = T.ps[u], lmj = lk[u];
+                    const Side J{aj.z, bj.z, bj.w, sj.x, sj.y, sj.z};
+                    pair_vjp2(rx, ry, d, G, Gp, vxk - bj.x, vyk - bj.y, K, J, lmk.x, lmk.y, lmk.z, lmj.x, lmj.y,
+                              lmj.z, ph, acc, ga, gc);
                 }
             }
         }
@@ -443,7 +452,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_pair(const TapeHdr *hdr, Tape
 #pragma unroll
         for (int q = 0; q < 5; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
     if (live && l == 0) {
-        if (label[k] == 0) {
+        if (ak.w >= 0.0) {
             gx[k] = acc[0];
             gy[k] = acc[1];
             gvx[k] = acc[2];
@@ -559,8 +568,8 @@ inline int redblk_(long long n) {
 int hipsph::tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s) {
     const int R = nblk_(c->n, kAdjBlock);
     k_tape_pair<<<R, kAdjBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->buf[1 - c->cur], c->perm, c->cxy_s,
-                                        c->cellStart, c->n, t->slot[slot], t->hdr + slot, t->cells, t->amax_v,
-                                        t->amax_i);
+                                        c->cellStart, c->n, t->slot[slot], t->hdr + slot, t->cells,
+                                        make_phys(c->P), t->amax_v, t->amax_i);
     k_tape_amax_final<<<1, 1024, 0, s>>>(t->amax_v, t->amax_i, R, t->hdr + slot);
     HIPCHK(hipGetLastError());
     return 0;
@@ -581,12 +590,12 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     t->ctx = ctx;
     t->nslots = nslots;
     t->cells = ctx->cap_cells + 2;
-    // per slot: 13 double arrays, id, cxy, cellStart
-    // (each carved array is rounded up to 256 bytes: 16 per slot, 16 fixed)
-    const size_t per = n * (13 * sizeof(double) + sizeof(int) + sizeof(int2)) + (size_t)t->cells * sizeof(int) +
-                       16 * 256;
+    // per slot: 10 double arrays, 3 double4 arrays, slot, id, cxy, cellStart
+    // (each carved array is rounded up to 256 bytes: 20 per slot, 20 fixed)
+    const size_t per = n * (10 * sizeof(double) + 3 * sizeof(double4) + 2 * sizeof(int) + sizeof(int2)) +
+                       (size_t)t->cells * sizeof(int) + 20 * 256;
     const size_t npart = (size_t)kAdjQ * (2 * kRedBlocks + nblk_((long long)ctx->n * kAdjLanes, kAdjBlock));
-    const size_t fixed = n * (2 * sizeof(double) + 1) + n * 10 * sizeof(double) + npart * sizeof(double) +
+    const size_t fixed = n + n * 7 * sizeof(double) + n * sizeof(double4) + npart * sizeof(double) +
                          sizeof(TapeHdr) * nslots + nblk_(ctx->n, kAdjBlock) * (sizeof(double) + sizeof(int)) + 20 * 256 + 4096;
     if (hipMalloc(&t->arena, per * nslots + fixed) != hipSuccess) {
         delete t;
@@ -604,19 +613,20 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     t->force_steps = (int *)take(sizeof(int));
     t->amax_v = (double *)take(nblk_(ctx->n, kAdjBlock) * sizeof(double));
     t->amax_i = (int *)take(nblk_(ctx->n, kAdjBlock) * sizeof(int));
-    t->m = (double *)take(n * sizeof(double));
-    t->h = (double *)take(n * sizeof(double));
     t->label = (int8_t *)take(n);
-    double **scr[] = {&t->lamx, &t->lamy, &t->kap, &t->vbx, &t->vby, &t->gx, &t->gy, &t->gvx, &t->gvy, &t->grho};
+    double **scr[] = {&t->vbx, &t->vby, &t->gx, &t->gy, &t->gvx, &t->gvy, &t->grho};
     for (double **d : scr) *d = (double *)take(n * sizeof(double));
+    t->lk = (double4 *)take(n * sizeof(double4));
     t->slot.resize(nslots);
     t->rec_exec.assign(2 * (size_t)nslots, nullptr);
     t->adj_exec.assign(nslots, nullptr);
     for (int k = 0; k < nslots; ++k) {
         TapeSlot &S = t->slot[k];
-        double **f[] = {&S.xh, &S.yh, &S.vxh, &S.vyh, &S.rhoh, &S.ax, &S.ay, &S.drho, &S.vx0, &S.vy0,
-                        &S.axp, &S.ayp, &S.drhop};
+        double **f[] = {&S.vxh, &S.vyh, &S.ax, &S.ay, &S.drho, &S.vx0, &S.vy0, &S.axp, &S.ayp, &S.drhop};
         for (double **d : f) *d = (double *)take(n * sizeof(double));
+        double4 **f4[] = {&S.pa, &S.pb, &S.ps};
+        for (double4 **d : f4) *d = (double4 *)take(n * sizeof(double4));
+        S.slot = (int *)take(n * sizeof(int));
         S.id = (int *)take(n * sizeof(int));
         S.cxy = (int2 *)take(n * sizeof(int2));
         S.cellStart = (int *)take((size_t)t->cells * sizeof(int));
@@ -632,7 +642,7 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
         delete t;
         return fail(SPH_EHIP, "tape init failed");
     }
-    k_tape_static<<<nblk_(ctx->n, kAdjBlock), kAdjBlock>>>(ctx->buf[ctx->cur], ctx->n, t->m, t->h, t->label);
+    k_tape_static<<<nblk_(ctx->n, kAdjBlock), kAdjBlock>>>(ctx->buf[ctx->cur], ctx->n, t->label);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         hipFree(t->arena);
         delete t;
@@ -690,23 +700,20 @@ int hipsph::adjoint_launch(sph_ctx *ctx, sph_tape *tape, int slot, const sph_adj
     Adj A{adj->x, adj->y, adj->vx, adj->vy, adj->rho, adj->ax, adj->ay, adj->drho};
     const Phys ph = make_phys(ctx->P);
     const int euler = ctx->P.integrator == SPH_INTEGRATOR_EULER;
-    k_adj_correct<<<R, kAdjBlock, 0, s>>>(H, T, tape->label, n, A, ctx->P.damping, euler, tape->lamx, tape->lamy,
-                                           tape->kap, tape->vbx, tape->vby, p1);
+    k_adj_correct<<<R, kAdjBlock, 0, s>>>(H, T, tape->label, n, A, ctx->P.damping, euler, tape->lk, tape->vbx,
+                                           tape->vby, p1);
     switch (ctx->P.kernel) {
         case SPH_KERNEL_GAUSSIAN:
-            k_adj_pair<SPH_KERNEL_GAUSSIAN><<<Rp, kAdjBlock, 0, s>>>(H, T, tape->m, tape->h, tape->label, n, ph,
-                                                                     tape->lamx, tape->lamy, tape->kap, tape->gx,
-                                                                     tape->gy, tape->gvx, tape->gvy, tape->grho, p2);
+            k_adj_pair<SPH_KERNEL_GAUSSIAN><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, tape->gx, tape->gy,
+                                                                     tape->gvx, tape->gvy, tape->grho, p2);
             break;
         case SPH_KERNEL_WENDLAND:
-            k_adj_pair<SPH_KERNEL_WENDLAND><<<Rp, kAdjBlock, 0, s>>>(H, T, tape->m, tape->h, tape->label, n, ph,
-                                                                     tape->lamx, tape->lamy, tape->kap, tape->gx,
-                                                                     tape->gy, tape->gvx, tape->gvy, tape->grho, p2);
+            k_adj_pair<SPH_KERNEL_WENDLAND><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, tape->gx, tape->gy,
+                                                                     tape->gvx, tape->gvy, tape->grho, p2);
             break;
         default:
-            k_adj_pair<SPH_KERNEL_CUBIC><<<Rp, kAdjBlock, 0, s>>>(H, T, tape->m, tape->h, tape->label, n, ph,
-                                                                  tape->lamx, tape->lamy, tape->kap, tape->gx,
-                                                                  tape->gy, tape->gvx, tape->gvy, tape->grho, p2);
+            k_adj_pair<SPH_KERNEL_CUBIC><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, tape->gx, tape->gy,
+                                                                  tape->gvx, tape->gvy, tape->grho, p2);
     }
     k_adj_predict<<<R, kAdjBlock, 0, s>>>(H, T, tape->label, n, A, ctx->P.damping, euler, tape->vbx, tape->vby,
                                            tape->gx, tape->gy, tape->gvx, tape->gvy, tape->grho, p3);

@@ -48,13 +48,19 @@ struct TapeHdr {
 // One recorded step. Per-particle values are stored in reference index
 // order (id); the walk structures in the cell order of the step's pair pass.
 struct TapeSlot {
-    double *xh, *yh, *vxh, *vyh, *rhoh;  // pair inputs: the predicted state
-    double *ax, *ay, *drho;              // pair outputs
-    double *vx0, *vy0;                   // velocity at the start of the step
-    double *axp, *ayp, *drhop;           // rates carried into the step (previous pair pass)
-    int *id;                             // cell slot -> id
-    int2 *cxy;                           // cell of each slot
-    int *cellStart;                      // cell offsets [0 .. ncells + 1]
+    // id order
+    double *vxh, *vyh;          // pair inputs: the predicted velocity
+    double *ax, *ay, *drho;     // pair outputs
+    double *vx0, *vy0;          // velocity at the start of the step
+    double *axp, *ayp, *drhop;  // rates carried into the step (previous pair pass)
+    int *slot;                  // id -> cell slot
+    // cell-slot order (the step's walk order, read by the pair adjoint)
+    double4 *pa;     // (x, y, h, w) of the predicted state; w = m for fluid, -label otherwise
+    double4 *pb;     // (vx, vy, rho, m) of the predicted state
+    double4 *ps;     // (p / rho^2, its rho- and c0-derivatives, 0) for fluid, 0 otherwise
+    int *id;         // cell slot -> id
+    int2 *cxy;       // cell of each slot
+    int *cellStart;  // cell offsets [0 .. ncells + 1]
 };
 
 }  // namespace hipsph
@@ -68,11 +74,12 @@ struct sph_tape {
     hipsph::TapeHdr *hdr = nullptr;  // device, one per slot
     std::vector<hipsph::TapeSlot> slot;
     // static per-particle data in id order
-    double *m = nullptr, *h = nullptr;
     int8_t *label = nullptr;
-    // reverse-pass scratch in id order
-    double *lamx = nullptr, *lamy = nullptr, *kap = nullptr, *vbx = nullptr, *vby = nullptr;
+    // reverse-pass scratch: id order, and (lam_x, lam_y, kap, 0) in the cell
+    // slot order of the step being reversed
+    double *vbx = nullptr, *vby = nullptr;
     double *gx = nullptr, *gy = nullptr, *gvx = nullptr, *gvy = nullptr, *grho = nullptr;
+    double4 *lk = nullptr;
     double *part = nullptr;  // reduction partials
     double *amax_v = nullptr;  // argmax partials (value, id) per workgroup
     int *amax_i = nullptr;
