@@ -46,31 +46,41 @@ struct AddOp {
     __device__ double operator()(double a, double b) const { return a + b; }
 };
 
-// Fold NQ quantities over a whole workgroup with one LDS round (the
-// single-workgroup finalisers are latency-bound: one barrier instead of one
-// per quantity). kind: 0 min, 1 max, 2 sum. The result is valid in every
-// thread. lds holds NQ * 16 doubles.
+__device__ inline double fold_op(int kind, double a, double b) {
+    return kind == 0 ? (b < a ? b : a) : kind == 1 ? (b > a ? b : a) : a + b;
+}
+
+// Fold NQ quantities over a whole workgroup (up to 16 wavefronts): a shuffle
+// tree inside each wavefront, one LDS round, then wavefront 0 folds the
+// per-wave values of four quantities at a time, one quantity per group of 16
+// lanes (the single-workgroup finalisers are latency-bound: no serial loop
+// over the waves). kind: 0 min, 1 max, 2 sum; every sum folded here is a
+// count, exact in any order. The result is valid in thread 0. lds holds
+// NQ * 16 doubles.
 template <int NQ>
 __device__ inline void block_fold(double (&v)[NQ], const int (&kind)[NQ], double *lds) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double u = __shfl_xor(v[q], o, 64);
-            v[q] = kind[q] == 0 ? (u < v[q] ? u : v[q]) : kind[q] == 1 ? (u > v[q] ? u : v[q]) : v[q] + u;
-        }
+        for (int o = 32; o > 0; o >>= 1) v[q] = fold_op(kind[q], v[q], __shfl_xor(v[q], o, 64));
         if (lane == 0) lds[q * 16 + w] = v[q];
     }
     __syncthreads();
+    if (w != 0) return;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double acc = lds[q * 16];
-        for (int k = 1; k < nw; ++k) {
-            const double u = lds[q * 16 + k];
-            acc = kind[q] == 0 ? (u < acc ? u : acc) : kind[q] == 1 ? (u > acc ? u : acc) : acc + u;
-        }
-        v[q] = acc;
+    for (int q0 = 0; q0 < NQ; q0 += 4) {
+        const int q = q0 + (lane >> 4), ww = lane & 15;
+        int kd = 2;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) kd = q == j ? kind[j] : kd;
+        double x = kd == 0 ? INFINITY : (kd == 1 ? -INFINITY : 0.0);
+        if (q < NQ && ww < nw) x = lds[q * 16 + ww];
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) x = fold_op(kd, x, __shfl_xor(x, o, 64));
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (q0 + j < NQ) v[q0 + j] = __shfl(x, j * 16, 64);
     }
 }
 

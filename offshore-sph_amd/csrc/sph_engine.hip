@@ -92,6 +92,11 @@ __global__ void k_dt_partial(const int8_t *label, const double *h, const double 
 // (:409) move here, so a captured step needs no host. One workgroup.
 __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *hist, const double *part, int R) {
     __shared__ double lds[kDtQ * 16];
+    // control fields are loaded with the partials (used after the fold)
+    unsigned flags = ctrl->flags;
+    const double t0 = ctrl->t, duration = ctrl->duration, fixed_dt = ctrl->fixed_dt, co = ctrl->co;
+    const long long step = ctrl->step;
+    const int advance_t = ctrl->advance_t;
     const int kind[kDtQ] = {0, 1, 2, 2};
     double v[kDtQ] = {INFINITY, -INFINITY, 0.0, 0.0};
     for (int k = threadIdx.x; k < R; k += blockDim.x) {
@@ -107,37 +112,38 @@ __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *h
     if (threadIdx.x != 0) return;
     const double mn = v[0], mx = v[1], cnt = v[2], bad = v[3];
     const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID;
-    if ((ctrl->flags & halt) || !(ctrl->t < ctrl->duration)) {
+    if ((flags & halt) || !(t0 < duration)) {
         ctrl->active = 0;
         return;
     }
     if (cnt == 0) {
-        ctrl->flags |= SPH_FLAG_NO_FLUID;
+        ctrl->flags = flags | SPH_FLAG_NO_FLUID;
         ctrl->active = 0;
         return;
     }
-    if (bad != 0) ctrl->flags |= SPH_FLAG_NONFINITE;
+    if (bad != 0) flags |= SPH_FLAG_NONFINITE;
     double dt, c, f;
-    if (ctrl->fixed_dt > 0.0) {
-        dt = ctrl->fixed_dt;  // src/Solver.py:370-372
+    if (fixed_dt > 0.0) {
+        dt = fixed_dt;  // src/Solver.py:370-372
         c = 0.0;
         f = 0.0;
     } else {
-        double max_c = ctrl->co;  // c = co for every fluid particle (WCSPH.py:110-129)
+        const double max_c = co;  // c = co for every fluid particle (WCSPH.py:110-129)
         c = gamma_c * mn / max_c;
         f = (mx < 1e-12) ? 1e10 : gamma_f * sqrt(mn / mx);
         dt = (f < c) ? f : c;
-        if (dt < 1e-6) ctrl->flags |= SPH_FLAG_DT_SMALL;  // src/Solver.py:220-224
+        if (dt < 1e-6) flags |= SPH_FLAG_DT_SMALL;  // src/Solver.py:220-224
     }
-    long long k = ctrl->step % SPH_DT_HISTORY;
+    const long long k = step % SPH_DT_HISTORY;
     hist[3 * k + 0] = dt;
     hist[3 * k + 1] = c;
     hist[3 * k + 2] = f;
+    ctrl->flags = flags;
     ctrl->dt = dt;
     ctrl->dt_c = c;
     ctrl->dt_f = f;
-    if (ctrl->advance_t) ctrl->t = ctrl->t + dt;
-    ctrl->step = ctrl->step + 1;
+    if (advance_t) ctrl->t = t0 + dt;
+    ctrl->step = step + 1;
     ctrl->active = 1;
 }
 
@@ -331,10 +337,10 @@ __device__ bool derive_grid(Ctrl *ctrl, double scale, long long cap, const doubl
     return true;
 }
 
-// Fold the R bounds partials and derive the grid. One workgroup; every load
-// is issued before the fold.
+// Fold the R bounds partials and derive the grid. One workgroup; every load,
+// the active flag's included, is issued before the fold.
 __global__ void k_grid(Ctrl *ctrl, const double *part, int R, double scale, long long cap) {
-    if (!ctrl->active) return;
+    const int active = ctrl->active;
     __shared__ double lds[kBoundsQ * 16];
     const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
     double v[kBoundsQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
@@ -347,7 +353,7 @@ __global__ void k_grid(Ctrl *ctrl, const double *part, int R, double scale, long
             v[q] = kind[q] == 0 ? (u[q] < v[q] ? u[q] : v[q]) : kind[q] == 1 ? (u[q] > v[q] ? u[q] : v[q]) : v[q] + u[q];
     }
     block_fold<kBoundsQ>(v, kind, lds);
-    if (threadIdx.x == 0) derive_grid(ctrl, scale, cap, v);
+    if (threadIdx.x == 0 && active) derive_grid(ctrl, scale, cap, v);
 }
 
 // _bin :129-141: cid = floor((x - xmin) / cs) per axis (:146-176), flat key
