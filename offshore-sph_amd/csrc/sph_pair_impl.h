@@ -34,7 +34,16 @@ namespace hipsph {
 namespace {
 
 constexpr int kPairThreads = 256;
-constexpr int kListLen = 32;  // list entries per thread before a flush
+constexpr int kListLen = 32;  // list entries per thread before a flush (largest case)
+// The float pass with a 2h support (~21 contributors) keeps 24 entries:
+// 24 KB of LDS per workgroup lets 6 workgroups share a CU instead of 5
+// (-3 % on the 1M dam break). With 16, lanes flush mid-walk at different
+// candidates and the pass takes twice as long; the double pass is
+// register-bound at 4 workgroups per CU either way.
+template <typename R, int KERNEL>
+__device__ constexpr int list_len() {
+    return (sizeof(R) == 4 && KERNEL != SPH_KERNEL_GAUSSIAN) ? 24 : kListLen;
+}
 constexpr double kPI = 3.141592653589793;
 constexpr int kFused = 0, kContinuity = 1, kMomentum = 2;
 
@@ -286,7 +295,7 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
             ++nlist;
             if constexpr (KERNEL != SPH_KERNEL_GAUSSIAN) {
                 // 2h support: ~21 contributors, the list rarely fills
-                if (nlist == kListLen) {
+                if (nlist == list_len<R, KERNEL>()) {
                     flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
                     nlist = 0;
                 }
@@ -361,7 +370,7 @@ __device__ inline void pair_target(const PairArgs<R> &P, const Phys &ph, int ncx
 // cost ~11% (measured on the 1M dam break).
 template <typename R, int KERNEL, bool EXACT, int MODE>
 __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArgs<R> P, Phys ph) {
-    __shared__ unsigned s_list[kListLen * kPairThreads];
+    __shared__ unsigned s_list[list_len<R, KERNEL>() * kPairThreads];
     if (!ctrl->active) return;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
