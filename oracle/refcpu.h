@@ -45,9 +45,20 @@
  *                         kernels + the invariants above
  *   solver step / run     the pieces above + mask test (walls fixed, fluid
  *                         only integrated)
- *   XSPH, summation density and dynamic h inside ref_loop / ref_solver_step:
- *                         PARITY UNPINNED (no reference test, no check here;
- *                         not on the device path).
+ *   XSPH (ref_xsph, the   dense-pair model in the all-pairs form of the
+ *   loop's velocity)      reference's momentum test (test/test_numba_momentum
+ *                         .py:117-123), walls included; Galilean invariance,
+ *                         momentum conservation, linearity in epsilon; zero
+ *                         without XSPH
+ *   summation density     dense-pair model (fluid neighbours, self included),
+ *                         lattice normalisation of all three kernels, the
+ *                         pressure of the loop taken from the old density,
+ *                         zero continuity, momentum with the new density
+ *   pec XSPH branch       reduces bit for bit to the plain branch when the
+ *                         XSPH velocity equals the one that branch uses
+ *   dynamic h inside ref_loop / ref_solver_step: PARITY UNPINNED (no
+ *                         reference test, no check here; not on the device
+ *                         path).
  */
 #ifndef REFCPU_H
 #define REFCPU_H

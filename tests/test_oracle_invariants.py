@@ -427,3 +427,229 @@ def test_torch_dense_model_matches_oracle_trajectory():
     for got, k in ((x, 'x'), (y, 'y'), (vx, 'vx'), (vy, 'vy'), (rho, 'rho')):
         want = o.pA[k][f]
         assert np.max(np.abs(got.numpy() - want)) <= 1e-11 * max(1.0, np.max(np.abs(want))), k
+
+
+# ---------------------------------------------------------------------------
+# XSPH and summation density (src/Equations/XSPH.py, SummationDensity.py;
+# src/Tools/SolverTools.py:129-140,173). The reference has no test of either
+# on the loop; its momentum test keeps an older all-pairs form of the XSPH
+# term, eta * sum m_j / rho_ij * W_ij * (v_j - v_i) with eta = 0.5
+# (test/test_numba_momentum.py:117-123), which the dense model below follows.
+# ---------------------------------------------------------------------------
+def dense_w(kernel, r, hij):
+    q = r / hij
+    if kernel == 'cubic':
+        a = 10.0 / (7.0 * np.pi) / (hij * hij)
+        return np.where(q > 2.0, 0.0, np.where(q > 1.0, a * 0.25 * (2.0 - q) ** 3,
+                                               a * (1.0 - 1.5 * q * q * (1.0 - 0.5 * q))))
+    if kernel == 'gaussian':
+        return np.where(q <= 3.0, np.exp(-q * q) / (np.pi * hij * hij), 0.0)
+    a = 9.0 / (4.0 * np.pi) / (hij * hij)
+    inner = 1.0 - 0.5 * q
+    return np.where(q >= 2.0, 0.0, a * inner ** 6 * (35.0 / 12.0 * q * q + 3.0 * q + 1.0))
+
+
+def dense_neighbours(pA, scale=2.0):
+    """Neighbour matrix of the 3x3-cell rule (q <= 3), distances, h_ij."""
+    x, y, h = pA['x'], pA['y'], pA['h']
+    xmin, ymin = x.min(), y.min()
+    cs = h.min() * scale
+    ncx = max(1, math.ceil((1.0 / cs) * (x.max() - xmin)))
+    ncy = max(1, math.ceil((1.0 / cs) * (y.max() - ymin)))
+    cx = np.floor((x - xmin) / cs).astype(np.int64)
+    cy = np.floor((y - ymin) / cs).astype(np.int64)
+    key = cx + ncx * cy
+    kx, ky = key % ncx, key // ncx
+    inside = (key >= 0) & (key < ncx * ncy)
+    block = inside[None, :] & (np.abs(kx[None, :] - cx[:, None]) <= 1) & (np.abs(ky[None, :] - cy[:, None]) <= 1)
+    r = np.hypot(x[:, None] - x[None, :], y[:, None] - y[None, :])
+    hij = 0.5 * (h[:, None] + h[None, :])
+    return block & (r / hij <= 3.0), r, hij
+
+
+def dense_xsph(pA, kernel, eps=0.5):
+    nb, r, hij = dense_neighbours(pA)
+    W = np.where(nb, dense_w(kernel, r, hij), 0.0)
+    rho, m = pA['rho'], pA['m']
+    fac = eps * m[None, :] / (0.5 * (rho[:, None] + rho[None, :])) * W
+    return (np.sum(fac * (pA['vx'][None, :] - pA['vx'][:, None]), axis=1),
+            np.sum(fac * (pA['vy'][None, :] - pA['vy'][:, None]), axis=1))
+
+
+def dense_summation_density(pA, kernel):
+    nb, r, hij = dense_neighbours(pA)
+    W = np.where(nb & (pA['label'][None, :] == 0), dense_w(kernel, r, hij), 0.0)
+    return np.sum(pA['m'][None, :] * W, axis=1)
+
+
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
+def test_xsph_vs_dense_pairs(kernel):
+    """XSPH over all near neighbours, walls included; xsph = v + correction."""
+    pA = _block_with_walls(21)
+    pA['m'] *= 1 + np.random.default_rng(3).uniform(-0.2, 0.2, len(pA))
+    want = dense_xsph(pA, kernel)
+    m = _M()
+    m.useXSPH = True
+    oracle.loop(pA, m, kernel)
+    f = pA['label'] == 0
+    for got, v, w in ((pA['xsphx'], pA['vx'], want[0]), (pA['xsphy'], pA['vy'], want[1])):
+        scale = np.max(np.abs(w[f]))
+        assert scale > 0
+        np.testing.assert_allclose(got[f] - v[f], w[f], rtol=0, atol=1e-12 * scale + 1e-15)
+    assert np.all(pA['xsphx'][~f] == 0.0)  # walls are not updated
+    m.useXSPH = False
+    oracle.loop(pA, m, kernel)
+    assert np.all(pA['xsphx'][f] == 0.0) and np.all(pA['xsphy'][f] == 0.0)  # WCSPH.py:188-189
+
+
+def test_xsph_galilean_invariance_and_momentum():
+    """The correction depends on v_j - v_i only, and sum_i m_i corr_i = 0 over
+    a fluid-only set with uniform h (the pair terms are antisymmetric)."""
+    pA = _block_with_walls(22)
+    pA = pA[pA['label'] == 0].copy()
+    pA['m'] *= 1 + np.random.default_rng(4).uniform(-0.3, 0.3, len(pA))
+    m = _M()
+    m.useXSPH = True
+    a = pA.copy()
+    oracle.loop(a, m, 'cubic')
+    cx, cy = a['xsphx'] - a['vx'], a['xsphy'] - a['vy']
+    scale = np.max(np.abs(cx))
+    assert scale > 0
+    assert abs(np.sum(a['m'] * cx)) < 1e-12 * np.sum(a['m'] * np.abs(cx))
+    assert abs(np.sum(a['m'] * cy)) < 1e-12 * np.sum(a['m'] * np.abs(cy))
+    b = pA.copy()
+    b['vx'] += 3.0
+    b['vy'] -= 1.5
+    oracle.loop(b, m, 'cubic')
+    np.testing.assert_allclose(b['xsphx'] - b['vx'], cx, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(b['xsphy'] - b['vy'], cy, rtol=0, atol=1e-12 * scale)
+    m.epsilon = 0.25  # linear in epsilon
+    c = pA.copy()
+    oracle.loop(c, m, 'cubic')
+    np.testing.assert_allclose(c['xsphx'] - c['vx'], 0.5 * cx, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
+def test_summation_density_vs_dense_pairs(kernel):
+    """rho_i = sum over fluid neighbours (self included) of m_j W_ij; the
+    pressure of the same loop is the one of the old density (SolverTools.py:
+    122-127 run before :129-140), continuity is 0 (WCSPH.py:200-201), and the
+    momentum equation sees the new densities."""
+    pA = _block_with_walls(23)
+    rho_old = pA['rho'].copy()
+    want_rho = dense_summation_density(pA, kernel)
+    m = _M()
+    m.useSummationDensity = True
+    b = pA.copy()
+    b['rho'] = np.where(b['label'] == 0, want_rho, b['rho'])
+    p_old = np.where(pA['label'] == 0, ((rho_old / m.rho0) ** m.gamma - 1.0) * m.B, 0.0) + m.Pb
+    oracle.loop(pA, m, kernel)
+    f = pA['label'] == 0
+    np.testing.assert_allclose(pA['rho'][f], want_rho[f], rtol=1e-13)
+    np.testing.assert_array_equal(pA['rho'][~f], rho_old[~f])
+    # numpy's pow against the oracle's left-to-right products: (rho/rho0)^7 - 1 cancels
+    np.testing.assert_allclose(pA['p'], p_old, rtol=1e-11)
+    assert np.all(pA['drho'][f] == 0.0)
+    # momentum with p from rho_old and rho_new in p / rho^2 and the AV mean
+    _, ax, ay = dense_rates_with_p(b, m, kernel, p_old)
+    scale = np.max(np.abs(ax[f]))
+    np.testing.assert_allclose(pA['ax'][f], ax[f], rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(pA['ay'][f], ay[f], rtol=0, atol=1e-12 * np.max(np.abs(ay[f])))
+
+
+def dense_rates_with_p(pA, method, kernel, p):
+    """dense_rates with a given pressure field instead of Tait(rho)."""
+    rho = pA['rho']
+    lab = pA['label']
+    x, y, h = pA['x'], pA['y'], pA['h']
+    nb, r, hij = dense_neighbours(pA)
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    q = r / hij
+    with np.errstate(divide='ignore', invalid='ignore'):
+        if kernel == 'cubic':
+            a = 10.0 / (7.0 * np.pi) / (hij * hij)
+            g = np.where(q <= 1.0, -3.0 * q * (1.0 - 0.75 * q), -0.75 * (2.0 - q) ** 2)
+            norm = np.where((q > 2.0) | (r < 1e-10), 0.0, a * g / (hij * r))
+        elif kernel == 'gaussian':
+            dwdq = -2.0 * q / (np.pi * hij * hij) * np.exp(-q * q)
+            norm = np.where((r * hij > 1e-12) & (q <= 3.0), dwdq / (r * hij), 0.0)
+        else:
+            a = 9.0 / (4.0 * np.pi) / (hij * hij)
+            inner = 1.0 - 0.5 * q
+            g = inner ** 5 * (-14.0 / 3.0) * q * (1 + 2.5 * q)
+            norm = np.where((q >= 2.0) | (r < 1e-10), 0.0, a * g / (hij * r))
+        dwx, dwy = norm * dx, norm * dy
+        vxij = pA['vx'][:, None] - pA['vx'][None, :]
+        vyij = pA['vy'][:, None] - pA['vy'][None, :]
+        W = nb & (lab[None, :] == 0)
+        prs = p / (rho * rho)
+        dv = vxij * dx + vyij * dy
+        hm = 0.5 * (h[:, None] + hij)
+        mu = hm * dv / (r * r + 0.01 * hm * hm)
+        PI = np.where(dv < 0, mu * (method.beta * mu - method.alpha * 0.5 * method.co) /
+                      (0.5 * (rho[:, None] + rho[None, :])), 0.0)
+        fac = -pA['m'][None, :] * (prs[:, None] + prs[None, :] + PI)
+        ax = np.sum(np.where(W, fac * dwx, 0.0), axis=1)
+        ay = np.sum(np.where(W, fac * dwy, 0.0), axis=1)
+        Bm = nb & (lab[None, :] != 0) & (r <= method.r0) & (r > 1e-12)
+        fr = method.r0 / r
+        lj = method.D * (fr ** method.p1 - fr ** method.p2) / (r * r)
+        fx = np.sum(np.where(Bm, lj * dx, 0.0), axis=1)
+        fy = np.sum(np.where(Bm, lj * dy, 0.0), axis=1)
+    return None, ax + fx, ay - G + fy
+
+
+def test_summation_density_lattice_normalisation():
+    """On a square lattice of spacing dx with h = 1.3 dx the interior density
+    is m / dx^2 times the lattice sum of W dx^2: within 1e-4 of 1 for the
+    cubic spline, 0.44 % for the Gaussian (its 3h support reaches past the
+    3x3 cells of size 2h, so the sum depends on the position in the cell) and
+    0.85 % for Wendland C4 at this h / dx. A wrong normalisation constant or a
+    missing self term (W(0) dx^2 / m ~ 0.27) is off by far more."""
+    dx, n = 0.02, 24
+    gx, gy = np.meshgrid(np.arange(n) * dx, np.arange(n) * dx, indexing='ij')
+    pA = _particles(gx.ravel(), gy.ravel(), 1.3 * dx, m=1000.0 * dx * dx)
+    m = _M()
+    m.useSummationDensity = True
+    for kernel, tol in (('cubic', 1e-4), ('gaussian', 5e-3), ('wendland', 1e-2)):
+        a = pA.copy()
+        oracle.loop(a, m, kernel)
+        ii = (np.abs(a['x'] - 0.5 * (n - 1) * dx) < 4 * dx) & (np.abs(a['y'] - 0.5 * (n - 1) * dx) < 4 * dx)
+        assert np.all(np.abs(a['rho'][ii] / 1000.0 - 1.0) < tol), kernel
+
+
+def test_pec_xsph_branch_reduces_to_plain_branch():
+    """PEC with XSPH moves x by the XSPH velocity (src/Integrators/PEC.py:
+    43-48,66-71): with xsph = v (the velocity the plain branch uses at that
+    point: the start velocity in the predictor, the half-step one in the
+    corrector) it gives the plain integrator bit for bit, and walls ignore
+    xsph."""
+    pA = _block_with_walls(24)
+    rng = np.random.default_rng(5)
+    pA['ax'] = rng.normal(0, 3, len(pA))
+    pA['ay'] = rng.normal(0, 3, len(pA))
+    pA['drho'] = rng.normal(0, 10, len(pA))
+    a = pA.copy()
+    a['xsphx'], a['xsphy'] = a['vx'], a['vy']
+    b = pA.copy()
+    b['xsphx'], b['xsphy'] = 7.0, -9.0  # ignored without XSPH
+    dt = 1e-4
+    oracle.pec_predict(a, dt, 0.02, useXSPH=True)
+    oracle.pec_predict(b, dt, 0.02, useXSPH=False)
+    for f in ('x', 'y', 'vx', 'vy', 'rho', 'x0', 'y0'):
+        np.testing.assert_array_equal(a[f], b[f], err_msg=f)
+    # a zero correction in the loop leaves xsph = the half-step velocity
+    a['xsphx'], a['xsphy'] = a['vx'], a['vy']
+    oracle.pec_correct(a, dt, 0.02, useXSPH=True)
+    oracle.pec_correct(b, dt, 0.02, useXSPH=False)
+    for f in ('x', 'y', 'vx', 'vy', 'rho'):
+        np.testing.assert_array_equal(a[f], b[f], err_msg=f)
+    c = pA.copy()
+    c['xsphx'] = c['vx'] + 1.0  # an XSPH velocity differing by 1 m/s moves fluid x by dt/2 more
+    d = pA.copy()
+    oracle.pec_predict(c, dt, 0.0, useXSPH=True)
+    oracle.pec_predict(d, dt, 0.0, useXSPH=False)
+    w = c['label'] != 0
+    np.testing.assert_array_equal(c['x'][w], d['x'][w])
+    np.testing.assert_allclose(c['x'][~w] - d['x'][~w], 0.5 * dt, rtol=1e-9)
