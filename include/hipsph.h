@@ -68,8 +68,12 @@ typedef struct {
     int32_t precision;       /* SPH_PREC_* */
     int32_t integrator;      /* SPH_INTEGRATOR_* */
     int32_t strict;          /* PEC(strict=...) : clamp rho at 0 */
-    int32_t use_xsph;        /* must be 0 (XSPH is not on this path) */
-    int32_t use_summation_density; /* must be 0 */
+    int32_t use_xsph;        /* 1: WCSPH.compute_velocity's XSPH velocity in the loop
+                              * (src/Methods/WCSPH.py:171-189, src/Equations/XSPH.py),
+                              * used by PEC for the positions (src/Integrators/PEC.py:
+                              * 43-48,66-71; the reference requires it on in both) */
+    int32_t use_summation_density; /* 1: density by summation before the loop's
+                              * pass (src/Tools/SolverTools.py:129-140), no continuity */
     double rho0, gamma, co, B, Pb, alpha, beta;
     double r0, D, p1, p2;    /* Lennard-Jones wall force */
     double gravity;          /* 9.81, subtracted from a_y */
@@ -80,6 +84,7 @@ typedef struct {
     double fixed_dt;         /* > 0: use this dt (Solver(timeStep=...)); else dynamic */
     double duration;         /* device-side stepping stops once t >= duration */
     int64_t cell_capacity;   /* max cells of the grid; 0 = automatic */
+    double epsilon;          /* XSPH factor (WCSPH.epsilon, 0.5; src/Methods/WCSPH.py:65) */
 } sph_params_t;
 
 /* Structure-of-arrays state in the active-particle order. Device pointers. */
@@ -87,6 +92,7 @@ typedef struct {
     double *x, *y, *vx, *vy, *rho, *ax, *ay, *drho, *m, *h;
     double *p;     /* download only: Tait pressure + Pb (src/Tools/SolverTools.py:122-127); may be NULL */
     int8_t *label;
+    double *xsphx, *xsphy; /* XSPH velocity (particle_dtype xsphx, xsphy); may be NULL (upload: 0) */
 } sph_state_t;
 
 /* ---- context ---- */

@@ -582,7 +582,7 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
         if (P.integrator != SPH_INTEGRATOR_PEC && P.integrator != SPH_INTEGRATOR_EULER)
             return fail(SPH_EINVAL, "unknown integrator %d", P.integrator);
         if (P.use_xsph || P.use_summation_density)
-            return fail(SPH_EUNSUPPORTED, "XSPH and summation density are not on the device path");
+            return fail(SPH_EUNSUPPORTED, "a batch runs the plain WCSPH step (no XSPH, no summation density)");
         if (!(P.scale > 0) || !(P.qmax > 0) || !(P.rho0 > 0))
             return fail(SPH_EINVAL, "instance %d: scale, qmax and rho0 must be positive", k);
         ntot += P.n;

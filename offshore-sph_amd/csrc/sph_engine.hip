@@ -50,6 +50,7 @@ Phys hipsph::make_phys(const sph_params_t &P) {
     ph.g = P.gravity;
     ph.qmax = P.qmax;
     ph.kernel = P.kernel;
+    ph.eps = P.epsilon;
     return ph;
 }
 
@@ -172,7 +173,7 @@ __global__ void k_ke_final(double *part, int R) {
 // ---------------------------------------------------------------------------
 // integrators (fluid only: src/Solver.py:380,396 pass pA[f_indexes])
 // ---------------------------------------------------------------------------
-__global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
+__global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict, int xs) {
     if (!ctrl->active) return;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n || s.label[j] != kFluid) return;
@@ -182,8 +183,8 @@ __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
     s.y0[j] = y;
     s.vx0[j] = vx;
     s.vy0[j] = vy;
-    s.x[j] = pec_half_pos(x, vx, dt);
-    s.y[j] = pec_half_pos(y, vy, dt);
+    s.x[j] = pec_half_pos(x, xs ? s.xsphx[j] : vx, dt);  // PEC.py:43-48
+    s.y[j] = pec_half_pos(y, xs ? s.xsphy[j] : vy, dt);
     s.vx[j] = pec_half_vel(vx, s.ax[j], dt, damping);
     s.vy[j] = pec_half_vel(vy, s.ay[j], dt, damping);
     s.rho0[j] = rho;
@@ -198,7 +199,8 @@ __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict) {
 // before the reorder at the particle's previous slot perm[j]: the same values.
 // The half-step velocity likewise comes from vx, vy or is formed again from
 // the start-of-step velocity and the previous rates there.
-__global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, int integrator, int strict,
+// xs: PEC moves x by the XSPH velocity of this step's loop (PEC.py:66-71).
+__global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, int integrator, int strict, int xs,
                           double *part) {
     if (!ctrl->active) return;
     __shared__ double lds[kDtQ * 16];
@@ -230,8 +232,8 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
                 vxh = s.vx[j];
                 vyh = s.vy[j];
             }
-            double mid_x = x0 + 0.5 * dt * vxh;
-            double mid_y = y0 + 0.5 * dt * vyh;
+            double mid_x = x0 + 0.5 * dt * (xs ? s.xsphx[j] : vxh);
+            double mid_y = y0 + 0.5 * dt * (xs ? s.xsphy[j] : vyh);
             double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
             double mid_vy = (vy0 + 0.5 * dt * ay) / (1 + 0.5 * damping);
             s.x[j] = 2 * mid_x - x0;
@@ -259,12 +261,13 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
 // ---------------------------------------------------------------------------
 // Position the neighbour update sees: the predicted one when the PEC
 // predictor is fused into the update (fluid only), else the stored one.
+// pred: 0 none, 1 PEC, 2 PEC moving x by the XSPH velocity (PEC.py:43-48).
 __device__ inline void upd_pos(const State &a, int i, int pred, double dt, double &x, double &y) {
     x = a.x[i];
     y = a.y[i];
     if (pred) {
         // the velocity is loaded with the label, not after it, and selected
-        const double vx = a.vx[i], vy = a.vy[i];
+        const double vx = pred == 2 ? a.xsphx[i] : a.vx[i], vy = pred == 2 ? a.xsphy[i] : a.vy[i];
         const bool f = a.label[i] == kFluid;
         x = f ? pec_half_pos(x, vx, dt) : x;
         y = f ? pec_half_pos(y, vy, dt) : y;
@@ -431,8 +434,9 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // the grid may find none and keep its previous rates (SolverTools.py:166-169).
 // The raw cell of each slot (for the pair pass's 3x3 walk) is formed again
 // from the position the keys were computed from.
+// pred as for upd_pos; xs: carry the XSPH velocity with the state.
 __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fp32,
-                          void *recA, void *recB, int pred, int strict, int lean) {
+                          void *recA, void *recB, int pred, int strict, int lean, int xs) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
     const int act = ctrl->active;
@@ -454,8 +458,8 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
             b.vy0[t] = vy;
             b.rho0[t] = rho;
         }
-        x = pec_half_pos(x, vx, dt);
-        y = pec_half_pos(y, vy, dt);
+        x = pec_half_pos(x, pred == 2 ? a.xsphx[s] : vx, dt);
+        y = pec_half_pos(y, pred == 2 ? a.xsphy[s] : vy, dt);
         vx = pec_half_vel(vx, ax, dt, damping);
         vy = pec_half_vel(vy, ay, dt, damping);
         rho = pec_half_rho(rho, drho, dt, strict);
@@ -479,6 +483,10 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
     b.h[t] = h;
     b.label[t] = lab;
     b.id[t] = a.id[s];
+    if (xs) {
+        b.xsphx[t] = a.xsphx[s];
+        b.xsphy[t] = a.xsphy[s];
+    }
     if (!skip || !(cc.x >= 0 && cc.x < g.ncx && cc.y >= 0 && cc.y < g.ncy && h > 0.0 && ph.qmax >= 0.0)) {
         b.ax[t] = ax;
         b.ay[t] = ay;
@@ -522,6 +530,8 @@ __global__ void k_upload(State d, sph_state_t in, int n) {
     d.m[i] = in.m[i];
     d.h[i] = in.h[i];
     d.label[i] = in.label[i];
+    d.xsphx[i] = in.xsphx ? in.xsphx[i] : 0.0;
+    d.xsphy[i] = in.xsphy ? in.xsphy[i] : 0.0;
     d.id[i] = i;
 }
 
@@ -550,6 +560,8 @@ __global__ void k_download(State s, sph_state_t out, Phys ph, int n) {
     if (out.m) out.m[i] = s.m[t];
     if (out.h) out.h[i] = s.h[t];
     if (out.label) out.label[i] = s.label[t];
+    if (out.xsphx) out.xsphx[i] = s.xsphx[t];
+    if (out.xsphy) out.xsphy[i] = s.xsphy[t];
     if (out.p) out.p[i] = tait_p(s.rho[t], s.label[t], ph.gamma, ph.B, ph.rho0, ph.Pb);
 }
 
@@ -741,8 +753,9 @@ static int check_params(const sph_params_t *P) {
         return fail(SPH_EINVAL, "unknown precision %d", P->precision);
     if (P->integrator != SPH_INTEGRATOR_PEC && P->integrator != SPH_INTEGRATOR_EULER)
         return fail(SPH_EINVAL, "unknown integrator %d", P->integrator);
-    if (P->use_xsph || P->use_summation_density)
-        return fail(SPH_EUNSUPPORTED, "XSPH and summation density are not implemented on the device path");
+    if ((P->use_xsph && !(P->epsilon == P->epsilon)) || P->use_xsph < 0 || P->use_xsph > 1 ||
+        P->use_summation_density < 0 || P->use_summation_density > 1)
+        return fail(SPH_EINVAL, "use_xsph / use_summation_density must be 0 or 1 (and epsilon finite)");
     if (!(P->scale > 0) || !(P->qmax > 0) || !(P->rho0 > 0))
         return fail(SPH_EINVAL, "scale, qmax and rho0 must be positive");
     return 0;
@@ -782,8 +795,8 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
     c->n = (int)params->n;
     const size_t n = (size_t)c->n;
     hipGetDevice(&c->dev);
-    // Two cell-ordered buffer sets: 15 double arrays + int8 label + int id.
-    const size_t nd = 15;
+    // Two cell-ordered buffer sets: 17 double arrays + int8 label + int id.
+    const size_t nd = 17;
     const size_t bytes = n * (nd * sizeof(double) + sizeof(int) + 8) + 1024;
     for (int k = 0; k < 2; ++k) {
         if (hipMalloc(&c->arena[k], bytes) != hipSuccess) {
@@ -793,7 +806,8 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
         char *p = (char *)c->arena[k];
         double **fields[] = {&c->buf[k].x,  &c->buf[k].y,   &c->buf[k].vx,  &c->buf[k].vy,  &c->buf[k].rho,
                              &c->buf[k].ax, &c->buf[k].ay,  &c->buf[k].drho, &c->buf[k].x0, &c->buf[k].y0,
-                             &c->buf[k].vx0, &c->buf[k].vy0, &c->buf[k].rho0, &c->buf[k].m, &c->buf[k].h};
+                             &c->buf[k].vx0, &c->buf[k].vy0, &c->buf[k].rho0, &c->buf[k].m, &c->buf[k].h,
+                             &c->buf[k].xsphx, &c->buf[k].xsphy};
         for (double **f : fields) {
             *f = (double *)p;
             p += n * sizeof(double);
@@ -937,6 +951,7 @@ static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipSt
 // pred != 0: the PEC predictor runs inside the update (bounds and keys see
 // the predicted positions, the reorder writes the predicted state).
 static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
+    if (pred && c->P.use_xsph) pred = 2;  // PEC with XSPH moves x by the XSPH velocity
     const int n = c->n;
     State a = c->buf[c->cur];
     State b = c->buf[1 - c->cur];
@@ -958,7 +973,7 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
                                                            c->binflag, c->bin, c->rank);
     k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_s, make_phys(c->P), n,
                                          c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict,
-                                         lean);
+                                         lean, c->P.use_xsph);
     HIPCHK(hipGetLastError());
     if (c->timing) {
         int rc = ev_record(c, c->ev_nn, c->n_nn, s);
@@ -968,17 +983,40 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
     return 0;
 }
 
+// _loop (src/Tools/SolverTools.py:120-175): the summation density first when
+// it is on (then the pass writes no continuity), the pair pass with the XSPH
+// velocity when XSPH is on. The split passes (mode 1, 2) are the plain
+// equations only.
 static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr, int mode = 0) {
+    const bool sd = c->P.use_summation_density, xs = c->P.use_xsph;
+    if (mode != 0 && (sd || xs))
+        return fail(SPH_EUNSUPPORTED, "the split passes have no XSPH or summation density form; use sph_compute");
     State st = c->buf[c->cur];
     PairIn in{c->recA, c->recB, st.h, c->cxy_s, c->cellStart};
-    PairOut out{st.ax, st.ay, st.drho, count, mode};
+    if (xs) {
+        in.M = st.m;
+        in.VX = st.vx;
+        in.VY = st.vy;
+    }
+    PairOut out{st.ax, st.ay, st.drho, count, sd ? 2 : mode};
+    if (xs) {
+        out.xsx = st.xsphx;
+        out.xsy = st.xsphy;
+    }
+    const bool f32 = c->P.precision == SPH_PREC_FP32C;
+    const Phys ph = make_phys(c->P);
     if (c->timing) {
         int rc = ev_record(c, c->ev_pair, c->n_pair, s);
         if (rc) return rc;
     }
-    hipError_t e = c->P.precision == SPH_PREC_FP32C
-                       ? launch_pair_fp32c(c->ctrl, in, out, make_phys(c->P), c->n, c->pair_threads, s)
-                       : launch_pair_fp64(c->ctrl, in, out, make_phys(c->P), c->n, c->pair_threads, s);
+    if (sd) {
+        const SdOut so{st.rho, st.drho, c->recB};
+        hipError_t e = f32 ? launch_sumden_fp32c(c->ctrl, in, so, ph, c->n, c->pair_threads, s)
+                           : launch_sumden_fp64(c->ctrl, in, so, ph, c->n, c->pair_threads, s);
+        if (e != hipSuccess) return fail(SPH_EHIP, "summation density launch: %s", hipGetErrorString(e));
+    }
+    hipError_t e = f32 ? launch_pair_fp32c(c->ctrl, in, out, ph, c->n, c->pair_threads, s)
+                       : launch_pair_fp64(c->ctrl, in, out, ph, c->n, c->pair_threads, s);
     if (e != hipSuccess) return fail(SPH_EHIP, "pair launch: %s", hipGetErrorString(e));
     if (c->timing) {
         int rc = ev_record(c, c->ev_pair, c->n_pair, s);
@@ -999,7 +1037,7 @@ static int dt_step(sph_ctx *c, hipStream_t s) {
 
 static int predict(sph_ctx *c, hipStream_t s) {
     if (c->P.integrator != SPH_INTEGRATOR_PEC) return 0;  // Euler.predict is a no-op
-    k_pec_predict<<<nblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, c->P.strict);
+    k_pec_predict<<<nblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->n, c->P.strict, c->P.use_xsph);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1009,7 +1047,7 @@ static int predict(sph_ctx *c, hipStream_t s) {
 static int correct(sph_ctx *c, hipStream_t s, bool lean = false) {
     k_correct<<<redblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->buf[1 - c->cur],
                                               lean ? c->perm : nullptr, c->n, c->P.integrator, c->P.strict,
-                                              c->part_d);
+                                              c->P.use_xsph, c->part_d);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1019,10 +1057,13 @@ static int correct(sph_ctx *c, hipStream_t s, bool lean = false) {
 static int one_step(sph_ctx *c, hipStream_t s) {
     int rc;
     const bool pec = c->P.integrator == SPH_INTEGRATOR_PEC;
+    // the lean reorder leaves the half-step velocity and density out of the
+    // state; XSPH and the summation density read them there
+    const bool lean = pec && !c->P.use_xsph && !c->P.use_summation_density;
     if ((rc = dt_step(c, s))) return rc;
-    if ((rc = nn_update(c, s, pec, pec))) return rc;
+    if ((rc = nn_update(c, s, pec, lean))) return rc;
     if ((rc = compute(c, s))) return rc;
-    if ((rc = correct(c, s, pec))) return rc;
+    if ((rc = correct(c, s, lean))) return rc;
     c->dt_fresh = true;
     return 0;
 }

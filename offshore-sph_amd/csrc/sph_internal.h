@@ -58,6 +58,7 @@ constexpr int kDtQ = 4;      // min h, max |a|^2, fluid count, non-finite count
 // One cell-ordered buffer set.
 struct State {
     double *x, *y, *vx, *vy, *rho, *ax, *ay, *drho, *x0, *y0, *vx0, *vy0, *rho0, *m, *h;
+    double *xsphx, *xsphy;  // XSPH velocity (particle_dtype xsphx, xsphy)
     int8_t *label;
     int *id;
 };
@@ -87,23 +88,37 @@ struct PairIn {
     const double *H;       // double h
     const int2 *cxy;       // raw cell coordinates (cx, cy) of each particle
     const int *cellStart;
+    // XSPH only: every particle's mass (walls included) and the half-step
+    // velocity in double
+    const double *M = nullptr, *VX = nullptr, *VY = nullptr;
 };
 
 struct PairOut {
     double *ax, *ay, *drho;
     int *count;  // optional: neighbour count per particle (cell order), -1 for walls
     int mode;    // 0 fused, 1 continuity only (drho), 2 momentum only (ax, ay)
+    double *xsx = nullptr, *xsy = nullptr;  // non-null: XSPH velocity out (fused and momentum modes)
+};
+
+// Summation density (state arrays of the current buffer and the B records).
+struct SdOut {
+    double *rho, *drho;
+    void *B;
 };
 
 struct Phys {
     double rho0, gamma, B, Pb, co, alpha, beta, r0, D, p1, p2, g, qmax;
     int kernel;
+    double eps;  // XSPH factor
 };
 
 // Launchers implemented in the pair translation units (threads: workgroup
 // size of the pair pass, 64, 128 or 256; chosen by the autotuner).
 hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
 hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
+// Summation density before the pair pass (src/Tools/SolverTools.py:129-140).
+hipError_t launch_sumden_fp64(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s);
+hipError_t launch_sumden_fp32c(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s);
 
 // One simulation of a batched ensemble (device array, one per instance).
 // Instance k owns the particle slots [off, off + n) of the concatenated
@@ -142,6 +157,7 @@ __host__ __device__ inline Phys inst_phys(const BInst &I) {
     ph.g = I.g;
     ph.qmax = I.qmax;
     ph.kernel = I.kernel;
+    ph.eps = 0.0;  // no XSPH in a batch
     return ph;
 }
 

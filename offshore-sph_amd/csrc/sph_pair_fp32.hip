@@ -19,35 +19,16 @@
 namespace hipsph {
 
 hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s) {
-    PairArgs<float> P;
-    P.A = (const Rec<float>::A *)in.A;
-    P.B = (const Rec<float>::B *)in.B;
-    P.H = in.H;
-    P.cxy = in.cxy;
-    P.cellStart = in.cellStart;
-    P.ax = out.ax;
-    P.ay = out.ay;
-    P.drho = out.drho;
-    P.count = out.count;
-    P.n = n;
-    return launch_pair<float, false>(ctrl, P, ph, threads, out.mode, s);
+    return launch_pair<float, false>(ctrl, pair_args<float>(in, &out, n), ph, threads, out.mode, s);
 }
 
+hipError_t launch_sumden_fp32c(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s) {
+    return launch_sumden<float, false>(ctrl, pair_args<float>(in, nullptr, n), out, ph, threads, s);
+}
 
-hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n, int threads,
-                  hipStream_t s) {
-    PairArgs<float> P;
-    P.A = (const Rec<float>::A *)in.A;
-    P.B = (const Rec<float>::B *)in.B;
-    P.H = in.H;
-    P.cxy = in.cxy;
-    P.cellStart = in.cellStart;
-    P.ax = out.ax;
-    P.ay = out.ay;
-    P.drho = out.drho;
-    P.count = out.count;
-    P.n = n;
-    return launch_pair_batch<float, false>(inst, instOf, P, kernel, threads, s);
+hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
+                                   int threads, hipStream_t s) {
+    return launch_pair_batch<float, false>(inst, instOf, pair_args<float>(in, &out, n), kernel, threads, s);
 }
 
 }  // namespace hipsph

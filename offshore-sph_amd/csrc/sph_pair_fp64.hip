@@ -19,35 +19,16 @@
 namespace hipsph {
 
 hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s) {
-    PairArgs<double> P;
-    P.A = (const Rec<double>::A *)in.A;
-    P.B = (const Rec<double>::B *)in.B;
-    P.H = in.H;
-    P.cxy = in.cxy;
-    P.cellStart = in.cellStart;
-    P.ax = out.ax;
-    P.ay = out.ay;
-    P.drho = out.drho;
-    P.count = out.count;
-    P.n = n;
-    return launch_pair<double, true>(ctrl, P, ph, threads, out.mode, s);
+    return launch_pair<double, true>(ctrl, pair_args<double>(in, &out, n), ph, threads, out.mode, s);
 }
 
+hipError_t launch_sumden_fp64(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s) {
+    return launch_sumden<double, true>(ctrl, pair_args<double>(in, nullptr, n), out, ph, threads, s);
+}
 
-hipError_t launch_pair_batch_fp64(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n, int threads,
-                  hipStream_t s) {
-    PairArgs<double> P;
-    P.A = (const Rec<double>::A *)in.A;
-    P.B = (const Rec<double>::B *)in.B;
-    P.H = in.H;
-    P.cxy = in.cxy;
-    P.cellStart = in.cellStart;
-    P.ax = out.ax;
-    P.ay = out.ay;
-    P.drho = out.drho;
-    P.count = out.count;
-    P.n = n;
-    return launch_pair_batch<double, true>(inst, instOf, P, kernel, threads, s);
+hipError_t launch_pair_batch_fp64(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
+                                  int threads, hipStream_t s) {
+    return launch_pair_batch<double, true>(inst, instOf, pair_args<double>(in, &out, n), kernel, threads, s);
 }
 
 }  // namespace hipsph

@@ -62,7 +62,34 @@ struct PairArgs {
     double *ax, *ay, *drho;
     int *count;  // optional diagnostic output
     int n;
+    // XSPH (XS instantiations only): masses of all particles, the double
+    // half-step velocity, and the XSPH velocity out
+    const double *M, *VX, *VY;
+    double *xsx, *xsy;
 };
+
+// PairArgs from the engine-side descriptors (out may be null for passes that
+// write no rates).
+template <typename R>
+inline PairArgs<R> pair_args(const PairIn &in, const PairOut *out, int n) {
+    PairArgs<R> P;
+    P.A = (const typename Rec<R>::A *)in.A;
+    P.B = (const typename Rec<R>::B *)in.B;
+    P.H = in.H;
+    P.cxy = in.cxy;
+    P.cellStart = in.cellStart;
+    P.ax = out ? out->ax : nullptr;
+    P.ay = out ? out->ay : nullptr;
+    P.drho = out ? out->drho : nullptr;
+    P.count = out ? out->count : nullptr;
+    P.n = n;
+    P.M = in.M;
+    P.VX = in.VX;
+    P.VY = in.VY;
+    P.xsx = out ? out->xsx : nullptr;
+    P.xsy = out ? out->xsy : nullptr;
+    return P;
+}
 
 template <typename R>
 struct Acc;
@@ -163,15 +190,68 @@ template <typename R>
 struct Target {
     double X, Y, H;            // double position and h
     R h, vx, vy, rho, slf;     // slf = p_i / rho_i^2 (Momentum.py:28)
+    int slot;                  // the target's own slot (XSPH: other particles at r = 0 count)
 };
 
 template <typename R>
 struct Sums {
     Acc<R> arho, a0, a1, f0, f1;
+    Acc<R> xs0, xs1;  // XSPH
 };
 
-// Pass 2: the pair arithmetic for candidate u.
-template <typename R, int KERNEL, int MODE>
+// Kernel values W (src/Kernels/CubicSpline.py:10-36, Gaussian.py:16-25,
+// Wendland.py:10-30) in the reference's expression order, for XSPH and the
+// summation density; zero outside the support.
+template <int KERNEL>
+__device__ inline double w_exact(double r, double h) {
+    if (KERNEL == SPH_KERNEL_GAUSSIAN) {
+        const double q = r / h;
+        if (q <= 3.0) {
+            const double alpha_c = (1.0 / kPI) / (h * h);
+            return alpha_c * exp(-q * q);
+        }
+        return 0.0;
+    } else if (KERNEL == SPH_KERNEL_WENDLAND) {
+        const double alpha = 9.0 / (4.0 * kPI) / (h * h);
+        const double q = r / h;
+        if (q >= 2.0) return 0.0;
+        const double inner = 1.0 - 0.5 * q;
+        return alpha * (ipow(inner, 6.0) * (35.0 / 12.0 * q * q + 3.0 * q + 1.0));
+    } else {
+        const double alpha = 10.0 / (7.0 * kPI) / (h * h);
+        const double q = r / h;
+        if (q > 2.0) return 0.0;
+        if (q > 1.0) return alpha * 0.25 * ipow(2.0 - q, 3.0);
+        return alpha * (1.0 - 1.5 * ipow(q, 2.0) * (1.0 - 0.5 * q));
+    }
+}
+
+// Float W with native exp / reciprocal (q = r / h, hinv = 1 / h).
+template <int KERNEL>
+__device__ inline float w_f(float q, float hinv) {
+    if (KERNEL == SPH_KERNEL_GAUSSIAN) {
+        if (!(q <= 3.0f)) return 0.0f;
+        return (float)(1.0 / kPI) * hinv * hinv * __expf(-q * q);
+    } else if (KERNEL == SPH_KERNEL_WENDLAND) {
+        if (q >= 2.0f) return 0.0f;
+        const float inner = 1.0f - 0.5f * q;
+        const float i2 = inner * inner, i6 = i2 * i2 * i2;
+        return (float)(9.0 / (4.0 * kPI)) * hinv * hinv * (i6 * ((35.0f / 12.0f) * q * q + 3.0f * q + 1.0f));
+    } else {
+        if (q > 2.0f) return 0.0f;
+        const float a = (float)(10.0 / (7.0 * kPI)) * hinv * hinv;
+        if (q > 1.0f) {
+            const float t = 2.0f - q;
+            return a * 0.25f * t * t * t;
+        }
+        return a * (1.0f - 1.5f * q * q * (1.0f - 0.5f * q));
+    }
+}
+
+// Pass 2: the pair arithmetic for candidate u. XS adds the XSPH term of
+// every listed neighbour, walls included (src/Equations/XSPH.py:26-31:
+// fac = -epsilon m_j W_ij / rho_ij times v_i - v_j).
+template <typename R, int KERNEL, int MODE, bool XS = false>
 __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int u, Sums<R> &S) {
     const typename Rec<R>::A a = P.A[u];
     if constexpr (sizeof(R) == 8) {
@@ -181,6 +261,15 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
         const double xij2 = dx * dx + dy * dy;
         const double r = sqrt(xij2);
         const double hij = 0.5 * ((double)T.h + (double)a.h);
+        if constexpr (XS) {
+            const typename Rec<R>::B bx = P.B[u];
+            const double w = w_exact<KERNEL>(r, hij);
+            const double mj = a.w >= 0 ? (double)a.w : P.M[u];
+            const double rho_ij = 0.5 * ((double)T.rho + (double)bx.rho);
+            const double fac = -ph.eps * mj * w / rho_ij;
+            S.xs0.add(fac * ((double)T.vx - (double)rec_vx(a, bx)));
+            S.xs1.add(fac * ((double)T.vy - (double)rec_vy(a, bx)));
+        }
         if (a.w >= 0) {
             const double q = r / hij;
             double dwx, dwy;
@@ -219,6 +308,15 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
         const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
         const float r = r2 * rinv;
         const float hij = 0.5f * ((float)T.h + (float)a.h);
+        if constexpr (XS) {
+            const typename Rec<R>::B bx = P.B[u];
+            const float w = w_f<KERNEL>(r * __builtin_amdgcn_rcpf(hij), __builtin_amdgcn_rcpf(hij));
+            const float mj = a.w >= 0.f ? (float)a.w : (float)P.M[u];
+            const float rho_ij = 0.5f * ((float)T.rho + (float)bx.rho);
+            const float fac = -(float)ph.eps * mj * w * __builtin_amdgcn_rcpf(rho_ij);
+            S.xs0.add(fac * ((float)T.vx - (float)rec_vx(a, bx)));
+            S.xs1.add(fac * ((float)T.vy - (float)rec_vy(a, bx)));
+        }
         if (a.w >= 0.f) {
             const float hinv = __builtin_amdgcn_rcpf(hij);
             const float q = r * hinv;
@@ -255,16 +353,16 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
     }
 }
 
-template <typename R, int KERNEL, int MODE>
+template <typename R, int KERNEL, int MODE, bool XS = false>
 __device__ inline void flush(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, const unsigned *list, int nlist,
                              Sums<R> &S) {
-    for (int j = 0; j < nlist; ++j) contribute<R, KERNEL, MODE>(P, T, ph, (int)list[j * blockDim.x], S);
+    for (int j = 0; j < nlist; ++j) contribute<R, KERNEL, MODE, XS>(P, T, ph, (int)list[j * blockDim.x], S);
 }
 
 // Pass 1 over the candidates [s, e) of one cell (descending for the reference
 // order): counts neighbours and appends contributors to this thread's list,
 // flushing it through pass 2 when full.
-template <typename R, int KERNEL, bool DESC, int MODE>
+template <typename R, int KERNEL, bool DESC, int MODE, bool XS = false>
 __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int s, int e, unsigned *list,
                                 int &nlist, Sums<R> &S) {
     constexpr double kEps = sizeof(R) == 8 ? 1e-12 : 1e-5;
@@ -281,7 +379,9 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
         const R qh = qmax * hij;
         const R lim = qh * qh;
         const R sh = ks * hij;
-        const R sup2 = a.w >= R(0) ? sh * sh * R(1.0 + kEps) : r0sq;
+        // XSPH also needs the walls inside the kernel support
+        const R sup2 = a.w >= R(0) ? sh * sh * R(1.0 + kEps)
+                                   : (XS ? fmax(sh * sh * R(1.0 + kEps), r0sq) : r0sq);
         bool in = r2 < lim * R(1.0 - kEps);
         if (!in && !(r2 > lim * R(1.0 + kEps))) {
             // within the rounding band of the cut-off: the reference's exact test
@@ -289,14 +389,16 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
             in = exact_q_test(T.X, T.Y, T.H, a.x, a.y, hj, ph.qmax);
         }
         cnt += in;
-        // r = 0 pairs contribute exactly nothing; walls add nothing to continuity
-        if (in && r2 > R(0) && !(r2 > sup2) && (MODE != kContinuity || a.w >= R(0))) {
+        // r = 0 pairs contribute exactly nothing (to XSPH they do, W(0) != 0,
+        // unless it is the target itself); walls add nothing to continuity
+        const bool apart = XS ? (r2 > R(0) || u != T.slot) : r2 > R(0);
+        if (in && apart && !(r2 > sup2) && (MODE != kContinuity || a.w >= R(0))) {
             list[nlist * blockDim.x] = (unsigned)u;
             ++nlist;
             if constexpr (KERNEL != SPH_KERNEL_GAUSSIAN) {
                 // 2h support: ~21 contributors, the list rarely fills
                 if (nlist == list_len<R, KERNEL>()) {
-                    flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
+                    flush<R, KERNEL, MODE, XS>(P, T, ph, list, nlist, S);
                     nlist = 0;
                 }
             }
@@ -307,7 +409,7 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
             // together; per-lane flushes at different candidates would run
             // the flush loop once per lane. A lane's sums keep their order.
             if (__ballot(nlist == kListLen) != 0) {
-                flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
+                flush<R, KERNEL, MODE, XS>(P, T, ph, list, nlist, S);
                 nlist = 0;
             }
         }
@@ -368,7 +470,7 @@ __device__ inline void pair_target(const PairArgs<R> &P, const Phys &ph, int ncx
 // The single-simulation pass keeps its own copy of the body: sharing
 // pair_target with the batched kernel changed its register allocation and
 // cost ~11% (measured on the 1M dam break).
-template <typename R, int KERNEL, bool EXACT, int MODE>
+template <typename R, int KERNEL, bool EXACT, int MODE, bool XS>
 __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArgs<R> P, Phys ph) {
     __shared__ unsigned s_list[list_len<R, KERNEL>() * kPairThreads];
     if (!ctrl->active) return;
@@ -391,6 +493,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArg
     T.vy = (R)rec_vy(at, bt);
     T.rho = (R)bt.rho;
     T.slf = (R)bt.prs;
+    T.slot = t;
     const int2 c = P.cxy[t];
     unsigned *list = s_list + threadIdx.x;
     int nlist = 0, cnt = 0;
@@ -406,11 +509,11 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArg
             const int cy = c.y + db;
             if (!(ncy > cy && cy > -1)) continue;
             const int k = cx + ncx * cy;
-            cnt += scan_cell<R, KERNEL, EXACT, MODE>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list, nlist,
-                                                      S);
+            cnt += scan_cell<R, KERNEL, EXACT, MODE, XS>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list,
+                                                          nlist, S);
         }
     }
-    flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
+    flush<R, KERNEL, MODE, XS>(P, T, ph, list, nlist, S);
     if (P.count) P.count[t] = cnt;
     if (cnt == 0) return;  // SolverTools.py:166-169: no neighbours, keep the previous values
     if constexpr (MODE != kMomentum) P.drho[t] = S.arho.get();
@@ -419,6 +522,102 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArg
         P.ax[t] = a0 + f0;
         P.ay[t] = (a1 - ph.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
     }
+    if constexpr (XS) {  // WCSPH.compute_velocity: xsph = v + correction (WCSPH.py:180-186)
+        const double vxi = sizeof(R) == 8 ? (double)T.vx : P.VX[t];
+        const double vyi = sizeof(R) == 8 ? (double)T.vy : P.VY[t];
+        P.xsx[t] = vxi + S.xs0.get();
+        P.xsy[t] = vyi + S.xs1.get();
+    }
+}
+
+// Summation density (src/Tools/SolverTools.py:129-140, src/Equations/
+// SummationDensity.py): for each fluid target, rho = the sum over its fluid
+// neighbours (same walk and neighbour rule as the pair pass, the target itself
+// included) of m_j W(r_ij, h_ij), in the reference's order for the double
+// pass. The loop's pressure is the one of the density before the sum
+// (:122-127 run first), so the target's record gets rho_new and
+// p(rho_old) / rho_new^2 for the pair pass that follows; only the A records
+// are read here, so rewriting B in place is safe. A target with neighbours
+// gets drho = 0 (WCSPH.compute_density_change, WCSPH.py:200-201); one with
+// none keeps its drho and, as in the reference, takes the empty sum 0 as its
+// density.
+template <typename R, int KERNEL, bool EXACT>
+__global__ __launch_bounds__(kPairThreads) void k_sumden(const Ctrl *ctrl, PairArgs<R> P, Phys ph, SdOut O) {
+    if (!ctrl->active) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const typename Rec<R>::A at = P.A[t];
+    if (!(at.w >= R(0))) return;  // fluid only
+    const Grid g = ctrl->grid;
+    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
+    const double X = at.x, Y = at.y, H = sizeof(R) == 8 ? (double)at.h : P.H[t];
+    constexpr double kEps = sizeof(R) == 8 ? 1e-12 : 1e-5;
+    const R qmax = (R)ph.qmax;
+    const R hi = (R)at.h;
+    Acc<R> rho;
+    int cnt = 0;
+    const int2 c = P.cxy[t];
+    for (int da = -1; da <= 1; ++da) {
+        const int cx = c.x + da;
+        if (!(ncx > cx && cx > -1)) continue;
+        for (int db = -1; db <= 1; ++db) {
+            const int cy = c.y + db;
+            if (!(ncy > cy && cy > -1)) continue;
+            const int k = cx + ncx * cy;
+            const int s0 = P.cellStart[k], e0 = P.cellStart[k + 1];
+            for (int i = 0; i < e0 - s0; ++i) {
+                const int u = EXACT ? e0 - 1 - i : s0 + i;
+                const typename Rec<R>::A a = P.A[u];
+                const R dx = (R)(X - a.x), dy = (R)(Y - a.y);
+                const R r2 = dx * dx + dy * dy;
+                const R hij = R(0.5) * (hi + (R)a.h);
+                const R qh = qmax * hij;
+                const R lim = qh * qh;
+                bool in = r2 < lim * R(1.0 - kEps);
+                if (!in && !(r2 > lim * R(1.0 + kEps))) {
+                    const double hj = sizeof(R) == 8 ? (double)a.h : P.H[u];
+                    in = exact_q_test(X, Y, H, a.x, a.y, hj, ph.qmax);
+                }
+                cnt += in;
+                if (!in || !(a.w >= R(0))) continue;
+                if constexpr (sizeof(R) == 8) {
+                    const double ex = X - a.x, ey = Y - a.y;
+                    const double r = sqrt(ex * ex + ey * ey);
+                    rho.add((double)a.w * w_exact<KERNEL>(r, 0.5 * (H + (double)a.h)));
+                } else {
+                    const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+                    const float hinv = __builtin_amdgcn_rcpf(hij);
+                    rho.add((float)a.w * w_f<KERNEL>(r2 * rinv * hinv, hinv));
+                }
+            }
+        }
+    }
+    const double rho_new = rho.get();
+    const double ratio = ipow(O.rho[t] / ph.rho0, ph.gamma);
+    const double p_old = (ratio - 1.0) * ph.B + ph.Pb;  // Tait + Pb of the density before the sum
+    O.rho[t] = rho_new;
+    typename Rec<R>::B *B = (typename Rec<R>::B *)O.B;
+    B[t].rho = (R)rho_new;
+    B[t].prs = (R)(p_old / (rho_new * rho_new));
+    if (cnt > 0) O.drho[t] = 0.0;
+}
+
+template <typename R, bool EXACT>
+hipError_t launch_sumden(const Ctrl *ctrl, const PairArgs<R> &P, const SdOut &O, const Phys &ph, int threads,
+                         hipStream_t s) {
+    if (threads != 64 && threads != 128 && threads != 256) threads = kPairThreads;
+    const int nb = xcd_blocks(P.n, threads);
+    switch (ph.kernel) {
+        case SPH_KERNEL_GAUSSIAN:
+            k_sumden<R, SPH_KERNEL_GAUSSIAN, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph, O);
+            break;
+        case SPH_KERNEL_WENDLAND:
+            k_sumden<R, SPH_KERNEL_WENDLAND, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph, O);
+            break;
+        default:
+            k_sumden<R, SPH_KERNEL_CUBIC, EXACT><<<nb, threads, 0, s>>>(ctrl, P, ph, O);
+    }
+    return hipGetLastError();
 }
 
 // Batched ensemble: each target walks its own instance's cells with its own
@@ -433,17 +632,17 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_batch(const BInst *inst, 
     pair_target<R, KERNEL, EXACT, kFused>(P, inst_phys(I), (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, s_list);
 }
 
-template <typename R, bool EXACT, int MODE>
+template <typename R, bool EXACT, int MODE, bool XS = false>
 void launch_pair_mode(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, int nb, int threads, hipStream_t s) {
     switch (ph.kernel) {
         case SPH_KERNEL_GAUSSIAN:
-            k_pair<R, SPH_KERNEL_GAUSSIAN, EXACT, MODE><<<nb, threads, 0, s>>>(ctrl, P, ph);
+            k_pair<R, SPH_KERNEL_GAUSSIAN, EXACT, MODE, XS><<<nb, threads, 0, s>>>(ctrl, P, ph);
             break;
         case SPH_KERNEL_WENDLAND:
-            k_pair<R, SPH_KERNEL_WENDLAND, EXACT, MODE><<<nb, threads, 0, s>>>(ctrl, P, ph);
+            k_pair<R, SPH_KERNEL_WENDLAND, EXACT, MODE, XS><<<nb, threads, 0, s>>>(ctrl, P, ph);
             break;
         default:
-            k_pair<R, SPH_KERNEL_CUBIC, EXACT, MODE><<<nb, threads, 0, s>>>(ctrl, P, ph);
+            k_pair<R, SPH_KERNEL_CUBIC, EXACT, MODE, XS><<<nb, threads, 0, s>>>(ctrl, P, ph);
     }
 }
 
@@ -469,12 +668,15 @@ template <typename R, bool EXACT>
 hipError_t launch_pair(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, int threads, int mode, hipStream_t s) {
     if (threads != 64 && threads != 128 && threads != 256) threads = kPairThreads;
     const int nb = xcd_blocks(P.n, threads);
+    const bool xs = P.xsx != nullptr;  // XSPH velocity requested (fused and momentum passes only)
     if (mode == kContinuity)
         launch_pair_mode<R, EXACT, kContinuity>(ctrl, P, ph, nb, threads, s);
     else if (mode == kMomentum)
-        launch_pair_mode<R, EXACT, kMomentum>(ctrl, P, ph, nb, threads, s);
+        xs ? launch_pair_mode<R, EXACT, kMomentum, true>(ctrl, P, ph, nb, threads, s)
+           : launch_pair_mode<R, EXACT, kMomentum>(ctrl, P, ph, nb, threads, s);
     else
-        launch_pair_mode<R, EXACT, kFused>(ctrl, P, ph, nb, threads, s);
+        xs ? launch_pair_mode<R, EXACT, kFused, true>(ctrl, P, ph, nb, threads, s)
+           : launch_pair_mode<R, EXACT, kFused>(ctrl, P, ph, nb, threads, s);
     return hipGetLastError();
 }
 

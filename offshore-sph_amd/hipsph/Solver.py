@@ -27,7 +27,9 @@ Differences from the reference, each deliberate:
   * ``p`` written back after ``run`` is the Tait pressure of the final
     density; the reference leaves the pressure of the last predicted density.
   * Not on the device path (raise ``NotImplementedError``): dynamic smoothing
-    length (``h=None``), XSPH, summation density and the coupling hooks.
+    length (``h=None``) and the coupling hooks. XSPH and summation density
+    run on the device; PEC's ``useXSPH`` must equal the method's (the
+    reference's WCSPH docstring requires both on), otherwise ``ValueError``.
   * ``save``/``load`` use h5py with the reference's dataset names when h5py
     is importable, else a ``.npz`` with the same names.
 """
@@ -125,11 +127,10 @@ class Solver:
         println('Starting setup.')
         if self.h is None:
             raise NotImplementedError('dynamic smoothing length (h=None) is not on the device path; pass h')
-        if getattr(self.method, 'useXSPH', False) or getattr(self.method, 'useSummationDensity', False):
-            raise NotImplementedError('XSPH and summation density are not on the device path')
-        if getattr(self.integrator, 'useXSPH', False):
-            raise NotImplementedError('the XSPH branch of the integrator is not on the device path '
-                                      '(construct PEC(useXSPH=False))')
+        if (self.integrator.name == 'pec' and
+                bool(getattr(self.integrator, 'useXSPH', False)) != bool(getattr(self.method, 'useXSPH', False))):
+            # src/Methods/WCSPH.py:56: "useXSPH has to be enabled in both the integrator and the method!"
+            raise ValueError('PEC(useXSPH=...) and WCSPH(useXSPH=...) must agree')
         pA = self.particleArray
         self.num_particles, self.indexes = findActive(self.num_particles, pA)
         self._masks()
@@ -173,7 +174,7 @@ class Solver:
         """Device state -> active records of particleArray."""
         d = e.download()
         idx = np.nonzero(self.indexes)[0]
-        for k in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'p'):
+        for k in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'p', 'xsphx', 'xsphy'):
             self.particleArray[k][idx] = d[k]
 
     def _steps(self, e, n):

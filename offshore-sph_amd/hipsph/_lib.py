@@ -47,11 +47,13 @@ class sph_params_t(C.Structure):
         ('gamma_c', C.c_double), ('gamma_f', C.c_double), ('damping', C.c_double),
         ('fixed_dt', C.c_double), ('duration', C.c_double),
         ('cell_capacity', C.c_int64),
+        ('epsilon', C.c_double),
     ]
 
 
 class sph_state_t(C.Structure):
-    _fields_ = [(f, C.c_void_p) for f in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p', 'label')]
+    _fields_ = [(f, C.c_void_p) for f in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p', 'label',
+                                           'xsphx', 'xsphy')]
 
 
 class sph_adjoint_t(C.Structure):

@@ -14,7 +14,7 @@ import numpy as np
 from . import _lib
 from ._lib import call, sph_params_t, sph_state_t
 
-IO_F64 = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p')
+IO_F64 = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p', 'xsphx', 'xsphy')
 
 KERNELS = {'cubic': _lib.SPH_KERNEL_CUBIC, 'gaussian': _lib.SPH_KERNEL_GAUSSIAN, 'wendland': _lib.SPH_KERNEL_WENDLAND}
 PRECISIONS = {'fp64': _lib.SPH_PREC_FP64, 'fp32c': _lib.SPH_PREC_FP32C}
@@ -56,6 +56,9 @@ class EngineParams:
     fixed_dt: float = 0.0
     duration: float = math.inf
     cell_capacity: int = 0
+    use_xsph: bool = False
+    use_summation_density: bool = False
+    epsilon: float = 0.5
 
     def to_c(self):
         p = sph_params_t()
@@ -64,19 +67,22 @@ class EngineParams:
         p.precision = PRECISIONS[self.precision] if isinstance(self.precision, str) else int(self.precision)
         p.integrator = INTEGRATORS[self.integrator] if isinstance(self.integrator, str) else int(self.integrator)
         p.strict = int(bool(self.strict))
-        p.use_xsph = 0
-        p.use_summation_density = 0
+        p.use_xsph = int(bool(self.use_xsph))
+        p.use_summation_density = int(bool(self.use_summation_density))
         for f in ('rho0', 'gamma', 'co', 'B', 'Pb', 'alpha', 'beta', 'r0', 'D', 'p1', 'p2', 'gravity', 'scale',
-                  'qmax', 'gamma_c', 'gamma_f', 'damping', 'fixed_dt', 'duration'):
+                  'qmax', 'gamma_c', 'gamma_f', 'damping', 'fixed_dt', 'duration', 'epsilon'):
             setattr(p, f, float(getattr(self, f)))
         p.cell_capacity = int(self.cell_capacity)
         return p
 
     @classmethod
     def from_method(cls, n, method, kernel='cubic', integrator='pec', precision='fp64', **kw):
-        """Take the WCSPH constants from a method object (src/Methods/WCSPH.py:35-79)."""
-        if getattr(method, 'useXSPH', False) or getattr(method, 'useSummationDensity', False):
-            raise NotImplementedError('XSPH and summation density are not on the device path')
+        """Take the WCSPH constants from a method object (src/Methods/WCSPH.py:35-79),
+        XSPH and summation density included (with PEC the integrator's useXSPH
+        must match the method's: the reference requires both on, WCSPH.py:56)."""
+        kw.setdefault('use_xsph', bool(getattr(method, 'useXSPH', False)))
+        kw.setdefault('use_summation_density', bool(getattr(method, 'useSummationDensity', False)))
+        kw.setdefault('epsilon', float(getattr(method, 'epsilon', 0.5)))
         return cls(n=n, kernel=kernel, integrator=integrator, precision=precision, rho0=method.rho0,
                    gamma=method.gamma, co=method.co, B=method.B, Pb=method.Pb, alpha=method.alpha,
                    beta=method.beta, r0=method.r0, D=method.D, p1=method.p1, p2=method.p2, **kw)
@@ -127,7 +133,7 @@ class Engine:
 
     # -- state -----------------------------------------------------------
     def upload(self, arrays, reset_clock=True):
-        """Copy x, y, vx, vy, rho, m, h, label (and ax, ay, drho if given) in."""
+        """Copy x, y, vx, vy, rho, m, h, label (and ax, ay, drho, xsphx, xsphy if given) in."""
         import torch
         need = ('x', 'y', 'vx', 'vy', 'rho', 'm', 'h', 'label')
         for k in need:
@@ -266,6 +272,7 @@ class Engine:
         return {'pair_ms': p.value, 'nn_ms': q.value, 'count': int(k.value)}
 
 
-def soa_from_particles(pA, fields=('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'label')):
+def soa_from_particles(pA, fields=('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'label', 'xsphx',
+                                   'xsphy')):
     """Structure-of-arrays view of a particle_dtype array (active records only)."""
     return {f: np.ascontiguousarray(pA[f]) for f in fields}

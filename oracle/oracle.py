@@ -172,14 +172,16 @@ class Solver:
     """The reference's Solver.run loop body on a particle array (CPU, in place)."""
 
     def __init__(self, pA, method, kernel='cubic', integrator='pec', h=None, damping=0.0, strict=False,
-                 nthreads=1, scale=2.0, gamma_c=0.25, gamma_f=0.25):
+                 nthreads=1, scale=2.0, gamma_c=0.25, gamma_f=0.25, pec_xsph=None):
+        """pec_xsph: PEC(useXSPH=...); default: the method's useXSPH (the
+        reference requires the two to agree)."""
         self.pA = as_particles(pA)
         self.m = wcsph(method)
         self.nn = NN(scale)
         s = ref_solver_t()
         s.kernel = KERNELS[kernel]
         s.integrator = INTEGRATORS[integrator]
-        s.pec_useXSPH = 0
+        s.pec_useXSPH = int(bool(getattr(method, 'useXSPH', False) if pec_xsph is None else pec_xsph))
         s.pec_strict = int(bool(strict))
         s.h_fixed = float(h) if h else 0.0
         s.damping = damping

@@ -1,0 +1,117 @@
+"""XSPH and summation density on the device against the CPU oracle.
+
+The oracle's XSPH (src/Equations/XSPH.py via WCSPH.compute_velocity), its
+summation density (src/Tools/SolverTools.py:129-140) and PEC's XSPH branch
+(src/Integrators/PEC.py:43-48,66-71) are pinned by a dense-pair model and
+invariants in tests/test_oracle_invariants.py. Here, through the C ABI:
+  * one loop on a perturbed dam break, every combination of the two: FP64
+    cubic bit-identical (XSPH velocity, summed density, rates), Gaussian and
+    Wendland FP64 to 1e-12, FP32C pairs to 2e-5 of the largest value;
+  * whole steps: FP64 cubic trajectories and dt series bit-identical over
+    200 steps for PEC with XSPH, PEC with summation density, both, and Euler
+    with summation density;
+  * hipGraph replay equals the eager steps with both on;
+  * what the device path does not cover fails loudly: the split passes and
+    the adjoint tape with either form on.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gpu_util import make_engine, oracle_loop, perturbed, rel_err, say
+from hipsph import scenarios
+from hipsph._lib import HipSphError
+
+pytestmark = pytest.mark.gpu
+
+OUT = ('xsphx', 'xsphy', 'rho', 'ax', 'ay', 'drho')
+STATE = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'xsphx', 'xsphy')
+
+
+def _method(sc, xsph, sd):
+    m = sc.method()
+    m.useXSPH = xsph
+    m.useSummationDensity = sd
+    return m
+
+
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
+@pytest.mark.parametrize('flags', [(True, False), (False, True), (True, True)])
+def test_loop_vs_oracle(kernel, flags):
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 7)
+    m = _method(sc, *flags)
+    ref, _ = oracle_loop(pA, m, kernel)
+    f = pA['label'] == 0
+    for prec in ('fp64', 'fp32c'):
+        say('loop', kernel, prec, 'xsph', flags[0], 'sd', flags[1])
+        e = make_engine(pA, m, kernel=kernel, precision=prec)
+        e.nn_update()
+        e.compute()
+        d = e.download()
+        e.close()
+        for k in OUT:
+            if prec == 'fp64' and kernel == 'cubic':
+                np.testing.assert_array_equal(d[k], ref[k], err_msg=k)
+            else:
+                tol = 1e-12 if prec == 'fp64' else 2e-5
+                assert rel_err(d[k], ref[k], f) <= tol, (k, rel_err(d[k], ref[k], f))
+        if flags[0]:
+            assert np.max(np.abs(ref['xsphx'][f] - ref['vx'][f])) > 0  # the correction is exercised
+        if flags[1]:
+            assert np.all(d['drho'][f] == 0.0)
+
+
+def _run_both(pA, m, h, steps, integrator='pec', graph=False):
+    e = make_engine(pA, m, integrator=integrator)
+    e.step(steps, graph=graph)
+    st = e.status()
+    out = e.download()
+    hist = e.dt_history(steps)
+    e.close()
+    s = oracle.Solver(pA, m, 'cubic', integrator, h=h, nthreads=8)
+    rhist = s.run(steps)
+    return out, hist, st, s.pA, rhist, s
+
+
+@pytest.mark.parametrize('case', ['pec_xsph', 'pec_sd', 'pec_xsph_sd', 'euler_sd'])
+def test_steps_fp64_bitexact(case):
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 3, vscale=0.1, rscale=0.001)
+    m = _method(sc, 'xsph' in case, 'sd' in case)
+    integrator = case.split('_')[0]
+    say('steps', case)
+    out, hist, st, ref, rhist, s = _run_both(pA, m, sc.h, 200, integrator)
+    assert st['steps'] == 200 and st['flags'] & 3 == 0
+    np.testing.assert_array_equal(hist, rhist)
+    assert st['t'] == s.t
+    for k in STATE:
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+def test_graph_replay_with_xsph_and_summation_density():
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 4, vscale=0.1, rscale=0.001)
+    m = _method(sc, True, True)
+    outs = []
+    for graph in (False, True):
+        e = make_engine(pA, m)
+        e.step(60, graph=graph)
+        outs.append(e.download())
+        e.close()
+    for k in STATE:
+        np.testing.assert_array_equal(outs[1][k], outs[0][k], err_msg=k)
+
+
+def test_unsupported_combinations_fail_loudly():
+    from hipsph.adjoint import Tape
+    sc = scenarios.dam_break_c1()
+    e = make_engine(sc.pA, _method(sc, True, False))
+    e.nn_update()
+    with pytest.raises(HipSphError):
+        e.continuity()
+    with pytest.raises(HipSphError):
+        e.momentum()
+    with pytest.raises(HipSphError):
+        Tape(e, 2)
+    e.close()

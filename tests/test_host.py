@@ -58,8 +58,9 @@ def test_engine_params_to_c():
     p = EngineParams.from_method(123, m, kernel='gaussian', precision='fp32c', integrator='euler').to_c()
     assert (p.n, p.kernel, p.precision, p.integrator) == (123, 1, 1, 1)
     assert p.co == m.co and p.B == m.B and p.D == m.D and p.qmax == 3.0 and p.scale == 2.0
-    with pytest.raises(NotImplementedError):
-        EngineParams.from_method(1, WCSPH(2.0, 0.02, 1000.0, useXSPH=True))
+    assert (p.use_xsph, p.use_summation_density, p.epsilon) == (0, 0, 0.5)
+    q = EngineParams.from_method(1, WCSPH(2.0, 0.02, 1000.0, useXSPH=True, useSummationDensity=True)).to_c()
+    assert (q.use_xsph, q.use_summation_density, q.epsilon) == (1, 1, 0.5)
 
 
 def test_solver_mirror_signature_and_host_policy():
