@@ -63,6 +63,35 @@ def test_solver_loop_fp64_bitexact():
         np.testing.assert_array_equal(got[f], o.pA[f], err_msg=f)
 
 
+def test_solver_loop_xsph_summation_density_fp64_bitexact():
+    """The same loop with WCSPH(useXSPH=True, useSummationDensity=True) and
+    PEC(useXSPH=True): XSPH velocities and summed densities on the device."""
+    sc = _small()
+    m = sc.method()
+    m.useXSPH = True
+    m.useSummationDensity = True
+    sv = Solver(m, PEC(useXSPH=True, strict=False), CubicSpline(), duration=1.0, incrementalWriteout=False,
+                exportProperties=[], h=sc.h, damping=0.0, settle=False, chunk=16)
+    pA = sc.pA.copy()
+    f = pA['label'] == 0
+    pA['vx'][f] = np.random.default_rng(8).normal(0, 0.1, f.sum())
+    sv.addParticles(pA)
+    sv.setup()
+    ref = _oracle_after_setup(sv)
+    K = 40
+    o = oracle.Solver(ref, m, 'cubic', 'pec', h=sc.h, damping=0.0)
+    hist = o.run(K)
+    sv.duration = _duration_for(hist, K)
+    say('solver loop xsph+sd: n', sv.num_particles, 'steps', K)
+    sv.run()
+    sv.close()
+    assert sv.t_step == K
+    np.testing.assert_array_equal(np.asarray(sv.dt_a), hist[:, 0])
+    got = sv.particleArray[sv.indexes]
+    for fld in FIELDS + ('xsphx', 'xsphy'):
+        np.testing.assert_array_equal(got[fld], o.pA[fld], err_msg=fld)
+
+
 def test_solver_settling_and_temp_boundary_fp64_bitexact():
     sc = _small()
     dx = sc.dx

@@ -88,3 +88,21 @@ def test_solver_mirror_signature_and_host_policy():
     sv2.h = None
     with pytest.raises(NotImplementedError):
         sv2.setup()
+
+
+def test_solver_rejects_mismatched_xsph_flags():
+    """src/Methods/WCSPH.py:56: XSPH has to be on in both the integrator and
+    the method; the mirror refuses a PEC/WCSPH pair that disagree."""
+    from hipsph import scenarios
+    from hipsph.Integrators.PEC import PEC
+    from hipsph.Kernels.CubicSpline import CubicSpline
+    from hipsph.Solver import Solver
+    sc = scenarios.dam_break(0.4, 0.5, 0.02, tank_w=1.0, tank_h=0.8)
+    for pec_xsph, m_xsph in ((True, False), (False, True)):
+        m = sc.method()
+        m.useXSPH = m_xsph
+        sv = Solver(m, PEC(useXSPH=pec_xsph, strict=False), CubicSpline(), duration=1.0, incrementalWriteout=False,
+                    exportProperties=[], h=sc.h)
+        sv.addParticles(sc.pA.copy())
+        with pytest.raises(ValueError):
+            sv.setup()
