@@ -712,10 +712,14 @@ static int solver_step_masked(ref_particle_t *pA, int64_t n, ref_solver_t *s, co
     if (s->h_fixed > 0) {
         for (int64_t j = 0; j < mk->nf; ++j) pA[mk->fidx[j]].h = s->h_fixed;
     } else {
-        /* SolverTools.computeH :106-118 with sigma 1.3, d = 2 */
+        /* SolverTools.computeH :106-118 with sigma 1.3, d = 2: (m/rho) ** (1/d)
+         * in a fastmath=True numba function, where the exponent folds to 0.5
+         * and LLVM's fast-math library-call simplification replaces
+         * pow(x, 0.5) by sqrt(x). The correctly rounded sqrt is used here;
+         * glibc's pow(x, 0.5) differs from it in ~0.08 % of arguments. */
         for (int64_t j = 0; j < mk->nf; ++j) {
             ref_particle_t *p = &pA[mk->fidx[j]];
-            p->h = (p->rho > 1e-12) ? 1.3 * pow(p->m / p->rho, 0.5) : 0.0;
+            p->h = (p->rho > 1e-12) ? 1.3 * sqrt(p->m / p->rho) : 0.0;
         }
     }
     gather(mk->active, pA, mk->aidx, mk->na);

@@ -56,9 +56,11 @@
  *                         zero continuity, momentum with the new density
  *   pec XSPH branch       reduces bit for bit to the plain branch when the
  *                         XSPH velocity equals the one that branch uses
- *   dynamic h inside ref_loop / ref_solver_step: PARITY UNPINNED (no
- *                         reference test, no check here; not on the device
- *                         path).
+ *   dynamic h (solver   h = 1.3 (m/rho)^(1/2) of the predicted density
+ *   step, h_fixed = 0)    after a step, walls unchanged; 1.3 dx on a lattice
+ *                         at rho = m / dx^2; 0 for a zero density. The square
+ *                         root is the correctly rounded sqrt (the reference's
+ *                         fastmath pow(x, 0.5), see ref_solver_step).
  */
 #ifndef REFCPU_H
 #define REFCPU_H

@@ -653,3 +653,34 @@ def test_pec_xsph_branch_reduces_to_plain_branch():
     w = c['label'] != 0
     np.testing.assert_array_equal(c['x'][w], d['x'][w])
     np.testing.assert_allclose(c['x'][~w] - d['x'][~w], 0.5 * dt, rtol=1e-9)
+
+
+def test_dynamic_h_from_predicted_density():
+    """Dynamic h (src/Tools/SolverTools.py:106-118, Solver._compute :242-246):
+    after a step every fluid h is 1.3 (m / rho)^(1/2) of the density the loop
+    saw, the predicted one rho + dt/2 drho (PEC.py:57), and walls keep theirs.
+    On a lattice at rho = m / dx^2 that is h = 1.3 dx (sigma and the 2D
+    exponent), and a zero density gives h = 0 (:115-116)."""
+    pA = _block_with_walls(25)
+    f = pA['label'] == 0
+    rng = np.random.default_rng(6)
+    pA['drho'][f] = rng.normal(0, 50, f.sum())
+    pA['c'][f] = _M().co  # the Courant dt reads c (WCSPH.compute_speed_of_sound)
+    h_wall = pA['h'][~f].copy()
+    s = oracle.Solver(pA, _M(), 'cubic', 'pec', h=None)
+    rho0, drho0 = s.pA['rho'].copy(), s.pA['drho'].copy()
+    hist = s.run(1)
+    dt = hist[0, 0]
+    rho_half = rho0 + 0.5 * dt * drho0
+    np.testing.assert_allclose(s.pA['h'][f], 1.3 * np.sqrt(s.pA['m'][f] / rho_half[f]), rtol=1e-15)
+    np.testing.assert_array_equal(s.pA['h'][~f], h_wall)
+    dx = 0.02
+    gx, gy = np.meshgrid(np.arange(12) * dx, np.arange(12) * dx, indexing='ij')
+    lat = _particles(gx.ravel(), gy.ravel(), 1.3 * dx, rho=1000.0, m=1000.0 * dx * dx)
+    lat['rho'][5] = 0.0
+    lat['c'] = _M().co
+    s = oracle.Solver(lat, _M(), 'cubic', 'euler', h=None)
+    s.run(1)
+    g = np.arange(len(lat)) != 5
+    np.testing.assert_allclose(s.pA['h'][g], 1.3 * dx, rtol=1e-15)
+    assert s.pA['h'][5] == 0.0
