@@ -85,6 +85,9 @@ typedef struct {
     double duration;         /* device-side stepping stops once t >= duration */
     int64_t cell_capacity;   /* max cells of the grid; 0 = automatic */
     double epsilon;          /* XSPH factor (WCSPH.epsilon, 0.5; src/Methods/WCSPH.py:65) */
+    int32_t dynamic_h;       /* 1: Solver(h=None): every step, after the neighbour update,
+                              * fluid h = 1.3 (m / rho)^(1/2) of the predicted density
+                              * (src/Tools/SolverTools.py:106-118, src/Solver.py:242-246) */
 } sph_params_t;
 
 /* Structure-of-arrays state in the active-particle order. Device pointers. */

@@ -434,9 +434,12 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // the grid may find none and keep its previous rates (SolverTools.py:166-169).
 // The raw cell of each slot (for the pair pass's 3x3 walk) is formed again
 // from the position the keys were computed from.
-// pred as for upd_pos; xs: carry the XSPH velocity with the state.
+// pred as for upd_pos; xs: carry the XSPH velocity with the state; dyn:
+// dynamic h for the fluid, formed here, after the keys were binned with the
+// previous h (Solver._compute: nn.update :238, then computeH :242-246) and
+// from the density the loop sees (the predicted one with PEC).
 __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fp32,
-                          void *recA, void *recB, int pred, int strict, int lean, int xs) {
+                          void *recA, void *recB, int pred, int strict, int lean, int xs, int dyn) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
     const int act = ctrl->active;
@@ -470,6 +473,9 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
         b.vy0[t] = a.vy0[s];
         b.rho0[t] = a.rho0[s];
     }
+    // SolverTools.computeH :106-118, sigma 1.3, d = 2 (the reference's fastmath
+    // (m / rho) ** 0.5 is the square root)
+    if (dyn && act && lab == kFluid) h = rho > 1e-12 ? 1.3 * sqrt(m / rho) : 0.0;
     const Grid &g = ctrl->grid;
     const int2 cc = act ? cell_of(x, y, g) : make_int2(0, 0);
     if (!skip) {
@@ -754,8 +760,8 @@ static int check_params(const sph_params_t *P) {
     if (P->integrator != SPH_INTEGRATOR_PEC && P->integrator != SPH_INTEGRATOR_EULER)
         return fail(SPH_EINVAL, "unknown integrator %d", P->integrator);
     if ((P->use_xsph && !(P->epsilon == P->epsilon)) || P->use_xsph < 0 || P->use_xsph > 1 ||
-        P->use_summation_density < 0 || P->use_summation_density > 1)
-        return fail(SPH_EINVAL, "use_xsph / use_summation_density must be 0 or 1 (and epsilon finite)");
+        P->use_summation_density < 0 || P->use_summation_density > 1 || P->dynamic_h < 0 || P->dynamic_h > 1)
+        return fail(SPH_EINVAL, "use_xsph / use_summation_density / dynamic_h must be 0 or 1 (and epsilon finite)");
     if (!(P->scale > 0) || !(P->qmax > 0) || !(P->rho0 > 0))
         return fail(SPH_EINVAL, "scale, qmax and rho0 must be positive");
     return 0;
@@ -973,7 +979,7 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
                                                            c->binflag, c->bin, c->rank);
     k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_s, make_phys(c->P), n,
                                          c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict,
-                                         lean, c->P.use_xsph);
+                                         lean, c->P.use_xsph, c->P.dynamic_h);
     HIPCHK(hipGetLastError());
     if (c->timing) {
         int rc = ev_record(c, c->ev_nn, c->n_nn, s);

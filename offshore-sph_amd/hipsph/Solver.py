@@ -26,9 +26,9 @@ Differences from the reference, each deliberate:
     the loop body this way.
   * ``p`` written back after ``run`` is the Tait pressure of the final
     density; the reference leaves the pressure of the last predicted density.
-  * Not on the device path (raise ``NotImplementedError``): dynamic smoothing
-    length (``h=None``) and the coupling hooks. XSPH and summation density
-    run on the device; PEC's ``useXSPH`` must equal the method's (the
+  * Not on the device path (raise ``NotImplementedError``): the coupling
+    hooks. Dynamic h (``h=None``), XSPH and summation density run on the
+    device; PEC's ``useXSPH`` must equal the method's (the
     reference's WCSPH docstring requires both on), otherwise ``ValueError``.
   * ``save``/``load`` use h5py with the reference's dataset names when h5py
     is importable, else a ``.npz`` with the same names.
@@ -125,8 +125,6 @@ class Solver:
     def setup(self):
         """src/Solver.py:173-208"""
         println('Starting setup.')
-        if self.h is None:
-            raise NotImplementedError('dynamic smoothing length (h=None) is not on the device path; pass h')
         if (self.integrator.name == 'pec' and
                 bool(getattr(self.integrator, 'useXSPH', False)) != bool(getattr(self.method, 'useXSPH', False))):
             # src/Methods/WCSPH.py:56: "useXSPH has to be enabled in both the integrator and the method!"
@@ -135,7 +133,14 @@ class Solver:
         self.num_particles, self.indexes = findActive(self.num_particles, pA)
         self._masks()
         println(f'{self.num_particles} total particles, {self.fluid_count} fluid particles.')
-        pA['h'][self.f_indexes] = self.h
+        if self.h is None:
+            # SolverTools.computeH (src/Tools/SolverTools.py:106-118) from the input density,
+            # before initialize (src/Solver.py:185-189); the device repeats it every step
+            f = pA[self.f_indexes]
+            pA['h'][self.f_indexes] = np.where(f['rho'] > 1e-12, 1.3 * np.sqrt(f['m'] / np.where(
+                f['rho'] > 1e-12, f['rho'], 1.0)), 0.0)
+        else:
+            pA['h'][self.f_indexes] = self.h
         pA[self.f_indexes] = self.method.initialize(pA[self.f_indexes])
         pA['p'][self.f_indexes] = self.method.compute_pressure(pA[self.f_indexes])
         pA['c'][self.f_indexes] = self.method.compute_speed_of_sound(pA[self.f_indexes])
@@ -161,6 +166,7 @@ class Solver:
                                      integrator=self.integrator.name, precision=self.precision,
                                      strict=bool(getattr(self.integrator, 'strict', False)))
         p.damping = damping
+        p.dynamic_h = self.h is None
         p.fixed_dt = float(self.timeStep) if self.timeStep is not None else 0.0
         p.duration = float(self.duration)
         e = Engine(p, device=self.device)
@@ -174,7 +180,7 @@ class Solver:
         """Device state -> active records of particleArray."""
         d = e.download()
         idx = np.nonzero(self.indexes)[0]
-        for k in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'p', 'xsphx', 'xsphy'):
+        for k in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'p', 'xsphx', 'xsphy', 'h'):
             self.particleArray[k][idx] = d[k]
 
     def _steps(self, e, n):

@@ -48,6 +48,7 @@ class sph_params_t(C.Structure):
         ('fixed_dt', C.c_double), ('duration', C.c_double),
         ('cell_capacity', C.c_int64),
         ('epsilon', C.c_double),
+        ('dynamic_h', C.c_int32),
     ]
 
 

@@ -59,6 +59,7 @@ class EngineParams:
     use_xsph: bool = False
     use_summation_density: bool = False
     epsilon: float = 0.5
+    dynamic_h: bool = False
 
     def to_c(self):
         p = sph_params_t()
@@ -69,6 +70,7 @@ class EngineParams:
         p.strict = int(bool(self.strict))
         p.use_xsph = int(bool(self.use_xsph))
         p.use_summation_density = int(bool(self.use_summation_density))
+        p.dynamic_h = int(bool(self.dynamic_h))
         for f in ('rho0', 'gamma', 'co', 'B', 'Pb', 'alpha', 'beta', 'r0', 'D', 'p1', 'p2', 'gravity', 'scale',
                   'qmax', 'gamma_c', 'gamma_f', 'damping', 'fixed_dt', 'duration', 'epsilon'):
             setattr(p, f, float(getattr(self, f)))

@@ -1,15 +1,19 @@
-"""XSPH and summation density on the device against the CPU oracle.
+"""The loop's other forms on the device against the CPU oracle: XSPH,
+summation density and dynamic h.
 
 The oracle's XSPH (src/Equations/XSPH.py via WCSPH.compute_velocity), its
-summation density (src/Tools/SolverTools.py:129-140) and PEC's XSPH branch
-(src/Integrators/PEC.py:43-48,66-71) are pinned by a dense-pair model and
-invariants in tests/test_oracle_invariants.py. Here, through the C ABI:
+summation density (src/Tools/SolverTools.py:129-140), PEC's XSPH branch
+(src/Integrators/PEC.py:43-48,66-71) and dynamic h (SolverTools.py:106-118,
+Solver._compute) are pinned by a dense-pair model and invariants in
+tests/test_oracle_invariants.py. Here, through the C ABI:
   * one loop on a perturbed dam break, every combination of the two: FP64
     cubic bit-identical (XSPH velocity, summed density, rates), Gaussian and
     Wendland FP64 to 1e-12, FP32C pairs to 2e-5 of the largest value;
   * whole steps: FP64 cubic trajectories and dt series bit-identical over
     200 steps for PEC with XSPH, PEC with summation density, both, and Euler
     with summation density;
+  * dynamic h: FP64 cubic trajectories, dt series and h bit-identical over
+    200 steps (PEC, Euler) and with XSPH and summation density on as well;
   * hipGraph replay equals the eager steps with both on;
   * what the device path does not cover fails loudly: the split passes and
     the adjoint tape with either form on.
@@ -62,14 +66,14 @@ def test_loop_vs_oracle(kernel, flags):
             assert np.all(d['drho'][f] == 0.0)
 
 
-def _run_both(pA, m, h, steps, integrator='pec', graph=False):
-    e = make_engine(pA, m, integrator=integrator)
+def _run_both(pA, m, h, steps, integrator='pec', graph=False, dynamic_h=False):
+    e = make_engine(pA, m, integrator=integrator, dynamic_h=dynamic_h)
     e.step(steps, graph=graph)
     st = e.status()
     out = e.download()
     hist = e.dt_history(steps)
     e.close()
-    s = oracle.Solver(pA, m, 'cubic', integrator, h=h, nthreads=8)
+    s = oracle.Solver(pA, m, 'cubic', integrator, h=None if dynamic_h else h, nthreads=8)
     rhist = s.run(steps)
     return out, hist, st, s.pA, rhist, s
 
@@ -86,6 +90,24 @@ def test_steps_fp64_bitexact(case):
     np.testing.assert_array_equal(hist, rhist)
     assert st['t'] == s.t
     for k in STATE:
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize('case', ['pec', 'euler', 'pec_xsph_sd'])
+def test_steps_dynamic_h_fp64_bitexact(case):
+    """Solver(h=None): the grid of each step from the previous h, then
+    h = 1.3 (m / rho)^(1/2) of the density the loop sees."""
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 5, vscale=0.1, rscale=0.01)
+    m = _method(sc, 'xsph' in case, 'sd' in case)
+    integrator = case.split('_')[0]
+    say('dynamic h steps', case)
+    out, hist, st, ref, rhist, s = _run_both(pA, m, sc.h, 200, integrator, dynamic_h=True)
+    assert st['steps'] == 200 and st['flags'] & 3 == 0
+    np.testing.assert_array_equal(hist, rhist)
+    f = pA['label'] == 0
+    assert np.ptp(out['h'][f]) > 0  # h does vary
+    for k in STATE + ('h',):
         np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
 
 
@@ -106,6 +128,10 @@ def test_graph_replay_with_xsph_and_summation_density():
 def test_unsupported_combinations_fail_loudly():
     from hipsph.adjoint import Tape
     sc = scenarios.dam_break_c1()
+    e = make_engine(sc.pA, sc.method(), dynamic_h=True)
+    with pytest.raises(HipSphError):
+        Tape(e, 2)
+    e.close()
     e = make_engine(sc.pA, _method(sc, True, False))
     e.nn_update()
     with pytest.raises(HipSphError):

@@ -92,6 +92,36 @@ def test_solver_loop_xsph_summation_density_fp64_bitexact():
         np.testing.assert_array_equal(got[fld], o.pA[fld], err_msg=fld)
 
 
+def test_solver_dynamic_h_fp64_bitexact():
+    """Solver(h=None): h from the input density at setup (src/Solver.py:185-189),
+    then every step on the device (:242-246)."""
+    sc = _small()
+    m = sc.method()
+    sv = Solver(m, PEC(useXSPH=False, strict=False), CubicSpline(), duration=1.0, incrementalWriteout=False,
+                exportProperties=[], h=None, damping=0.0, settle=False, chunk=16)
+    pA = sc.pA.copy()
+    f = pA['label'] == 0
+    pA['rho'][f] *= 1 + np.random.default_rng(9).uniform(-0.01, 0.01, f.sum())
+    h0 = 1.3 * np.sqrt(pA['m'][f] / pA['rho'][f])  # before initialize resets rho in place
+    sv.addParticles(pA)
+    sv.setup()
+    ref = _oracle_after_setup(sv)
+    np.testing.assert_allclose(ref['h'][ref['label'] == 0], h0, rtol=1e-15)
+    assert np.ptp(h0) > 0
+    K = 40
+    o = oracle.Solver(ref, m, 'cubic', 'pec', h=None, damping=0.0)
+    hist = o.run(K)
+    sv.duration = _duration_for(hist, K)
+    say('solver loop dynamic h: n', sv.num_particles, 'steps', K)
+    sv.run()
+    sv.close()
+    assert sv.t_step == K
+    np.testing.assert_array_equal(np.asarray(sv.dt_a), hist[:, 0])
+    got = sv.particleArray[sv.indexes]
+    for fld in FIELDS + ('h',):
+        np.testing.assert_array_equal(got[fld], o.pA[fld], err_msg=fld)
+
+
 def test_solver_settling_and_temp_boundary_fp64_bitexact():
     sc = _small()
     dx = sc.dx

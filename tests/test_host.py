@@ -85,9 +85,17 @@ def test_solver_mirror_signature_and_host_policy():
     assert sv._next_stop(3) == 1
     sv2 = Solver(None, None, None, exportProperties=[], incrementalWriteout=False, chunk=64)
     assert sv2._next_stop(7) == 64
-    sv2.h = None
+    # the coupling hooks (src/Solver.py:381-398) are not on the device path
+    from hipsph import scenarios
+    from hipsph.Integrators.PEC import PEC
+    from hipsph.Kernels.CubicSpline import CubicSpline
+    sc = scenarios.dam_break(0.4, 0.5, 0.02, tank_w=1.0, tank_h=0.8)
+    sv3 = Solver(sc.method(), PEC(useXSPH=False, strict=False), CubicSpline(), duration=1.0,
+                 incrementalWriteout=False, exportProperties=[], h=sc.h, coupling=lambda *a: None)
+    sv3.addParticles(sc.pA.copy())
+    sv3.num_particles = len(sv3.particleArray)  # as setup would leave it (setup needs the device)
     with pytest.raises(NotImplementedError):
-        sv2.setup()
+        sv3.run()
 
 
 def test_solver_rejects_mismatched_xsph_flags():
