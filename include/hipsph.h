@@ -55,6 +55,10 @@ enum { SPH_KERNEL_CUBIC = 0, SPH_KERNEL_GAUSSIAN = 1, SPH_KERNEL_WENDLAND = 2 };
 enum { SPH_PREC_FP64 = 0, SPH_PREC_FP32C = 1 };
 enum { SPH_INTEGRATOR_PEC = 0, SPH_INTEGRATOR_EULER = 1 };
 enum { SPH_FLUID = 0, SPH_BOUNDARY = 1, SPH_TEMPBOUNDARY = 2, SPH_COUPLED = 3 };
+/* Integrator of the particles labelled SPH_COUPLED: none (they stay where
+ * they are, as the reference leaves them without a coupling hook), or
+ * Newmark-beta (src/Integrators/NewmarkBeta.py:41-66). */
+enum { SPH_COUPLED_NONE = 0, SPH_COUPLED_NEWMARK = 1 };
 
 typedef struct sph_ctx sph_ctx_t;
 
@@ -88,6 +92,10 @@ typedef struct {
     int32_t dynamic_h;       /* 1: Solver(h=None): every step, after the neighbour update,
                               * fluid h = 1.3 (m / rho)^(1/2) of the predicted density
                               * (src/Tools/SolverTools.py:106-118, src/Solver.py:242-246) */
+    int32_t coupled_integrator; /* SPH_COUPLED_*: Solver(couplingIntegrator=NewmarkBeta(...))
+                              * predicts and corrects the coupled particles with the
+                              * fluid (src/Solver.py:381-382,397-398) */
+    double newmark_beta, newmark_gamma; /* NewmarkBeta(beta, gamma); 2 beta >= gamma */
 } sph_params_t;
 
 /* Structure-of-arrays state in the active-particle order. Device pointers. */
@@ -103,7 +111,7 @@ typedef struct {
 int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out);
 int sph_ctx_destroy(sph_ctx_t *ctx);
 /* Update constants that may change between steps (damping, alpha, beta, co,
- * B, duration, fixed_dt). n, kernel and precision are fixed at creation. */
+ * B, duration, fixed_dt). n, kernel, precision and use_xsph are fixed at creation. */
 int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params);
 
 /* Copy the state in (reference index order) / out (un-permuted back to that
@@ -172,6 +180,46 @@ int sph_correct(sph_ctx_t *ctx, void *stream);
  * dt -> predict -> nn_update -> compute -> correct, all on the device.
  * use_graph != 0 captures the step in a hipGraph and replays it. */
 int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream);
+
+/* One loop body split at the coupling hook (src/Solver.py:389-392):
+ * sph_step_begin runs dt -> predict (fluid and, with coupled_integrator,
+ * the coupled particles) -> nn_update -> compute and leaves the state of the
+ * hook's call: the predicted positions, velocities and density and this
+ * step's rates. The caller may then read and write records
+ * (sph_state_gather / sph_state_scatter, sph_state_download), query the
+ * step's grid (sph_near_pos, sph_probe_pressure) and must call
+ * sph_step_end, which runs the correctors with the values it finds and
+ * advances the clock. begin + end give sph_step's result bit for bit.
+ * Other step calls fail with SPH_ESTATE while a split step is open. */
+int sph_step_begin(sph_ctx_t *ctx, void *stream);
+int sph_step_end(sph_ctx_t *ctx, void *stream);
+
+/* ---- records by reference index: the coupling hook's view of the state ---- */
+/* The hook reads and writes particleArray[cInd] (examples/IceBreak.py:110-178,
+ * src/Solver.py:391). `ids` (host, k entries) are indices into the uploaded
+ * active array; the non-NULL fields of `st` are device arrays of k entries in
+ * the order of `ids`. Gather fills them (p is the Tait pressure + Pb of the
+ * record's density); scatter writes x, y, vx, vy, rho, ax, ay, drho, m, h,
+ * xsphx, xsphy into the current state (label and p must be NULL; ids must be
+ * distinct). Synchronise `stream`. */
+int sph_state_gather(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_state_t *st, void *stream);
+int sph_state_scatter(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_state_t *st, void *stream);
+
+/* NNLinkedList.nearPos (src/Tools/NNLinkedList.py:41-80) for npts points
+ * (x, y, h: device arrays) against the grid of the last neighbour update and
+ * the current positions: as sph_nn_dump, off (npts + 1 entries) and, when
+ * the total fits in cap, idx (reference indices), q and r in the reference's
+ * order. Returns SPH_ECAPACITY with *total set when it does not fit. */
+int sph_near_pos(sph_ctx_t *ctx, int64_t npts, const double *x, const double *y, const double *h, int64_t *off,
+                 int32_t *idx, double *q, double *r, int64_t cap, int64_t *total, void *stream);
+/* The SPH pressure probe of the coupled example (pressure_SPH,
+ * examples/IceBreak.py:253-285) at npts points, one lane each: nearPos of
+ * (x, y) with the probe's h, W(r, h) of the context's kernel, the Shepard
+ * normalisation (src/Equations/Shepard.py:5-12), the summation density over
+ * the fluid (src/Equations/SummationDensity.py:5-13) and
+ * p = ((rho / rho0)^gamma - 1) B. rho, p: device arrays of npts. */
+int sph_probe_pressure(sph_ctx_t *ctx, int64_t npts, const double *x, const double *y, const double *h,
+                       double *rho, double *p, void *stream);
 
 /* Device clock and flags: out_d = {t, dt, dt_c, dt_f}, out_i = {steps, flags}.
  * Synchronises `stream`. */

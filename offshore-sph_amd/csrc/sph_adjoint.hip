@@ -584,9 +584,9 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     if (!ctx || !out) return fail(SPH_EINVAL, "NULL argument");
     *out = nullptr;
     if (nslots < 1 || nslots > 100000) return fail(SPH_EINVAL, "nslots=%d out of range", nslots);
-    if (ctx->P.use_xsph || ctx->P.use_summation_density || ctx->P.dynamic_h)
+    if (ctx->P.use_xsph || ctx->P.use_summation_density || ctx->P.dynamic_h || ctx->P.coupled_integrator)
         return fail(SPH_EUNSUPPORTED,
-                    "the adjoint covers the plain WCSPH step (no XSPH, summation density or dynamic h)");
+                    "the adjoint covers the plain WCSPH step (no XSPH, summation density, dynamic h or coupled integrator)");
     const size_t n = (size_t)ctx->n;
     sph_tape *t = new (std::nothrow) sph_tape;
     if (!t) return fail(SPH_ENOMEM, "host allocation failed");

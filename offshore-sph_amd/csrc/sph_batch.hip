@@ -581,8 +581,8 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
             return fail(SPH_EINVAL, "unknown precision %d", P.precision);
         if (P.integrator != SPH_INTEGRATOR_PEC && P.integrator != SPH_INTEGRATOR_EULER)
             return fail(SPH_EINVAL, "unknown integrator %d", P.integrator);
-        if (P.use_xsph || P.use_summation_density || P.dynamic_h)
-            return fail(SPH_EUNSUPPORTED, "a batch runs the plain WCSPH step (no XSPH, summation density or dynamic h)");
+        if (P.use_xsph || P.use_summation_density || P.dynamic_h || P.coupled_integrator)
+            return fail(SPH_EUNSUPPORTED, "a batch runs the plain WCSPH step (no XSPH, summation density, dynamic h or coupled integrator)");
         if (!(P.scale > 0) || !(P.qmax > 0) || !(P.rho0 > 0))
             return fail(SPH_EINVAL, "instance %d: scale, qmax and rho0 must be positive", k);
         ntot += P.n;

@@ -108,6 +108,17 @@ __device__ inline double pec_half_rho(double rho, double drho, double dt, int st
     return r;
 }
 
+// src/Integrators/NewmarkBeta.py:41-66, one coupled particle: the predictor
+// x + v dt + a (1/2 - beta) dt^2, v + a (1 - gamma) dt and the corrector
+// x + a beta dt^2, v + a gamma dt, left to right, dt^2 the product dt dt (a
+// power with the integer literal 2 in the numba source).
+__device__ inline double nm_pred_pos(double x, double v, double a, double beta, double dt) {
+    return x + v * dt + a * (0.5 - beta) * (dt * dt);
+}
+__device__ inline double nm_pred_vel(double v, double a, double gamma, double dt) { return v + a * (1 - gamma) * dt; }
+__device__ inline double nm_corr_pos(double x, double a, double beta, double dt) { return x + a * beta * (dt * dt); }
+__device__ inline double nm_corr_vel(double v, double a, double gamma, double dt) { return v + a * gamma * dt; }
+
 // Exclusive prefix scan of counts[0 .. ncells] into cellStart, in tiles of
 // kScanTile bins: the tile sums, then each tile's scan offset by the sum of
 // the tile sums before it (a few hundred integers, read by that tile's

@@ -23,6 +23,8 @@ struct sph_ctx {
     double *part_k = nullptr;                     // kinetic-energy partials + result
     long long cap_cells = 0;
     bool dt_fresh = false;  // part_d holds the dt partials of the current state (left by the corrector)
+    bool mid = false;       // a split step is open (sph_step_begin ran, sph_step_end not yet)
+    bool xs_fluid_zero = false;  // XSPH off and a reorder ran: the fluid's XSPH velocity reads as 0
     void *recA = nullptr, *recB = nullptr;  // packed pair-pass records (Rec<double> / Rec<float>)
     int *slot_of = nullptr;
     hipStream_t own = nullptr;

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "sph_ctx.h"
@@ -200,14 +201,28 @@ __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict, int 
 // The half-step velocity likewise comes from vx, vy or is formed again from
 // the start-of-step velocity and the previous rates there.
 // xs: PEC moves x by the XSPH velocity of this step's loop (PEC.py:66-71).
+// With Newmark-beta on, the coupled particles' corrector
+// (NewmarkBeta.py:54-66, src/Solver.py:397-398) with the accelerations the
+// state holds now (the coupling hook's).
 __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, int integrator, int strict, int xs,
                           double *part) {
     if (!ctrl->active) return;
     __shared__ double lds[kDtQ * 16];
     const double dt = ctrl->dt, damping = ctrl->damping;
+    const int nm = ctrl->nm;
     DtVars v;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        if (s.label[j] != kFluid) continue;
+        const int8_t lab = s.label[j];
+        if (lab != kFluid) {
+            if (nm && lab == kCoupled) {
+                const double nb = ctrl->nm_beta, ng = ctrl->nm_gamma, ax = s.ax[j], ay = s.ay[j];
+                s.x[j] = nm_corr_pos(s.x[j], ax, nb, dt);
+                s.y[j] = nm_corr_pos(s.y[j], ay, nb, dt);
+                s.vx[j] = nm_corr_vel(s.vx[j], ax, ng, dt);
+                s.vy[j] = nm_corr_vel(s.vy[j], ay, ng, dt);
+            }
+            continue;
+        }
         const double ax = s.ax[j], ay = s.ay[j];
         v.add(s.h[j], ax, ay);
         if (integrator == SPH_INTEGRATOR_PEC) {
@@ -262,7 +277,10 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
 // Position the neighbour update sees: the predicted one when the PEC
 // predictor is fused into the update (fluid only), else the stored one.
 // pred: 0 none, 1 PEC, 2 PEC moving x by the XSPH velocity (PEC.py:43-48).
-__device__ inline void upd_pos(const State &a, int i, int pred, double dt, double &x, double &y) {
+// With Newmark-beta on (ctrl->nm) the coupled particles' predicted position
+// (NewmarkBeta.py:46-47).
+__device__ inline void upd_pos(const Ctrl *ctrl, const State &a, int i, int pred, double dt, double &x,
+                               double &y) {
     x = a.x[i];
     y = a.y[i];
     if (pred) {
@@ -271,6 +289,11 @@ __device__ inline void upd_pos(const State &a, int i, int pred, double dt, doubl
         const bool f = a.label[i] == kFluid;
         x = f ? pec_half_pos(x, vx, dt) : x;
         y = f ? pec_half_pos(y, vy, dt) : y;
+    }
+    if (ctrl->nm && a.label[i] == kCoupled) {
+        const double b = ctrl->nm_beta;
+        x = nm_pred_pos(x, a.vx[i], a.ax[i], b, dt);
+        y = nm_pred_pos(y, a.vy[i], a.ay[i], b, dt);
     }
 }
 
@@ -294,7 +317,7 @@ __global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *par
             const int i = i0 + u * stride;
             if (i < n) {
                 double xi, yi;
-                upd_pos(a, i, pred, dt, xi, yi);
+                upd_pos(ctrl, a, i, pred, dt, xi, yi);
                 const double hi = a.h[i];
                 if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;  // NaN or Inf
                 xmn = xi < xmn ? xi : xmn;
@@ -382,7 +405,7 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
     int b = -1;
     if (t < n) {
         double x, y;
-        upd_pos(a, t, pred, ctrl->dt, x, y);
+        upd_pos(ctrl, a, t, pred, ctrl->dt, x, y);
         const int2 c = cell_of(x, y, g);
         const long long key = (long long)c.x + g.ncx * (long long)c.y;
         if (key >= 0 && key < g.ncells) {
@@ -467,6 +490,14 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
         vy = pec_half_vel(vy, ay, dt, damping);
         rho = pec_half_rho(rho, drho, dt, strict);
     } else {
+        if (act && lab == kCoupled && ctrl->nm) {
+            // NewmarkBeta.predict (src/Integrators/NewmarkBeta.py:41-52), src/Solver.py:381-382
+            const double dt = ctrl->dt, nb = ctrl->nm_beta, ng = ctrl->nm_gamma;
+            x = nm_pred_pos(x, vx, ax, nb, dt);
+            y = nm_pred_pos(y, vy, ay, nb, dt);
+            vx = nm_pred_vel(vx, ax, ng, dt);
+            vy = nm_pred_vel(vy, ay, ng, dt);
+        }
         b.x0[t] = a.x0[s];
         b.y0[t] = a.y0[s];
         b.vx0[t] = a.vx0[s];
@@ -489,7 +520,7 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
     b.h[t] = h;
     b.label[t] = lab;
     b.id[t] = a.id[s];
-    if (xs) {
+    if (xs || lab != kFluid) {  // without XSPH the fluid's is 0 after every loop (WCSPH.py:187-189)
         b.xsphx[t] = a.xsphx[s];
         b.xsphy[t] = a.xsphy[s];
     }
@@ -551,7 +582,15 @@ __global__ void k_restore_ctrl(Ctrl *ctrl, double t) {
     ctrl->advance_t = 1;
 }
 
-__global__ void k_download(State s, sph_state_t out, Phys ph, int n) {
+// fz: XSPH is off and a reorder has run since the upload, so the fluid's
+// XSPH velocity is not carried and reads as the 0 that WCSPH.compute_velocity
+// gives every fluid particle with a neighbour (src/Methods/WCSPH.py:187-189;
+// an in-grid particle always has itself). Walls keep theirs.
+__device__ inline double xs_read(const State &s, const double *v, int t, int fz) {
+    return fz && s.label[t] == kFluid ? 0.0 : v[t];
+}
+
+__global__ void k_download(State s, sph_state_t out, Phys ph, int n, int fz) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i = s.id[t];
@@ -566,8 +605,8 @@ __global__ void k_download(State s, sph_state_t out, Phys ph, int n) {
     if (out.m) out.m[i] = s.m[t];
     if (out.h) out.h[i] = s.h[t];
     if (out.label) out.label[i] = s.label[t];
-    if (out.xsphx) out.xsphx[i] = s.xsphx[t];
-    if (out.xsphy) out.xsphy[i] = s.xsphy[t];
+    if (out.xsphx) out.xsphx[i] = xs_read(s, s.xsphx, t, fz);
+    if (out.xsphy) out.xsphy[i] = xs_read(s, s.xsphy, t, fz);
     if (out.p) out.p[i] = tait_p(s.rho[t], s.label[t], ph.gamma, ph.B, ph.rho0, ph.Pb);
 }
 
@@ -579,22 +618,27 @@ __global__ void k_inverse(const int *id, int *slot_of, int n) {
     if (t < n) slot_of[id[t]] = t;
 }
 
-// nearPos (src/Tools/NNLinkedList.py:41-80) for particle i. mode 0 counts,
-// mode 1 writes indices (and q, r) from off[i].
-__global__ void k_nn_dump(const Ctrl *ctrl, State s, const int *cellStart, const int *slot_of, int n, int mode,
-                          long long *keys, long long *off, int *idx, double *qo, double *ro, double qmax) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Grid g = ctrl->grid;
-    int t = slot_of[i];
-    double x = s.x[t], y = s.y[t], h = s.h[t];
+// Raw cell of a point (NNLinkedList._find_cell_id_raw :146-176), clamped
+// to +-1e9 so the cast is defined (such a cell has no valid neighbour cell).
+__device__ inline void raw_cell(const Grid &g, double x, double y, long long &cx0, long long &cy0) {
     double fx = floor((x - g.xmin) / g.cs);
     double fy = floor((y - g.ymin) / g.cs);
     fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
     fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
-    long long cx0 = (long long)fx, cy0 = (long long)fy;
-    if (mode == 0) keys[i] = cx0 + g.ncx * cy0;
-    long long k = 0, base = mode ? off[i] : 0;
+    cx0 = (long long)fx;
+    cy0 = (long long)fy;
+}
+
+// nearPos (src/Tools/NNLinkedList.py:41-80) of the point (x, y) with length
+// h against the last update: the 3 x 3 cells around its raw cell, x shift
+// outer and y shift inner, each cell from its highest reference index down
+// (the slots of a cell ascend in index), every particle with
+// q = r / (0.5 (h + h_j)) <= qmax. f(u, q, r) per neighbour slot u.
+template <typename F>
+__device__ inline void near_walk(const Grid &g, const State &s, const int *cellStart, double x, double y, double h,
+                                 double qmax, F &&f) {
+    long long cx0, cy0;
+    raw_cell(g, x, y, cx0, cy0);
     for (int a = -1; a <= 1; ++a) {
         for (int b = -1; b <= 1; ++b) {
             long long cx = cx0 + a, cy = cy0 + b;
@@ -606,17 +650,54 @@ __global__ void k_nn_dump(const Ctrl *ctrl, State s, const int *cellStart, const
                 double r = sqrt(xij2);
                 double h_ij = 0.5 * (h + s.h[u]);
                 double q = r / h_ij;
-                if (q <= qmax) {
-                    if (mode) {
-                        idx[base + k] = s.id[u];
-                        if (qo) qo[base + k] = q;
-                        if (ro) ro[base + k] = r;
-                    }
-                    ++k;
-                }
+                if (q <= qmax) f(u, q, r);
             }
         }
     }
+}
+
+// near(i) = nearPos of particle i (:82-84). mode 0 counts, mode 1 writes
+// indices (and q, r) from off[i].
+__global__ void k_nn_dump(const Ctrl *ctrl, State s, const int *cellStart, const int *slot_of, int n, int mode,
+                          long long *keys, long long *off, int *idx, double *qo, double *ro, double qmax) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Grid g = ctrl->grid;
+    int t = slot_of[i];
+    double x = s.x[t], y = s.y[t], h = s.h[t];
+    if (mode == 0) {
+        long long cx0, cy0;
+        raw_cell(g, x, y, cx0, cy0);
+        keys[i] = cx0 + g.ncx * cy0;
+    }
+    long long k = 0, base = mode ? off[i] : 0;
+    near_walk(g, s, cellStart, x, y, h, qmax, [&](int u, double q, double r) {
+        if (mode) {
+            idx[base + k] = s.id[u];
+            if (qo) qo[base + k] = q;
+            if (ro) ro[base + k] = r;
+        }
+        ++k;
+    });
+    if (mode == 0) off[i] = k;
+}
+
+// nearPos for npts arbitrary points (px, py, ph), as k_nn_dump.
+__global__ void k_near_pos(const Ctrl *ctrl, State s, const int *cellStart, int npts, const double *px,
+                           const double *py, const double *ph, int mode, long long *off, int *idx, double *qo,
+                           double *ro, double qmax) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const Grid g = ctrl->grid;
+    long long k = 0, base = mode ? off[i] : 0;
+    near_walk(g, s, cellStart, px[i], py[i], ph[i], qmax, [&](int u, double q, double r) {
+        if (mode) {
+            idx[base + k] = s.id[u];
+            if (qo) qo[base + k] = q;
+            if (ro) ro[base + k] = r;
+        }
+        ++k;
+    });
     if (mode == 0) off[i] = k;
 }
 
@@ -698,6 +779,74 @@ __global__ void k_tait(int n, double gamma, double B, double rho0, double Pb, co
     if (j < n) p[j] = tait_p(rho[j], label[j], gamma, B, rho0, Pb);
 }
 
+// pressure_SPH of the coupled example (examples/IceBreak.py:253-285), one lane
+// per point: the Shepard normaliser sum of W(r, h) m / rho over the fluid
+// neighbours with rho >= 1e-3 (src/Equations/Shepard.py:5-12), then the
+// summation density sum of m (W / normaliser) over the fluid neighbours
+// (src/Equations/SummationDensity.py:5-13), both in nearPos order with the
+// probe's own h in W, and p = ((rho / rho0)^gamma - 1) B (:282-284).
+__global__ void k_probe(const Ctrl *ctrl, State s, const int *cellStart, int npts, const double *px, const double *py,
+                        const double *ph, int kernel, double qmax, double rho0, double gamma, double B, double *rho_out,
+                        double *p_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const Grid g = ctrl->grid;
+    const double x = px[i], y = py[i], hp = ph[i];
+    double norm = 0.0;
+    near_walk(g, s, cellStart, x, y, hp, qmax, [&](int u, double, double r) {
+        const double rj = s.rho[u];
+        if (rj < 1e-3 || s.label[u] != kFluid) return;
+        norm += dev_kernel_w(kernel, r, hp) * s.m[u] / rj;
+    });
+    double rho = 0.0;
+    near_walk(g, s, cellStart, x, y, hp, qmax, [&](int u, double, double r) {
+        if (s.label[u] != kFluid) return;
+        rho += s.m[u] * (dev_kernel_w(kernel, r, hp) / norm);
+    });
+    rho_out[i] = rho;
+    p_out[i] = (ipow(rho / rho0, gamma) - 1.0) * B;
+}
+
+// Records by reference index: ids[j] -> slot slot_of[ids[j]] of the current
+// buffer (the coupling hook's particleArray[cInd] reads and writes).
+__global__ void k_gather_ids(State s, const int *slot_of, const int *ids, int k, sph_state_t out, Phys ph, int fz) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    const int t = slot_of[ids[j]];
+    if (out.x) out.x[j] = s.x[t];
+    if (out.y) out.y[j] = s.y[t];
+    if (out.vx) out.vx[j] = s.vx[t];
+    if (out.vy) out.vy[j] = s.vy[t];
+    if (out.rho) out.rho[j] = s.rho[t];
+    if (out.ax) out.ax[j] = s.ax[t];
+    if (out.ay) out.ay[j] = s.ay[t];
+    if (out.drho) out.drho[j] = s.drho[t];
+    if (out.m) out.m[j] = s.m[t];
+    if (out.h) out.h[j] = s.h[t];
+    if (out.label) out.label[j] = s.label[t];
+    if (out.xsphx) out.xsphx[j] = xs_read(s, s.xsphx, t, fz);
+    if (out.xsphy) out.xsphy[j] = xs_read(s, s.xsphy, t, fz);
+    if (out.p) out.p[j] = tait_p(s.rho[t], s.label[t], ph.gamma, ph.B, ph.rho0, ph.Pb);
+}
+
+__global__ void k_scatter_ids(State s, const int *slot_of, const int *ids, int k, sph_state_t in) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    const int t = slot_of[ids[j]];
+    if (in.x) s.x[t] = in.x[j];
+    if (in.y) s.y[t] = in.y[j];
+    if (in.vx) s.vx[t] = in.vx[j];
+    if (in.vy) s.vy[t] = in.vy[j];
+    if (in.rho) s.rho[t] = in.rho[j];
+    if (in.ax) s.ax[t] = in.ax[j];
+    if (in.ay) s.ay[t] = in.ay[j];
+    if (in.drho) s.drho[t] = in.drho[j];
+    if (in.m) s.m[t] = in.m[j];
+    if (in.h) s.h[t] = in.h[j];
+    if (in.xsphx) s.xsphx[t] = in.xsphx[j];
+    if (in.xsphy) s.xsphy[t] = in.xsphy[j];
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -764,10 +913,20 @@ static int check_params(const sph_params_t *P) {
         return fail(SPH_EINVAL, "use_xsph / use_summation_density / dynamic_h must be 0 or 1 (and epsilon finite)");
     if (!(P->scale > 0) || !(P->qmax > 0) || !(P->rho0 > 0))
         return fail(SPH_EINVAL, "scale, qmax and rho0 must be positive");
+    if (P->coupled_integrator != SPH_COUPLED_NONE && P->coupled_integrator != SPH_COUPLED_NEWMARK)
+        return fail(SPH_EINVAL, "unknown coupled_integrator %d", P->coupled_integrator);
+    // NewmarkBeta.__init__ asserts beta * 2 >= gamma (src/Integrators/NewmarkBeta.py:33)
+    if (P->coupled_integrator == SPH_COUPLED_NEWMARK &&
+        !(P->newmark_beta * 2 >= P->newmark_gamma && P->newmark_beta - P->newmark_beta == 0.0 &&
+          P->newmark_gamma - P->newmark_gamma == 0.0))
+        return fail(SPH_EINVAL, "Newmark-beta needs finite beta, gamma with 2 beta >= gamma");
     return 0;
 }
 
 static void ctrl_from_params(Ctrl &h, const sph_params_t &P) {
+    h.nm = P.coupled_integrator == SPH_COUPLED_NEWMARK;
+    h.nm_beta = P.newmark_beta;
+    h.nm_gamma = P.newmark_gamma;
     h.duration = P.duration;
     h.fixed_dt = P.fixed_dt;
     h.damping = P.damping;
@@ -878,8 +1037,9 @@ int sph_ctx_destroy(sph_ctx_t *ctx) {
 
 int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params) {
     if (!ctx || !params) return fail(SPH_EINVAL, "NULL argument");
-    if (params->n != ctx->P.n || params->kernel != ctx->P.kernel || params->precision != ctx->P.precision)
-        return fail(SPH_EINVAL, "n, kernel and precision are fixed at creation");
+    if (params->n != ctx->P.n || params->kernel != ctx->P.kernel || params->precision != ctx->P.precision ||
+        params->use_xsph != ctx->P.use_xsph)
+        return fail(SPH_EINVAL, "n, kernel, precision and use_xsph are fixed at creation");
     int rc = check_params(params);
     if (rc) return rc;
     ctx->P = *params;
@@ -902,6 +1062,8 @@ int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock,
     hipStream_t s = S(stream);
     ctx->cur = 0;
     ctx->dt_fresh = false;
+    ctx->mid = false;
+    ctx->xs_fluid_zero = false;
     k_upload<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->buf[0], *st, ctx->n);
     HIPCHK(hipGetLastError());
     Ctrl h;
@@ -931,6 +1093,8 @@ int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *str
     hipStream_t s = S(stream);
     ctx->cur = 0;
     ctx->dt_fresh = false;
+    ctx->mid = false;
+    ctx->xs_fluid_zero = false;
     k_upload<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->buf[0], *st, ctx->n);
     k_restore_ctrl<<<1, 1, 0, s>>>(ctx->ctrl, t);
     HIPCHK(hipGetLastError());
@@ -939,7 +1103,8 @@ int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *str
 
 int sph_state_download(sph_ctx_t *ctx, const sph_state_t *st, void *stream) {
     if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
-    k_download<<<nblk(ctx->n), kBlock, 0, S(stream)>>>(ctx->buf[ctx->cur], *st, make_phys(ctx->P), ctx->n);
+    k_download<<<nblk(ctx->n), kBlock, 0, S(stream)>>>(ctx->buf[ctx->cur], *st, make_phys(ctx->P), ctx->n,
+                                                        ctx->xs_fluid_zero);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -952,6 +1117,11 @@ static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipSt
     }
     HIPCHK(hipEventRecord(v[used++], s));
     return 0;
+}
+
+// Step calls other than sph_step_end are refused while a split step is open.
+static int closed(sph_ctx *c) {
+    return c->mid ? fail(SPH_ESTATE, "a split step is open: call sph_step_end first") : 0;
 }
 
 // pred != 0: the PEC predictor runs inside the update (bounds and keys see
@@ -981,6 +1151,7 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
                                          c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict,
                                          lean, c->P.use_xsph, c->P.dynamic_h);
     HIPCHK(hipGetLastError());
+    if (!c->P.use_xsph) c->xs_fluid_zero = true;
     if (c->timing) {
         int rc = ev_record(c, c->ev_nn, c->n_nn, s);
         if (rc) return rc;
@@ -1078,6 +1249,7 @@ int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream
     if (!ctx || !tape) return fail(SPH_EINVAL, "NULL argument");
     if (tape->ctx != ctx) return fail(SPH_EINVAL, "tape belongs to another context");
     if (slot < 0 || slot >= tape->nslots) return fail(SPH_EINVAL, "slot %d out of range", slot);
+    if (closed(ctx)) return SPH_ESTATE;
     hipStream_t s = S(stream);
     int rc;
     if ((rc = dt_step(ctx, s))) return rc;
@@ -1120,6 +1292,7 @@ static int graph_launch(hipGraphExec_t *exec, hipStream_t g, F body) {
 }
 
 int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const sph_adjoint_t *adj, void *stream) {
+    if (ctx && closed(ctx)) return SPH_ESTATE;
     if (!ctx || !tape || !adj) return fail(SPH_EINVAL, "NULL argument");
     if (tape->ctx != ctx) return fail(SPH_EINVAL, "tape belongs to another context");
     if (nsteps < 1 || nsteps > tape->nslots) return fail(SPH_EINVAL, "nsteps %d out of 1..%d", nsteps, tape->nslots);
@@ -1161,39 +1334,46 @@ int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const 
 
 int sph_nn_update(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     return nn_update(ctx, S(stream));
 }
 
 int sph_compute(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     ctx->dt_fresh = false;
     return compute(ctx, S(stream));
 }
 
 int sph_continuity(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     ctx->dt_fresh = false;
     return compute(ctx, S(stream), nullptr, 1);
 }
 
 int sph_momentum(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     ctx->dt_fresh = false;
     return compute(ctx, S(stream), nullptr, 2);
 }
 
 int sph_dt(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     return dt_step(ctx, S(stream));
 }
 
 int sph_predict(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     return predict(ctx, S(stream));
 }
 
 int sph_correct(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     int rc = correct(ctx, S(stream));
     if (!rc) ctx->dt_fresh = true;
     return rc;
@@ -1201,6 +1381,7 @@ int sph_correct(sph_ctx_t *ctx, void *stream) {
 
 int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
     if (nsteps < 0) return fail(SPH_EINVAL, "nsteps < 0");
     hipStream_t s = S(stream);
     if (!use_graph || ctx->timing) {
@@ -1245,6 +1426,133 @@ int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream) {
     HIPCHK(hipStreamWaitEvent(s, e1, 0));
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    return 0;
+}
+
+// The loop body split at the coupling hook (src/Solver.py:389-392). The
+// first half writes the whole predicted state (no lean reorder), so what a
+// caller changes in between reaches the correctors as in the reference.
+int sph_step_begin(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (closed(ctx)) return SPH_ESTATE;
+    hipStream_t s = S(stream);
+    int rc;
+    if ((rc = dt_step(ctx, s))) return rc;
+    if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
+    if ((rc = compute(ctx, s))) return rc;
+    ctx->mid = true;
+    return 0;
+}
+
+int sph_step_end(sph_ctx_t *ctx, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (!ctx->mid) return fail(SPH_ESTATE, "no split step is open: call sph_step_begin first");
+    ctx->mid = false;
+    int rc = correct(ctx, S(stream));
+    if (!rc) ctx->dt_fresh = true;
+    return rc;
+}
+
+// Stage `ids` (host, validated) in ctx->bin and map every reference index to
+// its slot of the current buffer in ctx->slot_of. bin is scratch outside the
+// neighbour update.
+static int stage_ids(sph_ctx *c, const int64_t *ids, int64_t k, bool distinct, hipStream_t s) {
+    if (k < 0 || k > c->n) return fail(SPH_EINVAL, "k=%lld out of range (0 .. n=%d)", (long long)k, c->n);
+    if (k > 0 && !ids) return fail(SPH_EINVAL, "ids is NULL");
+    std::vector<int> v((size_t)k);
+    for (int64_t j = 0; j < k; ++j) {
+        if (ids[j] < 0 || ids[j] >= c->n)
+            return fail(SPH_EINVAL, "ids[%lld]=%lld outside 0 .. n-1=%d", (long long)j, (long long)ids[j], c->n - 1);
+        v[j] = (int)ids[j];
+    }
+    if (distinct) {
+        std::vector<int> w(v);
+        std::sort(w.begin(), w.end());
+        if (std::adjacent_find(w.begin(), w.end()) != w.end()) return fail(SPH_EINVAL, "ids repeat an index");
+    }
+    if (k == 0) return 0;
+    HIPCHK(hipMemcpyAsync(c->bin, v.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+    k_inverse<<<nblk(c->n), kBlock, 0, s>>>(c->buf[c->cur].id, c->slot_of, c->n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));  // v is freed on return
+    return 0;
+}
+
+int sph_state_gather(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_state_t *st, void *stream) {
+    if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
+    hipStream_t s = S(stream);
+    int rc = stage_ids(ctx, ids, k, false, s);
+    if (rc || k == 0) return rc;
+    k_gather_ids<<<nblk(k), kBlock, 0, s>>>(ctx->buf[ctx->cur], ctx->slot_of, ctx->bin, (int)k, *st,
+                                            make_phys(ctx->P), ctx->xs_fluid_zero);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int sph_state_scatter(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_state_t *st, void *stream) {
+    if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
+    if (st->label || st->p) return fail(SPH_EINVAL, "label and p cannot be written (p follows from rho)");
+    if (!ctx->P.use_xsph && (st->xsphx || st->xsphy))
+        return fail(SPH_EUNSUPPORTED, "without XSPH the XSPH velocity is not kept on the device");
+    hipStream_t s = S(stream);
+    int rc = stage_ids(ctx, ids, k, true, s);
+    if (rc || k == 0) return rc;
+    k_scatter_ids<<<nblk(k), kBlock, 0, s>>>(ctx->buf[ctx->cur], ctx->slot_of, ctx->bin, (int)k, *st);
+    HIPCHK(hipGetLastError());
+    ctx->dt_fresh = false;  // h or a may have changed under the corrector's dt partials
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int sph_near_pos(sph_ctx_t *ctx, int64_t npts, const double *x, const double *y, const double *h, int64_t *off,
+                 int32_t *idx, double *q, double *r, int64_t cap, int64_t *total, void *stream) {
+    if (!ctx || !off || !total) return fail(SPH_EINVAL, "NULL argument");
+    if (npts < 0 || npts > (1LL << 30)) return fail(SPH_EINVAL, "npts=%lld out of range", (long long)npts);
+    if (npts > 0 && (!x || !y || !h)) return fail(SPH_EINVAL, "x, y and h are needed");
+    hipStream_t s = S(stream);
+    int rc = sph_nn_grid(ctx, nullptr, nullptr, stream);
+    if (rc) return rc;
+    const int np = (int)npts;
+    State st = ctx->buf[ctx->cur];
+    if (np > 0)
+        k_near_pos<<<nblk(np), kBlock, 0, s>>>(ctx->ctrl, st, ctx->cellStart, np, x, y, h, 0, (long long *)off,
+                                                nullptr, nullptr, nullptr, ctx->P.qmax);
+    HIPCHK(hipGetLastError());
+    std::vector<long long> cnt((size_t)np + 1);
+    if (np > 0) HIPCHK(hipMemcpyAsync(cnt.data(), off, sizeof(long long) * np, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    long long acc = 0;
+    for (int i = 0; i < np; ++i) {
+        long long k = cnt[i];
+        cnt[i] = acc;
+        acc += k;
+    }
+    cnt[np] = acc;
+    *total = acc;
+    HIPCHK(hipMemcpyAsync(off, cnt.data(), sizeof(long long) * (np + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (acc > cap) return fail(SPH_ECAPACITY, "nearPos needs %lld entries, cap is %lld", acc, (long long)cap);
+    if (acc == 0) return 0;
+    if (!idx) return fail(SPH_EINVAL, "idx is NULL");
+    k_near_pos<<<nblk(np), kBlock, 0, s>>>(ctx->ctrl, st, ctx->cellStart, np, x, y, h, 1, (long long *)off, idx, q,
+                                            r, ctx->P.qmax);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int sph_probe_pressure(sph_ctx_t *ctx, int64_t npts, const double *x, const double *y, const double *h,
+                       double *rho, double *p, void *stream) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    if (npts < 0 || npts > (1LL << 30)) return fail(SPH_EINVAL, "npts=%lld out of range", (long long)npts);
+    if (npts > 0 && (!x || !y || !h || !rho || !p)) return fail(SPH_EINVAL, "x, y, h, rho and p are needed");
+    int rc = sph_nn_grid(ctx, nullptr, nullptr, stream);
+    if (rc || npts == 0) return rc;
+    const sph_params_t &P = ctx->P;
+    k_probe<<<nblk(npts), kBlock, 0, S(stream)>>>(ctx->ctrl, ctx->buf[ctx->cur], ctx->cellStart, (int)npts, x, y, h,
+                                                  P.kernel, P.qmax, P.rho0, P.gamma, P.B, rho, p);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

@@ -23,6 +23,7 @@
 namespace hipsph {
 
 constexpr int kFluid = 0;
+constexpr int kCoupled = 3;
 constexpr int kBlock = 256;        // threads per block of streaming kernels
 constexpr int kScanItems = 8;      // bins per thread in the cell scan
 constexpr int kScanTile = kBlock * kScanItems;
@@ -44,6 +45,8 @@ struct Ctrl {
     double alpha, beta, co, B, Pb;
     int advance_t;       // 0 while settling (t does not advance)
     int nn_valid;        // cell order and offsets match the positions
+    int nm;              // Newmark-beta on the coupled particles
+    double nm_beta, nm_gamma;
 };
 
 // Grid-wide reductions (bounds, dt) run over at most kRedBlocks workgroups in

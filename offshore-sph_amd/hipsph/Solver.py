@@ -26,10 +26,20 @@ Differences from the reference, each deliberate:
     the loop body this way.
   * ``p`` written back after ``run`` is the Tait pressure of the final
     density; the reference leaves the pressure of the last predicted density.
-  * Not on the device path (raise ``NotImplementedError``): the coupling
-    hooks. Dynamic h (``h=None``), XSPH and summation density run on the
-    device; PEC's ``useXSPH`` must equal the method's (the
-    reference's WCSPH docstring requires both on), otherwise ``ValueError``.
+  * Dynamic h (``h=None``), XSPH and summation density run on the device;
+    PEC's ``useXSPH`` must equal the method's (the reference's WCSPH
+    docstring requires both on), otherwise ``ValueError``.
+  * A coupling hook (``coupling``, ``couplingIntegrator=NewmarkBeta(...)``)
+    runs on the host between the device's loop and correctors, every step:
+    the active records come down, the hook edits the array, and the changed
+    x, y, vx, vy, rho, ax, ay, drho, m, h (and xsphx, xsphy with XSPH) go
+    back; its p (and xsphx, xsphy without XSPH) stay on the host; a change to
+    any other field raises ``NotImplementedError``.
+    Inside the hook ``solver.nn`` answers ``nearPos`` from the step's device
+    grid with the active records' positions (the reference reads the array it
+    is passed at the active indices, which differ from the full array's once
+    TempBoundary particles are deleted); ``hipsph.coupling.probe_pressure``
+    is the example's pressure probe on the device.
   * ``save``/``load`` use h5py with the reference's dataset names when h5py
     is importable, else a ``.npz`` with the same names.
 """
@@ -38,6 +48,7 @@ from time import perf_counter
 
 import numpy as np
 
+from . import coupling as _coupling
 from .Common import ParticleType, particle_dtype
 
 _TIMING_KEYS = ('total', 'storage', 'integrate_correction', 'integrate_prediction', 'compute', 'time_step',
@@ -100,7 +111,10 @@ class Solver:
         self.export = {}
         self.settleTime = 0.0
         self.dt_a, self.dt_c, self.dt_f = [], [], []
+        self.dt = 0.0
+        self.nn = None
         self._engine = None
+        self._host_hook = {}
 
     # -- particles ---------------------------------------------------------
     def load(self, file: str):
@@ -157,6 +171,12 @@ class Solver:
         self.fluid_count = int(np.sum(self.f_indexes))
 
     # -- device plumbing ------------------------------------------------------
+    def _check_coupling(self):
+        ci = self.couplingIntegrator
+        if ci is None or not hasattr(ci, 'beta') or not hasattr(ci, 'gamma'):
+            raise NotImplementedError('a coupling hook needs couplingIntegrator=NewmarkBeta(...): the '
+                                      'device integrates the coupled particles with Newmark-beta')
+
     def _make_engine(self, damping, advance_time):
         from .device import Engine, EngineParams, soa_from_particles
         if self._engine is not None:
@@ -167,6 +187,11 @@ class Solver:
                                      strict=bool(getattr(self.integrator, 'strict', False)))
         p.damping = damping
         p.dynamic_h = self.h is None
+        if self.coupling is not None:
+            self._check_coupling()
+            p.coupled_integrator = 'newmark'
+            p.newmark_beta = float(self.couplingIntegrator.beta)
+            p.newmark_gamma = float(self.couplingIntegrator.gamma)
         p.fixed_dt = float(self.timeStep) if self.timeStep is not None else 0.0
         p.duration = float(self.duration)
         e = Engine(p, device=self.device)
@@ -180,11 +205,15 @@ class Solver:
         """Device state -> active records of particleArray."""
         d = e.download()
         idx = np.nonzero(self.indexes)[0]
-        for k in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'p', 'xsphx', 'xsphy', 'h'):
+        for k in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'p', 'xsphx', 'xsphy', 'h', 'm'):
             self.particleArray[k][idx] = d[k]
+        for f, (rows, vals) in self._host_hook.items():  # the last coupling hook's values stand
+            self.particleArray[f][rows] = vals
 
     def _steps(self, e, n):
         """Launch n device steps; returns the steps actually taken."""
+        if self.coupling is not None:
+            return sum(self._coupled_step(e) for _ in range(n))
         before = e.status()['steps']
         e.step(n, graph=self.graph and n > 1)
         st = e.status()
@@ -197,6 +226,64 @@ class Solver:
         taken = st['steps'] - before
         self._collect_dt(e, st['steps'])
         self.t = st['t']
+        return taken
+
+    _HOOK_WRITABLE = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'xsphx', 'xsphy')
+    _HOOK_HOST_ONLY = ('p', 'c')
+
+    def _coupled_step(self, e):
+        """One step split at the coupling hook (src/Solver.py:374-398): the
+        device runs dt, the predictors (fluid and Newmark-beta) and the loop;
+        the hook sees and edits the particle array; what it changed in the
+        active records goes back to the device before the correctors. The
+        hook's pressures (and, without XSPH, XSPH velocities) are kept on the
+        host: the device derives p from rho and keeps no XSPH velocity then."""
+        before = e.status()['steps']
+        e.step_begin()
+        st = e.status()
+        taken = st['steps'] - before
+        if st['flags'] & 7:
+            e.step_end()
+            raise RuntimeError(f'device step halted (flags {st["flags"]:#x})')
+        if taken == 0:  # past the duration: nothing ran
+            e.step_end()
+            return 0
+        self._collect_dt(e, st['steps'])
+        self.dt = st['dt']
+        t0 = perf_counter()
+        self._host_hook = {}
+        self._pull(e)
+        idx = np.nonzero(self.indexes)[0]
+        snap = self.particleArray.copy()
+        self.nn = _coupling.EngineNN(e, self)
+        out = self.coupling(self.particleArray, self)
+        if out is not None:
+            self.particleArray = out
+        pA = self.particleArray
+        host_only = self._HOOK_HOST_ONLY
+        writable = self._HOOK_WRITABLE
+        if not e.params.use_xsph:  # the device keeps no XSPH velocity then
+            host_only = host_only + ('xsphx', 'xsphy')
+            writable = tuple(f for f in writable if f not in ('xsphx', 'xsphy'))
+        for f in pA.dtype.names:
+            if f in writable or f in host_only:
+                continue
+            if not _same(pA[f], snap[f]).all():
+                raise NotImplementedError(f'the coupling hook changed {f!r}; the device takes back only '
+                                          + ', '.join(writable))
+        rows = np.zeros(len(idx), dtype=bool)
+        for f in writable:
+            rows |= ~_same(pA[f][idx], snap[f][idx])
+        if rows.any():
+            sel = np.nonzero(rows)[0]
+            e.scatter(sel, {f: pA[f][idx[sel]] for f in writable})
+        for f in host_only:
+            ch = ~_same(pA[f][idx], snap[f][idx])
+            if ch.any():
+                self._host_hook[f] = (idx[ch], pA[f][idx[ch]].copy())
+        self.timing_data['coupling'] += perf_counter() - t0
+        e.step_end()
+        self.t = e.status()['t']
         return taken
 
     def _collect_dt(self, e, total):
@@ -245,7 +332,7 @@ class Solver:
                 int(np.sum(~self.particleArray['deleted'])) != self.num_particles:
             raise Exception('No or invalid particles set!')
         if self.coupling is not None:
-            raise NotImplementedError('coupling hooks are not on the device path')
+            self._check_coupling()
         println('Started solving...')
         t_step = 0
         self.t = 0.0
@@ -344,6 +431,11 @@ class Solver:
         if self._engine is not None:
             self._engine.close()
             self._engine = None
+
+
+def _same(a, b):
+    """Element-wise bit-level equality that treats NaN as equal to NaN."""
+    return (a == b) | ((a != a) & (b != b))
 
 
 def _write_store(location, d):

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, 'libhipsph.so')
 SPH_KERNEL_CUBIC, SPH_KERNEL_GAUSSIAN, SPH_KERNEL_WENDLAND = 0, 1, 2
 SPH_PREC_FP64, SPH_PREC_FP32C = 0, 1
 SPH_INTEGRATOR_PEC, SPH_INTEGRATOR_EULER = 0, 1
+SPH_COUPLED_NONE, SPH_COUPLED_NEWMARK = 0, 1
 
 SPH_FLAG_CAPACITY = 1
 SPH_FLAG_NONFINITE = 2
@@ -49,6 +50,8 @@ class sph_params_t(C.Structure):
         ('cell_capacity', C.c_int64),
         ('epsilon', C.c_double),
         ('dynamic_h', C.c_int32),
+        ('coupled_integrator', C.c_int32),
+        ('newmark_beta', C.c_double), ('newmark_gamma', C.c_double),
     ]
 
 
@@ -81,6 +84,12 @@ _PROTOS = {
     'sph_predict': (C.c_int, [_P, _P]),
     'sph_correct': (C.c_int, [_P, _P]),
     'sph_step': (C.c_int, [_P, _I64, _I32, _P]),
+    'sph_step_begin': (C.c_int, [_P, _P]),
+    'sph_step_end': (C.c_int, [_P, _P]),
+    'sph_state_gather': (C.c_int, [_P, _P, _I64, C.POINTER(sph_state_t), _P]),
+    'sph_state_scatter': (C.c_int, [_P, _P, _I64, C.POINTER(sph_state_t), _P]),
+    'sph_near_pos': (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _I64, C.POINTER(_I64), _P]),
+    'sph_probe_pressure': (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, _P]),
     'sph_status': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), _P]),
     'sph_dt_history': (C.c_int, [_P, C.POINTER(_D), _I64, _P]),
     'sph_set_clock': (C.c_int, [_P, _D, C.c_int32, _P]),

@@ -19,6 +19,9 @@ IO_F64 = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p', 'xsphx
 KERNELS = {'cubic': _lib.SPH_KERNEL_CUBIC, 'gaussian': _lib.SPH_KERNEL_GAUSSIAN, 'wendland': _lib.SPH_KERNEL_WENDLAND}
 PRECISIONS = {'fp64': _lib.SPH_PREC_FP64, 'fp32c': _lib.SPH_PREC_FP32C}
 INTEGRATORS = {'pec': _lib.SPH_INTEGRATOR_PEC, 'euler': _lib.SPH_INTEGRATOR_EULER}
+COUPLED = {'none': _lib.SPH_COUPLED_NONE, 'newmark': _lib.SPH_COUPLED_NEWMARK}
+GATHER_FIELDS = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p', 'label', 'xsphx', 'xsphy')
+SCATTER_FIELDS = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'xsphx', 'xsphy')
 
 
 def _torch():
@@ -60,6 +63,9 @@ class EngineParams:
     use_summation_density: bool = False
     epsilon: float = 0.5
     dynamic_h: bool = False
+    coupled_integrator: str = 'none'
+    newmark_beta: float = 0.25
+    newmark_gamma: float = 0.5
 
     def to_c(self):
         p = sph_params_t()
@@ -71,6 +77,10 @@ class EngineParams:
         p.use_xsph = int(bool(self.use_xsph))
         p.use_summation_density = int(bool(self.use_summation_density))
         p.dynamic_h = int(bool(self.dynamic_h))
+        ci = self.coupled_integrator
+        p.coupled_integrator = COUPLED[ci] if isinstance(ci, str) else int(ci)
+        p.newmark_beta = float(self.newmark_beta)
+        p.newmark_gamma = float(self.newmark_gamma)
         for f in ('rho0', 'gamma', 'co', 'B', 'Pb', 'alpha', 'beta', 'r0', 'D', 'p1', 'p2', 'gravity', 'scale',
                   'qmax', 'gamma_c', 'gamma_f', 'damping', 'fixed_dt', 'duration', 'epsilon'):
             setattr(p, f, float(getattr(self, f)))
@@ -208,6 +218,92 @@ class Engine:
 
     def step(self, nsteps=1, graph=False):
         call('sph_step', self._ctx, int(nsteps), int(bool(graph)), self._stream())
+
+    def step_begin(self):
+        """First half of a step, up to the coupling hook (sph_step_begin)."""
+        call('sph_step_begin', self._ctx, self._stream())
+
+    def step_end(self):
+        """Second half: the correctors and the clock (sph_step_end)."""
+        call('sph_step_end', self._ctx, self._stream())
+
+    # -- records by reference index (the coupling hook's view) ---------------
+    @staticmethod
+    def _ids(ids):
+        return np.ascontiguousarray(np.asarray(ids, dtype=np.int64).reshape(-1))
+
+    def gather(self, ids, fields=('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay')):
+        """{field: numpy array} of the records with reference indices ids."""
+        import torch
+        ids = self._ids(ids)
+        k = len(ids)
+        out = {f: torch.empty(k, dtype=torch.int8 if f == 'label' else torch.float64, device=self.device)
+               for f in fields}
+        st = sph_state_t()
+        for f in fields:
+            if f not in GATHER_FIELDS:
+                raise KeyError(f)
+            setattr(st, f, out[f].data_ptr() if k else None)
+        call('sph_state_gather', self._ctx, ids.ctypes.data_as(C.c_void_p), k, C.byref(st), self._stream())
+        return {f: v.cpu().numpy() for f, v in out.items()}
+
+    def scatter(self, ids, values):
+        """Write {field: array in the order of ids} into the records ids."""
+        import torch
+        ids = self._ids(ids)
+        k = len(ids)
+        st = sph_state_t()
+        keep = []
+        for f, v in values.items():
+            if f not in SCATTER_FIELDS:
+                raise KeyError(f'{f} cannot be written (writable: {", ".join(SCATTER_FIELDS)})')
+            t = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64) if not isinstance(v, torch.Tensor) else v,
+                                dtype=torch.float64).to(self.device).reshape(-1).contiguous()
+            if t.numel() != k:
+                raise ValueError(f'{f} has {t.numel()} entries for {k} ids')
+            keep.append(t)
+            setattr(st, f, t.data_ptr() if k else None)
+        call('sph_state_scatter', self._ctx, ids.ctypes.data_as(C.c_void_p), k, C.byref(st), self._stream())
+
+    def near_pos(self, x, y, h):
+        """NNLinkedList.nearPos for points against the last update: CSR (off, idx, q, r), numpy."""
+        import torch
+        dev = self.device
+        pts = [torch.as_tensor(np.atleast_1d(np.asarray(a, dtype=np.float64))).to(dev).contiguous() for a in (x, y)]
+        if pts[0].shape != pts[1].shape or pts[0].dim() != 1:
+            raise ValueError('x and y must be 1-D and of one length')
+        hh = torch.as_tensor(np.broadcast_to(np.asarray(h, dtype=np.float64), pts[0].shape).copy()).to(dev)
+        npts = pts[0].numel()
+        off = torch.zeros(npts + 1, dtype=torch.int64, device=dev)
+        total = C.c_int64(0)
+        args = (pts[0].data_ptr(), pts[1].data_ptr(), hh.data_ptr(), off.data_ptr())
+        rc = _lib.load().sph_near_pos(self._ctx, npts, *args, None, None, None, 0, C.byref(total), self._stream())
+        if rc not in (0, -4):
+            _lib.check(rc)
+        cap = max(int(total.value), 1)
+        idx = torch.zeros(cap, dtype=torch.int32, device=dev)
+        q = torch.zeros(cap, dtype=torch.float64, device=dev)
+        r = torch.zeros(cap, dtype=torch.float64, device=dev)
+        call('sph_near_pos', self._ctx, npts, *args, idx.data_ptr(), q.data_ptr(), r.data_ptr(), cap,
+             C.byref(total), self._stream())
+        t = int(total.value)
+        return off.cpu().numpy(), idx[:t].cpu().numpy(), q[:t].cpu().numpy(), r[:t].cpu().numpy()
+
+    def probe_pressure(self, x, y, h):
+        """pressure_SPH of the coupled example at points (x, y) with length h: (rho, p), numpy."""
+        import torch
+        dev = self.device
+        px, py = [torch.as_tensor(np.atleast_1d(np.asarray(a, dtype=np.float64))).to(dev).contiguous()
+                  for a in (x, y)]
+        if px.shape != py.shape or px.dim() != 1:
+            raise ValueError('x and y must be 1-D and of one length')
+        ph = torch.as_tensor(np.broadcast_to(np.asarray(h, dtype=np.float64), px.shape).copy()).to(dev)
+        n = px.numel()
+        rho = torch.empty(n, dtype=torch.float64, device=dev)
+        p = torch.empty(n, dtype=torch.float64, device=dev)
+        call('sph_probe_pressure', self._ctx, n, px.data_ptr(), py.data_ptr(), ph.data_ptr(), rho.data_ptr(),
+             p.data_ptr(), self._stream())
+        return rho.cpu().numpy(), p.cpu().numpy()
 
     def status(self):
         d = (C.c_double * 4)()

@@ -43,7 +43,8 @@ class ref_solver_t(C.Structure):
     _fields_ = [('kernel', C.c_int32), ('integrator', C.c_int32), ('pec_useXSPH', C.c_int32),
                 ('pec_strict', C.c_int32), ('h_fixed', C.c_double), ('damping', C.c_double),
                 ('gamma_c', C.c_double), ('gamma_f', C.c_double), ('t', C.c_double), ('t_step', C.c_int64),
-                ('nthreads', C.c_int32)]
+                ('nthreads', C.c_int32), ('coupled_integrator', C.c_int32), ('nm_beta', C.c_double),
+                ('nm_gamma', C.c_double), ('dt', C.c_double)]
 
 
 _lib = None
@@ -81,6 +82,12 @@ def load():
                                       C.POINTER(ref_nn_t), p]),
         'ref_solver_run': (C.c_int, [p, i64, C.POINTER(ref_solver_t), C.POINTER(ref_wcsph_t),
                                      C.POINTER(ref_nn_t), i64, p]),
+        'ref_solver_begin': (C.c_int, [p, i64, C.POINTER(ref_solver_t), C.POINTER(ref_wcsph_t),
+                                       C.POINTER(ref_nn_t), p]),
+        'ref_solver_end': (C.c_int, [p, i64, C.POINTER(ref_solver_t)]),
+        'ref_nn_near_pos': (i64, [C.POINTER(ref_nn_t), d, d, d, p, p, p, p, p, i64]),
+        'ref_newmark_predict': (None, [d, d, d, p, i64]), 'ref_newmark_correct': (None, [d, d, d, p, i64]),
+        'ref_probe_pressure': (i64, [C.POINTER(ref_nn_t), p, i64, i32, C.POINTER(ref_wcsph_t), d, d, d, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -151,6 +158,14 @@ class NN:
         assert tot >= 0
         return {'keys': keys, 'off': off, 'idx': idx[:tot], 'q': q[:tot], 'r': r[:tot]}
 
+    def near_pos(self, x, y, h, pA):
+        """nearPos (src/Tools/NNLinkedList.py:41-80): (h_ij, q, r, idx) against the last update."""
+        lib = load()
+        k = lib.ref_nn_near_pos(C.byref(self.s), x, y, h, _ptr(pA), None, None, None, None, 0)
+        hh, q, r, idx = np.zeros(k), np.zeros(k), np.zeros(k), np.zeros(k, np.int64)
+        lib.ref_nn_near_pos(C.byref(self.s), x, y, h, _ptr(pA), _ptr(hh), _ptr(q), _ptr(r), _ptr(idx), k)
+        return hh, q, r, idx
+
     def __del__(self):
         try:
             load().ref_nn_free(C.byref(self.s))
@@ -172,9 +187,10 @@ class Solver:
     """The reference's Solver.run loop body on a particle array (CPU, in place)."""
 
     def __init__(self, pA, method, kernel='cubic', integrator='pec', h=None, damping=0.0, strict=False,
-                 nthreads=1, scale=2.0, gamma_c=0.25, gamma_f=0.25, pec_xsph=None):
+                 nthreads=1, scale=2.0, gamma_c=0.25, gamma_f=0.25, pec_xsph=None, newmark=None):
         """pec_xsph: PEC(useXSPH=...); default: the method's useXSPH (the
-        reference requires the two to agree)."""
+        reference requires the two to agree). newmark: (beta, gamma) of the
+        coupling integrator for the coupled particles, None for none."""
         self.pA = as_particles(pA)
         self.m = wcsph(method)
         self.nn = NN(scale)
@@ -188,7 +204,11 @@ class Solver:
         s.gamma_c = gamma_c
         s.gamma_f = gamma_f
         s.nthreads = nthreads
+        if newmark is not None:
+            s.coupled_integrator = 1
+            s.nm_beta, s.nm_gamma = float(newmark[0]), float(newmark[1])
         self.s = s
+        self.kernel = kernel
 
     def run(self, nsteps):
         hist = np.zeros((nsteps, 3))
@@ -197,6 +217,26 @@ class Solver:
         if rc:
             raise RuntimeError(f'ref_solver_run failed: {rc}')
         return hist
+
+    def begin(self):
+        """The loop body up to the coupling hook (src/Solver.py:366-392); returns (dt, dt_c, dt_f)."""
+        out = np.zeros(3)
+        rc = load().ref_solver_begin(_ptr(self.pA), len(self.pA), C.byref(self.s), C.byref(self.m),
+                                     C.byref(self.nn.s), _ptr(out))
+        if rc:
+            raise RuntimeError(f'ref_solver_begin failed: {rc}')
+        return out
+
+    def end(self):
+        """The loop body after the coupling hook (src/Solver.py:394-410)."""
+        rc = load().ref_solver_end(_ptr(self.pA), len(self.pA), C.byref(self.s))
+        if rc:
+            raise RuntimeError(f'ref_solver_end failed: {rc}')
+
+    def probe(self, x, y, h):
+        """IceBreak's pressure_SPH at points (x, y) with smoothing length h,
+        against this solver's last neighbour update; returns (rho, p)."""
+        return probe_pressure(self.nn, self.pA, self.kernel, self.m, x, y, h)
 
     @property
     def t(self):
@@ -271,3 +311,29 @@ def pec_correct(pA, dt, damping=0.0, useXSPH=False, strict=False):
 def euler_correct(pA, dt, damping=0.0):
     load().ref_euler_correct(dt, _ptr(pA), len(pA), damping)
     return pA
+
+
+def newmark_predict(pA, dt, beta, gamma):
+    load().ref_newmark_predict(beta, gamma, dt, _ptr(pA), len(pA))
+    return pA
+
+
+def newmark_correct(pA, dt, beta, gamma):
+    load().ref_newmark_correct(beta, gamma, dt, _ptr(pA), len(pA))
+    return pA
+
+
+def probe_pressure(nn, pA, kernel, method, x, y, h):
+    """examples/IceBreak.py:253-285 at every point (x[k], y[k]): (rho, p)."""
+    m = method if isinstance(method, ref_wcsph_t) else wcsph(method)
+    x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    y = np.atleast_1d(np.asarray(y, dtype=np.float64))
+    h = np.broadcast_to(np.asarray(h, dtype=np.float64), x.shape)
+    rho, p = np.zeros(len(x)), np.zeros(len(x))
+    out = np.zeros(2)
+    for k in range(len(x)):
+        if load().ref_probe_pressure(C.byref(nn.s), _ptr(pA), len(pA), KERNELS[kernel], C.byref(m), x[k], y[k],
+                                     h[k], _ptr(out)) < 0:
+            raise MemoryError('ref_probe_pressure')
+        rho[k], p[k] = out
+    return rho, p

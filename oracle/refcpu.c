@@ -660,26 +660,35 @@ int ref_loop(ref_particle_t *pA, int64_t n, int32_t kernel, const ref_wcsph_t *m
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-    ref_particle_t *fluid, *active;
-    int64_t *fidx, *aidx;
-    int64_t nf, na, cap;
+    ref_particle_t *fluid, *active, *coupled;
+    int64_t *fidx, *aidx, *cidx;
+    int64_t nf, na, nc, cap;
 } ref_masks_t;
 
+static void masks_free(ref_masks_t *mk) {
+    free(mk->fluid); free(mk->active); free(mk->coupled);
+    free(mk->fidx); free(mk->aidx); free(mk->cidx);
+}
+
+/* f_indexes, c_indexes and indexes (src/Solver.py:178-181) */
 static int masks_build(ref_masks_t *mk, const ref_particle_t *pA, int64_t n) {
     if (n > mk->cap) {
-        free(mk->fluid); free(mk->active); free(mk->fidx); free(mk->aidx);
+        masks_free(mk);
         mk->fluid = (ref_particle_t *)malloc(sizeof(ref_particle_t) * n);
         mk->active = (ref_particle_t *)malloc(sizeof(ref_particle_t) * n);
+        mk->coupled = (ref_particle_t *)malloc(sizeof(ref_particle_t) * n);
         mk->fidx = (int64_t *)malloc(sizeof(int64_t) * n);
         mk->aidx = (int64_t *)malloc(sizeof(int64_t) * n);
+        mk->cidx = (int64_t *)malloc(sizeof(int64_t) * n);
         mk->cap = n;
-        if (!mk->fluid || !mk->active || !mk->fidx || !mk->aidx) return -1;
+        if (!mk->fluid || !mk->active || !mk->coupled || !mk->fidx || !mk->aidx || !mk->cidx) return -1;
     }
-    mk->nf = mk->na = 0;
+    mk->nf = mk->na = mk->nc = 0;
     for (int64_t i = 0; i < n; ++i) {
         if (pA[i].deleted) continue;
         mk->aidx[mk->na++] = i;
         if (pA[i].label == REF_FLUID) mk->fidx[mk->nf++] = i;
+        if (pA[i].label == REF_COUPLED) mk->cidx[mk->nc++] = i;
     }
     return 0;
 }
@@ -692,19 +701,27 @@ static void scatter(ref_particle_t *dst, const ref_particle_t *src, const int64_
     for (int64_t j = 0; j < k; ++j) dst[idx[j]] = src[j];
 }
 
-static int solver_step_masked(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn,
-                              ref_masks_t *mk, double dt_out[3]) {
-    (void)pA; (void)n;
+/* The loop body up to the coupling hook (src/Solver.py:366-392): dt, the
+ * predictors, _compute. The dt is kept in s->dt for the second half. */
+static int solver_begin_masked(ref_particle_t *pA, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn,
+                               ref_masks_t *mk, double dt_out[3]) {
     double ts[3];
     /* :210-230 _minTimeStep on the fluid copy */
     gather(mk->fluid, pA, mk->fidx, mk->nf);
     ref_timestep(mk->nf, mk->fluid, s->gamma_c, s->gamma_f, ts);
     double dt = ts[0];
+    s->dt = dt;
     if (dt_out) { dt_out[0] = ts[0]; dt_out[1] = ts[1]; dt_out[2] = ts[2]; }
     /* :380 predict on the fluid copy */
     if (s->integrator == REF_INTEGRATOR_PEC) {
         ref_pec_predict(s->pec_useXSPH, s->pec_strict, dt, mk->fluid, mk->nf, s->damping);
         scatter(pA, mk->fluid, mk->fidx, mk->nf);
+    }
+    /* :381-382 the coupling integrator's predict on the coupled copy */
+    if (s->coupled_integrator) {
+        gather(mk->coupled, pA, mk->cidx, mk->nc);
+        ref_newmark_predict(s->nm_beta, s->nm_gamma, dt, mk->coupled, mk->nc);
+        scatter(pA, mk->coupled, mk->cidx, mk->nc);
     }
     /* :232-256 _compute: nn.update on the active copy, h for fluid, _loop */
     gather(mk->active, pA, mk->aidx, mk->na);
@@ -725,6 +742,13 @@ static int solver_step_masked(ref_particle_t *pA, int64_t n, ref_solver_t *s, co
     gather(mk->active, pA, mk->aidx, mk->na);
     if (ref_loop(mk->active, mk->na, s->kernel, m, nn, s->nthreads)) return -1;
     scatter(pA, mk->active, mk->aidx, mk->na);
+    return 0;
+}
+
+/* The loop body after the coupling hook (src/Solver.py:394-410): the
+ * correctors with the dt of the first half, then the clock. */
+static void solver_end_masked(ref_particle_t *pA, ref_solver_t *s, ref_masks_t *mk) {
+    const double dt = s->dt;
     /* :396 correct on the fluid copy */
     gather(mk->fluid, pA, mk->fidx, mk->nf);
     if (s->integrator == REF_INTEGRATOR_PEC)
@@ -732,17 +756,23 @@ static int solver_step_masked(ref_particle_t *pA, int64_t n, ref_solver_t *s, co
     else
         ref_euler_correct(dt, mk->fluid, mk->nf, s->damping);
     scatter(pA, mk->fluid, mk->fidx, mk->nf);
+    /* :397-398 the coupling integrator's correct on the coupled copy */
+    if (s->coupled_integrator) {
+        gather(mk->coupled, pA, mk->cidx, mk->nc);
+        ref_newmark_correct(s->nm_beta, s->nm_gamma, dt, mk->coupled, mk->nc);
+        scatter(pA, mk->coupled, mk->cidx, mk->nc);
+    }
     s->t += dt;
     s->t_step += 1;
-    return 0;
 }
 
 int ref_solver_step(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn, double dt_out[3]) {
     ref_masks_t mk;
     memset(&mk, 0, sizeof(mk));
     int rc = masks_build(&mk, pA, n);
-    if (!rc) rc = solver_step_masked(pA, n, s, m, nn, &mk, dt_out);
-    free(mk.fluid); free(mk.active); free(mk.fidx); free(mk.aidx);
+    if (!rc) rc = solver_begin_masked(pA, s, m, nn, &mk, dt_out);
+    if (!rc) solver_end_masked(pA, s, &mk);
+    masks_free(&mk);
     return rc;
 }
 
@@ -751,8 +781,91 @@ int ref_solver_run(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcs
     ref_masks_t mk;
     memset(&mk, 0, sizeof(mk));
     int rc = masks_build(&mk, pA, n);
-    for (int64_t k = 0; k < nsteps && !rc; ++k)
-        rc = solver_step_masked(pA, n, s, m, nn, &mk, dt_hist ? dt_hist + 3 * k : NULL);
-    free(mk.fluid); free(mk.active); free(mk.fidx); free(mk.aidx);
+    for (int64_t k = 0; k < nsteps && !rc; ++k) {
+        rc = solver_begin_masked(pA, s, m, nn, &mk, dt_hist ? dt_hist + 3 * k : NULL);
+        if (!rc) solver_end_masked(pA, s, &mk);
+    }
+    masks_free(&mk);
     return rc;
+}
+
+int ref_solver_begin(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn,
+                     double dt_out[3]) {
+    ref_masks_t mk;
+    memset(&mk, 0, sizeof(mk));
+    int rc = masks_build(&mk, pA, n);
+    if (!rc) rc = solver_begin_masked(pA, s, m, nn, &mk, dt_out);
+    masks_free(&mk);
+    return rc;
+}
+
+int ref_solver_end(ref_particle_t *pA, int64_t n, ref_solver_t *s) {
+    ref_masks_t mk;
+    memset(&mk, 0, sizeof(mk));
+    int rc = masks_build(&mk, pA, n);
+    if (!rc) solver_end_masked(pA, s, &mk);
+    masks_free(&mk);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Coupling: Newmark-beta (src/Integrators/NewmarkBeta.py) and the SPH      */
+/* pressure probe of the coupled IceBreak example (examples/IceBreak.py).   */
+/* ------------------------------------------------------------------------ */
+
+/* NewmarkBeta.predict :41-52 (damping is not used there) */
+void ref_newmark_predict(double beta, double gamma, double dt, ref_particle_t *pA, int64_t J) {
+    for (int64_t j = 0; j < J; ++j) {
+        ref_particle_t *p = &pA[j];
+        p->x = p->x + p->vx * dt + p->ax * (0.5 - beta) * (dt * dt);
+        p->y = p->y + p->vy * dt + p->ay * (0.5 - beta) * (dt * dt);
+        p->vx = p->vx + p->ax * (1 - gamma) * dt;
+        p->vy = p->vy + p->ay * (1 - gamma) * dt;
+    }
+}
+
+/* NewmarkBeta.correct :54-66 */
+void ref_newmark_correct(double beta, double gamma, double dt, ref_particle_t *pA, int64_t J) {
+    for (int64_t j = 0; j < J; ++j) {
+        ref_particle_t *p = &pA[j];
+        p->x = p->x + p->ax * beta * (dt * dt);
+        p->y = p->y + p->ay * beta * (dt * dt);
+        p->vx = p->vx + p->ax * gamma * dt;
+        p->vy = p->vy + p->ay * gamma * dt;
+    }
+}
+
+/* pressure_SPH (examples/IceBreak.py:253-285) at the point (x, y): the
+ * neighbours of nearPos (src/Tools/NNLinkedList.py:41-80) against the last
+ * update, W(r, h) with the probe's own h (:274), the Shepard-normalised
+ * weights (src/Equations/Shepard.py:5-12: fluid with rho >= 1e-3 in the
+ * normaliser), the summation density over the fluid
+ * (src/Equations/SummationDensity.py:5-13) and the Tait pressure
+ * ((rho / rho0)^gamma - 1) B (:282-284). out = (rho, p); returns the
+ * neighbour count or -1. */
+int64_t ref_probe_pressure(const ref_nn_t *nn, const ref_particle_t *pA, int64_t n, int32_t kernel,
+                           const ref_wcsph_t *m, double x, double y, double h, double out[2]) {
+    int64_t J = ref_nn_near_pos(nn, x, y, h, pA, NULL, NULL, NULL, NULL, 0);
+    double *r = (double *)malloc(sizeof(double) * (J > 0 ? J : 1));
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (J > 0 ? J : 1));
+    if (!r || !idx) { free(r); free(idx); return -1; }
+    ref_nn_near_pos(nn, x, y, h, pA, NULL, NULL, r, idx, J);
+    (void)n;
+    double norm = 0.0;
+    for (int64_t j = 0; j < J; ++j) {
+        const ref_particle_t *b = &pA[idx[j]];
+        if (b->rho < 1e-3 || b->label != REF_FLUID) continue;
+        norm += kernel_w(kernel, r[j], h) * b->m / b->rho;
+    }
+    double rho = 0.0;
+    for (int64_t j = 0; j < J; ++j) {
+        const ref_particle_t *b = &pA[idx[j]];
+        if (b->label != REF_FLUID) continue;
+        rho += b->m * (kernel_w(kernel, r[j], h) / norm);
+    }
+    out[0] = rho;
+    out[1] = ref_tait_eos(m->gamma, m->B, m->rho0, rho, REF_FLUID);
+    free(r);
+    free(idx);
+    return J;
 }

@@ -56,6 +56,17 @@
  *                         zero continuity, momentum with the new density
  *   pec XSPH branch       reduces bit for bit to the plain branch when the
  *                         XSPH velocity equals the one that branch uses
+ *   newmark predict /   the textbook Newmark-beta update with a_n and
+ *   correct               a_n+1 (closed form); constant acceleration exact;
+ *                         an undamped multi-node oscillator (average
+ *                         acceleration, beta 1/4, gamma 1/2) conserves its
+ *                         discrete energy
+ *   solver begin / end    begin + end = step, bit for bit; coupled particles
+ *                         integrated only with the coupling integrator
+ *   probe pressure        Shepard + summation density of a uniform density
+ *                         field returns it (Tait 0 at rho0); brute force over
+ *                         the 3x3-cell rule with random densities; no fluid
+ *                         neighbour gives rho 0, p = -B
  *   dynamic h (solver   h = 1.3 (m/rho)^(1/2) of the predicted density
  *   step, h_fixed = 0)    after a step, walls unchanged; 1.3 dx on a lattice
  *                         at rho = m / dx^2; 0 for a zero density. The square
@@ -117,6 +128,10 @@ typedef struct {
     double t;
     int64_t t_step;
     int32_t nthreads;    /* 1 = serial like the reference; >1 = OpenMP over fluid */
+    int32_t coupled_integrator; /* 1: Newmark-beta on the coupled particles (Solver(coupling=...,
+                                 * couplingIntegrator=NewmarkBeta(...)), src/Solver.py:381-398) */
+    double nm_beta, nm_gamma;
+    double dt;           /* the dt of the current step (set by the first half) */
 } ref_solver_t;
 
 /* --- L1: kernels (src/Kernels/CubicSpline.py, src/Kernels/Gaussian.py) --- */
@@ -167,6 +182,17 @@ int ref_loop(ref_particle_t *pA, int64_t n, int32_t kernel, const ref_wcsph_t *m
 int ref_solver_step(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn, double dt_out[3]);
 int ref_solver_run(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn,
                    int64_t nsteps, double *dt_hist);
+/* The same body split at the coupling hook (src/Solver.py:389-392): begin = dt,
+ * predictors, _compute; end = correctors and the clock. */
+int ref_solver_begin(ref_particle_t *pA, int64_t n, ref_solver_t *s, const ref_wcsph_t *m, ref_nn_t *nn,
+                     double dt_out[3]);
+int ref_solver_end(ref_particle_t *pA, int64_t n, ref_solver_t *s);
+
+/* --- coupling: src/Integrators/NewmarkBeta.py, examples/IceBreak.py:253-285 --- */
+void ref_newmark_predict(double beta, double gamma, double dt, ref_particle_t *pA, int64_t J);
+void ref_newmark_correct(double beta, double gamma, double dt, ref_particle_t *pA, int64_t J);
+int64_t ref_probe_pressure(const ref_nn_t *nn, const ref_particle_t *pA, int64_t n, int32_t kernel,
+                           const ref_wcsph_t *m, double x, double y, double h, double out[2]);
 
 #ifdef __cplusplus
 }

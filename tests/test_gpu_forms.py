@@ -15,6 +15,8 @@ tests/test_oracle_invariants.py. Here, through the C ABI:
   * dynamic h: FP64 cubic trajectories, dt series and h bit-identical over
     200 steps (PEC, Euler) and with XSPH and summation density on as well;
   * hipGraph replay equals the eager steps with both on;
+  * without XSPH the fluid's XSPH velocity reads 0 after a loop, the walls'
+    keeps its value;
   * what the device path does not cover fails loudly: the split passes and
     the adjoint tape with either form on.
 """
@@ -123,6 +125,35 @@ def test_graph_replay_with_xsph_and_summation_density():
         e.close()
     for k in STATE:
         np.testing.assert_array_equal(outs[1][k], outs[0][k], err_msg=k)
+
+
+def test_xsph_velocity_without_xsph():
+    """Without XSPH the loop sets the fluid's XSPH velocity to 0 and leaves
+    the walls' (WCSPH.py:187-189): after steps the download says so, however
+    the buffer memory was used before; use_xsph is fixed for a context."""
+    sc = scenarios.dam_break_c1()
+    pA = sc.pA.copy()
+    rng = np.random.default_rng(12)
+    pA['xsphx'] = rng.normal(size=len(pA))
+    pA['xsphy'] = rng.normal(size=len(pA))
+    warm = make_engine(pA, _method(sc, True, False))  # leaves XSPH velocities in device memory
+    warm.step(3)
+    warm.close()
+    e = make_engine(pA, sc.method())
+    d0 = e.download()
+    np.testing.assert_array_equal(d0['xsphx'], pA['xsphx'])  # before any loop: as uploaded
+    e.step(5)
+    d = e.download()
+    f = pA['label'] == 0
+    assert np.all(d['xsphx'][f] == 0.0) and np.all(d['xsphy'][f] == 0.0)
+    np.testing.assert_array_equal(d['xsphx'][~f], pA['xsphx'][~f])
+    np.testing.assert_array_equal(d['xsphy'][~f], pA['xsphy'][~f])
+    ids = np.nonzero(f)[0][:5]
+    with pytest.raises(HipSphError):
+        e.scatter(ids, {'xsphx': np.ones(5)})
+    with pytest.raises(HipSphError):
+        e.set_params(use_xsph=True)
+    e.close()
 
 
 def test_unsupported_combinations_fail_loudly():

@@ -85,7 +85,8 @@ def test_solver_mirror_signature_and_host_policy():
     assert sv._next_stop(3) == 1
     sv2 = Solver(None, None, None, exportProperties=[], incrementalWriteout=False, chunk=64)
     assert sv2._next_stop(7) == 64
-    # the coupling hooks (src/Solver.py:381-398) are not on the device path
+    # a coupling hook (src/Solver.py:381-398) needs the coupled particles'
+    # integrator on the device: Newmark-beta; without one, run() refuses
     from hipsph import scenarios
     from hipsph.Integrators.PEC import PEC
     from hipsph.Kernels.CubicSpline import CubicSpline
