@@ -26,7 +26,8 @@
 //     the support needs no cut-off test (-6 % on the FP64 pass). The float
 //     pass walks the three cells of each row as one run of slots, and the
 //     walk loads the next candidate's record before testing the current one
-//     (-11 % cubic, -25 % Gaussian on the float pass). Pass 2
+//     (-11 % cubic, -25 % Gaussian on the float pass); the float cubic and
+//     Wendland list loops do the same (-4.5 %). Pass 2
 //     runs the costly pair arithmetic over the list only, so lanes of a wave
 //     stay busy instead of idling through candidates that add nothing. The list
 //     is flushed when full, which keeps the order of the sums.
@@ -270,8 +271,8 @@ __device__ inline float w_f(float q, float hinv) {
 // every listed neighbour, walls included (src/Equations/XSPH.py:26-31:
 // fac = -epsilon m_j W_ij / rho_ij times v_i - v_j).
 template <typename R, int KERNEL, int MODE, bool XS = false>
-__device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, unsigned u, Sums<R> &S) {
-    const typename Rec<R>::A a = P.A[u];
+__device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, unsigned u,
+                                  const typename Rec<R>::A &a, Sums<R> &S) {
     if constexpr (sizeof(R) == 8) {
         // Reference expression order: Continuity.py:13-16, Momentum.py:36-56,
         // BoundaryForce.py:31-41.
@@ -374,7 +375,32 @@ __device__ inline void contribute(const PairArgs<R> &P, const Target<R> &T, cons
 template <typename R, int KERNEL, int MODE, bool XS = false>
 __device__ inline void flush(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, const unsigned *list, int nlist,
                              Sums<R> &S) {
-    for (int j = 0; j < nlist; ++j) contribute<R, KERNEL, MODE, XS>(P, T, ph, list[j * blockDim.x], S);
+    // The double pass is register-bound, and the Gaussian float pass (42
+    // contributors, lists flushed together) lost 2.5 % to the lower occupancy:
+    // no second record in flight there.
+    if constexpr (sizeof(R) == 8 || KERNEL == SPH_KERNEL_GAUSSIAN) {
+        for (int j = 0; j < nlist; ++j) {
+            const unsigned u = list[j * blockDim.x];
+            contribute<R, KERNEL, MODE, XS>(P, T, ph, u, P.A[u], S);
+        }
+        return;
+    }
+    // software-pipelined as the walk: the next listed record is in flight
+    unsigned un = 0;
+    typename Rec<R>::A an{};
+    if (nlist > 0) {
+        un = list[0];
+        an = P.A[un];
+    }
+    for (int j = 0; j < nlist; ++j) {
+        const unsigned u = un;
+        const typename Rec<R>::A a = an;
+        if (j + 1 < nlist) {
+            un = list[(j + 1) * blockDim.x];
+            an = P.A[un];
+        }
+        contribute<R, KERNEL, MODE, XS>(P, T, ph, u, a, S);
+    }
 }
 
 // Pass 1 over the candidates [s, e) of one cell (descending for the reference
