@@ -418,14 +418,21 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
     const R r0sq = (R)ph.r0 * (R)ph.r0 * R(1.0 + kEps);
     int cnt = 0;
     const int len = e - s;
-    // software-pipelined: the next candidate's record is in flight while this
-    // one is tested
-    typename Rec<R>::A nxt{};
+    // software-pipelined: the next candidate's record (the float pass: the
+    // next two) is in flight while this one is tested
+    constexpr int kAhead = sizeof(R) == 4 ? 2 : 1;
+    typename Rec<R>::A nxt{}, nxt2{};
     if (len > 0) nxt = P.A[(unsigned)(DESC ? e - 1 : s)];
+    if (kAhead == 2 && len > 1) nxt2 = P.A[(unsigned)(DESC ? e - 2 : s + 1)];
     for (int k = 0; k < len; ++k) {
         const unsigned u = (unsigned)(DESC ? e - 1 - k : s + k);
         const typename Rec<R>::A a = nxt;
-        if (k + 1 < len) nxt = P.A[(unsigned)(DESC ? e - 2 - k : s + k + 1)];
+        if constexpr (kAhead == 2) {
+            nxt = nxt2;
+            if (k + 2 < len) nxt2 = P.A[(unsigned)(DESC ? e - 3 - k : s + k + 2)];
+        } else {
+            if (k + 1 < len) nxt = P.A[(unsigned)(DESC ? e - 2 - k : s + k + 1)];
+        }
         const R dx = (R)(T.X - a.x), dy = (R)(T.Y - a.y);
         const R r2 = dx * dx + dy * dy;
         const R hij = R(0.5) * (T.h + (R)a.h);
