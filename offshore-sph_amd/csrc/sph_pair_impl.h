@@ -615,6 +615,55 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(const Ctrl *ctrl, PairArg
 // gets drho = 0 (WCSPH.compute_density_change, WCSPH.py:200-201); one with
 // none keeps its drho and, as in the reference, takes the empty sum 0 as its
 // density.
+// One run of candidate slots [s0, e0) of the summation density's walk (DESC:
+// from the highest index down), next record in flight. `implied` as in
+// scan_cell: a fluid candidate inside the support is a neighbour, one outside
+// adds W = 0 exactly, walls add nothing, and the count is not formed.
+template <typename R, int KERNEL, bool DESC>
+__device__ inline int sumden_run(const PairArgs<R> &P, double X, double Y, double H, R hi, const Phys &ph, int s0,
+                                 int e0, Acc<R> &rho, bool implied) {
+    constexpr double kEps = sizeof(R) == 8 ? 1e-12 : 1e-5;
+    const R qmax = (R)ph.qmax;
+    const R ks = (R)support_q<KERNEL>();
+    int cnt = 0;
+    const int len = e0 - s0;
+    typename Rec<R>::A nxt{};
+    if (len > 0) nxt = P.A[(unsigned)(DESC ? e0 - 1 : s0)];
+    for (int i = 0; i < len; ++i) {
+        const unsigned u = (unsigned)(DESC ? e0 - 1 - i : s0 + i);
+        const typename Rec<R>::A a = nxt;
+        if (i + 1 < len) nxt = P.A[(unsigned)(DESC ? e0 - 2 - i : s0 + i + 1)];
+        const R dx = (R)(X - a.x), dy = (R)(Y - a.y);
+        const R r2 = dx * dx + dy * dy;
+        const R hij = R(0.5) * (hi + (R)a.h);
+        if (implied) {
+            if (!(a.w >= R(0))) continue;
+            const R sh = ks * hij;
+            if (r2 > sh * sh * R(1.0 + kEps)) continue;
+        } else {
+            const R qh = qmax * hij;
+            const R lim = qh * qh;
+            bool in = r2 < lim * R(1.0 - kEps);
+            if (!in && !(r2 > lim * R(1.0 + kEps))) {
+                const double hj = sizeof(R) == 8 ? (double)a.h : P.H[u];
+                in = exact_q_test(X, Y, H, a.x, a.y, hj, ph.qmax);
+            }
+            cnt += in;
+            if (!in || !(a.w >= R(0))) continue;
+        }
+        if constexpr (sizeof(R) == 8) {
+            const double ex = X - a.x, ey = Y - a.y;
+            const double r = sqrt(ex * ex + ey * ey);
+            rho.add((double)a.w * w_exact<KERNEL>(r, 0.5 * (H + (double)a.h)));
+        } else {
+            const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+            const float hinv = __builtin_amdgcn_rcpf(hij);
+            rho.add((float)a.w * w_f<KERNEL>(r2 * rinv * hinv, hinv));
+        }
+    }
+    return cnt;
+}
+
 template <typename R, int KERNEL, bool EXACT>
 __global__ __launch_bounds__(kPairThreads) void k_sumden(const Ctrl *ctrl, PairArgs<R> P, Phys ph, SdOut O) {
     if (!ctrl->active) return;
@@ -625,45 +674,32 @@ __global__ __launch_bounds__(kPairThreads) void k_sumden(const Ctrl *ctrl, PairA
     const Grid g = ctrl->grid;
     const int ncx = (int)g.ncx, ncy = (int)g.ncy;
     const double X = at.x, Y = at.y, H = sizeof(R) == 8 ? (double)at.h : P.H[t];
-    constexpr double kEps = sizeof(R) == 8 ? 1e-12 : 1e-5;
-    const R qmax = (R)ph.qmax;
     const R hi = (R)at.h;
     Acc<R> rho;
     int cnt = 0;
     const int2 c = P.cxy[t];
-    for (int da = -1; da <= 1; ++da) {
-        const int cx = c.x + da;
-        if (!(ncx > cx && cx > -1)) continue;
-        for (int db = -1; db <= 1; ++db) {
+    const bool implied = cutoff_implied<KERNEL>(nullptr, ph.qmax, c, ncx, ncy, H);
+    if constexpr (EXACT) {  // the reference's cell order (as walk_cells)
+        for (int da = -1; da <= 1; ++da) {
+            const int cx = c.x + da;
+            if (!(ncx > cx && cx > -1)) continue;
+            for (int db = -1; db <= 1; ++db) {
+                const int cy = c.y + db;
+                if (!(ncy > cy && cy > -1)) continue;
+                const int k = cx + ncx * cy;
+                cnt += sumden_run<R, KERNEL, true>(P, X, Y, H, hi, ph, P.cellStart[k], P.cellStart[k + 1], rho,
+                                                   implied);
+            }
+        }
+    } else {  // row runs
+        const int lo = c.x - 1 > 0 ? c.x - 1 : 0;
+        const int hi_x = c.x + 1 < ncx - 1 ? c.x + 1 : ncx - 1;
+        for (int db = -1; db <= 1 && lo <= hi_x; ++db) {
             const int cy = c.y + db;
             if (!(ncy > cy && cy > -1)) continue;
-            const int k = cx + ncx * cy;
-            const int s0 = P.cellStart[k], e0 = P.cellStart[k + 1];
-            for (int i = 0; i < e0 - s0; ++i) {
-                const int u = EXACT ? e0 - 1 - i : s0 + i;
-                const typename Rec<R>::A a = P.A[u];
-                const R dx = (R)(X - a.x), dy = (R)(Y - a.y);
-                const R r2 = dx * dx + dy * dy;
-                const R hij = R(0.5) * (hi + (R)a.h);
-                const R qh = qmax * hij;
-                const R lim = qh * qh;
-                bool in = r2 < lim * R(1.0 - kEps);
-                if (!in && !(r2 > lim * R(1.0 + kEps))) {
-                    const double hj = sizeof(R) == 8 ? (double)a.h : P.H[u];
-                    in = exact_q_test(X, Y, H, a.x, a.y, hj, ph.qmax);
-                }
-                cnt += in;
-                if (!in || !(a.w >= R(0))) continue;
-                if constexpr (sizeof(R) == 8) {
-                    const double ex = X - a.x, ey = Y - a.y;
-                    const double r = sqrt(ex * ex + ey * ey);
-                    rho.add((double)a.w * w_exact<KERNEL>(r, 0.5 * (H + (double)a.h)));
-                } else {
-                    const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
-                    const float hinv = __builtin_amdgcn_rcpf(hij);
-                    rho.add((float)a.w * w_f<KERNEL>(r2 * rinv * hinv, hinv));
-                }
-            }
+            const int row = ncx * cy;
+            cnt += sumden_run<R, KERNEL, false>(P, X, Y, H, hi, ph, P.cellStart[row + lo],
+                                                P.cellStart[row + hi_x + 1], rho, implied);
         }
     }
     const double rho_new = rho.get();
@@ -673,7 +709,7 @@ __global__ __launch_bounds__(kPairThreads) void k_sumden(const Ctrl *ctrl, PairA
     typename Rec<R>::B *B = (typename Rec<R>::B *)O.B;
     B[t].rho = (R)rho_new;
     B[t].prs = (R)(p_old / (rho_new * rho_new));
-    if (cnt > 0) O.drho[t] = 0.0;
+    if (implied || cnt > 0) O.drho[t] = 0.0;
 }
 
 template <typename R, bool EXACT>
