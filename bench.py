@@ -313,16 +313,27 @@ def stress(reps):
                       'stages': rows}), flush=True)
 
 
-def traffic_from_profiles(precision):
-    p = os.path.join(ROOT, 'profiles', 'pair_traffic.json')
+def _profile_field(name, precision, field):
+    p = os.path.join(ROOT, 'profiles', name)
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
             d = json.load(f)
-        return d.get(precision, {}).get('hbm_bytes_per_launch')
+        return d.get(precision, {}).get(field)
     except Exception:
         return None
+
+
+def traffic_from_profiles(precision):
+    """HBM bytes per pair launch from the calibrated FETCH_SIZE/WRITE_SIZE runs."""
+    return _profile_field('pair_traffic.json', precision, 'hbm_bytes_per_launch')
+
+
+def valu_from_profiles(precision):
+    """Fraction of the SIMD cycles the pair pass issues VALU work (SQ counters;
+    the pass is VALU-bound, the HBM fraction beside it is small by design)."""
+    return _profile_field('pair_valu.json', precision, 'valu_issue_frac')
 
 
 def main():
@@ -420,7 +431,8 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': ach32, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': ach32 / HBM_PEAK_GBS, 'traffic': traffic_from_profiles('fp32c'),
                      'kernel': 'k_pair_fp32c<cubic>', 'algorithmic_bytes_per_launch': b32,
-                     'avg_launch_ms': r32['pair_ms'], 'launches': r32['timed_launches']},
+                     'avg_launch_ms': r32['pair_ms'], 'launches': r32['timed_launches'],
+                     'valu_issue_frac': valu_from_profiles('fp32c')},
         'cpu_baseline': cpu,
         'speedup_vs_cpu': (value / cpu['value']) if cpu else None,
         'cpu_baseline_omp': cpu_omp,
@@ -435,7 +447,7 @@ def main():
                  'roofline': {'bound': 'hbm', 'achieved': ach64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                               'frac': ach64 / HBM_PEAK_GBS, 'traffic': traffic_from_profiles('fp64'),
                               'kernel': 'k_pair_fp64<cubic>', 'algorithmic_bytes_per_launch': b64,
-                              'avg_launch_ms': r64['pair_ms']},
+                              'avg_launch_ms': r64['pair_ms'], 'valu_issue_frac': valu_from_profiles('fp64')},
                  'stage_ms': {'pair': r64['pair_ms'], 'neighbour_update': r64['nn_ms']}},
         'precision': prec_report,
         'fwd_adjoint_over_fwd': adj['ratio'] if adj else None,
