@@ -282,6 +282,29 @@ def test_step_c1_fp64_bitexact_1000():
         np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
 
 
+def test_step_c2_500k_fp64_bitexact_100():
+    """C2 at 500k fluid (1.0 x 2.0 column, dx 0.002, 4 x 3 tank), cubic, PEC,
+    100 steps with the graph replay: every field and the dt series bit-identical
+    to the oracle (its pair loop on 16 host threads)."""
+    sc = scenarios.dam_break_c2(million=False)
+    m = sc.method()
+    say('c2 500k: n', sc.n, '100 steps fp64')
+    e = make_engine(sc.pA, m)
+    e.step(100, graph=True)
+    st = e.status()
+    out = e.download()
+    hist = e.dt_history(100)
+    e.close()
+    assert st['steps'] == 100 and st['flags'] & 3 == 0
+    s = oracle.Solver(sc.pA, m, 'cubic', 'pec', h=sc.h, nthreads=16)
+    rhist = s.run(100)
+    say('c2 500k oracle done')
+    np.testing.assert_array_equal(hist, rhist)
+    assert st['t'] == s.t
+    for k in FIELDS:
+        np.testing.assert_array_equal(out[k], s.pA[k], err_msg=k)
+
+
 def test_step_euler_and_gaussian():
     sc = scenarios.dam_break_c1()
     say('c1 euler 100')
