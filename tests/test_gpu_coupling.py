@@ -117,8 +117,8 @@ class _OracleSolver:
         self.dt = 0.0
 
 
-def _oracle_coupled(pA, method, nm, props, steps, hook):
-    s = oracle.Solver(pA, method, 'cubic', 'pec', h=1.3 * R0, newmark=(nm.beta, nm.gamma))
+def _oracle_coupled(pA, method, nm, props, steps, hook, damping=0.0):
+    s = oracle.Solver(pA, method, 'cubic', 'pec', h=1.3 * R0, newmark=(nm.beta, nm.gamma), damping=damping)
     view = _OracleSolver(s, method, nm, props)
     hist = np.zeros((steps, 3))
     for k in range(steps):
@@ -130,22 +130,23 @@ def _oracle_coupled(pA, method, nm, props, steps, hook):
     return s, hist
 
 
-def test_split_step_equals_step_and_oracle():
+@pytest.mark.parametrize('integrator', ['pec', 'euler'])
+def test_split_step_equals_step_and_oracle(integrator):
     sc, pA = _scene()
     c = pA['label'] == ParticleType.Coupled
     pA['ay'][c] = np.linspace(-2.0, 2.0, c.sum())
     pA['vy'][c] = 0.05
     y0 = pA['y'][c].copy()  # the oracle below steps pA in place
     kw = dict(coupled_integrator='newmark', newmark_beta=0.25, newmark_gamma=0.5)
-    a = make_engine(pA, sc.method(), **kw)
-    b = make_engine(pA, sc.method(), **kw)
-    say('split step: n', len(pA), 'coupled', int(c.sum()))
+    a = make_engine(pA, sc.method(), integrator=integrator, **kw)
+    b = make_engine(pA, sc.method(), integrator=integrator, **kw)
+    say('split step', integrator, ': n', len(pA), 'coupled', int(c.sum()))
     a.step(30)
     for _ in range(30):
         b.step_begin()
         b.step_end()
     da, db = a.download(), b.download()
-    s = oracle.Solver(pA, sc.method(), 'cubic', 'pec', h=sc.h, newmark=(0.25, 0.5))
+    s = oracle.Solver(pA, sc.method(), 'cubic', integrator, h=sc.h, newmark=(0.25, 0.5))
     s.run(30)
     for k in FIELDS + ('h',):
         np.testing.assert_array_equal(da[k], db[k], err_msg=k)
@@ -281,6 +282,42 @@ def test_solver_mirror_coupling_hook_fp64_bitexact():
     for a, b in zip(props['Pressure'], props_o['Pressure']):
         np.testing.assert_array_equal(a, b)
     assert np.ptp(got['y'][got['label'] == 3] - props['y0']) > 0  # the sheet moved
+
+
+def test_solver_mirror_coupling_with_settling_fp64_bitexact():
+    """The hook also runs while settling (src/Solver.py:374-398 with damping,
+    the clock held): maxSettle + 1 damped coupled steps, then undamped ones."""
+    sc, pA = _scene()
+    m = sc.method()
+    c = pA['label'] == ParticleType.Coupled
+    M, K, Cd = _beam(int(c.sum()))
+    nm = NewmarkBeta(0.25, 0.5, M, K, Cd)
+    props = _props(pA)
+    max_settle = 3
+    sv = Solver(m, PEC(useXSPH=False, strict=False), CubicSpline(), duration=1.0, incrementalWriteout=False,
+                exportProperties=[], h=1.3 * R0, coupling=ice_hook, couplingIntegrator=nm,
+                couplingProperties=props, damping=0.05, maxSettle=max_settle, chunk=8)
+    sv.addParticles(pA.copy())
+    sv.setup()
+    ref = oracle.as_particles(sv.particleArray[sv.indexes].copy())
+    props_o = _props(pA)
+    n_settle = max_settle + 1  # KE0 = 0: the check fires at t_step > maxSettle
+    s1, h1 = _oracle_coupled(ref, m, nm, props_o, n_settle, ice_hook, damping=0.05)
+    K_ = 12
+    s2, h2 = _oracle_coupled(oracle.as_particles(s1.pA.copy()), m, nm, props_o, K_, ice_hook)
+    t = np.cumsum(h2[:, 0])
+    sv.duration = 0.5 * (t[K_ - 2] + t[K_ - 1])
+    say('coupled solver with settling: settle', n_settle, 'steps', K_)
+    sv.run()
+    sv.close()
+    assert sv.settled and sv.t_step == n_settle + K_
+    np.testing.assert_array_equal(np.asarray(sv.dt_a), np.concatenate((h1[:, 0], h2[:, 0])))
+    got = sv.particleArray[sv.indexes]
+    for f in FIELDS:
+        np.testing.assert_array_equal(got[f], s2.pA[f], err_msg=f)
+    assert len(props['Pressure']) == n_settle + K_
+    for a, b in zip(props['Pressure'], props_o['Pressure']):
+        np.testing.assert_array_equal(a, b)
 
 
 def test_engine_device_probe_hook_fp64_bitexact():
