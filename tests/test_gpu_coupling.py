@@ -222,8 +222,8 @@ def test_near_pos_vs_oracle():
 
 @pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
 def test_probe_vs_oracle(kernel):
-    """On a binned, perturbed state: cubic bit-exact; Gaussian (exp) and
-    Wendland to 1e-12 relative, as their element-wise kernels."""
+    """On a binned, perturbed state: cubic and Wendland bit-exact; Gaussian
+    (exp) to 1e-12 relative, as its element-wise kernel."""
     sc, pA = _scene()
     pA = oracle.as_particles(pA)
     f = pA['label'] == 0
@@ -240,7 +240,7 @@ def test_probe_vs_oracle(kernel):
     say('probe', kernel, 'points', len(px))
     rho, p = e.probe_pressure(px, py, h)
     ro, po = oracle.probe_pressure(nn, pA, kernel, oracle.wcsph(sc.method()), px, py, h)
-    if kernel == 'cubic':
+    if kernel != 'gaussian':  # the Gaussian's exp is the device's
         np.testing.assert_array_equal(rho, ro)
         np.testing.assert_array_equal(p, po)
     else:
