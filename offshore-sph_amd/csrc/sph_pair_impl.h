@@ -4,9 +4,10 @@
 // -ffp-contract=off) and sph_pair_fp32.hip (R = float, -ffp-contract=fast-honor-pragmas).
 //
 // The pass is bound by the VALU, not by HBM (1M dam break, FP32C cubic: VALU
-// instructions x 4 cycles fill ~84 % of the SIMD cycles once the cached loads
-// are software-pipelined, ~57 % before; HBM traffic ~0.65 TB/s), so the
-// design minimises instructions per candidate and keeps loads in flight:
+// instructions x 4 cycles fill at least 78 % of the SIMD cycles once the
+// cached loads are software-pipelined, ~57 % before; HBM traffic ~0.65 TB/s),
+// so the design minimises instructions per candidate and keeps loads in
+// flight:
 //   * One thread per particle of the cell-ordered state; fluid particles are
 //     targets. Consecutive threads hold neighbouring particles, so the
 //     candidates of a wavefront overlap and are served by L1/L2.
