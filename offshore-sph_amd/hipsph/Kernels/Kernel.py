@@ -9,6 +9,7 @@ import abc
 import numpy as np
 
 from .. import _lib
+from ..device import host_array
 
 
 def _dev_call(name, kid, arrays):
@@ -17,7 +18,7 @@ def _dev_call(name, kid, arrays):
         raise _lib.HipSphUnavailable('no GPU visible to torch: kernels evaluate on the device only')
     dev = torch.device('cuda:0')
     host = not isinstance(arrays[0], torch.Tensor)
-    ts = [torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64) if host else a, dtype=torch.float64).to(dev).contiguous()
+    ts = [torch.as_tensor(host_array(a, np.float64) if host else a, dtype=torch.float64).to(dev).contiguous()
           for a in arrays]
     n = ts[0].numel()
     out = torch.empty(n, dtype=torch.float64, device=dev)

@@ -24,6 +24,13 @@ GATHER_FIELDS = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'p',
 SCATTER_FIELDS = ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'xsphx', 'xsphy')
 
 
+def host_array(a, dtype=None):
+    """A fresh C-ordered copy with the element's own stride. A field of a
+    particle_dtype record array is strided by the 154-byte record, and numpy
+    calls a one-element such view contiguous, which torch rejects."""
+    return np.array(a, dtype=dtype, order='C', copy=True)
+
+
 def _torch():
     import torch
     if not torch.cuda.is_available():
@@ -155,7 +162,7 @@ class Engine:
         for k, v in arrays.items():
             if k not in self.io:
                 continue
-            t = torch.as_tensor(np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v)
+            t = torch.as_tensor(host_array(v) if isinstance(v, np.ndarray) else v)
             if t.numel() != self.n:
                 raise ValueError(f'{k} has {t.numel()} entries, engine has n={self.n}')
             self.io[k].copy_(t.to(self.io[k].dtype).reshape(-1))
@@ -257,7 +264,7 @@ class Engine:
         for f, v in values.items():
             if f not in SCATTER_FIELDS:
                 raise KeyError(f'{f} cannot be written (writable: {", ".join(SCATTER_FIELDS)})')
-            t = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64) if not isinstance(v, torch.Tensor) else v,
+            t = torch.as_tensor(host_array(v, np.float64) if not isinstance(v, torch.Tensor) else v,
                                 dtype=torch.float64).to(self.device).reshape(-1).contiguous()
             if t.numel() != k:
                 raise ValueError(f'{f} has {t.numel()} entries for {k} ids')
@@ -269,7 +276,7 @@ class Engine:
         """NNLinkedList.nearPos for points against the last update: CSR (off, idx, q, r), numpy."""
         import torch
         dev = self.device
-        pts = [torch.as_tensor(np.atleast_1d(np.asarray(a, dtype=np.float64))).to(dev).contiguous() for a in (x, y)]
+        pts = [torch.as_tensor(np.atleast_1d(host_array(a, np.float64))).to(dev).contiguous() for a in (x, y)]
         if pts[0].shape != pts[1].shape or pts[0].dim() != 1:
             raise ValueError('x and y must be 1-D and of one length')
         hh = torch.as_tensor(np.broadcast_to(np.asarray(h, dtype=np.float64), pts[0].shape).copy()).to(dev)
@@ -293,7 +300,7 @@ class Engine:
         """pressure_SPH of the coupled example at points (x, y) with length h: (rho, p), numpy."""
         import torch
         dev = self.device
-        px, py = [torch.as_tensor(np.atleast_1d(np.asarray(a, dtype=np.float64))).to(dev).contiguous()
+        px, py = [torch.as_tensor(np.atleast_1d(host_array(a, np.float64))).to(dev).contiguous()
                   for a in (x, y)]
         if px.shape != py.shape or px.dim() != 1:
             raise ValueError('x and y must be 1-D and of one length')
@@ -373,4 +380,4 @@ class Engine:
 def soa_from_particles(pA, fields=('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho', 'm', 'h', 'label', 'xsphx',
                                    'xsphy')):
     """Structure-of-arrays view of a particle_dtype array (active records only)."""
-    return {f: np.ascontiguousarray(pA[f]) for f in fields}
+    return {f: host_array(pA[f]) for f in fields}

@@ -400,6 +400,47 @@ def test_duration_stops_the_clock():
     e.close()
 
 
+def test_no_fluid_halts_without_touching_the_state():
+    """Walls only: the reference's _minTimeStep has no fluid to reduce over
+    (an empty min); the device raises SPH_FLAG_NO_FLUID and steps nothing."""
+    from hipsph import _lib
+    sc = scenarios.dam_break_c1()
+    walls = sc.pA[sc.pA['label'] != 0].copy()
+    e = make_engine(walls, sc.method())
+    before = e.download()
+    e.step(3)
+    st = e.status()
+    assert st['flags'] & _lib.SPH_FLAG_NO_FLUID and st['steps'] == 0 and st['t'] == 0.0
+    after = e.download()
+    for k in FIELDS:
+        np.testing.assert_array_equal(after[k], before[k], err_msg=k)
+    e.close()
+
+
+def test_single_particle_free_fall_fp64_bitexact():
+    """One fluid particle, no walls: a one-cell grid, the particle its own
+    only neighbour (no pair term), gravity alone; 50 steps bit-identical to
+    the oracle and, to rounding, the closed-form free fall."""
+    sc = scenarios.dam_break_c1()
+    one = sc.pA[sc.pA['label'] == 0][:1].copy()
+    one['vx'], one['vy'] = 0.3, 0.0
+    m = sc.method()
+    say('single particle, 50 steps')
+    e = make_engine(one, m)
+    e.step(50)
+    st = e.status()
+    out = e.download()
+    e.close()
+    s = oracle.Solver(one, m, 'cubic', 'pec', h=sc.h)
+    hist = s.run(50)
+    assert st['steps'] == 50 and st['t'] == s.t
+    for k in FIELDS:
+        np.testing.assert_array_equal(out[k], s.pA[k], err_msg=k)
+    t = float(np.sum(hist[:, 0]))
+    assert out['vy'][0] == pytest.approx(-9.81 * t, rel=1e-12)
+    assert out['x'][0] == pytest.approx(sc.pA['x'][sc.pA['label'] == 0][0] + 0.3 * t, rel=1e-12)
+
+
 def test_nonfinite_position_flags():
     from hipsph import _lib
     sc = scenarios.dam_break_c1()
