@@ -296,6 +296,10 @@ int sph_batch_step(sph_batch_t *batch, int64_t nsteps, int32_t use_graph, void *
 int sph_batch_status(sph_batch_t *batch, double *out_d, int64_t *out_i, void *stream);
 /* Pair-pass workgroup size (64, 128, 256), as sph_ctx_set_tuning. */
 int sph_batch_set_tuning(sph_batch_t *batch, int32_t pair_threads);
+/* Pair-pass contributor list length per thread (the LDS tile: list_len x 4 B
+ * x the workgroup): 16, 24 or 32, or 0 for the single-simulation pass's
+ * (24 for the float cubic/Wendland pass, else 32). Chosen by the autotuner. */
+int sph_batch_set_list_len(sph_batch_t *batch, int32_t list_len);
 int sph_batch_timing_enable(sph_batch_t *batch, int32_t on);
 int sph_batch_timing_read(sph_batch_t *batch, double *avg_ms_pair, int64_t *count, int32_t reset);
 

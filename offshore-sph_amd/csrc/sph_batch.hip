@@ -464,6 +464,7 @@ struct sph_batch {
     hipStream_t own = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int pair_threads = 256;
+    int list_len = 0;  // 0: the single-simulation pass's length
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t n_ev = 0;
@@ -532,8 +533,10 @@ int b_pair(sph_batch *b, hipStream_t s) {
         HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
     }
     hipError_t e = b->precision == SPH_PREC_FP32C
-                       ? launch_pair_batch_fp32c(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads, s)
-                       : launch_pair_batch_fp64(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads, s);
+                       ? launch_pair_batch_fp32c(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads,
+                                                 b->list_len, s)
+                       : launch_pair_batch_fp64(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads,
+                                                b->list_len, s);
     if (e != hipSuccess) return fail(SPH_EHIP, "batch pair launch: %s", hipGetErrorString(e));
     if (b->timing) {
         if (b->n_ev >= b->ev.size()) {
@@ -825,6 +828,18 @@ int sph_batch_set_tuning(sph_batch_t *b, int32_t pair_threads) {
         return fail(SPH_EINVAL, "pair_threads must be 64, 128 or 256");
     if (pair_threads != b->pair_threads) {
         b->pair_threads = pair_threads;
+        for (auto &g : b->gexec)
+            if (g) hipGraphExecDestroy(g), g = nullptr;
+    }
+    return 0;
+}
+
+int sph_batch_set_list_len(sph_batch_t *b, int32_t list_len) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    if (list_len != 0 && list_len != 16 && list_len != 24 && list_len != 32)
+        return fail(SPH_EINVAL, "list_len must be 0, 16, 24 or 32");
+    if (list_len != b->list_len) {
+        b->list_len = list_len;
         for (auto &g : b->gexec)
             if (g) hipGraphExecDestroy(g), g = nullptr;
     }

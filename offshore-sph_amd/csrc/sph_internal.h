@@ -165,10 +165,11 @@ __host__ __device__ inline Phys inst_phys(const BInst &I) {
 }
 
 // The pair pass over a batch: instOf[t] is the instance of slot t.
+// list: the per-thread contributor list length (16, 24, 32; 0 = default).
 hipError_t launch_pair_batch_fp64(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
-                                  int threads, hipStream_t s);
+                                  int threads, int list, hipStream_t s);
 hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
-                                   int threads, hipStream_t s);
+                                   int threads, int list, hipStream_t s);
 
 // Number of blocks for a grid over n items, padded to a multiple of the XCD
 // count so that the block remap below is a bijection.

@@ -27,8 +27,8 @@ hipError_t launch_sumden_fp32c(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, 
 }
 
 hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
-                                   int threads, hipStream_t s) {
-    return launch_pair_batch<float, false>(inst, instOf, pair_args<float>(in, &out, n), kernel, threads, s);
+                                   int threads, int list, hipStream_t s) {
+    return launch_pair_batch<float, false>(inst, instOf, pair_args<float>(in, &out, n), kernel, threads, list, s);
 }
 
 }  // namespace hipsph

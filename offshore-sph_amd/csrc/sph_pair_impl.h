@@ -49,7 +49,7 @@ constexpr int kListLen = 32;  // list entries per thread before a flush (largest
 // candidates and the pass takes twice as long; the double pass is
 // register-bound at 4 workgroups per CU either way.
 template <typename R, int KERNEL>
-__device__ constexpr int list_len() {
+__host__ __device__ constexpr int list_len() {
     return (sizeof(R) == 4 && KERNEL != SPH_KERNEL_GAUSSIAN) ? 24 : kListLen;
 }
 constexpr double kPI = 3.141592653589793;
@@ -409,7 +409,7 @@ __device__ inline void flush(const PairArgs<R> &P, const Target<R> &T, const Phy
 // kernel support lies well inside the neighbour cut-off (2h against 3h for
 // the cubic spline and Wendland), so a fluid candidate inside the support is
 // a neighbour and the cut-off test is skipped for it.
-template <typename R, int KERNEL, bool DESC, int MODE, bool XS = false>
+template <typename R, int KERNEL, bool DESC, int MODE, bool XS = false, int LIST = list_len<R, KERNEL>()>
 __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int s, int e, unsigned *list,
                                 int &nlist, Sums<R> &S, bool implied) {
     constexpr double kEps = sizeof(R) == 8 ? 1e-12 : 1e-5;
@@ -460,7 +460,7 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
             ++nlist;
             if constexpr (KERNEL != SPH_KERNEL_GAUSSIAN) {
                 // 2h support: ~21 contributors, the list rarely fills
-                if (nlist == list_len<R, KERNEL>()) {
+                if (nlist == LIST) {
                     flush<R, KERNEL, MODE, XS>(P, T, ph, list, nlist, S);
                     nlist = 0;
                 }
@@ -471,7 +471,7 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
             // When one lane's list is full, every active lane flushes
             // together; per-lane flushes at different candidates would run
             // the flush loop once per lane. A lane's sums keep their order.
-            if (__ballot(nlist == kListLen) != 0) {
+            if (__ballot(nlist == LIST) != 0) {
                 flush<R, KERNEL, MODE, XS>(P, T, ph, list, nlist, S);
                 nlist = 0;
             }
@@ -486,7 +486,7 @@ __device__ inline int scan_cell(const PairArgs<R> &P, const Target<R> &T, const 
 // each cell from its highest index down (head-inserted lists, :129-141).
 // Otherwise (the float pass needs only a fixed order) the three cells of each
 // row, which are consecutive bins, as one run of slots.
-template <typename R, int KERNEL, bool EXACT, int MODE, bool XS = false>
+template <typename R, int KERNEL, bool EXACT, int MODE, bool XS = false, int LIST = list_len<R, KERNEL>()>
 __device__ inline int walk_cells(const PairArgs<R> &P, const Target<R> &T, const Phys &ph, int2 c, int ncx, int ncy,
                                  long long base, unsigned *list, int &nlist, Sums<R> &S, bool implied) {
     int cnt = 0;
@@ -498,8 +498,8 @@ __device__ inline int walk_cells(const PairArgs<R> &P, const Target<R> &T, const
                 const int cy = c.y + db;
                 if (!(ncy > cy && cy > -1)) continue;
                 const long long k = base + cx + (long long)ncx * cy;
-                cnt += scan_cell<R, KERNEL, true, MODE, XS>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list, nlist,
-                                                             S, implied);
+                cnt += scan_cell<R, KERNEL, true, MODE, XS, LIST>(P, T, ph, P.cellStart[k], P.cellStart[k + 1], list,
+                                                                   nlist, S, implied);
             }
         }
     } else {
@@ -510,8 +510,8 @@ __device__ inline int walk_cells(const PairArgs<R> &P, const Target<R> &T, const
             const int cy = c.y + db;
             if (!(ncy > cy && cy > -1)) continue;
             const long long row = base + (long long)ncx * cy;
-            cnt += scan_cell<R, KERNEL, false, MODE, XS>(P, T, ph, P.cellStart[row + lo], P.cellStart[row + hi + 1],
-                                                          list, nlist, S, implied);
+            cnt += scan_cell<R, KERNEL, false, MODE, XS, LIST>(P, T, ph, P.cellStart[row + lo],
+                                                                P.cellStart[row + hi + 1], list, nlist, S, implied);
         }
     }
     return cnt;
@@ -519,7 +519,7 @@ __device__ inline int walk_cells(const PairArgs<R> &P, const Target<R> &T, const
 
 // The pass for target slot t: the 3x3 cells of its grid (ncx x ncy cells
 // starting at bin `base`), sums, outputs.
-template <typename R, int KERNEL, bool EXACT, int MODE>
+template <typename R, int KERNEL, bool EXACT, int MODE, int LIST = list_len<R, KERNEL>()>
 __device__ inline void pair_target(const PairArgs<R> &P, const Phys &ph, int ncx, int ncy, long long base, int t,
                                    unsigned *s_list) {
     const typename Rec<R>::A at = P.A[t];
@@ -542,7 +542,7 @@ __device__ inline void pair_target(const PairArgs<R> &P, const Phys &ph, int ncx
     int nlist = 0, cnt = 0;
     Sums<R> S;
     const bool implied = cutoff_implied<KERNEL>(P.count, ph.qmax, c, ncx, ncy, T.H);
-    cnt = walk_cells<R, KERNEL, EXACT, MODE>(P, T, ph, c, ncx, ncy, base, list, nlist, S, implied);
+    cnt = walk_cells<R, KERNEL, EXACT, MODE, false, LIST>(P, T, ph, c, ncx, ncy, base, list, nlist, S, implied);
     flush<R, KERNEL, MODE>(P, T, ph, list, nlist, S);
     if (P.count) P.count[t] = cnt;
     if (!implied && cnt == 0) return;  // SolverTools.py:166-169: no neighbours, keep the previous values
@@ -731,15 +731,31 @@ hipError_t launch_sumden(const Ctrl *ctrl, const PairArgs<R> &P, const SdOut &O,
 }
 
 // Batched ensemble: each target walks its own instance's cells with its own
-// constants; targets of finished or halted instances do nothing.
-template <typename R, int KERNEL, bool EXACT>
+// constants; targets of finished or halted instances do nothing. LIST (the
+// per-thread contributor list, LIST x 4 B x the workgroup in LDS) is a tile
+// size the batch's autotuner chooses.
+template <typename R, int KERNEL, bool EXACT, int LIST>
 __global__ __launch_bounds__(kPairThreads) void k_pair_batch(const BInst *inst, const int *instOf, PairArgs<R> P) {
-    __shared__ unsigned s_list[kListLen * kPairThreads];
+    __shared__ unsigned s_list[LIST * kPairThreads];
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const BInst &I = inst[instOf[t]];
     if (!I.active) return;
-    pair_target<R, KERNEL, EXACT, kFused>(P, inst_phys(I), (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, s_list);
+    pair_target<R, KERNEL, EXACT, kFused, LIST>(P, inst_phys(I), (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t,
+                                                s_list);
+}
+
+// The batch's list lengths; 0 picks the single-simulation pass's.
+template <typename R, int KERNEL, bool EXACT>
+void launch_batch_list(const BInst *inst, const int *instOf, const PairArgs<R> &P, int list, int nb, int threads,
+                       hipStream_t s) {
+    if (list == 0) list = list_len<R, KERNEL>();
+    if (list == 16)
+        k_pair_batch<R, KERNEL, EXACT, 16><<<nb, threads, 0, s>>>(inst, instOf, P);
+    else if (list == 24)
+        k_pair_batch<R, KERNEL, EXACT, 24><<<nb, threads, 0, s>>>(inst, instOf, P);
+    else
+        k_pair_batch<R, KERNEL, EXACT, 32><<<nb, threads, 0, s>>>(inst, instOf, P);
 }
 
 template <typename R, bool EXACT, int MODE, bool XS = false>
@@ -758,18 +774,18 @@ void launch_pair_mode(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, in
 
 template <typename R, bool EXACT>
 hipError_t launch_pair_batch(const BInst *inst, const int *instOf, const PairArgs<R> &P, int kernel, int threads,
-                             hipStream_t s) {
+                             int list, hipStream_t s) {
     if (threads != 64 && threads != 128 && threads != 256) threads = kPairThreads;
     const int nb = xcd_blocks(P.n, threads);
     switch (kernel) {
         case SPH_KERNEL_GAUSSIAN:
-            k_pair_batch<R, SPH_KERNEL_GAUSSIAN, EXACT><<<nb, threads, 0, s>>>(inst, instOf, P);
+            launch_batch_list<R, SPH_KERNEL_GAUSSIAN, EXACT>(inst, instOf, P, list, nb, threads, s);
             break;
         case SPH_KERNEL_WENDLAND:
-            k_pair_batch<R, SPH_KERNEL_WENDLAND, EXACT><<<nb, threads, 0, s>>>(inst, instOf, P);
+            launch_batch_list<R, SPH_KERNEL_WENDLAND, EXACT>(inst, instOf, P, list, nb, threads, s);
             break;
         default:
-            k_pair_batch<R, SPH_KERNEL_CUBIC, EXACT><<<nb, threads, 0, s>>>(inst, instOf, P);
+            launch_batch_list<R, SPH_KERNEL_CUBIC, EXACT>(inst, instOf, P, list, nb, threads, s);
     }
     return hipGetLastError();
 }

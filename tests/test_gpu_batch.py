@@ -96,10 +96,22 @@ def test_batch_duration_per_instance():
 
 
 def test_batch_autotune_keeps_results():
+    """The launch shape (workgroup size, contributor-list length) changes no
+    result: a list flushes in listing order, a lane's target is its own."""
     mem = _members()
     b = BatchEngine([params_for(sc, alpha, precision='fp32c') for sc, alpha in mem])
+    runs = []
+    for threads, ll in ((256, 24), (64, 16), (128, 32)):
+        b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+        b.set_tuning(threads, ll)
+        b.step(12, graph=True)
+        runs.append(b.download())
+    for other in runs[1:]:
+        for k, (got, want) in enumerate(zip(other, runs[0])):
+            for fld in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho'):
+                np.testing.assert_array_equal(got[fld], want[fld], err_msg=f'{k} {fld}')
     b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
     best, table = autotune(b, steps=2)
     say('batch autotune', best, table)
-    assert best in (64, 128, 256) and len(table) == 3
+    assert best[0] in (64, 128, 256) and best[1] in (16, 24, 32) and len(table) == 9
     b.close()
