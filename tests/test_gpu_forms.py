@@ -15,6 +15,8 @@ tests/test_oracle_invariants.py. Here, through the C ABI:
   * dynamic h: FP64 cubic trajectories, dt series and h bit-identical over
     200 steps (PEC, Euler) and with XSPH and summation density on as well;
   * hipGraph replay equals the eager steps with both on;
+  * FP32 pairs against FP64 with each form on: relative density within 1e-5
+    and positions within 1e-3 dx after 1,000 steps;
   * without XSPH the fluid's XSPH velocity reads 0 after a loop, the walls'
     keeps its value;
   * what the device path does not cover fails loudly: the split passes and
@@ -111,6 +113,30 @@ def test_steps_dynamic_h_fp64_bitexact(case):
     assert np.ptp(out['h'][f]) > 0  # h does vary
     for k in STATE + ('h',):
         np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+@pytest.mark.parametrize('case', ['xsph', 'sd', 'xsph_sd_dynh'])
+def test_fp32c_forms_vs_fp64_after_1000_steps(case):
+    """FP32 pairs with FP64 integration against the FP64 engine with the
+    loop's other forms on: relative density within 1e-5 and positions within
+    1e-3 dx after 1,000 steps, the bar of the plain step."""
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 11, vscale=0.1, rscale=0.002)
+    m = _method(sc, 'xsph' in case, 'sd' in case)
+    dyn = 'dynh' in case
+    a = make_engine(pA, m, precision='fp64', dynamic_h=dyn)
+    b = make_engine(pA, m, precision='fp32c', dynamic_h=dyn)
+    a.step(1000)
+    b.step(1000)
+    oa, ob = a.download(), b.download()
+    a.close()
+    b.close()
+    f = pA['label'] == 0
+    drho = np.max(np.abs(ob['rho'][f] - oa['rho'][f]) / oa['rho'][f])
+    dx = np.max(np.hypot(ob['x'][f] - oa['x'][f], ob['y'][f] - oa['y'][f])) / sc.dx
+    say('fp32c', case, 'after 1000 steps: max rel rho', drho, 'max |dx|/dx', dx)
+    assert drho < 1e-5
+    assert dx < 1e-3
 
 
 def test_graph_replay_with_xsph_and_summation_density():
