@@ -35,7 +35,8 @@ extern "C" {
 #define SPH_EHIP (-2)        /* HIP runtime error */
 #define SPH_ENOMEM (-3)      /* allocation failed */
 #define SPH_ECAPACITY (-4)   /* cell grid larger than the context's capacity */
-#define SPH_ESTATE (-5)      /* call out of order (e.g. compute before nn_update) */
+#define SPH_ESTATE (-5)      /* call out of order (e.g. compute before nn_update, a step or
+                                download before the first state upload) */
 #define SPH_ENUMERIC (-6)    /* non-finite value met (position, dt) */
 #define SPH_EUNSUPPORTED (-7)
 
@@ -111,7 +112,8 @@ typedef struct {
 int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out);
 int sph_ctx_destroy(sph_ctx_t *ctx);
 /* Update constants that may change between steps (damping, alpha, beta, co,
- * B, duration, fixed_dt). n, kernel, precision and use_xsph are fixed at creation. */
+ * B, duration, fixed_dt). n, kernel, precision and use_xsph are fixed at creation.
+ * Synchronises the device first (the control block is rewritten). */
 int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params);
 
 /* Copy the state in (reference index order) / out (un-permuted back to that

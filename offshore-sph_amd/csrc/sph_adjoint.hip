@@ -577,6 +577,24 @@ int hipsph::tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s) {
     return 0;
 }
 
+// The forms the adjoint covers; the parameters may change after the tape is
+// made, so recording checks again.
+int hipsph::tape_supported(const sph_ctx *c) {
+    if (c->P.use_xsph || c->P.use_summation_density || c->P.dynamic_h || c->P.coupled_integrator)
+        return fail(SPH_EUNSUPPORTED,
+                    "the adjoint covers the plain WCSPH step (no XSPH, summation density, dynamic h or coupled integrator)");
+    return 0;
+}
+
+// The fluid mask by reference index, from the current state: taken when a
+// recording starts (slot 0), so a tape made before the state was uploaded,
+// or kept across uploads, sees the labels of the steps it records.
+int hipsph::tape_refresh_labels(sph_ctx *c, sph_tape *t, hipStream_t s) {
+    k_tape_static<<<nblk_(c->n, kAdjBlock), kAdjBlock, 0, s>>>(c->buf[c->cur], c->n, t->label);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -586,9 +604,8 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     if (!ctx || !out) return fail(SPH_EINVAL, "NULL argument");
     *out = nullptr;
     if (nslots < 1 || nslots > 100000) return fail(SPH_EINVAL, "nslots=%d out of range", nslots);
-    if (ctx->P.use_xsph || ctx->P.use_summation_density || ctx->P.dynamic_h || ctx->P.coupled_integrator)
-        return fail(SPH_EUNSUPPORTED,
-                    "the adjoint covers the plain WCSPH step (no XSPH, summation density, dynamic h or coupled integrator)");
+    int rc = tape_supported(ctx);
+    if (rc) return rc;
     const size_t n = (size_t)ctx->n;
     sph_tape *t = new (std::nothrow) sph_tape;
     if (!t) return fail(SPH_ENOMEM, "host allocation failed");
@@ -642,12 +659,14 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
         return fail(SPH_EINVAL, "internal: tape arena too small");
     }
     if (hipMemset(t->hdr, 0, sizeof(TapeHdr) * nslots) != hipSuccess ||
-        hipMemset(t->force_steps, 0, sizeof(int)) != hipSuccess) {
+        hipMemset(t->force_steps, 0, sizeof(int)) != hipSuccess || hipMemset(t->label, 0, n) != hipSuccess) {
         hipFree(t->arena);
         delete t;
         return fail(SPH_EHIP, "tape init failed");
     }
-    k_tape_static<<<nblk_(ctx->n, kAdjBlock), kAdjBlock>>>(ctx->buf[ctx->cur], ctx->n, t->label);
+    // the labels of the current state, if there is one (a recording from
+    // slot 0 takes them again)
+    if (ctx->loaded) k_tape_static<<<nblk_(ctx->n, kAdjBlock), kAdjBlock>>>(ctx->buf[ctx->cur], ctx->n, t->label);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         hipFree(t->arena);
         delete t;

@@ -461,6 +461,7 @@ struct sph_batch {
     void *recA = nullptr, *recB = nullptr;
     long long capBins = 0;
     bool dt_fresh = false;
+    bool loaded = false;  // a state was uploaded: every id indexes its instance
     hipStream_t own = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int pair_threads = 256;
@@ -730,6 +731,7 @@ int sph_batch_upload(sph_batch_t *b, const sph_state_t *st, void *stream) {
     b->dt_fresh = false;
     kb_upload<<<nblk_b(b->n), kBlock, 0, s>>>(b->buf[0], *st, b->n);
     HIPCHK(hipGetLastError());
+    b->loaded = true;
     HIPCHK(hipStreamSynchronize(s));
     std::vector<BInst> hI(b->K);
     HIPCHK(hipMemcpy(hI.data(), b->inst, sizeof(BInst) * b->K, hipMemcpyDeviceToHost));
@@ -746,6 +748,7 @@ int sph_batch_upload(sph_batch_t *b, const sph_state_t *st, void *stream) {
 
 int sph_batch_download(sph_batch_t *b, const sph_state_t *st, void *stream) {
     if (!b || !st) return fail(SPH_EINVAL, "NULL argument");
+    if (!b->loaded) return fail(SPH_ESTATE, "no state: call sph_batch_upload first");
     kb_download<<<nblk_b(b->n), kBlock, 0, (hipStream_t)stream>>>(b->buf[b->cur], *st, b->inst, b->instOf, b->n);
     HIPCHK(hipGetLastError());
     return 0;
@@ -754,6 +757,7 @@ int sph_batch_download(sph_batch_t *b, const sph_state_t *st, void *stream) {
 int sph_batch_step(sph_batch_t *b, int64_t nsteps, int32_t use_graph, void *stream) {
     if (!b) return fail(SPH_EINVAL, "batch is NULL");
     if (nsteps < 0) return fail(SPH_EINVAL, "nsteps < 0");
+    if (!b->loaded) return fail(SPH_ESTATE, "no state: call sph_batch_upload first");
     hipStream_t s = (hipStream_t)stream;
     if (!use_graph || b->timing) {
         for (int64_t k = 0; k < nsteps; ++k) {
@@ -776,24 +780,27 @@ int sph_batch_step(sph_batch_t *b, int64_t nsteps, int32_t use_graph, void *stre
     for (int64_t k = 0; k < nsteps && !rc; ++k) {
         const int start = b->cur;
         if (!b->gexec[start]) {
-            hipGraph_t graph;
-            HIPCHK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+            // the capture ends, and the graph is released, whatever fails
+            hipGraph_t graph = nullptr;
+            hipError_t ce = hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal);
+            if (ce != hipSuccess) {
+                rc = fail(SPH_EHIP, "begin capture: %s", hipGetErrorString(ce));
+                break;
+            }
             rc = b_step(b, g);
-            hipError_t ce = hipStreamEndCapture(g, &graph);
-            if (rc) break;
-            if (ce != hipSuccess) {
-                rc = fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
-                break;
-            }
-            ce = hipGraphInstantiate(&b->gexec[start], graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
-            if (ce != hipSuccess) {
+            ce = hipStreamEndCapture(g, &graph);
+            b->cur = start;  // capture flipped it without running anything
+            if (!rc && ce != hipSuccess) rc = fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
+            if (!rc && (ce = hipGraphInstantiate(&b->gexec[start], graph, nullptr, nullptr, 0)) != hipSuccess) {
+                b->gexec[start] = nullptr;
                 rc = fail(SPH_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
-                break;
             }
+            if (graph) hipGraphDestroy(graph);
+            if (rc) break;
         }
-        HIPCHK(hipGraphLaunch(b->gexec[start], g));
-        b->cur = 1 - start;
+        hipError_t le = hipGraphLaunch(b->gexec[start], g);
+        if (le != hipSuccess) rc = fail(SPH_EHIP, "graph launch: %s", hipGetErrorString(le));
+        else b->cur = 1 - start;
     }
     hipEventRecord(e1, g);
     hipStreamWaitEvent(s, e1, 0);

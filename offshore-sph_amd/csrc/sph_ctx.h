@@ -23,6 +23,7 @@ struct sph_ctx {
     double *part_k = nullptr;                     // kinetic-energy partials + result
     long long cap_cells = 0;
     bool dt_fresh = false;  // part_d holds the dt partials of the current state (left by the corrector)
+    bool loaded = false;    // a state was uploaded: every id indexes 0 .. n-1
     bool mid = false;       // a split step is open (sph_step_begin ran, sph_step_end not yet)
     bool xs_fluid_zero = false;  // XSPH off and a reorder ran: the fluid's XSPH velocity reads as 0
     void *recA = nullptr, *recB = nullptr;  // packed pair-pass records (Rec<double> / Rec<float>)
@@ -103,6 +104,8 @@ int tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s);
 // The reverse of one recorded step (the kernels of sph_adjoint_step).
 int adjoint_launch(sph_ctx *c, sph_tape *t, int slot, const sph_adjoint_t *adj, hipStream_t s);
 void tape_drop_graphs(sph_tape *t);
+int tape_supported(const sph_ctx *c);
+int tape_refresh_labels(sph_ctx *c, sph_tape *t, hipStream_t s);
 
 }  // namespace hipsph
 

@@ -937,6 +937,17 @@ static void ctrl_from_params(Ctrl &h, const sph_params_t &P) {
     h.Pb = P.Pb;
 }
 
+// Calls that index by particle id (steps, downloads, gathers) need a state:
+// before the first upload the buffers hold no valid ids.
+static int unloaded(sph_ctx *c) {
+    return c->loaded ? 0 : fail(SPH_ESTATE, "no state: call sph_state_upload first");
+}
+// Step calls other than sph_step_end are refused while a split step is open.
+static int closed(sph_ctx *c) {
+    if (!c->loaded) return unloaded(c);
+    return c->mid ? fail(SPH_ESTATE, "a split step is open: call sph_step_end first") : 0;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1042,6 +1053,10 @@ int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params) {
         return fail(SPH_EINVAL, "n, kernel, precision and use_xsph are fixed at creation");
     int rc = check_params(params);
     if (rc) return rc;
+    // The control block is rewritten whole: steps still in flight on any
+    // stream (the caller's, or the context's own graph stream, which the
+    // null stream does not wait for) must finish first.
+    HIPCHK(hipDeviceSynchronize());
     ctx->P = *params;
     ctx->dt_fresh = false;
     Ctrl h;
@@ -1066,6 +1081,7 @@ int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock,
     ctx->xs_fluid_zero = false;
     k_upload<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->buf[0], *st, ctx->n);
     HIPCHK(hipGetLastError());
+    ctx->loaded = true;
     Ctrl h;
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
@@ -1098,11 +1114,13 @@ int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *str
     k_upload<<<nblk(ctx->n), kBlock, 0, s>>>(ctx->buf[0], *st, ctx->n);
     k_restore_ctrl<<<1, 1, 0, s>>>(ctx->ctrl, t);
     HIPCHK(hipGetLastError());
+    ctx->loaded = true;
     return 0;
 }
 
 int sph_state_download(sph_ctx_t *ctx, const sph_state_t *st, void *stream) {
     if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
+    if (unloaded(ctx)) return SPH_ESTATE;
     k_download<<<nblk(ctx->n), kBlock, 0, S(stream)>>>(ctx->buf[ctx->cur], *st, make_phys(ctx->P), ctx->n,
                                                         ctx->xs_fluid_zero);
     HIPCHK(hipGetLastError());
@@ -1119,10 +1137,6 @@ static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipSt
     return 0;
 }
 
-// Step calls other than sph_step_end are refused while a split step is open.
-static int closed(sph_ctx *c) {
-    return c->mid ? fail(SPH_ESTATE, "a split step is open: call sph_step_end first") : 0;
-}
 
 // pred != 0: the PEC predictor runs inside the update (bounds and keys see
 // the predicted positions, the reorder writes the predicted state).
@@ -1252,6 +1266,8 @@ int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream
     if (closed(ctx)) return SPH_ESTATE;
     hipStream_t s = S(stream);
     int rc;
+    if ((rc = tape_supported(ctx))) return rc;
+    if (slot == 0 && (rc = tape_refresh_labels(ctx, tape, s))) return rc;
     if ((rc = dt_step(ctx, s))) return rc;
     if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
     if ((rc = compute(ctx, s))) return rc;
@@ -1273,19 +1289,32 @@ static int record_step(sph_ctx *ctx, sph_tape *tape, int slot, hipStream_t s) {
     return 0;
 }
 
+// Capture `body` on stream g into *exec. The capture always ends, and the
+// graph is released, whatever fails.
+extern "C++" template <typename F>
+static int capture_step(hipGraphExec_t *exec, hipStream_t g, F body) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+    int rc = body();
+    hipError_t ce = hipStreamEndCapture(g, &graph);
+    if (!rc && ce != hipSuccess) rc = fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
+    if (!rc) {
+        ce = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            *exec = nullptr;
+            rc = fail(SPH_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+        }
+    }
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+}
+
 // Capture `body` on stream g into *exec (once) and launch it.
 extern "C++" template <typename F>
 static int graph_launch(hipGraphExec_t *exec, hipStream_t g, F body) {
     if (!*exec) {
-        hipGraph_t graph;
-        HIPCHK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
-        int rc = body();
-        hipError_t ce = hipStreamEndCapture(g, &graph);
+        int rc = capture_step(exec, g, body);
         if (rc) return rc;
-        if (ce != hipSuccess) return fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
-        ce = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
-        if (ce != hipSuccess) return fail(SPH_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
     }
     HIPCHK(hipGraphLaunch(*exec, g));
     return 0;
@@ -1300,6 +1329,8 @@ int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const 
         !adj->g_alpha || !adj->g_co)
         return fail(SPH_EINVAL, "every sph_adjoint_t pointer is required");
     if (ctx->P.strict) return fail(SPH_EUNSUPPORTED, "the adjoint of the strict density clamp is not implemented");
+    int rc = tape_supported(ctx);
+    if (rc) return rc;
     if (std::memcmp(&tape->adj_key, adj, sizeof *adj) != 0) {  // other buffers: the reverse graphs are stale
         for (auto &g : tape->adj_exec)
             if (g) hipGraphExecDestroy(g), g = nullptr;
@@ -1311,7 +1342,7 @@ int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const 
     HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
     HIPCHK(hipEventRecord(e0, s));
     HIPCHK(hipStreamWaitEvent(g, e0, 0));
-    int rc = 0;
+    rc = tape_refresh_labels(ctx, tape, g);
     for (int k = 0; k < nsteps && !rc; ++k) {
         if (!ctx->dt_fresh) {  // a captured step expects the corrector's dt partials
             rc = record_step(ctx, tape, k, g);
@@ -1319,14 +1350,14 @@ int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const 
         }
         const int start = ctx->cur;
         rc = graph_launch(&tape->rec_exec[2 * (size_t)k + start], g, [&] { return record_step(ctx, tape, k, g); });
-        ctx->cur = 1 - start;  // a capture flips it without running; either way the step ends on the other buffer
+        // a capture flips it without running; a launched step ends on the other buffer
+        ctx->cur = rc ? start : 1 - start;
     }
     for (int k = nsteps - 1; k >= 0 && !rc; --k)
         rc = graph_launch(&tape->adj_exec[k], g, [&] { return adjoint_launch(ctx, tape, k, adj, g); });
-    if (!rc) {
-        hipEventRecord(e1, g);
-        hipStreamWaitEvent(s, e1, 0);
-    }
+    // the caller's stream waits for whatever was queued, also after an error
+    hipEventRecord(e1, g);
+    hipStreamWaitEvent(s, e1, 0);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return rc;
@@ -1400,33 +1431,28 @@ int sph_step(sph_ctx_t *ctx, int64_t nsteps, int32_t use_graph, void *stream) {
     HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
     HIPCHK(hipEventRecord(e0, s));
     HIPCHK(hipStreamWaitEvent(g, e0, 0));
+    int rc = 0;
     if (!ctx->dt_fresh && nsteps > 0) {  // the captured step expects the corrector's dt partials
-        int rc = one_step(ctx, g);
-        if (rc) return rc;
+        rc = one_step(ctx, g);
         --nsteps;
     }
-    for (int64_t k = 0; k < nsteps; ++k) {
-        int start = ctx->cur;
+    for (int64_t k = 0; k < nsteps && !rc; ++k) {
+        const int start = ctx->cur;
         if (!ctx->gexec[start]) {
-            hipGraph_t graph;
-            HIPCHK(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
-            int rc = one_step(ctx, g);
-            hipError_t ce = hipStreamEndCapture(g, &graph);
-            if (rc) return rc;
-            if (ce != hipSuccess) return fail(SPH_EHIP, "end capture: %s", hipGetErrorString(ce));
-            ce = hipGraphInstantiate(&ctx->gexec[start], graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
-            if (ce != hipSuccess) return fail(SPH_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+            rc = capture_step(&ctx->gexec[start], g, [&] { return one_step(ctx, g); });
             ctx->cur = start;  // capture flipped it without running anything
+            if (rc) break;
         }
-        HIPCHK(hipGraphLaunch(ctx->gexec[start], g));
-        ctx->cur = 1 - start;
+        hipError_t le = hipGraphLaunch(ctx->gexec[start], g);
+        if (le != hipSuccess) rc = fail(SPH_EHIP, "graph launch: %s", hipGetErrorString(le));
+        else ctx->cur = 1 - start;
     }
-    HIPCHK(hipEventRecord(e1, g));
-    HIPCHK(hipStreamWaitEvent(s, e1, 0));
+    // the caller's stream waits for whatever was queued, also after an error
+    hipEventRecord(e1, g);
+    hipStreamWaitEvent(s, e1, 0);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    return 0;
+    return rc;
 }
 
 // The loop body split at the coupling hook (src/Solver.py:389-392). The
@@ -1480,6 +1506,7 @@ static int stage_ids(sph_ctx *c, const int64_t *ids, int64_t k, bool distinct, h
 
 int sph_state_gather(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_state_t *st, void *stream) {
     if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
+    if (unloaded(ctx)) return SPH_ESTATE;
     hipStream_t s = S(stream);
     int rc = stage_ids(ctx, ids, k, false, s);
     if (rc || k == 0) return rc;
@@ -1492,6 +1519,7 @@ int sph_state_gather(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_st
 
 int sph_state_scatter(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_state_t *st, void *stream) {
     if (!ctx || !st) return fail(SPH_EINVAL, "NULL argument");
+    if (unloaded(ctx)) return SPH_ESTATE;
     if (st->label || st->p) return fail(SPH_EINVAL, "label and p cannot be written (p follows from rho)");
     if (!ctx->P.use_xsph && (st->xsphx || st->xsphy))
         return fail(SPH_EUNSUPPORTED, "without XSPH the XSPH velocity is not kept on the device");
@@ -1587,6 +1615,7 @@ int sph_set_clock(sph_ctx_t *ctx, double t, int32_t advance_time, void *stream) 
 
 int sph_kinetic_energy(sph_ctx_t *ctx, double *out, void *stream) {
     if (!ctx || !out) return fail(SPH_EINVAL, "NULL argument");
+    if (unloaded(ctx)) return SPH_ESTATE;
     hipStream_t s = S(stream);
     const int R = redblk(ctx->n);
     k_ke_partial<<<R, kBlock, 0, s>>>(ctx->buf[ctx->cur], ctx->n, ctx->part_k);
@@ -1638,6 +1667,7 @@ int sph_nn_grid(sph_ctx_t *ctx, double out_d[5], int64_t out_i[3], void *stream)
 int sph_nn_dump(sph_ctx_t *ctx, int64_t *keys, int64_t *off, int32_t *idx, double *q, double *r, int64_t cap,
                 int64_t *total, void *stream) {
     if (!ctx || !keys || !off || !total) return fail(SPH_EINVAL, "NULL argument");
+    if (unloaded(ctx)) return SPH_ESTATE;
     hipStream_t s = S(stream);
     int rc = sph_nn_grid(ctx, nullptr, nullptr, stream);
     if (rc) return rc;
@@ -1659,10 +1689,8 @@ int sph_nn_dump(sph_ctx_t *ctx, int64_t *keys, int64_t *off, int32_t *idx, doubl
     cnt[n] = acc;
     *total = acc;
     HIPCHK(hipMemcpyAsync(off, cnt.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, s));
-    if (acc > cap) {
-        HIPCHK(hipStreamSynchronize(s));
-        return fail(SPH_ECAPACITY, "dump needs %lld entries, cap is %lld", acc, (long long)cap);
-    }
+    HIPCHK(hipStreamSynchronize(s));  // cnt is freed on return
+    if (acc > cap) return fail(SPH_ECAPACITY, "dump needs %lld entries, cap is %lld", acc, (long long)cap);
     if (!idx) return fail(SPH_EINVAL, "idx is NULL");
     k_nn_dump<<<nblk(n), kBlock, 0, s>>>(ctx->ctrl, st, ctx->cellStart, ctx->slot_of, n, 1, nullptr,
                                           (long long *)off, idx, q, r, ctx->P.qmax);
@@ -1678,6 +1706,7 @@ __global__ void k_unpermute_int(const int *src, const int *id, int *dst, int n) 
 
 int sph_compute_counts(sph_ctx_t *ctx, int32_t *counts, void *stream) {
     if (!ctx || !counts) return fail(SPH_EINVAL, "NULL argument");
+    if (closed(ctx)) return SPH_ESTATE;
     hipStream_t s = S(stream);
     int rc = sph_nn_grid(ctx, nullptr, nullptr, stream);
     if (rc) return rc;

@@ -128,6 +128,31 @@ def test_adjoint_checkpoint_interval_is_exact():
     assert torch.equal(grads[0][2], grads[1][2])
 
 
+def test_tape_made_before_the_upload_takes_the_labels():
+    """A tape made before any state is on the device holds no fluid mask; a
+    recording from slot 0 takes the labels of the state it records, so the
+    gradient is the one of a tape made afterwards, bit for bit (walls and
+    fluid told apart), with and without hipGraph segments."""
+    from hipsph.adjoint import Tape
+    sc = scenarios.dam_break_c5(small=True)
+    m = sc.method()
+    target = _target(sc, 12)
+    v0 = torch.zeros((sc.n_fluid, 2), dtype=torch.float64, device='cuda')
+    out = []
+    for early in (False, True, True):
+        run = SPHRun(sc.pA, m, 12, checkpoint_every=6)
+        if early:
+            run._tape = Tape(run.engine, run.k)
+        xt = run.forward(m.alpha, m.co, v0).clone().requires_grad_(True)
+        (g,) = torch.autograd.grad(_mse(xt, target), [xt])
+        ga, gc, gv = run.backward(g, graphs=len(out) != 2)
+        out.append((float(ga), float(gc), gv.clone()))
+        run.close()
+    for o in out[1:]:
+        assert o[0] == out[0][0] and o[1] == out[0][1]
+        assert torch.equal(o[2], out[0][2])
+
+
 def test_adjoint_graph_segments_match_eager_steps():
     """sph_adjoint_segment (hipGraph replays) gives the same bits as
     sph_step_record / sph_adjoint_step launched one by one."""

@@ -417,6 +417,35 @@ def test_no_fluid_halts_without_touching_the_state():
     e.close()
 
 
+def test_calls_before_upload_are_refused():
+    """Before the first upload the device buffers hold no particle ids: every
+    call that indexes by id (steps, download, gather, the neighbour dump, the
+    kinetic energy) is refused with ESTATE and launches nothing; after the
+    upload the same engine steps as usual. Likewise for a batch."""
+    from hipsph import _lib
+    from hipsph.batch import BatchEngine
+    from hipsph.device import Engine, EngineParams, soa_from_particles
+    sc = scenarios.dam_break_c1()
+    m = sc.method()
+    p = EngineParams.from_method(len(sc.pA), m)
+    e = Engine(p)
+    for fn in (lambda: e.step(1), lambda: e.step(2, graph=True), e.step_begin, e.nn_update, e.compute, e.dt,
+               e.download, lambda: e.gather([0]), e.kinetic_energy, e.nn_dump):
+        with pytest.raises(_lib.HipSphError) as ex:
+            fn()
+        assert _lib.ERRORS[ex.value.code] == 'ESTATE'
+    e.upload(soa_from_particles(sc.pA))
+    e.step(2)
+    assert e.status()['steps'] == 2
+    e.close()
+    b = BatchEngine([p, p])
+    for fn in (lambda: b.step(1), b.download):
+        with pytest.raises(_lib.HipSphError) as ex:
+            fn()
+        assert _lib.ERRORS[ex.value.code] == 'ESTATE'
+    b.close()
+
+
 def test_single_particle_free_fall_fp64_bitexact():
     """One fluid particle, no walls: a one-cell grid, the particle its own
     only neighbour (no pair term), gravity alone; 50 steps bit-identical to
