@@ -199,8 +199,8 @@ def ensemble(steps=500, count=256, seq_steps=10, seq_instances=64):
     """The batched ensemble: `count` dam breaks (width U(0.5, 1.5), height
     U(0.5, 2.5), dx from {0.005, 0.01, 0.02}, alpha U(0.005, 0.05), seeded per
     instance), FP32C pairs and FP64 integrator, `steps` steps each in one batch
-    (one launch per stage, hipGraph replay, pair workgroup size and
-    contributor-list length autotuned).
+    (one launch per stage, hipGraph replay, pair workgroup size,
+    contributor-list length and schedule autotuned).
     Beside it: the same instances as separate engines one after another
     (graph replay each), timed on the first `seq_instances` for `seq_steps`."""
     import torch
@@ -252,7 +252,7 @@ def ensemble(steps=500, count=256, seq_steps=10, seq_instances=64):
            'min_steps': int(min(steps_done)), 'halted_instances': halted,
            'ms_per_step': el / steps * 1e3,
            'particle_steps_per_s': nf * steps / el, 'pair_ms': pair_ms,
-           'pair_threads': best[0], 'pair_list_len': best[1], 'autotune': table,
+           'pair_threads': best[0], 'pair_list_len': best[1], 'pair_persistent': best[2], 'autotune': table,
            'sequential_engines_ms_per_step_est': seq_full_ms,
            'sequential_sample': f'{seq_instances} instances x {seq_steps} steps, scaled by fluid count',
            'batch_speedup_vs_sequential': seq_full_ms / (el / steps * 1e3)}

@@ -302,6 +302,11 @@ int sph_batch_set_tuning(sph_batch_t *batch, int32_t pair_threads);
  * x the workgroup): 16, 24 or 32, or 0 for the single-simulation pass's
  * (24 for the float cubic/Wendland pass, else 32). Chosen by the autotuner. */
 int sph_batch_set_list_len(sph_batch_t *batch, int32_t list_len);
+/* Pair-pass schedule: 0 = one thread per slot (one workgroup per tile of
+ * slots), 1 = a persistent kernel of about one resident occupancy whose
+ * workgroups take tiles from a per-XCD work queue until it is drained.
+ * Results are identical either way. Chosen by the autotuner. */
+int sph_batch_set_persistent(sph_batch_t *batch, int32_t on);
 int sph_batch_timing_enable(sph_batch_t *batch, int32_t on);
 int sph_batch_timing_read(sph_batch_t *batch, double *avg_ms_pair, int64_t *count, int32_t reset);
 

@@ -466,6 +466,10 @@ struct sph_batch {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int pair_threads = 256;
     int list_len = 0;  // 0: the single-simulation pass's length
+    int persistent = 0;    // the pair pass as a persistent work-queue kernel
+    int *queue = nullptr;  // its counters (9 x 32 ints), zero between launches
+    int cus = 0;           // compute units of the batch's device
+    int queue_grid = 0;    // its grid (0: to be sized before the next launch)
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t n_ev = 0;
@@ -482,12 +486,23 @@ void free_batch(sph_batch *b) {
     }
     void *ptrs[] = {b->inst, b->view, b->instOf, b->cs, b->chunkInst, b->chunk0, b->bin, b->rank, b->perm,
                     b->permKey, b->cxy_s, b->counts, b->cellStart, b->tileSums, b->binflag,
-                    b->part_b, b->part_d, b->recA, b->recB};
+                    b->part_b, b->part_d, b->recA, b->recB, b->queue};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : b->ev) hipEventDestroy(e);
     if (b->own) hipStreamDestroy(b->own);
     delete b;
+}
+
+// The persistent pair pass's grid: one resident occupancy of the device
+// (hipOccupancy for the chosen kernel, list and workgroup size). Host only;
+// sized before a step is launched or captured.
+void size_queue(sph_batch *b) {
+    if (!b->persistent || b->queue_grid > 0) return;
+    const int per = b->precision == SPH_PREC_FP32C
+                        ? pair_batch_queue_occupancy_fp32c(b->kernel, b->list_len, b->pair_threads)
+                        : pair_batch_queue_occupancy_fp64(b->kernel, b->list_len, b->pair_threads);
+    b->queue_grid = (b->cus > 0 ? b->cus : kXcds) * per;
 }
 
 int b_dt(sph_batch *b, hipStream_t s) {
@@ -533,11 +548,12 @@ int b_pair(sph_batch *b, hipStream_t s) {
         }
         HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
     }
+    int *q = b->persistent ? b->queue : nullptr;
     hipError_t e = b->precision == SPH_PREC_FP32C
                        ? launch_pair_batch_fp32c(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads,
-                                                 b->list_len, s)
+                                                 b->list_len, q, b->queue_grid, s)
                        : launch_pair_batch_fp64(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads,
-                                                b->list_len, s);
+                                                b->list_len, q, b->queue_grid, s);
     if (e != hipSuccess) return fail(SPH_EHIP, "batch pair launch: %s", hipGetErrorString(e));
     if (b->timing) {
         if (b->n_ev >= b->ev.size()) {
@@ -689,7 +705,8 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMalloc(&b->binflag, sizeof(int) * (capTot + 1)) == hipSuccess &&
          hipMalloc(&b->part_b, sizeof(double) * kBoundsQ * b->NC) == hipSuccess &&
          hipMalloc(&b->part_d, sizeof(double) * kDtQ * b->NC) == hipSuccess &&
-         hipMalloc(&b->recA, ra * n) == hipSuccess && hipMalloc(&b->recB, rb * n) == hipSuccess;
+         hipMalloc(&b->recA, ra * n) == hipSuccess && hipMalloc(&b->recB, rb * n) == hipSuccess &&
+         hipMalloc(&b->queue, sizeof(int) * kQueueInts) == hipSuccess;
     if (!ok) {
         free_batch(b);
         return fail(SPH_ENOMEM, "device allocation for a batch of %lld particles failed", ntot);
@@ -707,7 +724,12 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMemset(b->counts, 0, sizeof(int) * (capTot + 1)) == hipSuccess &&
          hipMemset(b->cellStart, 0, sizeof(int) * (capTot + 2)) == hipSuccess &&
          hipMemset(b->binflag, 0, sizeof(int) * (capTot + 1)) == hipSuccess &&
+         hipMemset(b->queue, 0, sizeof(int) * kQueueInts) == hipSuccess &&
          hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) == hipSuccess;
+    int dev = 0;
+    if (ok && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        b->cus = 0;
     if (!ok) {
         free_batch(b);
         return fail(SPH_EHIP, "batch initialisation failed");
@@ -758,6 +780,7 @@ int sph_batch_step(sph_batch_t *b, int64_t nsteps, int32_t use_graph, void *stre
     if (!b) return fail(SPH_EINVAL, "batch is NULL");
     if (nsteps < 0) return fail(SPH_EINVAL, "nsteps < 0");
     if (!b->loaded) return fail(SPH_ESTATE, "no state: call sph_batch_upload first");
+    size_queue(b);
     hipStream_t s = (hipStream_t)stream;
     if (!use_graph || b->timing) {
         for (int64_t k = 0; k < nsteps; ++k) {
@@ -835,6 +858,7 @@ int sph_batch_set_tuning(sph_batch_t *b, int32_t pair_threads) {
         return fail(SPH_EINVAL, "pair_threads must be 64, 128 or 256");
     if (pair_threads != b->pair_threads) {
         b->pair_threads = pair_threads;
+        b->queue_grid = 0;
         for (auto &g : b->gexec)
             if (g) hipGraphExecDestroy(g), g = nullptr;
     }
@@ -847,6 +871,19 @@ int sph_batch_set_list_len(sph_batch_t *b, int32_t list_len) {
         return fail(SPH_EINVAL, "list_len must be 0, 16, 24 or 32");
     if (list_len != b->list_len) {
         b->list_len = list_len;
+        b->queue_grid = 0;
+        for (auto &g : b->gexec)
+            if (g) hipGraphExecDestroy(g), g = nullptr;
+    }
+    return 0;
+}
+
+int sph_batch_set_persistent(sph_batch_t *b, int32_t on) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    if (on != 0 && on != 1) return fail(SPH_EINVAL, "persistent must be 0 or 1");
+    if (on != b->persistent) {
+        b->persistent = on;
+        b->queue_grid = 0;
         for (auto &g : b->gexec)
             if (g) hipGraphExecDestroy(g), g = nullptr;
     }

@@ -166,14 +166,25 @@ __host__ __device__ inline Phys inst_phys(const BInst &I) {
 
 // The pair pass over a batch: instOf[t] is the instance of slot t.
 // list: the per-thread contributor list length (16, 24, 32; 0 = default).
+// queue (9 counters, 32 ints apart, zero before the first launch): the
+// persistent work-queue form over about `grid` workgroups; null: one thread
+// per slot.
 hipError_t launch_pair_batch_fp64(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
-                                  int threads, int list, hipStream_t s);
+                                  int threads, int list, int *queue, int grid, hipStream_t s);
 hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
-                                   int threads, int list, hipStream_t s);
+                                   int threads, int list, int *queue, int grid, hipStream_t s);
+// Resident workgroups per CU of the persistent form (not to be called under
+// stream capture).
+int pair_batch_queue_occupancy_fp64(int kernel, int list, int threads);
+int pair_batch_queue_occupancy_fp32c(int kernel, int list, int threads);
 
 // Number of blocks for a grid over n items, padded to a multiple of the XCD
 // count so that the block remap below is a bijection.
 constexpr int kXcds = 8;
+// The persistent batch pair pass's work queue: one counter per XCD and one
+// for the workgroups that finished, 128 B apart.
+constexpr int kQueueStride = 32;
+constexpr int kQueueInts = (kXcds + 1) * kQueueStride;
 inline int xcd_blocks(long long n, int bs) {
     long long b = (n + bs - 1) / bs;
     if (b < 1) b = 1;

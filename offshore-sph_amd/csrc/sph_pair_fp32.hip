@@ -27,8 +27,13 @@ hipError_t launch_sumden_fp32c(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, 
 }
 
 hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
-                                   int threads, int list, hipStream_t s) {
-    return launch_pair_batch<float, false>(inst, instOf, pair_args<float>(in, &out, n), kernel, threads, list, s);
+                                   int threads, int list, int *queue, int grid, hipStream_t s) {
+    return launch_pair_batch<float, false>(inst, instOf, pair_args<float>(in, &out, n), kernel, threads, list, queue,
+                                           grid, s);
+}
+
+int pair_batch_queue_occupancy_fp32c(int kernel, int list, int threads) {
+    return queue_occupancy<float, false>(kernel, list, threads);
 }
 
 }  // namespace hipsph

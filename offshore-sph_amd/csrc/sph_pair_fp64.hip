@@ -27,8 +27,13 @@ hipError_t launch_sumden_fp64(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, i
 }
 
 hipError_t launch_pair_batch_fp64(const BInst *inst, const int *instOf, PairIn in, PairOut out, int kernel, int n,
-                                  int threads, int list, hipStream_t s) {
-    return launch_pair_batch<double, true>(inst, instOf, pair_args<double>(in, &out, n), kernel, threads, list, s);
+                                  int threads, int list, int *queue, int grid, hipStream_t s) {
+    return launch_pair_batch<double, true>(inst, instOf, pair_args<double>(in, &out, n), kernel, threads, list, queue,
+                                           grid, s);
+}
+
+int pair_batch_queue_occupancy_fp64(int kernel, int list, int threads) {
+    return queue_occupancy<double, true>(kernel, list, threads);
 }
 
 }  // namespace hipsph

@@ -745,17 +745,96 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_batch(const BInst *inst, 
                                                 s_list);
 }
 
-// The batch's list lengths; 0 picks the single-simulation pass's.
+// The same pass as a persistent kernel: a grid of about one resident
+// occupancy (CUs x workgroups per CU) takes tiles of blockDim.x consecutive
+// slots from a work queue until it is drained. The queue is one counter per
+// XCD, each over a contiguous eighth of the tiles, and the workgroups the
+// dispatcher hands to an XCD (blockIdx % 8) take from that XCD's counter, so
+// neighbouring tiles still share an L2. Every workgroup ends once its
+// counter has passed the end of its range; none waits for another. The last
+// workgroup out (a separate counter) clears all counters, so the next launch,
+// graph replays included, starts from zero without a memset. Each target's
+// sums do not depend on which workgroup runs it: results are those of
+// k_pair_batch bit for bit.
+template <typename R, int KERNEL, bool EXACT, int LIST>
+__global__ __launch_bounds__(kPairThreads) void k_pair_batch_queue(const BInst *inst, const int *instOf,
+                                                                   PairArgs<R> P, int *queue) {
+    __shared__ unsigned s_list[LIST * kPairThreads];
+    __shared__ int s_tile;
+    const int ntiles = (int)((P.n + (long long)blockDim.x - 1) / blockDim.x);
+    const int xcd = (int)(blockIdx.x % kXcds);
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int lo = xcd * per;
+    const int hi = lo + per < ntiles ? lo + per : ntiles;
+    int *ctr = queue + xcd * kQueueStride;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = lo + atomicAdd(ctr, 1);
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads();  // every lane has read s_tile before lane 0 writes the next
+        if (tile >= hi) break;
+        const int t = tile * (int)blockDim.x + (int)threadIdx.x;
+        if (t < P.n) {
+            const BInst &I = inst[instOf[t]];
+            if (I.active)
+                pair_target<R, KERNEL, EXACT, kFused, LIST>(P, inst_phys(I), (int)I.grid.ncx, (int)I.grid.ncy,
+                                                            I.cellBase, t, s_list);
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(queue + kXcds * kQueueStride, 1) == (int)gridDim.x - 1)
+            for (int k = 0; k <= kXcds; ++k) atomicExch(queue + k * kQueueStride, 0);
+    }
+}
+
+// The batch's list lengths; 0 picks the single-simulation pass's. queue:
+// the persistent form over `grid` workgroups (a multiple of 8), else one
+// thread per slot.
 template <typename R, int KERNEL, bool EXACT>
 void launch_batch_list(const BInst *inst, const int *instOf, const PairArgs<R> &P, int list, int nb, int threads,
-                       hipStream_t s) {
+                       int *queue, int grid, hipStream_t s) {
     if (list == 0) list = list_len<R, KERNEL>();
+    if (queue) {
+        if (list == 16)
+            k_pair_batch_queue<R, KERNEL, EXACT, 16><<<grid, threads, 0, s>>>(inst, instOf, P, queue);
+        else if (list == 24)
+            k_pair_batch_queue<R, KERNEL, EXACT, 24><<<grid, threads, 0, s>>>(inst, instOf, P, queue);
+        else
+            k_pair_batch_queue<R, KERNEL, EXACT, 32><<<grid, threads, 0, s>>>(inst, instOf, P, queue);
+        return;
+    }
     if (list == 16)
         k_pair_batch<R, KERNEL, EXACT, 16><<<nb, threads, 0, s>>>(inst, instOf, P);
     else if (list == 24)
         k_pair_batch<R, KERNEL, EXACT, 24><<<nb, threads, 0, s>>>(inst, instOf, P);
     else
         k_pair_batch<R, KERNEL, EXACT, 32><<<nb, threads, 0, s>>>(inst, instOf, P);
+}
+
+// Resident workgroups per CU of the persistent batch pass (hipOccupancy), for
+// sizing its grid outside any stream capture.
+template <typename R, int KERNEL, bool EXACT>
+int queue_occupancy_list(int list, int threads) {
+    if (list == 0) list = list_len<R, KERNEL>();
+    int per = 0;
+    hipError_t e;
+    if (list == 16)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pair_batch_queue<R, KERNEL, EXACT, 16>, threads, 0);
+    else if (list == 24)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pair_batch_queue<R, KERNEL, EXACT, 24>, threads, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pair_batch_queue<R, KERNEL, EXACT, 32>, threads, 0);
+    return e == hipSuccess && per > 0 ? per : 1;
+}
+
+template <typename R, bool EXACT>
+int queue_occupancy(int kernel, int list, int threads) {
+    switch (kernel) {
+        case SPH_KERNEL_GAUSSIAN: return queue_occupancy_list<R, SPH_KERNEL_GAUSSIAN, EXACT>(list, threads);
+        case SPH_KERNEL_WENDLAND: return queue_occupancy_list<R, SPH_KERNEL_WENDLAND, EXACT>(list, threads);
+        default: return queue_occupancy_list<R, SPH_KERNEL_CUBIC, EXACT>(list, threads);
+    }
 }
 
 template <typename R, bool EXACT, int MODE, bool XS = false>
@@ -774,18 +853,23 @@ void launch_pair_mode(const Ctrl *ctrl, const PairArgs<R> &P, const Phys &ph, in
 
 template <typename R, bool EXACT>
 hipError_t launch_pair_batch(const BInst *inst, const int *instOf, const PairArgs<R> &P, int kernel, int threads,
-                             int list, hipStream_t s) {
+                             int list, int *queue, int grid, hipStream_t s) {
     if (threads != 64 && threads != 128 && threads != 256) threads = kPairThreads;
     const int nb = xcd_blocks(P.n, threads);
+    if (queue) {  // no more workgroups than tiles; a multiple of the XCD count, so every queue has takers
+        if (grid > nb) grid = nb;
+        grid = grid / kXcds * kXcds;
+        if (grid < kXcds) grid = kXcds;
+    }
     switch (kernel) {
         case SPH_KERNEL_GAUSSIAN:
-            launch_batch_list<R, SPH_KERNEL_GAUSSIAN, EXACT>(inst, instOf, P, list, nb, threads, s);
+            launch_batch_list<R, SPH_KERNEL_GAUSSIAN, EXACT>(inst, instOf, P, list, nb, threads, queue, grid, s);
             break;
         case SPH_KERNEL_WENDLAND:
-            launch_batch_list<R, SPH_KERNEL_WENDLAND, EXACT>(inst, instOf, P, list, nb, threads, s);
+            launch_batch_list<R, SPH_KERNEL_WENDLAND, EXACT>(inst, instOf, P, list, nb, threads, queue, grid, s);
             break;
         default:
-            launch_batch_list<R, SPH_KERNEL_CUBIC, EXACT>(inst, instOf, P, list, nb, threads, s);
+            launch_batch_list<R, SPH_KERNEL_CUBIC, EXACT>(inst, instOf, P, list, nb, threads, queue, grid, s);
     }
     return hipGetLastError();
 }

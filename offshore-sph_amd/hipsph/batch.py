@@ -7,8 +7,9 @@ replays as one hipGraph; each instance evolves exactly as an ``Engine`` with
 the same parameters would advance it alone.
 
 ``autotune(batch)`` picks the pair pass's launch shape for the batch's own
-workload, the workgroup size and the per-thread contributor-list length (its
-LDS tile), by timing each candidate on the device.
+workload, the workgroup size, the per-thread contributor-list length (its
+LDS tile) and the schedule (one thread per slot, or a persistent kernel
+draining a work queue), by timing each candidate on the device.
 """
 import ctypes as C
 
@@ -89,10 +90,12 @@ class BatchEngine:
         return [{'t': d[4 * k], 'dt': d[4 * k + 1], 'dt_c': d[4 * k + 2], 'dt_f': d[4 * k + 3],
                  'steps': int(i[2 * k]), 'flags': int(i[2 * k + 1])} for k in range(self.K)]
 
-    def set_tuning(self, pair_threads, list_len=None):
+    def set_tuning(self, pair_threads, list_len=None, persistent=None):
         call('sph_batch_set_tuning', self._b, int(pair_threads))
         if list_len is not None:
             call('sph_batch_set_list_len', self._b, int(list_len))
+        if persistent is not None:
+            call('sph_batch_set_persistent', self._b, int(bool(persistent)))
 
     def timing_enable(self, on=True):
         call('sph_batch_timing_enable', self._b, int(bool(on)))
@@ -115,25 +118,27 @@ class BatchEngine:
             pass
 
 
-def autotune(batch, candidates=(64, 128, 256), lists=(16, 24, 32), steps=4):
-    """Pick the pair pass's workgroup size and contributor-list length by
-    timing each pair over a few eager steps of the batch itself (advances the
-    batch by len(candidates) * len(lists) * (steps + 1) steps; the results do
-    not depend on either choice); returns ((threads, list_len), table of
-    (threads, list_len, ms))."""
+def autotune(batch, candidates=(64, 128, 256), lists=(16, 24, 32), schedules=(False, True), steps=4):
+    """Pick the pair pass's workgroup size, contributor-list length and
+    schedule (persistent work-queue kernel or not) by timing each combination
+    over a few eager steps of the batch itself (advances the batch by
+    len(candidates) * len(lists) * len(schedules) * (steps + 1) steps; no
+    choice changes a result); returns ((threads, list_len, persistent), table
+    of (threads, list_len, persistent, ms))."""
     table = []
     for threads in candidates:
         for ll in lists:
-            batch.set_tuning(threads, ll)
-            batch.step(1, graph=False)
-            batch.timing_enable(True)
-            batch.step(steps, graph=False)
-            ms = batch.timing_read()['pair_ms']
-            batch.timing_enable(False)
-            table.append((threads, ll, ms))
-    best = min(table, key=lambda r: r[2])
-    batch.set_tuning(best[0], best[1])
-    return (best[0], best[1]), table
+            for per in schedules:
+                batch.set_tuning(threads, ll, per)
+                batch.step(1, graph=False)
+                batch.timing_enable(True)
+                batch.step(steps, graph=False)
+                ms = batch.timing_read()['pair_ms']
+                batch.timing_enable(False)
+                table.append((threads, ll, bool(per), ms))
+    best = min(table, key=lambda r: r[3])
+    batch.set_tuning(best[0], best[1], best[2])
+    return (best[0], best[1], best[2]), table
 
 
 def params_for(scenario, alpha=None, **kw):

@@ -5,7 +5,8 @@ resolution, height (hence c0, B, D) and alpha, and compares every instance
 with a single-simulation engine (which is itself bit-exact against the CPU
 oracle) on the same initial state:
   * FP64 pair pass: bit-identical state and clock after 40 steps, eager and
-    hipGraph replay;
+    hipGraph replay, with the pair pass one thread per slot and as the
+    persistent work-queue kernel;
   * FP32C pair pass: the same pair code per particle, equal to 1e-12;
   * an instance whose duration ends early stops while the others go on,
     exactly as a single engine with that duration.
@@ -44,10 +45,12 @@ def _single(sc, alpha, steps, precision, duration=np.inf):
 
 @pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
 @pytest.mark.parametrize('graph', [False, True])
-def test_batch_instances_match_single_engines(precision, graph):
+@pytest.mark.parametrize('persistent', [False, True])
+def test_batch_instances_match_single_engines(precision, graph, persistent):
     mem = _members()
     K = 40
     b = BatchEngine([params_for(sc, alpha, precision=precision) for sc, alpha in mem])
+    b.set_tuning(256, 0, persistent)
     b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
     b.step(K, graph=graph)
     got = b.download()
@@ -96,14 +99,18 @@ def test_batch_duration_per_instance():
 
 
 def test_batch_autotune_keeps_results():
-    """The launch shape (workgroup size, contributor-list length) changes no
-    result: a list flushes in listing order, a lane's target is its own."""
+    """The launch shape (workgroup size, contributor-list length, one thread
+    per slot or the persistent work-queue kernel) changes no result: a list
+    flushes in listing order, a lane's target is its own, whichever
+    workgroup takes its tile. The persistent runs replay their graph 12
+    times, so the queue's self-reset is exercised too."""
     mem = _members()
     b = BatchEngine([params_for(sc, alpha, precision='fp32c') for sc, alpha in mem])
     runs = []
-    for threads, ll in ((256, 24), (64, 16), (128, 32)):
+    for threads, ll, per in ((256, 24, False), (64, 16, False), (128, 32, False), (256, 24, True),
+                             (64, 32, True), (128, 16, True)):
         b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
-        b.set_tuning(threads, ll)
+        b.set_tuning(threads, ll, per)
         b.step(12, graph=True)
         runs.append(b.download())
     for other in runs[1:]:
@@ -113,5 +120,6 @@ def test_batch_autotune_keeps_results():
     b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
     best, table = autotune(b, steps=2)
     say('batch autotune', best, table)
-    assert best[0] in (64, 128, 256) and best[1] in (16, 24, 32) and len(table) == 9
+    assert best[0] in (64, 128, 256) and best[1] in (16, 24, 32) and best[2] in (False, True)
+    assert len(table) == 18
     b.close()
