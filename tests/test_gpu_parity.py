@@ -446,6 +446,28 @@ def test_calls_before_upload_are_refused():
     b.close()
 
 
+def test_kinetic_energy_of_the_fluid():
+    """sph_kinetic_energy is KineticEnergy (src/Equations/KineticEnergy.py:5-12)
+    over the fluid records, as the Solver passes them (src/Solver.py:202,416):
+    vx = 1 on every fluid particle gives n_fluid * 0.5 * m (walls moving at 5
+    add nothing); random velocities match the host sum. Both to the rounding
+    of a different summation order."""
+    from hipsph.Solver import KineticEnergy
+    sc = scenarios.dam_break_c1()
+    pA = sc.pA.copy()
+    f = pA['label'] == 0
+    pA['vx'][f] = 1.0
+    pA['vx'][~f] = 5.0
+    e = make_engine(pA, sc.method())
+    assert e.kinetic_energy() == pytest.approx(f.sum() * 0.5 * pA['m'][f][0], rel=1e-13)
+    e.close()
+    q = perturbed(sc.pA, 4)
+    e = make_engine(q, sc.method())
+    want = KineticEnergy(int(f.sum()), q[f])
+    assert e.kinetic_energy() == pytest.approx(want, rel=1e-13)
+    e.close()
+
+
 def test_single_particle_free_fall_fp64_bitexact():
     """One fluid particle, no walls: a one-cell grid, the particle its own
     only neighbour (no pair term), gravity alone; 50 steps bit-identical to

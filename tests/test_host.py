@@ -115,3 +115,34 @@ def test_solver_rejects_mismatched_xsph_flags():
         sv.addParticles(sc.pA.copy())
         with pytest.raises(ValueError):
             sv.setup()
+
+
+def test_kinetic_energy_and_find_active_known_answers():
+    """The reference's own known answers: test/test_kinetic_energy.py:11-23
+    (zero records give 0; m = 1, vx = 1 on 100 records give 100 * 0.5) and
+    test/test_tools.py:7-21 (ten records, one deleted: count 9, the mask
+    false at the deleted one)."""
+    from hipsph.Solver import KineticEnergy, findActive
+    pA = np.zeros(100, dtype=Common.particle_dtype)
+    assert KineticEnergy(len(pA), pA) == 0
+    pA['m'] = 1
+    pA['vx'] = 1
+    assert KineticEnergy(len(pA), pA) == 100 * 0.5 * 1 * 1
+    no = np.zeros(10, dtype=Common.particle_dtype)
+    count, indexes = findActive(len(no), no)
+    assert count == 10 and len(indexes) == 10 and indexes.all()
+    no['deleted'][5] = True
+    count, indexes = findActive(len(no), no)
+    assert count == 9 and len(indexes) == 10 and not indexes[5] and indexes.sum() == 9
+
+
+def test_courant_known_answers():
+    """test/test_eq_courant.py:8-11 (alpha h / c for h, c = 0, 1; 1, 1; 2, 4)
+    on TimeStep.courant (src/Equations/TimeStep.py:47-49), the form the
+    solver's time step uses."""
+    from hipsph.Equations.TimeStep import TimeStep
+    ts = TimeStep()
+    assert ts.courant(0.4, 0.0, 1.0) == 0.0
+    assert ts.courant(0.4, 1.0, 1.0) == 0.4
+    assert ts.courant(0.4, 2.0, 4.0) == 0.5 * 0.4
+    assert ts.force(0.25, 1.0, 0.0) == 1e10 and ts.force(0.25, 4.0, 1.0) == 0.5
