@@ -304,8 +304,9 @@ __global__ void k_cellsort(const Ctrl *ctrl, const int *cellStart, int *perm, in
 __device__ inline int2 cell_of(double x, double y, const Grid &g) {
     double fx = floor((x - g.xmin) / g.cs);
     double fy = floor((y - g.ymin) / g.cs);
-    fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
-    fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
+    // clamped so the integer conversion is defined; NaN goes below the grid
+    fx = !(fx >= -1.0e9) ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
+    fy = !(fy >= -1.0e9) ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
     return make_int2((int)(long long)fx, (int)(long long)fy);
 }
 

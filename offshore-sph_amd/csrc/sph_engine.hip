@@ -623,8 +623,9 @@ __global__ void k_inverse(const int *id, int *slot_of, int n) {
 __device__ inline void raw_cell(const Grid &g, double x, double y, long long &cx0, long long &cy0) {
     double fx = floor((x - g.xmin) / g.cs);
     double fy = floor((y - g.ymin) / g.cs);
-    fx = fx < -1.0e9 ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
-    fy = fy < -1.0e9 ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
+    // clamped so the integer conversion is defined; NaN goes below the grid
+    fx = !(fx >= -1.0e9) ? -1.0e9 : (fx > 1.0e9 ? 1.0e9 : fx);
+    fy = !(fy >= -1.0e9) ? -1.0e9 : (fy > 1.0e9 ? 1.0e9 : fy);
     cx0 = (long long)fx;
     cy0 = (long long)fy;
 }

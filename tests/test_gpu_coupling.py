@@ -353,3 +353,18 @@ def test_engine_device_probe_hook_fp64_bitexact():
         s.end()
     for f in FIELDS:
         np.testing.assert_array_equal(d[f], s.pA[f], err_msg=f)
+
+
+def test_near_pos_and_probe_outside_the_grid():
+    """Points far outside the grid, or NaN, have no cells to walk: nearPos
+    is empty and the probe gives the empty sum (rho = 0, p = -B), as for a
+    point whose 3 x 3 cells hold no fluid (examples/IceBreak.py:253-285)."""
+    sc, pA = _scene()
+    m = sc.method()
+    e = make_engine(pA, m)
+    e.step(2)
+    off, idx, q, r = e.near_pos([1e12, np.nan, -5.0], [0.1, 0.1, np.nan], 0.026)
+    assert len(idx) == 0 and list(off) == [0, 0, 0, 0]
+    rho, p = e.probe_pressure([1e12, np.nan, 0.1], [0.1, 0.1, -1e300], 0.026)
+    assert np.all(rho == 0.0) and np.all(p == -m.B)
+    e.close()
