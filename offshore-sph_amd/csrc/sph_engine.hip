@@ -971,7 +971,10 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
     c->P = *params;
     c->n = (int)params->n;
     const size_t n = (size_t)c->n;
-    hipGetDevice(&c->dev);
+    if (hipGetDevice(&c->dev) != hipSuccess) {
+        delete c;
+        return fail(SPH_EHIP, "no current HIP device");
+    }
     // Two cell-ordered buffer sets: 17 double arrays + int8 label + int id.
     const size_t nd = 17;
     const size_t bytes = n * (nd * sizeof(double) + sizeof(int) + 8) + 1024;
