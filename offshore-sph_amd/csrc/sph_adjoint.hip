@@ -642,6 +642,7 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     t->slot.resize(nslots);
     t->rec_exec.assign(2 * (size_t)nslots, nullptr);
     t->adj_exec.assign(nslots, nullptr);
+    t->gen = ctx->gen;
     for (int k = 0; k < nslots; ++k) {
         TapeSlot &S = t->slot[k];
         double **f[] = {&S.vxh, &S.vyh, &S.ax, &S.ay, &S.drho, &S.vx0, &S.vy0, &S.axp, &S.ayp, &S.drhop};

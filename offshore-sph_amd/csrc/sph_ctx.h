@@ -30,6 +30,10 @@ struct sph_ctx {
     int *slot_of = nullptr;
     hipStream_t own = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // bumped whenever values that captured graphs bake into kernel arguments
+    // (constants, launch shape) change; graphs captured under another value
+    // are stale
+    unsigned long long gen = 0;
     int pair_threads = 256;
     // timing of the pair pass and of the neighbour update
     bool timing = false;
@@ -91,6 +95,7 @@ struct sph_tape {
     // parity), reversal per slot for the adjoint buffers in adj_key
     std::vector<hipGraphExec_t> rec_exec, adj_exec;
     sph_adjoint_t adj_key{};
+    unsigned long long gen = 0;  // the context's gen when the cached graphs were captured
 };
 
 namespace hipsph {

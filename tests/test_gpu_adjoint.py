@@ -173,6 +173,40 @@ def test_adjoint_graph_segments_match_eager_steps():
         assert torch.equal(o[2], out[0][2])
 
 
+def test_adjoint_reused_run_after_new_constants():
+    """A run differentiated at (alpha1, c1) and then run forward and backward
+    at (alpha2, c2) gives the gradient of a fresh run at (alpha2, c2) bit for
+    bit, with graph segments and with one launch per kernel: the segment
+    graphs cached in the tape at the first constants are not replayed."""
+    sc = scenarios.dam_break_c5(small=True)
+    m = sc.method()
+    target = _target(sc, 24)
+    v0 = torch.zeros((sc.n_fluid, 2), dtype=torch.float64, device='cuda')
+    a1, c1 = m.alpha, m.co
+    a2, c2 = 0.037, m.co * 1.07
+
+    def grads(run, alpha, co, graphs):
+        xt = run.forward(alpha, co, v0).clone().requires_grad_(True)
+        (g,) = torch.autograd.grad(_mse(xt, target), [xt])
+        ga, gc, gv = run.backward(g, graphs=graphs)
+        return float(ga), float(gc), gv.clone()
+
+    fresh = SPHRun(sc.pA, m, 24, checkpoint_every=8)
+    ref = grads(fresh, a2, c2, True)
+    ref_eager = grads(fresh, a2, c2, False)
+    fresh.close()
+    reused = SPHRun(sc.pA, m, 24, checkpoint_every=8)
+    first = grads(reused, a1, c1, True)
+    second = grads(reused, a2, c2, True)
+    second_eager = grads(reused, a2, c2, False)
+    reused.close()
+    say('reused run: alpha grads', first[0], second[0], ref[0], 'c0 grads', first[1], second[1], ref[1])
+    assert first[0] != ref[0]  # the two constant sets give different gradients
+    for o in (ref_eager, second, second_eager):
+        assert o[0] == ref[0] and o[1] == ref[1]
+        assert torch.equal(o[2], ref[2])
+
+
 @pytest.mark.parametrize('dt_mode', ['fixed', 'dynamic'])
 def test_adjoint_dam_break_2k_vs_torch_dense(dt_mode):
     """Fixed dt from a generic initial velocity field (v0 ~ N(0, 0.05 m/s)):
