@@ -86,7 +86,7 @@ __global__ void kb_dt_final(BInst *inst, const int *chunk0, const double *part, 
     block_fold<4>(v, kind, lds);
     if (threadIdx.x != 0) return;
     const double mn = v[0], mx = v[1], cnt = v[2], bad = v[3];
-    const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID;
+    const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID | SPH_FLAG_ZERO_DENSITY;
     if ((I.flags & halt) || !(I.t < I.duration)) {
         I.active = 0;
         return;
@@ -248,21 +248,26 @@ __global__ void kb_bases(BInst *inst, int K, Ctrl *view, long long capBins) {
 }
 
 // _bin :129-141 with the instance's grid; rank by runs as k_keys does.
-__global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, int pec, int *counts, int *bin,
+__global__ void kb_keys(BInst *inst, const int *instOf, State a, int n, int pec, int *counts, int *bin,
                         int *rank, int *binflag, const Ctrl *view) {
     if (!view->active) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     int b = -1;
     if (t < n) {
-        const BInst &I = inst[instOf[t]];
+        BInst &I = inst[instOf[t]];
         const Grid g = I.grid;
         if (I.active) {
             double x, y;
             bupd_pos(a, t, pec != 0, I.dt, x, y);
             const int2 c = cell_of(x, y, g);
             const long long key = (long long)c.x + g.ncx * (long long)c.y;
-            b = (int)(I.cellBase + ((key >= 0 && key < g.ncells) ? key : g.ncells));
+            if (key >= 0 && key < g.ncells) {
+                b = (int)(I.cellBase + key);
+            } else {  // as k_keys: the overflow bin, and the instance's flag
+                b = (int)(I.cellBase + g.ncells);
+                atomicOr(&I.flags, (unsigned)SPH_FLAG_OOB_KEY);
+            }
         } else {
             b = (int)I.cellBase;  // parked: the instance's single bin
         }
@@ -294,11 +299,12 @@ __global__ void kb_keys(const BInst *inst, const int *instOf, State a, int n, in
 // x, y, vx, vy, rho (kb_correct forms the half-step velocity again and
 // overwrites the rest), no rates of a particle inside its instance's grid
 // (written by the pair pass).
-__global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State b, const int *perm, int2 *cxy_s,
+__global__ void kb_reorder(BInst *inst, const int *instOf, State a, State b, const int *perm, int2 *cxy_s,
                            int n, int fp32, void *recA, void *recB, int pec) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const BInst &I = inst[instOf[t]];  // slots never change instance
+    const int k = instOf[t];  // slots never change instance
+    const BInst &I = inst[k];
     const int s = perm[t];
     double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
     const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
@@ -336,6 +342,7 @@ __global__ void kb_reorder(const BInst *inst, const int *instOf, State a, State 
     const double p = tait_p(rho, lab, I.gamma, I.B, I.rho0, I.Pb);
     const double pr = p / (rho * rho);
     const double w = lab == kFluid ? m : -(double)lab;
+    if (I.active && lab == kFluid && !isfinite(pr)) atomicOr(&inst[k].flags, (unsigned)SPH_FLAG_ZERO_DENSITY);
     if (fp32) {
         Rec<float>::A ra;
         ra.x = x;

@@ -108,13 +108,14 @@ def dam_break_c5(small=False):
     return dam_break(0.5, 1.0, 0.005, tank_w=2.0, tank_h=1.5, name='dam_break_20k_dx0.005')
 
 
-def ensemble_c4(count=256, seed=SEED, dxs=(0.005, 0.01, 0.02)):
+def ensemble_c4(count=256, seed=SEED, dxs=(0.005, 0.01, 0.02), members=None):
     """Batched ensemble: dam breaks with column width U(0.5, 1.5) m, height
     U(0.5, 2.5) m and dx drawn from dxs in a 4.0 x 3.0 tank; instance k uses
-    PCG64(seed + k), which also draws its alpha ~ U(0.005, 0.05). Returns
-    (scenarios, alphas)."""
+    PCG64(seed + k), which also draws its alpha ~ U(0.005, 0.05). `members`
+    (instance numbers) builds only those instances of the ensemble, each as
+    it is in the whole one. Returns (scenarios, alphas)."""
     out, alphas = [], []
-    for k in range(count):
+    for k in (range(count) if members is None else members):
         rng = np.random.Generator(np.random.PCG64(seed + k))
         w = float(rng.uniform(0.5, 1.5))
         hgt = float(rng.uniform(0.5, 2.5))
@@ -122,6 +123,39 @@ def ensemble_c4(count=256, seed=SEED, dxs=(0.005, 0.01, 0.02)):
         alphas.append(float(rng.uniform(0.005, 0.05)))
         out.append(dam_break(w, hgt, dx, name=f'ensemble_{k}'))
     return out, alphas
+
+
+def dam_break_example(N=50, obstacle=False, rho0=1000.0):
+    """The reference's own 2-D dam-break example (examples/DamBreak.py:16-53,
+    57-64): a 25 m x 25 m column of N x N hex-packed fluid particles
+    (r0 = 25 / N, mass 25^2 / N^2 rho0 each), a bottom row from x = -r0 to 180 m, a left column at x = -r0 and a
+    TempBoundary gate at x = 25 + r0, both up to y = 30 m, and optionally
+    the example's 2.5 m triangular obstacle at x = 50 m. The walls keep the
+    example's zero mass, zero density and zero h (Helpers.rect leaves h at 0),
+    so the cell size falls back to 1 m (src/Tools/NNLinkedList.py:103-110).
+    The column height is 25 m; the example runs Wendland, XSPH and
+    h = 1.6 r0. Returns (r0, particle array) in the example's order: fluid,
+    bottom, left, gate[, obstacle]."""
+    r0 = 25 / N
+    mass = 25 * 25 / N ** 2 * rho0
+    lo = -1 * r0
+    fluid = Helpers.rect(xmin=0, xmax=25, ymin=0, ymax=25, r0=r0, mass=mass, rho0=rho0, pack=True)
+    walls = [Helpers.rect(xmin=lo, xmax=180, ymin=lo, ymax=lo, r0=r0, mass=0.0, rho0=0.0,
+                          label=ParticleType.Boundary),
+             Helpers.rect(xmin=lo, xmax=lo, ymin=lo, ymax=30, r0=r0, mass=0.0, rho0=0.0,
+                          label=ParticleType.Boundary),
+             Helpers.rect(xmin=25 - lo, xmax=25 - lo, ymin=lo, ymax=30, r0=r0, mass=0.0, rho0=0.0,
+                          label=ParticleType.TempBoundary)]
+    if obstacle:
+        side = 2.5
+        k = int(np.ceil(side / r0))
+        run = np.linspace(0.0, side, k)
+        ob = np.zeros(3 * k, dtype=particle_dtype)
+        ob['x'] = np.concatenate((50 + run, np.full(k, 50 + side), 50 + run))  # diagonal, right side, base
+        ob['y'] = np.concatenate((run, run, np.zeros(k)))
+        ob['label'] = ParticleType.Boundary
+        walls.append(ob)
+    return r0, np.concatenate([fluid] + walls)
 
 
 def stress_lattice(nside=2000, dx=1e-3, H=2.0, layers=3, rho0=1000.0, hfac=1.3, seed=SEED):
