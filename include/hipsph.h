@@ -46,6 +46,11 @@ extern "C" {
 #define SPH_FLAG_OOB_KEY 4u      /* a particle's flat cell key fell outside [0, n_cells) */
 #define SPH_FLAG_DT_SMALL 8u     /* dt < 1e-6 (src/Solver.py:220-224) */
 #define SPH_FLAG_NO_FLUID 16u    /* dt reduction over zero fluid particles */
+#define SPH_FLAG_ZERO_DENSITY 32u /* a fluid particle reached the pair pass with p / rho^2 not finite
+                                    (rho = 0, e.g. after PEC's strict clamp): the reference's momentum
+                                    sums turn NaN from there on, because its zero-gradient terms
+                                    multiply p / rho^2 by 0 (src/Equations/Momentum.py:36-56); the
+                                    step completes and the next one halts */
 
 /* ---- enums ---- */
 enum { SPH_KERNEL_CUBIC = 0, SPH_KERNEL_GAUSSIAN = 1, SPH_KERNEL_WENDLAND = 2 };

@@ -113,7 +113,7 @@ __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *h
     block_fold<kDtQ>(v, kind, lds);
     if (threadIdx.x != 0) return;
     const double mn = v[0], mx = v[1], cnt = v[2], bad = v[3];
-    const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID;
+    const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID | SPH_FLAG_ZERO_DENSITY;
     if ((flags & halt) || !(t0 < duration)) {
         ctrl->active = 0;
         return;
@@ -534,6 +534,7 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
     const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
     const double pr = p / (rho * rho);
     const double w = lab == kFluid ? m : -(double)lab;
+    if (lab == kFluid && !isfinite(pr)) atomicOr(&ctrl->flags, (unsigned)SPH_FLAG_ZERO_DENSITY);
     if (fp32) {
         Rec<float>::A ra;
         ra.x = x;
@@ -1072,6 +1073,7 @@ int sph_ctx_set_params(sph_ctx_t *ctx, const sph_params_t *params) {
             hipGraphExecDestroy(ctx->gexec[k]);
             ctx->gexec[k] = nullptr;
         }
+    ++ctx->gen;  // a tape's cached segment graphs hold the old constants
     return 0;
 }
 
@@ -1335,6 +1337,10 @@ int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const 
     if (ctx->P.strict) return fail(SPH_EUNSUPPORTED, "the adjoint of the strict density clamp is not implemented");
     int rc = tape_supported(ctx);
     if (rc) return rc;
+    if (tape->gen != ctx->gen) {  // constants or launch shape changed since the capture: every graph is stale
+        tape_drop_graphs(tape);
+        tape->gen = ctx->gen;
+    }
     if (std::memcmp(&tape->adj_key, adj, sizeof *adj) != 0) {  // other buffers: the reverse graphs are stale
         for (auto &g : tape->adj_exec)
             if (g) hipGraphExecDestroy(g), g = nullptr;
@@ -1768,6 +1774,7 @@ int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads) {
             hipGraphExecDestroy(ctx->gexec[k]);
             ctx->gexec[k] = nullptr;
         }
+    ++ctx->gen;
     return 0;
 }
 

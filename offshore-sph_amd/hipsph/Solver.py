@@ -51,6 +51,10 @@ import numpy as np
 from . import coupling as _coupling
 from .Common import ParticleType, particle_dtype
 
+# device flags the host loop stops on: cell capacity, non-finite position,
+# a key outside the grid (the reference writes out of bounds there), zero
+# fluid density (the reference's sums turn NaN there)
+_HALT_FLAGS = 1 | 2 | 4 | 32
 _TIMING_KEYS = ('total', 'storage', 'integrate_correction', 'integrate_prediction', 'compute', 'time_step',
                 'neighbour_hood', 'coupling')
 
@@ -217,10 +221,10 @@ class Solver:
         before = e.status()['steps']
         e.step(n, graph=self.graph and n > 1)
         st = e.status()
-        bad = st['flags'] & 7
+        bad = st['flags'] & _HALT_FLAGS
         if bad:
             raise RuntimeError(f'device step halted (flags {st["flags"]:#x}: 1 cell capacity, 2 non-finite, '
-                               f'4 key outside the grid)')
+                               f'4 key outside the grid, 32 zero fluid density)')
         if st['flags'] & 8 and self.ts_error == 0:
             self.ts_error = len(self.dt_a)  # src/Solver.py:220-224
         taken = st['steps'] - before
@@ -242,7 +246,7 @@ class Solver:
         e.step_begin()
         st = e.status()
         taken = st['steps'] - before
-        if st['flags'] & 7:
+        if st['flags'] & _HALT_FLAGS:
             e.step_end()
             raise RuntimeError(f'device step halted (flags {st["flags"]:#x})')
         if taken == 0:  # past the duration: nothing ran

@@ -20,6 +20,7 @@ SPH_FLAG_NONFINITE = 2
 SPH_FLAG_OOB_KEY = 4
 SPH_FLAG_DT_SMALL = 8
 SPH_FLAG_NO_FLUID = 16
+SPH_FLAG_ZERO_DENSITY = 32
 SPH_DT_HISTORY = 65536
 
 ERRORS = {-1: 'EINVAL', -2: 'EHIP', -3: 'ENOMEM', -4: 'ECAPACITY', -5: 'ESTATE', -6: 'ENUMERIC',
