@@ -336,6 +336,33 @@ int sph_tait_eos(int64_t n, double gamma, double B, double rho0, double Pb, cons
 int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads);
 int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads);
 
+/* ---- nn.update + _loop with no allocation ----
+ * No counterpart in the reference (whose _loop allocates per particle). For
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc. One call does what sph_nn_update
+ * then sph_compute do on a context holding the same state
+ * (src/Tools/NNLinkedList.py:37-39, src/Tools/SolverTools.py:120-175), with
+ * the same kernels and the same results, on arrays the caller passes, and
+ * with every piece of workspace in one static device array of the library
+ * (about 6 MB per device, held until the process ends): at most
+ * SPH_NOALLOC_MAX_PARTICLES particles (more is refused with SPH_EINVAL) and
+ * SPH_NOALLOC_MAX_CELLS grid cells. params: n, kernel, precision and the
+ * WCSPH constants (no XSPH, summation density or dynamic h). st: device x,
+ * y, vx, vy, rho, m, h, label and, optionally, ax, ay, drho (the rates a
+ * target keeps when it finds no neighbour; NULL reads as 0), n entries in
+ * the reference's active order. drho, ax, ay: device outputs of n entries.
+ * status: a device int32 that receives the SPH_FLAG_* bits of the call; with
+ * SPH_FLAG_CAPACITY (a grid of more than SPH_NOALLOC_MAX_CELLS cells) or
+ * SPH_FLAG_NONFINITE the outputs are not written. The call allocates
+ * nothing and does not synchronise; the first call on a device resolves the
+ * array's address (hipGetSymbolAddress), so make that one outside a graph
+ * capture. Calls on one device share the array: calls that overlap on one
+ * device race. */
+#define SPH_NOALLOC_MAX_PARTICLES 32768
+#define SPH_NOALLOC_MAX_CELLS 131072
+int sph_loop_noalloc(const sph_params_t *params, const sph_state_t *st, double *drho, double *ax, double *ay,
+                     int32_t *status, void *stream);
+
 /* ---- diagnostics ---- */
 const char *sph_last_error(void);
 /* Library build string (compile flags, target). */

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "sph_ctx.h"
@@ -461,6 +462,31 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // dynamic h for the fluid, formed here, after the keys were binned with the
 // previous h (Solver._compute: nn.update :238, then computeH :242-246) and
 // from the density the loop sees (the predicted one with PEC).
+// The packed pair-pass records of slot t: w = m for fluid and -label
+// otherwise; p / rho^2 with p the Tait pressure + Pb (src/Tools/
+// SolverTools.py:122-127, src/Equations/Momentum.py:28,36). Returns p / rho^2.
+__device__ inline double write_pair_records(int t, double x, double y, double vx, double vy, double rho, double m,
+                                            double h, int8_t lab, const Phys &ph, int fp32, void *recA, void *recB) {
+    const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
+    const double pr = p / (rho * rho);
+    const double w = lab == kFluid ? m : -(double)lab;
+    if (fp32) {
+        Rec<float>::A ra;
+        ra.x = x;
+        ra.y = y;
+        ra.h = (float)h;
+        ra.w = (float)w;
+        ra.vx = (float)vx;
+        ra.vy = (float)vy;
+        ((Rec<float>::A *)recA)[t] = ra;
+        ((Rec<float>::B *)recB)[t] = Rec<float>::B{(float)rho, (float)pr};
+    } else {
+        ((Rec<double>::A *)recA)[t] = Rec<double>::A{x, y, h, w};
+        ((Rec<double>::B *)recB)[t] = Rec<double>::B{vx, vy, rho, pr};
+    }
+    return pr;
+}
+
 __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fp32,
                           void *recA, void *recB, int pred, int strict, int lean, int xs, int dyn) {
     // An inactive step (halted, or past the duration) still flips the
@@ -531,24 +557,8 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
     }
     if (!act) return;
     cxy_s[t] = cc;
-    const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
-    const double pr = p / (rho * rho);
-    const double w = lab == kFluid ? m : -(double)lab;
+    const double pr = write_pair_records(t, x, y, vx, vy, rho, m, h, lab, ph, fp32, recA, recB);
     if (lab == kFluid && !isfinite(pr)) atomicOr(&ctrl->flags, (unsigned)SPH_FLAG_ZERO_DENSITY);
-    if (fp32) {
-        Rec<float>::A ra;
-        ra.x = x;
-        ra.y = y;
-        ra.h = (float)h;
-        ra.w = (float)w;
-        ra.vx = (float)vx;
-        ra.vy = (float)vy;
-        ((Rec<float>::A *)recA)[t] = ra;
-        ((Rec<float>::B *)recB)[t] = Rec<float>::B{(float)rho, (float)pr};
-    } else {
-        ((Rec<double>::A *)recA)[t] = Rec<double>::A{x, y, h, w};
-        ((Rec<double>::B *)recB)[t] = Rec<double>::B{vx, vy, rho, pr};
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1814,3 +1824,175 @@ int sph_timing_read(sph_ctx_t *ctx, double *avg_ms_pair, double *avg_ms_nn, int6
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// nn.update + _loop with no allocation (sph_loop_noalloc)
+//
+// The workspace of sph_nn_update + sph_compute (cell table, sort arrays,
+// cell-ordered pair records, partials, control block) lives in one static
+// device array, `scratch`, sized for SPH_NOALLOC_MAX_PARTICLES particles and
+// SPH_NOALLOC_MAX_CELLS cells. Host code never passes the array's name to a
+// kernel: its device address comes from hipGetSymbolAddress, once per device
+// (the first call on that device), and kernels receive that address. Every
+// call on a device shares that device's array, so calls that overlap on one
+// device race; each call clears what must start at zero.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long long kNaParticles = SPH_NOALLOC_MAX_PARTICLES;
+constexpr long long kNaCells = SPH_NOALLOC_MAX_CELLS;
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+static_assert(sizeof(Rec<double>::A) == 32 && sizeof(Rec<double>::B) == 32 && sizeof(Rec<float>::A) == 32 &&
+                  sizeof(Rec<float>::B) <= 32,
+              "pair records are sized at 32 bytes in the workspace");
+constexpr size_t kNaOffCtrl = 0;
+constexpr size_t kNaOffPart = kNaOffCtrl + al256(sizeof(Ctrl));
+constexpr size_t kNaOffId = kNaOffPart + al256(sizeof(double) * kBoundsQ * kBoundsBlocks);
+constexpr size_t kNaOffBin = kNaOffId + al256(4 * kNaParticles);
+constexpr size_t kNaOffRank = kNaOffBin + al256(4 * kNaParticles);
+constexpr size_t kNaOffPerm = kNaOffRank + al256(4 * kNaParticles);
+constexpr size_t kNaOffPermKey = kNaOffPerm + al256(4 * kNaParticles);
+constexpr size_t kNaOffCxy = kNaOffPermKey + al256(4 * kNaParticles);
+constexpr size_t kNaOffRecA = kNaOffCxy + al256(8 * kNaParticles);
+constexpr size_t kNaOffRecB = kNaOffRecA + al256(32 * kNaParticles);
+constexpr size_t kNaOffH = kNaOffRecB + al256(32 * kNaParticles);
+constexpr size_t kNaOffAx = kNaOffH + al256(8 * kNaParticles);
+constexpr size_t kNaOffAy = kNaOffAx + al256(8 * kNaParticles);
+constexpr size_t kNaOffDrho = kNaOffAy + al256(8 * kNaParticles);
+constexpr size_t kNaOffCounts = kNaOffDrho + al256(8 * kNaParticles);
+constexpr size_t kNaOffCellStart = kNaOffCounts + al256(4 * (kNaCells + 1));
+constexpr size_t kNaOffBinflag = kNaOffCellStart + al256(4 * (kNaCells + 2));
+constexpr size_t kNaOffTiles = kNaOffBinflag + al256(4 * (kNaCells + 1));
+constexpr size_t kNaBytes = kNaOffTiles + al256(4 * ((kNaCells + 1 + kScanTile - 1) / kScanTile + 1));
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaDevices = 64;
+static std::mutex g_na_mu;
+static void *g_na_ptr[kNaDevices];  // device address of `scratch` per device ordinal
+
+// The current device's address of `scratch` (hipGetSymbolAddress on the
+// first call per device, with that device current).
+static int na_workspace(unsigned char **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaDevices) return fail(SPH_EINVAL, "device ordinal %d beyond %d", dev, kNaDevices);
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SPH_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_na_ptr[dev] = p;
+    }
+    *out = (unsigned char *)g_na_ptr[dev];
+    return 0;
+}
+
+// Start of a call: an active control block with no flags, no predictor and
+// no coupled integrator; the cell counts and bin flags cleared over the whole
+// capacity (the workspace keeps the last call's contents); the identity as
+// each particle's reference index.
+__global__ void k_na_init(Ctrl *ctrl, int *counts, int *binflag, int *id, int n, long long ncap) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        Ctrl c{};
+        c.active = 1;
+        *ctrl = c;
+    }
+    if (i <= ncap) {
+        counts[i] = 0;
+        binflag[i] = 0;
+    }
+    if (i < n) id[i] = (int)i;
+}
+
+// The cell-ordered inputs of the pair pass, gathered from the caller's
+// arrays through the permutation: records (as k_reorder writes them without
+// a predictor), the double h, each slot's raw cell, and the rates a target
+// keeps when it finds no neighbour (src/Tools/SolverTools.py:166-169).
+__global__ void k_na_reorder(const Ctrl *ctrl, sph_state_t in, const int *perm, int2 *cxy, double *H, double *ax,
+                             double *ay, double *drho, Phys ph, int n, int fp32, void *recA, void *recB) {
+    if (!ctrl->active) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int s = perm[t];
+    const double x = in.x[s], y = in.y[s], vx = in.vx[s], vy = in.vy[s], rho = in.rho[s], m = in.m[s], h = in.h[s];
+    const int8_t lab = in.label[s];
+    ax[t] = in.ax ? in.ax[s] : 0.0;
+    ay[t] = in.ay ? in.ay[s] : 0.0;
+    drho[t] = in.drho ? in.drho[s] : 0.0;
+    cxy[t] = cell_of(x, y, ctrl->grid);
+    H[t] = h;
+    write_pair_records(t, x, y, vx, vy, rho, m, h, lab, ph, fp32, recA, recB);
+}
+
+// The rates back to the caller's order, and the status word.
+__global__ void k_na_finish(const Ctrl *ctrl, const int *perm, const double *ax_s, const double *ay_s,
+                            const double *drho_s, double *ax, double *ay, double *drho, int n, int32_t *status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *status = (int32_t)ctrl->flags;
+    if (!ctrl->active || t >= n) return;
+    const int s = perm[t];
+    ax[s] = ax_s[t];
+    ay[s] = ay_s[t];
+    drho[s] = drho_s[t];
+}
+
+extern "C" int sph_loop_noalloc(const sph_params_t *params, const sph_state_t *st, double *drho, double *ax,
+                                double *ay, int32_t *status, void *stream) {
+    if (!params || !st || !drho || !ax || !ay || !status) return fail(SPH_EINVAL, "NULL argument");
+    const long long n = params->n;
+    if (n < 1 || n > kNaParticles)
+        return fail(SPH_EINVAL, "sph_loop_noalloc: n = %lld outside 1 .. %lld (its static workspace)", n,
+                    kNaParticles);
+    if (!st->x || !st->y || !st->vx || !st->vy || !st->rho || !st->m || !st->h || !st->label)
+        return fail(SPH_EINVAL, "sph_loop_noalloc needs x, y, vx, vy, rho, m, h and label");
+    int rc = check_params(params);
+    if (rc) return rc;
+    if (params->use_xsph || params->use_summation_density || params->dynamic_h)
+        return fail(SPH_EUNSUPPORTED, "sph_loop_noalloc runs the plain WCSPH loop (no XSPH, summation density or "
+                                      "dynamic h)");
+    unsigned char *w = nullptr;
+    if ((rc = na_workspace(&w))) return rc;
+    Ctrl *ctrl = (Ctrl *)(w + kNaOffCtrl);
+    double *part = (double *)(w + kNaOffPart);
+    int *id = (int *)(w + kNaOffId), *bin = (int *)(w + kNaOffBin), *rank = (int *)(w + kNaOffRank);
+    int *perm = (int *)(w + kNaOffPerm), *permKey = (int *)(w + kNaOffPermKey);
+    int2 *cxy = (int2 *)(w + kNaOffCxy);
+    void *recA = w + kNaOffRecA, *recB = w + kNaOffRecB;
+    double *H = (double *)(w + kNaOffH), *ax_s = (double *)(w + kNaOffAx), *ay_s = (double *)(w + kNaOffAy);
+    double *drho_s = (double *)(w + kNaOffDrho);
+    int *counts = (int *)(w + kNaOffCounts), *cellStart = (int *)(w + kNaOffCellStart);
+    int *binflag = (int *)(w + kNaOffBinflag), *tileSums = (int *)(w + kNaOffTiles);
+    hipStream_t s = S(stream);
+    const int nn = (int)n;
+    State a{};  // the caller's arrays, read by the bounds and keys kernels
+    a.x = st->x;
+    a.y = st->y;
+    a.h = st->h;
+    a.label = st->label;
+    a.id = id;
+    const long long ninit = (kNaCells + 1 > n ? kNaCells + 1 : n);
+    k_na_init<<<nblk(ninit), kBlock, 0, s>>>(ctrl, counts, binflag, id, nn, kNaCells);
+    const int R = redblk(n) < kBoundsBlocks ? redblk(n) : kBoundsBlocks;
+    k_bounds<<<R, kBlock, 0, s>>>(ctrl, a, nn, 0, part);
+    k_grid<<<1, 1024, 0, s>>>(ctrl, part, R, params->scale, kNaCells);
+    k_keys<<<nblk(n), kBlock, 0, s>>>(ctrl, a, nn, 0, counts, bin, rank, binflag);
+    const int tiles = (int)((kNaCells + 1 + kScanTile - 1) / kScanTile);
+    k_scan_reduce<<<tiles, kBlock, 0, s>>>(ctrl, counts, tileSums);
+    k_scan_apply<<<tiles, kBlock, 0, s>>>(ctrl, counts, tileSums, cellStart);
+    k_scatter<<<nblk(n), kBlock, 0, s>>>(ctrl, cellStart, bin, rank, id, perm, permKey, nn);
+    k_cellsort<<<redblk(kNaCells + 1), kBlock, 0, s>>>(ctrl, cellStart, perm, permKey, binflag, bin, rank);
+    const bool f32 = params->precision == SPH_PREC_FP32C;
+    const Phys ph = make_phys(*params);
+    k_na_reorder<<<nblk(n), kBlock, 0, s>>>(ctrl, *st, perm, cxy, H, ax_s, ay_s, drho_s, ph, nn, f32, recA, recB);
+    HIPCHK(hipGetLastError());
+    PairIn in{recA, recB, H, cxy, cellStart};
+    PairOut out{ax_s, ay_s, drho_s, nullptr, 0};
+    hipError_t e = f32 ? launch_pair_fp32c(ctrl, in, out, ph, nn, 256, s) : launch_pair_fp64(ctrl, in, out, ph, nn, 256, s);
+    if (e != hipSuccess) return fail(SPH_EHIP, "pair launch: %s", hipGetErrorString(e));
+    k_na_finish<<<nblk(n), kBlock, 0, s>>>(ctrl, perm, ax_s, ay_s, drho_s, ax, ay, drho, nn, status);
+    HIPCHK(hipGetLastError());
+    return 0;
+}

@@ -22,6 +22,8 @@ SPH_FLAG_DT_SMALL = 8
 SPH_FLAG_NO_FLUID = 16
 SPH_FLAG_ZERO_DENSITY = 32
 SPH_DT_HISTORY = 65536
+SPH_NOALLOC_MAX_PARTICLES = 32768
+SPH_NOALLOC_MAX_CELLS = 131072
 
 ERRORS = {-1: 'EINVAL', -2: 'EHIP', -3: 'ENOMEM', -4: 'ECAPACITY', -5: 'ESTATE', -6: 'ENUMERIC',
           -7: 'EUNSUPPORTED'}
@@ -125,6 +127,7 @@ _PROTOS = {
     'sph_ctx_get_tuning': (C.c_int, [_P, C.POINTER(_I32)]),
     'sph_timing_enable': (C.c_int, [_P, _I32]),
     'sph_timing_read': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I64), _I32]),
+    'sph_loop_noalloc': (C.c_int, [C.POINTER(sph_params_t), C.POINTER(sph_state_t), _P, _P, _P, _P, _P]),
 }
 
 EXPORTED = tuple(_PROTOS)
