@@ -35,6 +35,12 @@ struct sph_ctx {
     // are stale
     unsigned long long gen = 0;
     int pair_threads = 256;
+    // one h for every particle and one mass for every fluid particle in the
+    // current state (found at upload, restore and scatter): the fused float
+    // pass then takes its uniform form, with kRecUniform records
+    bool uni_hm = false;
+    double uni_h = 0.0, uni_m = 0.0;
+    int rec_fmt = 0;  // format of the records the last reorder wrote
     // timing of the pair pass and of the neighbour update
     bool timing = false;
     std::vector<hipEvent_t> ev_pair, ev_nn;

@@ -462,15 +462,23 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // dynamic h for the fluid, formed here, after the keys were binned with the
 // previous h (Solver._compute: nn.update :238, then computeH :242-246) and
 // from the density the loop sees (the predicted one with PEC).
-// The packed pair-pass records of slot t: w = m for fluid and -label
-// otherwise; p / rho^2 with p the Tait pressure + Pb (src/Tools/
-// SolverTools.py:122-127, src/Equations/Momentum.py:28,36). Returns p / rho^2.
+// The packed pair-pass records of slot t in format fmt (kRecFp64,
+// kRecFp32, kRecUniform): w = m for fluid and -label otherwise; p / rho^2
+// with p the Tait pressure + Pb (src/Tools/SolverTools.py:122-127,
+// src/Equations/Momentum.py:28,36). Returns p / rho^2.
 __device__ inline double write_pair_records(int t, double x, double y, double vx, double vy, double rho, double m,
-                                            double h, int8_t lab, const Phys &ph, int fp32, void *recA, void *recB) {
+                                            double h, int8_t lab, const Phys &ph, int fmt, void *recA, void *recB) {
     const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
     const double pr = p / (rho * rho);
     const double w = lab == kFluid ? m : -(double)lab;
-    if (fp32) {
+    if (fmt == kRecUniform) {
+        ((double2 *)recA)[t] = make_double2(x, y);
+        ((float4 *)recB)[t] = make_float4((float)vx, (float)vy, (float)rho,
+                                          lab == kFluid ? (float)pr : __int_as_float(0x7fc00000));
+    } else if (fmt == kRecUniform64) {
+        ((double2 *)recA)[t] = make_double2(x, y);
+        ((double4 *)recB)[t] = make_double4(vx, vy, rho, lab == kFluid ? pr : __longlong_as_double(0x7ff8000000000000ll));
+    } else if (fmt == kRecFp32) {
         Rec<float>::A ra;
         ra.x = x;
         ra.y = y;
@@ -487,7 +495,7 @@ __device__ inline double write_pair_records(int t, double x, double y, double vx
     return pr;
 }
 
-__global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fp32,
+__global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fmt,
                           void *recA, void *recB, int pred, int strict, int lean, int xs, int dyn) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
@@ -557,8 +565,39 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
     }
     if (!act) return;
     cxy_s[t] = cc;
-    const double pr = write_pair_records(t, x, y, vx, vy, rho, m, h, lab, ph, fp32, recA, recB);
+    const double pr = write_pair_records(t, x, y, vx, vy, rho, m, h, lab, ph, fmt, recA, recB);
     if (lab == kFluid && !isfinite(pr)) atomicOr(&ctrl->flags, (unsigned)SPH_FLAG_ZERO_DENSITY);
+}
+
+// The records of the current cell-ordered state in format fmt (after a
+// neighbour update, when a pass needs another format than the reorder wrote).
+__global__ void k_records(const Ctrl *ctrl, State st, Phys ph, int n, int fmt, void *recA, void *recB) {
+    if (!ctrl->active) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    write_pair_records(t, st.x[t], st.y[t], st.vx[t], st.vy[t], st.rho[t], st.m[t], st.h[t], st.label[t], ph, fmt,
+                       recA, recB);
+}
+
+// min and max h over all particles and min and max m over the fluid, in
+// out[0..3] (one workgroup; upload, restore and scatter only).
+__global__ void k_hm_range(State st, int n, double *out) {
+    __shared__ double lds[4 * 16];
+    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double h = st.h[i];
+        v[0] = h < v[0] ? h : v[0];
+        v[1] = h > v[1] ? h : v[1];
+        if (st.label[i] == kFluid) {
+            const double m = st.m[i];
+            v[2] = m < v[2] ? m : v[2];
+            v[3] = m > v[3] ? m : v[3];
+        }
+    }
+    const int kind[4] = {0, 1, 0, 1};
+    block_fold<4>(v, kind, lds);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) out[q] = v[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -935,6 +974,8 @@ static int check_params(const sph_params_t *P) {
     return 0;
 }
 
+static int refresh_uniform(sph_ctx *c, hipStream_t s);
+
 static void ctrl_from_params(Ctrl &h, const sph_params_t &P) {
     h.nm = P.coupled_integrator == SPH_COUPLED_NEWMARK;
     h.nm_beta = P.newmark_beta;
@@ -1021,7 +1062,8 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
     if (ok) {
         const bool f32 = params->precision == SPH_PREC_FP32C;
         const size_t ra = f32 ? sizeof(Rec<float>::A) : sizeof(Rec<double>::A);
-        const size_t rb = f32 ? sizeof(Rec<float>::B) : sizeof(Rec<double>::B);
+        // the float pass's B records are 8 bytes, the uniform pass's 16
+        const size_t rb = f32 ? sizeof(float4) : sizeof(Rec<double>::B);
         ok = hipMalloc(&c->recA, ra * n) == hipSuccess && hipMalloc(&c->recB, rb * n) == hipSuccess;
     }
     if (!ok) {
@@ -1116,7 +1158,7 @@ int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock,
             hipGraphExecDestroy(ctx->gexec[k]);
             ctx->gexec[k] = nullptr;
         }
-    return 0;
+    return refresh_uniform(ctx, s);
 }
 
 int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *stream) {
@@ -1131,7 +1173,7 @@ int sph_state_restore(sph_ctx_t *ctx, const sph_state_t *st, double t, void *str
     k_restore_ctrl<<<1, 1, 0, s>>>(ctx->ctrl, t);
     HIPCHK(hipGetLastError());
     ctx->loaded = true;
-    return 0;
+    return refresh_uniform(ctx, s);
 }
 
 int sph_state_download(sph_ctx_t *ctx, const sph_state_t *st, void *stream) {
@@ -1153,6 +1195,78 @@ static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipSt
     return 0;
 }
 
+
+// Whether the fused pass takes its uniform form (sph_pair_uniform.h for
+// FP32C, sph_pair_uniform64.h for FP64): the cubic spline or Wendland, the
+// plain loop (no XSPH, summation density or dynamic h), one h > 0 for every
+// particle and one fluid mass in the state, and both the support 2h and the
+// wall force's r0 inside the neighbour cut-off qmax h.
+static bool uniform_pass(const sph_ctx *c) {
+    const sph_params_t &P = c->P;
+    if (P.kernel != SPH_KERNEL_CUBIC && P.kernel != SPH_KERNEL_WENDLAND) return false;
+    if (P.use_xsph || P.use_summation_density || P.dynamic_h || !c->uni_hm) return false;
+    const double cut = 0.99 * P.qmax * c->uni_h;
+    return c->uni_h > 0.0 && 2.0 * c->uni_h < cut && P.r0 < cut;
+}
+
+static int rec_format(const sph_ctx *c) {
+    const bool uni = uniform_pass(c);
+    if (c->P.precision == SPH_PREC_FP32C) return uni ? kRecUniform : kRecFp32;
+    return uni ? kRecUniform64 : kRecFp64;
+}
+
+static UniformParams uniform_params(const sph_ctx *c) {
+    const sph_params_t &P = c->P;
+    const double h = c->uni_h, pi = 3.141592653589793;
+    UniformParams u;
+    const double lim = 2.0 * h > P.r0 ? 2.0 * h : P.r0;
+    u.lim2 = lim * lim * (1.0 + 1e-5);
+    u.h = h;
+    u.m = c->uni_m;
+    u.qmax = P.qmax;
+    u.g = P.gravity;
+    u.ck = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h * h);
+    u.ac = P.alpha * (0.5 * P.co);
+    // the double pass's constants with the reference's operations
+    // (CubicSpline.py:55 / Wendland.py:34, Momentum.py:43-46)
+    u.alpha_k = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h);
+    u.c01 = 0.01 * h * h;
+    u.cij = 0.5 * (P.co + 0.0);
+    u.alpha = P.alpha;
+    if (P.precision != SPH_PREC_FP32C) u.lim2 = lim * lim * (1.0 + 1e-12);
+    u.beta = P.beta;
+    u.r0 = P.r0;
+    u.D = P.D;
+    u.p1 = P.p1;
+    u.p2 = P.p2;
+    return u;
+}
+
+static void drop_graphs(sph_ctx *c) {
+    for (int k = 0; k < 2; ++k)
+        if (c->gexec[k]) {
+            hipGraphExecDestroy(c->gexec[k]);
+            c->gexec[k] = nullptr;
+        }
+    ++c->gen;
+}
+
+// The h range of the state and the mass range of its fluid: one h and one
+// mass make the fused float pass uniform. Synchronises s; a change of form
+// drops the captured steps.
+static int refresh_uniform(sph_ctx *c, hipStream_t s) {
+    const int before = rec_format(c);
+    k_hm_range<<<1, 1024, 0, s>>>(c->buf[c->cur], c->n, c->part_k);
+    HIPCHK(hipGetLastError());
+    double v[4];
+    HIPCHK(hipMemcpyAsync(v, c->part_k, sizeof v, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->uni_hm = v[0] == v[1] && v[2] == v[3];
+    c->uni_h = v[0];
+    c->uni_m = v[2];
+    if (rec_format(c) != before) drop_graphs(c);
+    return 0;
+}
 
 // pred != 0: the PEC predictor runs inside the update (bounds and keys see
 // the predicted positions, the reorder writes the predicted state).
@@ -1177,9 +1291,9 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
     // bin/rank are free once scattered: they stage the rare bins of more than 64
     k_cellsort<<<redblk(c->cap_cells + 1), kBlock, 0, s>>>(c->ctrl, c->cellStart, c->perm, c->permKey,
                                                            c->binflag, c->bin, c->rank);
-    k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_s, make_phys(c->P), n,
-                                         c->P.precision == SPH_PREC_FP32C, c->recA, c->recB, pred, c->P.strict,
-                                         lean, c->P.use_xsph, c->P.dynamic_h);
+    c->rec_fmt = rec_format(c);
+    k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_s, make_phys(c->P), n, c->rec_fmt,
+                                         c->recA, c->recB, pred, c->P.strict, lean, c->P.use_xsph, c->P.dynamic_h);
     HIPCHK(hipGetLastError());
     if (!c->P.use_xsph) c->xs_fluid_zero = true;
     if (c->timing) {
@@ -1212,9 +1326,27 @@ static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr, int mode = 0
     }
     const bool f32 = c->P.precision == SPH_PREC_FP32C;
     const Phys ph = make_phys(c->P);
+    const bool uni_fmt = c->rec_fmt == kRecUniform || c->rec_fmt == kRecUniform64;
+    const bool uni = mode == 0 && !count && uni_fmt && rec_format(c) == c->rec_fmt;
+    if (!uni && uni_fmt) {  // a pass that reads the general records
+        const int fmt = f32 ? kRecFp32 : kRecFp64;
+        k_records<<<nblk(c->n), kBlock, 0, s>>>(c->ctrl, st, ph, c->n, fmt, c->recA, c->recB);
+        HIPCHK(hipGetLastError());
+        c->rec_fmt = fmt;
+    }
     if (c->timing) {
         int rc = ev_record(c, c->ev_pair, c->n_pair, s);
         if (rc) return rc;
+    }
+    if (uni) {
+        hipError_t e = f32 ? launch_pair_fp32u(c->ctrl, in, out, uniform_params(c), c->P.kernel, c->n, s)
+                           : launch_pair_fp64u(c->ctrl, in, out, uniform_params(c), c->P.kernel, c->n, s);
+        if (e != hipSuccess) return fail(SPH_EHIP, "pair launch: %s", hipGetErrorString(e));
+        if (c->timing) {
+            int rc = ev_record(c, c->ev_pair, c->n_pair, s);
+            if (rc) return rc;
+        }
+        return 0;
     }
     if (sd) {
         const SdOut so{st.rho, st.drho, c->recB};
@@ -1550,7 +1682,7 @@ int sph_state_scatter(sph_ctx_t *ctx, const int64_t *ids, int64_t k, const sph_s
     HIPCHK(hipGetLastError());
     ctx->dt_fresh = false;  // h or a may have changed under the corrector's dt partials
     HIPCHK(hipStreamSynchronize(s));
-    return 0;
+    return (st->h || st->m) ? refresh_uniform(ctx, s) : 0;
 }
 
 int sph_near_pos(sph_ctx_t *ctx, int64_t npts, const double *x, const double *y, const double *h, int64_t *off,

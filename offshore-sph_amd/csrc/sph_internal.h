@@ -85,6 +85,29 @@ struct Rec<float> {
     struct __align__(8) B { float rho, prs; };
 };
 
+// Record formats written by the reorder: the double pass's, the general
+// float pass's, and the float pass's for one h and one fluid mass
+// (A = double2 (x, y), B = float4 (vx, vy, rho, p / rho^2), p / rho^2 NaN for
+// particles that are not fluid).
+constexpr int kRecFp64 = 0, kRecFp32 = 1, kRecUniform = 2;
+// the double pass's for one h and one fluid mass: A = double2 (x, y),
+// B = double4 (vx, vy, rho, p / rho^2), p / rho^2 NaN when not fluid
+constexpr int kRecUniform64 = 3;
+
+// Constants of the uniform float pass (one h, one fluid mass), in double.
+struct UniformParams {
+    double lim2;   // max((2h)^2, r0^2) (1 + 1e-5; 1 + 1e-12 for the double pass)
+    double h, m, qmax, g;
+    double ck;     // the kernel's normalisation / h^3
+    double ac;     // alpha c0 / 2
+    double beta, r0, D, p1, p2;
+    // the double pass's constants, formed with the reference's operations
+    double alpha_k;  // 10 / (7 pi) / (h h), Wendland 9 / (4 pi) / (h h)
+    double c01;      // (0.01 h) h
+    double cij;      // 0.5 (c0 + 0)
+    double alpha;
+};
+
 // Inputs of the pair pass, cell ordered.
 struct PairIn {
     const void *A, *B;     // Rec<double> or Rec<float> records
@@ -119,6 +142,14 @@ struct Phys {
 // size of the pair pass, 64, 128 or 256; chosen by the autotuner).
 hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
 hipError_t launch_pair_fp32c(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
+// The fused float pass for one h and one fluid mass (kRecUniform records;
+// cubic spline or Wendland).
+hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                             hipStream_t s);
+// The fused double pass for one h and one fluid mass (kRecUniform64 records;
+// cubic spline or Wendland), bit-identical to launch_pair_fp64's.
+hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                             hipStream_t s);
 // Summation density before the pair pass (src/Tools/SolverTools.py:129-140).
 hipError_t launch_sumden_fp64(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s);
 hipError_t launch_sumden_fp32c(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s);

@@ -13,8 +13,10 @@
 //     1e-5 (relative) of (3 h_ij)^2 are re-tested with the reference's double
 //     expression under contraction off.
 // Built with -ffp-contract=fast-honor-pragmas (the pragmas below switch
-// contraction off where exactness matters).
+// contraction off where exactness matters). The uniform-h, uniform-mass form
+// of the same pass is in sph_pair_uniform.h.
 #include "sph_pair_impl.h"
+#include "sph_pair_uniform.h"
 
 namespace hipsph {
 
@@ -30,6 +32,41 @@ hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn 
                                    int threads, int list, int *queue, int grid, hipStream_t s) {
     return launch_pair_batch<float, false>(inst, instOf, pair_args<float>(in, &out, n), kernel, threads, list, queue,
                                            grid, s);
+}
+
+hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                             hipStream_t s) {
+    UniformArgs P;
+    P.A = (const double2 *)in.A;
+    P.B = (const float4 *)in.B;
+    P.cxy = in.cxy;
+    P.cellStart = in.cellStart;
+    P.ax = out.ax;
+    P.ay = out.ay;
+    P.drho = out.drho;
+    P.n = n;
+    UniformConst K;
+    K.lim2 = u.lim2;
+    K.hd = u.h;
+    K.qmax = u.qmax;
+    K.g = u.g;
+    K.h = (float)u.h;
+    K.hinv = (float)(1.0 / u.h);
+    K.m = (float)u.m;
+    K.ck = (float)u.ck;
+    K.c01 = (float)(0.01 * u.h * u.h);
+    K.ac = (float)u.ac;
+    K.beta = (float)u.beta;
+    K.r0 = (float)u.r0;
+    K.D = (float)u.D;
+    K.p1 = (float)u.p1;
+    K.p2 = (float)u.p2;
+    const int nb = xcd_blocks(n, 256);
+    if (kernel == SPH_KERNEL_WENDLAND)
+        k_pair_uniform<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K);
+    else
+        k_pair_uniform<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K);
+    return hipGetLastError();
 }
 
 int pair_batch_queue_occupancy_fp32c(int kernel, int list, int threads) {

@@ -14,12 +14,48 @@
 // with r^2 > (3 h_ij)^2 (1 + 1e-12) cannot pass that test and pairs with
 // r^2 < (3 h_ij)^2 (1 - 1e-12) cannot fail it (the three roundings are below
 // 1e-15 relative); only pairs in between evaluate the exact expression.
+// The form for one h and one fluid mass is in sph_pair_uniform64.h.
 #include "sph_pair_impl.h"
+#include "sph_pair_uniform64.h"
 
 namespace hipsph {
 
 hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s) {
     return launch_pair<double, true>(ctrl, pair_args<double>(in, &out, n), ph, threads, out.mode, s);
+}
+
+hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                             hipStream_t s) {
+    Uniform64Args P;
+    P.A = (const double2 *)in.A;
+    P.B = (const double4 *)in.B;
+    P.cxy = in.cxy;
+    P.cellStart = in.cellStart;
+    P.ax = out.ax;
+    P.ay = out.ay;
+    P.drho = out.drho;
+    P.n = n;
+    Uniform64Const K;
+    K.lim2 = u.lim2;
+    K.h = u.h;
+    K.m = u.m;
+    K.alpha_k = u.alpha_k;
+    K.c01 = u.c01;
+    K.cij = u.cij;
+    K.alpha = u.alpha;
+    K.beta = u.beta;
+    K.r0 = u.r0;
+    K.D = u.D;
+    K.p1 = u.p1;
+    K.p2 = u.p2;
+    K.g = u.g;
+    K.qmax = u.qmax;
+    const int nb = xcd_blocks(n, 256);
+    if (kernel == SPH_KERNEL_WENDLAND)
+        k_pair_uniform64<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K);
+    else
+        k_pair_uniform64<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K);
+    return hipGetLastError();
 }
 
 hipError_t launch_sumden_fp64(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s) {

@@ -1,0 +1,209 @@
+// sph_pair_uniform.h — the float pair pass for the common case of one
+// smoothing length for every particle and one mass for every fluid particle
+// (every dam break and stress lattice here): the fused pass of
+// sph_pair_impl.h (Tait p/rho^2, continuity, momentum + artificial
+// viscosity + gravity, Lennard-Jones wall force) with the cubic spline or
+// Wendland C4, in fewer instructions per candidate and per contributor.
+// Included by sph_pair_fp32.hip only.
+//
+// What the uniform case removes:
+//   * h_ij = h and m_j = m are constants: the candidate test compares r^2
+//     with one constant, max(2h, r0)^2; the kernel's normalisation, 1/h and
+//     the viscosity's 0.01 h^2 are hoisted.
+//   * Both limits lie inside the cut-off 3h, so no candidate takes the cut-off
+//     test (a target whose own cell is outside the grid counts its neighbours
+//     in a separate walk, only to decide "no neighbours, keep the previous
+//     rates", src/Tools/SolverTools.py:166-169).
+//   * Records are 16 bytes each: A = (x, y) in double, B = (vx, vy, rho,
+//     p/rho^2) in float with a NaN in place of p/rho^2 for every particle
+//     that is not fluid. The walk reads A only and lists every particle
+//     within max(2h, r0) of the target; a contributor reads both and takes
+//     the wall force for the non-fluid ones.
+//   * The candidate test runs in double (no conversions): on CDNA4 a double
+//     add or FMA issues like a float one.
+//   * The density and force sums take each float term into a double
+//     accumulator: the term is exact in double, so the sum's error is the
+//     double rounding of each add (2^-53 relative), tighter than TwoSum's
+//     float compensation, in two instructions per term instead of seven.
+//     The wall force joins the momentum sum (the general pass keeps it apart
+//     until the end; in float the order is not the reference's anyway).
+//   * The walk loads UNROLL records before testing them (no rotating
+//     registers), and the list of contributors advances by pointer.
+#pragma once
+#include "sph_internal.h"
+
+namespace hipsph {
+namespace {
+
+#ifndef HIPSPH_UNIFORM_UNROLL
+#define HIPSPH_UNIFORM_UNROLL 4
+#endif
+// Contributor-list entries per thread (28 KB of LDS per workgroup); a list is
+// flushed before a group of UNROLL candidates would overflow it.
+constexpr int kUniformList = 28;
+
+struct UniformArgs {
+    const double2 *A;  // (x, y)
+    const float4 *B;   // (vx, vy, rho, p / rho^2); p / rho^2 is NaN for particles that are not fluid
+    const int2 *cxy;
+    const int *cellStart;
+    double *ax, *ay, *drho;
+    int n;
+};
+
+struct UniformConst {
+    double lim2;         // max((2h)^2, r0^2) (1 + 1e-5): the listed pairs
+    double hd, qmax, g;  // h, the cut-off in units of h, gravity
+    float h, hinv, m, ck;  // ck: the kernel's normalisation / h^3 (dW/dr / r = ck g(q) / r)
+    float c01, ac, beta;   // 0.01 h^2, alpha c0 / 2, beta
+    float r0, D;
+    float p1, p2;
+};
+
+__device__ inline bool not_fluid(float prs) { return prs != prs; }
+
+// The gradient factor g(q) of the kernel's dW/dq (zero outside the support).
+template <int KERNEL>
+__device__ inline float grad_q(float q) {
+    if (KERNEL == SPH_KERNEL_WENDLAND) {
+        const float inner = 1.0f - 0.5f * q;
+        const float i2 = inner * inner;
+        const float gq = i2 * i2 * inner * (-14.0f / 3.0f) * q * (1.0f + 2.5f * q);
+        return q < 2.0f ? gq : 0.0f;
+    } else {
+        const float t = 2.0f - q;
+        const float g2 = -0.75f * t * t;
+        const float g1 = -3.0f * q * (1.0f - 0.75f * q);
+        return q > 2.0f ? 0.0f : (q > 1.0f ? g2 : g1);
+    }
+}
+
+// The terms of one listed pair (dx, dy: x_i - x_j): (m v_ij . grad W,
+// momentum + viscosity along x and y) for a fluid neighbour, (0, the
+// Lennard-Jones force) for a wall within r0, zero otherwise.
+template <int KERNEL>
+__device__ inline float3 uniform_terms(const UniformConst &K, float dx, float dy, float vxi, float vyi, float rhoi,
+                                       float slf, const float4 &b) {
+    const float r2 = dx * dx + dy * dy;
+    const float rinv = __builtin_amdgcn_rsqf(r2);  // listed pairs have r > 0
+    const float r = r2 * rinv;
+    if (!not_fluid(b.w)) {
+        const float q = r * K.hinv;
+        const float norm = K.ck * grad_q<KERNEL>(q) * rinv;
+        const float dwx = norm * dx, dwy = norm * dy;
+        const float vx = vxi - b.x, vy = vyi - b.y;
+        const float dv = vx * dx + vy * dy;
+        float pi = 0.0f;
+        if (dv < 0.0f) {
+            // mu (beta mu - alpha c_ij) / rho_ij, mu = h dv / (r^2 + 0.01 h^2), one reciprocal
+            const float rhoij = 0.5f * (rhoi + b.z);
+            const float dn = r2 + K.c01;
+            const float hv = K.h * dv;
+            pi = hv * (K.beta * hv - K.ac * dn) * __builtin_amdgcn_rcpf(dn * dn * rhoij);
+        }
+        const float fm = -K.m * (slf + b.w + pi);
+        return make_float3(K.m * (vx * dwx + vy * dwy), fm * dwx, fm * dwy);
+    }
+    if (r <= K.r0 && r > 1e-12f) {
+        const float frac = K.r0 * rinv;
+        const float tmp = ipow(frac, K.p1) - ipow(frac, K.p2);
+        const float fac = K.D * tmp * (rinv * rinv);
+        return make_float3(0.0f, fac * dx, fac * dy);
+    }
+    return make_float3(0.0f, 0.0f, 0.0f);
+}
+
+// The reference's neighbour count over the 3 x 3 cells of an out-of-grid
+// target (q = r / h <= qmax, the exact double test in the rounding band).
+__device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K, double X, double Y, int2 c, int ncx,
+                                    int ncy) {
+    const int lo = c.x - 1 > 0 ? c.x - 1 : 0;
+    const int hi = c.x + 1 < ncx - 1 ? c.x + 1 : ncx - 1;
+    const double lim = (K.qmax * K.hd) * (K.qmax * K.hd);
+    int cnt = 0;
+    for (int db = -1; db <= 1 && lo <= hi; ++db) {
+        const int cy = c.y + db;
+        if (!(ncy > cy && cy > -1)) continue;
+        const int s = P.cellStart[ncx * cy + lo], e = P.cellStart[ncx * cy + hi + 1];
+        for (int k = s; k < e; ++k) {
+            const double2 a = P.A[k];
+            const double dx = X - a.x, dy = Y - a.y;
+            const double r2 = dx * dx + dy * dy;
+            bool in = r2 < lim * (1.0 - 1e-12);
+            if (!in && !(r2 > lim * (1.0 + 1e-12))) in = exact_q_test(X, Y, K.hd, a.x, a.y, K.hd, K.qmax);
+            cnt += in;
+        }
+    }
+    return cnt;
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformArgs P, UniformConst K) {
+    __shared__ unsigned s_list[kUniformList * 256];
+    if (!ctrl->active) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const float4 bt = P.B[t];
+    if (not_fluid(bt.w)) return;  // walls are not targets
+    const double2 at = P.A[t];
+    const double X = at.x, Y = at.y;
+    const Grid &g = ctrl->grid;
+    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
+    const int2 c = P.cxy[t];
+    const unsigned stride = blockDim.x;
+    unsigned *const list0 = s_list + threadIdx.x;
+    unsigned *lp = list0;
+    double srho = 0.0, sax = 0.0, say = 0.0;
+    const int lo = c.x - 1 > 0 ? c.x - 1 : 0;
+    const int hi = c.x + 1 < ncx - 1 ? c.x + 1 : ncx - 1;
+    auto flush = [&]() {
+        for (unsigned *q = list0; q != lp; q += stride) {
+            const unsigned u = *q;
+            const double2 a = P.A[u];
+            const float4 b = P.B[u];
+            const float3 f = uniform_terms<KERNEL>(K, (float)(X - a.x), (float)(Y - a.y), bt.x, bt.y, bt.z, bt.w, b);
+            srho += (double)f.x;
+            sax += (double)f.y;
+            say += (double)f.z;
+        }
+        lp = list0;
+    };
+    constexpr int U = HIPSPH_UNIFORM_UNROLL;
+    static_assert(U >= 1 && U < kUniformList, "unroll within the list");
+    unsigned *const full = list0 + (kUniformList - U) * stride;  // past this, U appends could overflow
+    auto test = [&](double2 a, int k) {
+        const double dx = X - a.x, dy = Y - a.y;
+        const double r2 = dx * dx + dy * dy;
+        if (r2 <= K.lim2 && r2 > 0.0) {
+            *lp = (unsigned)k;
+            lp += stride;
+        }
+    };
+    for (int db = -1; db <= 1 && lo <= hi; ++db) {
+        const int cy = c.y + db;
+        if (!(ncy > cy && cy > -1)) continue;
+        const int s = P.cellStart[ncx * cy + lo], e = P.cellStart[ncx * cy + hi + 1];
+        int k = s;
+        for (; k + U <= e; k += U) {
+            if (lp > full) flush();
+            double2 a[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) a[j] = P.A[k + j];
+#pragma unroll
+            for (int j = 0; j < U; ++j) test(a[j], k + j);
+        }
+        if (k < e) {
+            if (lp > full) flush();
+            for (; k < e; ++k) test(P.A[k], k);  // fewer than U
+        }
+    }
+    flush();
+    const bool inside = c.x >= 0 && c.x < ncx && c.y >= 0 && c.y < ncy;  // it finds itself: a neighbour
+    if (!inside && uniform_count(P, K, X, Y, c, ncx, ncy) == 0) return;  // SolverTools.py:166-169
+    P.drho[t] = srho;
+    P.ax[t] = sax;
+    P.ay[t] = say - K.g;  // WCSPH.py:169
+}
+
+}  // namespace
+}  // namespace hipsph

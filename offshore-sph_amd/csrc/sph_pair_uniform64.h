@@ -1,0 +1,184 @@
+// sph_pair_uniform64.h — the double pair pass for one smoothing length for
+// every particle and one mass for every fluid particle, bit-identical to the
+// general double pass (sph_pair_impl.h) and so to the reference's loop
+// (src/Tools/SolverTools.py:120-175). Included by sph_pair_fp64.hip only
+// (-ffp-contract=off).
+//
+// Every contribution keeps the reference's operation order; what the uniform
+// case changes is everything around it:
+//   * h_ij = 0.5 (h + h) = h, the re-averaged h of the viscosity likewise,
+//     m_j = m: the kernel normalisation 10 / (7 pi) / (h h) (or Wendland's),
+//     0.01 h h and c_ij = c0 / 2 are formed once on the host with the same
+//     operations, so their bits are the ones the reference forms per pair;
+//   * both the support 2h and the wall range r0 lie inside the cut-off 3h:
+//     the walk lists every particle within max(2h, r0) (a double r^2 test,
+//     no cut-off test, no square root), and a listed pair outside the support
+//     or the wall range adds nothing, as in the reference;
+//   * records: A = (x, y), B = (vx, vy, rho, p / rho^2) with a NaN in place of
+//     p / rho^2 for particles that are not fluid;
+//   * the walk visits the reference's nine cells in its order (x shift outer,
+//     y shift inner, each cell from its highest slot down), UNROLL records
+//     loaded before they are tested.
+#pragma once
+#include "sph_internal.h"
+
+namespace hipsph {
+namespace {
+
+#ifndef HIPSPH_UNIFORM64_UNROLL
+#define HIPSPH_UNIFORM64_UNROLL 4
+#endif
+constexpr int kUniform64List = 32;  // contributor-list entries per thread (32 KB of LDS per workgroup)
+
+struct Uniform64Args {
+    const double2 *A;  // (x, y)
+    const double4 *B;  // (vx, vy, rho, p / rho^2); p / rho^2 is NaN for particles that are not fluid
+    const int2 *cxy;
+    const int *cellStart;
+    double *ax, *ay, *drho;
+    int n;
+};
+
+struct Uniform64Const {
+    double lim2;           // max((2h)^2, r0^2) (1 + 1e-12): the listed pairs
+    double h, m, alpha_k;  // alpha_k: the kernel normalisation 10 / (7 pi) / (h h) (Wendland: 9 / (4 pi) / (h h))
+    double c01;            // 0.01 h h, in the reference's order (0.01 h) h
+    double cij, alpha, beta;
+    double r0, D, p1, p2, g, qmax;
+};
+
+__device__ inline bool not_fluid64(double prs) { return prs != prs; }
+
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Uniform64Args P, Uniform64Const K) {
+    __shared__ unsigned s_list[kUniform64List * 256];
+    if (!ctrl->active) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const double4 bt = P.B[t];
+    if (not_fluid64(bt.w)) return;  // walls are not targets
+    const double2 at = P.A[t];
+    const double X = at.x, Y = at.y;
+    const Grid &g = ctrl->grid;
+    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
+    const int2 c = P.cxy[t];
+    const unsigned stride = blockDim.x;
+    unsigned *const list0 = s_list + threadIdx.x;
+    unsigned *lp = list0;
+    double arho = 0.0, a0 = 0.0, a1 = 0.0, f0 = 0.0, f1 = 0.0;
+    // The reference's expressions (Continuity.py:13-16, Momentum.py:36-56,
+    // BoundaryForce.py:31-41, CubicSpline.py:53-70 / Wendland.py:32-64) for
+    // each listed pair, in listing order.
+    auto flush = [&]() {
+        for (unsigned *q = list0; q != lp; q += stride) {
+            const unsigned u = *q;
+            const double2 a = P.A[u];
+            const double4 b = P.B[u];
+            const double dx = X - a.x, dy = Y - a.y;
+            const double xij2 = dx * dx + dy * dy;
+            const double r = sqrt(xij2);
+            if (!not_fluid64(b.w)) {
+                const double qq = r / K.h;
+                double grad;
+                if (KERNEL == SPH_KERNEL_WENDLAND) {
+                    if ((qq >= 2.0) || (r < 1e-10)) continue;
+                    const double inner = 1.0 - 0.5 * qq;
+                    grad = ipow(inner, 5.0) * (-14.0 / 3.0) * qq * (1 + 2.5 * qq);
+                } else {
+                    if ((qq > 2.0) || (r < 1e-10)) continue;
+                    grad = (qq > 1.0) ? -0.75 * ((2.0 - qq) * (2.0 - qq)) : -3.0 * qq * (1.0 - 0.75 * qq);
+                }
+                const double norm = K.alpha_k * grad / (K.h * r);
+                const double dwx = norm * dx, dwy = norm * dy;
+                const double vx = bt.x - b.x, vy = bt.y - b.y;
+                const double dot = vx * dwx + vy * dwy;
+                arho += K.m * dot;
+                const double dv = vx * dx + vy * dy;
+                double PI_ij = 0.0;
+                if (dv < 0) {
+                    const double rhoij = 0.5 * (bt.z + b.z);
+                    const double muij = K.h * dv / (r * r + K.c01);
+                    const double ppij = muij * (K.beta * muij - K.alpha * K.cij);
+                    PI_ij = ppij / rhoij;
+                }
+                const double factor = bt.w + b.w + PI_ij;
+                a0 += -K.m * factor * dwx;
+                a1 += -K.m * factor * dwy;
+            } else if (!(r > K.r0) && r > 1e-12) {
+                const double frac = K.r0 / r;
+                const double tmp = ipow(frac, K.p1) - ipow(frac, K.p2);
+                const double fac = K.D * tmp;
+                f0 += fac * dx / (r * r);
+                f1 += fac * dy / (r * r);
+            }
+        }
+        lp = list0;
+    };
+    constexpr int U = HIPSPH_UNIFORM64_UNROLL;
+    static_assert(U >= 1 && U < kUniform64List, "unroll within the list");
+    unsigned *const full = list0 + (kUniform64List - U) * stride;  // past this, U appends could overflow
+    auto test = [&](double2 a, int k) {
+        const double dx = X - a.x, dy = Y - a.y;
+        const double r2 = dx * dx + dy * dy;
+        if (r2 <= K.lim2 && r2 > 0.0) {
+            *lp = (unsigned)k;
+            lp += stride;
+        }
+    };
+    // src/Tools/NNLinkedList.py:50-56: x shift outer, y shift inner, cells at
+    // the grid's edge dropped; each cell from its highest slot down (:129-141)
+    for (int da = -1; da <= 1; ++da) {
+        const int cx = c.x + da;
+        if (!(ncx > cx && cx > -1)) continue;
+        for (int db = -1; db <= 1; ++db) {
+            const int cy = c.y + db;
+            if (!(ncy > cy && cy > -1)) continue;
+            const int cell = cx + ncx * cy;
+            const int s = P.cellStart[cell], e = P.cellStart[cell + 1];
+            int k = e - 1;
+            for (; k - U + 1 >= s; k -= U) {
+                if (lp > full) flush();
+                double2 a[U];
+#pragma unroll
+                for (int j = 0; j < U; ++j) a[j] = P.A[k - j];
+#pragma unroll
+                for (int j = 0; j < U; ++j) test(a[j], k - j);
+            }
+            if (k >= s) {
+                if (lp > full) flush();
+                for (; k >= s; --k) test(P.A[k], k);  // fewer than U
+            }
+        }
+    }
+    flush();
+    // a target whose own cell is in the grid finds itself (q = 0): it has a
+    // neighbour; one outside counts them (SolverTools.py:166-169)
+    if (!(c.x >= 0 && c.x < ncx && c.y >= 0 && c.y < ncy)) {
+        const double lim = (K.qmax * K.h) * (K.qmax * K.h);
+        int cnt = 0;
+        for (int da = -1; da <= 1; ++da) {
+            const int cx = c.x + da;
+            if (!(ncx > cx && cx > -1)) continue;
+            for (int db = -1; db <= 1; ++db) {
+                const int cy = c.y + db;
+                if (!(ncy > cy && cy > -1)) continue;
+                const int cell = cx + ncx * cy;
+                for (int k = P.cellStart[cell]; k < P.cellStart[cell + 1]; ++k) {
+                    const double2 a = P.A[k];
+                    const double ddx = X - a.x, ddy = Y - a.y;
+                    const double r2 = ddx * ddx + ddy * ddy;
+                    bool in = r2 < lim * (1.0 - 1e-12);
+                    if (!in && !(r2 > lim * (1.0 + 1e-12))) in = exact_q_test(X, Y, K.h, a.x, a.y, K.h, K.qmax);
+                    cnt += in;
+                }
+            }
+        }
+        if (cnt == 0) return;
+    }
+    P.drho[t] = arho;
+    P.ax[t] = a0 + f0;
+    P.ay[t] = (a1 - K.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+}
+
+}  // namespace
+}  // namespace hipsph

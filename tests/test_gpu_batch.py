@@ -7,7 +7,9 @@ oracle) on the same initial state:
   * FP64 pair pass: bit-identical state and clock after 40 steps, eager and
     hipGraph replay, with the pair pass one thread per slot and as the
     persistent work-queue kernel;
-  * FP32C pair pass: the same pair code per particle, equal to 1e-12;
+  * FP32C pair pass: within 1e-6 of the state's largest magnitude and 1e-4
+    of the rates' (the single engine's float pass, in its uniform form, rounds
+    the same pairs differently);
   * an instance whose duration ends early stops while the others go on,
     exactly as a single engine with that duration.
 """
@@ -66,10 +68,14 @@ def test_batch_instances_match_single_engines(precision, graph, persistent):
             for fld in FIELDS:
                 np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
         else:
-            assert st[k]['t'] == pytest.approx(wst['t'], rel=1e-12)
+            # the single engine takes the float pass's uniform form (one h, one
+            # fluid mass: double accumulation of the float terms), the batch the
+            # general one (TwoSum): the same pairs, rounded differently
+            assert st[k]['t'] == pytest.approx(wst['t'], rel=1e-6)
             for fld in FIELDS:
                 scale = max(1.0, float(np.max(np.abs(want[fld][f]))))
-                assert np.max(np.abs(got[k][fld] - want[fld])) <= 1e-12 * scale, (k, fld)
+                tol = 1e-4 if fld in ('ax', 'ay', 'drho') else 1e-6
+                assert np.max(np.abs(got[k][fld] - want[fld])) <= tol * scale, (k, fld)
 
 
 def test_batch_duration_per_instance():

@@ -143,3 +143,31 @@ def test_stress_lattice_200_vs_oracle(kernel, precision):
     say('stress 200^2 errors', kernel, precision, errs)
     for k, v in errs.items():
         assert v <= tol, (k, v)
+
+
+def test_fp32c_out_of_grid_targets_vs_oracle():
+    """One h and one mass for every particle (the float pass's uniform form),
+    with targets whose own cell is outside the grid (x on the far edge: the
+    key aliases, and the corner's key lies past the grid): each keeps its
+    previous rates only when it has no neighbour at all, as in the
+    reference; the rates of the others agree with the oracle to the float
+    pass's 2e-5."""
+    sc = scenarios.dam_break_c1()
+    x = np.array([0.0, 1.5, 2.5, 4.0, 0.2, 4.0, 3.9, 3.7, 1.2, 1.7])
+    y = np.array([0.0, 0.5, 1.5, 0.5, 1.6, 2.0, 1.9, 0.6, 0.4, 0.6])
+    pA = np.zeros(len(x), dtype=sc.pA.dtype)
+    pA['x'], pA['y'], pA['h'], pA['rho'], pA['m'] = x, y, 0.5, 1000.0, 1.0
+    pA['ax'], pA['ay'], pA['drho'] = 7.0, -3.0, 11.0  # kept where a target finds no neighbour
+    pA['vx'] = np.linspace(-1.0, 1.0, len(x))
+    m = sc.method()
+    e = make_engine(pA, m, precision='fp32c')
+    e.nn_update()
+    e.compute()
+    out = e.download()
+    e.close()
+    ref, _ = oracle_loop(pA, m, 'cubic')
+    say('fp32c out of grid: ax', out['ax'], 'oracle', ref['ax'])
+    for k in ('drho', 'ax', 'ay'):
+        kept = ref[k] == pA[k]
+        np.testing.assert_array_equal(out[k][kept], pA[k][kept], err_msg=k)
+        assert rel_err(out[k], ref[k], ~kept) < 2e-5, k

@@ -3,8 +3,9 @@ array, against sph_nn_update + sph_compute on a context (the entry points it
 is a variant of), through the C ABI.
 
 Two calls in a row, the second with a smaller input (the static array keeps
-the first call's contents), each equal to the context path bit for bit, in
-FP64 and FP32C; then the limits: more particles than the workspace holds is
+the first call's contents), each equal to the context path bit for bit in
+FP64, and in FP32C within the float pair pass's 2e-5 (the context takes the
+pass's uniform form, this entry point its general one); then the limits: more particles than the workspace holds is
 refused on the host, and a grid of more cells than it holds comes back as
 SPH_FLAG_CAPACITY in the status word with the outputs untouched.
 """
@@ -71,7 +72,12 @@ def test_noalloc_loop_matches_context_twice(precision):
         want = _context(pA, m, precision)
         assert status & 3 == 0, status
         for fld in ('drho', 'ax', 'ay'):
-            np.testing.assert_array_equal(got[fld], want[fld], err_msg=f'{precision} call {k} {fld}')
+            if precision == 'fp64':
+                np.testing.assert_array_equal(got[fld], want[fld], err_msg=f'{precision} call {k} {fld}')
+            else:  # the context takes the float pass's uniform form, this entry point the general one
+                scale = float(np.max(np.abs(want[fld][f])))
+                assert np.max(np.abs(got[fld] - want[fld])) <= 2e-5 * scale, (k, fld)
+                np.testing.assert_array_equal(got[fld][~f], want[fld][~f])
 
 
 def test_noalloc_limits():
