@@ -9,18 +9,24 @@ A "step" is one pass of the Solver loop body on the device: dt reduction,
 predict, neighbour search (bounds, keys, counting sort, reorder), fused pair
 pass (Tait EOS, continuity, momentum + artificial viscosity + gravity,
 Lennard-Jones wall force), correct. The state is resident in HBM before the
-timed region; W untimed steps precede K timed ones, bracketed by device
-synchronisation.
+timed region; W untimed steps (which capture the step as a hipGraph) precede
+K timed ones replayed from the graphs with no per-launch timing, bracketed by
+device synchronisation.
 
 `value` is particle-steps/s (fluid particles x steps / s) with the FP32
 compensated pair pass and the FP64 integrator; the FP64 pair pass is
 reported beside it. `roofline` is for the dominant kernel (the fused pair
 pass): algorithmic bytes per launch (21 N + 12 N_f + 8 C for FP32C,
 41 N + 24 N_f + 8 C for FP64; N all particles, N_f fluid, C grid cells)
-over its mean launch time, measured with HIP events on its launch stream
-inside the timed region. `cpu_baseline` is the CPU oracle (a C port of the
-reference step, one thread like the reference's serial numba loop) on the
-same initial state, over a bounded number of steps.
+over its mean launch time, measured with HIP events on its launch stream in
+a separate pass of eager steps right after the timed region. `traffic` and
+`valu_issue_frac` come from the committed counter summaries in profiles/ and
+are reported only when those were made with this library build (the source
+hash sph_version() reports). `cpu_baseline` is the CPU oracle (a C port of
+the reference step, one thread like the reference's serial numba loop) on the
+same initial state, over a bounded number of steps. The default run also
+reports the 4M stress lattice's stage times (`stress_4M`) and the 256-
+instance ensemble's throughput (`ensemble_256`).
 """
 import argparse
 import json
@@ -41,34 +47,39 @@ def algorithmic_pair_bytes(precision, n, nf, cells):
     return 41 * n + 24 * nf + 8 * cells
 
 
-def run_gpu(sc, precision, steps, warmup, graph_steps, host_reps=0):
+def run_gpu(sc, precision, steps, warmup, timing_steps=50, host_reps=0):
+    """The headline timing: W warm-up steps (the first captures the step as a
+    hipGraph per buffer parity), then K steps replayed from the graphs with
+    no per-launch timing, bracketed by device synchronisation. Then, apart
+    from that timed region, `timing_steps` eager steps with HIP events around
+    every pair pass and neighbour update (on the stream they run on) for the
+    per-stage times and the roofline."""
     import torch
     from hipsph.device import Engine, EngineParams, soa_from_particles
     m = sc.method()
     e = Engine(EngineParams.from_method(sc.n, m, kernel='cubic', precision=precision, integrator='pec'))
     e.upload(soa_from_particles(sc.pA))
-    e.step(warmup)
-    torch.cuda.synchronize()
-    e.timing_enable(True)
+    e.step(max(warmup, 3), graph=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    e.step(steps)
+    e.step(steps, graph=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    tm = e.timing_read()
-    e.timing_enable(False)
     st = e.status()
     grid = e.grid()
-    res = {'elapsed_s': t1 - t0, 'ms_per_step': (t1 - t0) / steps * 1e3, 'pair_ms': tm['pair_ms'],
-           'nn_ms': tm['nn_ms'], 'timed_launches': tm['count'], 'status': st, 'grid': grid}
-    if graph_steps:
-        e.step(2, graph=True)  # capture both buffer parities
-        torch.cuda.synchronize()
-        g0 = time.perf_counter()
-        e.step(graph_steps, graph=True)
-        torch.cuda.synchronize()
-        g1 = time.perf_counter()
-        res['graph_ms_per_step'] = (g1 - g0) / graph_steps * 1e3
+    res = {'elapsed_s': t1 - t0, 'ms_per_step': (t1 - t0) / steps * 1e3, 'status': st, 'grid': grid,
+           'pair_form': e.pair_form()}
+    # per-stage times, outside the timed region
+    e.timing_enable(True)
+    torch.cuda.synchronize()
+    q0 = time.perf_counter()
+    e.step(timing_steps)
+    torch.cuda.synchronize()
+    q1 = time.perf_counter()
+    tm = e.timing_read()
+    e.timing_enable(False)
+    res.update({'pair_ms': tm['pair_ms'], 'nn_ms': tm['nn_ms'], 'timed_launches': tm['count'],
+                'eager_events_ms_per_step': (q1 - q0) / timing_steps * 1e3})
     if host_reps:
         # a caller that hands over host arrays every step (the reference's
         # boundary): upload from numpy, one step, download to numpy; not `value`
@@ -110,8 +121,12 @@ def run_cpu(sc, budget_s=12.0, max_steps=20, nthreads=1):
 
 def profile_pair(sc, launches):
     """Counter-collection driver: `launches` pair passes per precision on the
-    1M workload (after 3 steps), then a plain 256 MiB device copy whose
-    FETCH_SIZE / WRITE_SIZE calibrate the counters to bytes."""
+    1M workload (after 3 steps and a neighbour update), then two kernels of
+    known bytes that calibrate the read and write counters: a plain 256 MiB
+    device copy (torch copy_, streaming requests) and a gather of 32M doubles
+    through a permutation that is random inside each block of 16 elements
+    (torch index_select: every 128-byte line of the source is read whole,
+    in narrow requests, as the reorder and corrector gather)."""
     import torch
     from hipsph.device import Engine, EngineParams, soa_from_particles
     for prec in ('fp32c', 'fp64'):
@@ -122,15 +137,23 @@ def profile_pair(sc, launches):
         for _ in range(launches):
             e.compute()
         torch.cuda.synchronize()
-        print(f'# profiled {launches} pair passes, {prec}', file=sys.stderr, flush=True)
+        print(f'# profiled {launches} pair passes, {prec} ({e.pair_form()})', file=sys.stderr, flush=True)
         e.close()
-    src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device='cuda')
+    n = 32 * 1024 * 1024
+    src = torch.empty(n, dtype=torch.float64, device='cuda')
     dst = torch.empty_like(src)
     src.fill_(1.0)
     for _ in range(3):
         dst.copy_(src)
+    blk = 16
+    g = torch.Generator(device='cpu').manual_seed(7)
+    local = torch.argsort(torch.rand(n // blk, blk, generator=g), dim=1)
+    perm = (local + torch.arange(0, n, blk).unsqueeze(1)).reshape(-1).to('cuda')
+    for _ in range(3):
+        torch.index_select(src, 0, perm, out=dst)
     torch.cuda.synchronize()
-    print(f'# calibration copy: {src.numel() * 8} bytes read and written per launch', file=sys.stderr, flush=True)
+    print(f'# calibration copy: {n * 8} bytes read and written per launch; gather: {n * 8} bytes of doubles and '
+          f'{n * 8} bytes of int64 indices read, {n * 8} bytes written per launch', file=sys.stderr, flush=True)
 
 
 def precision_report(sc, checkpoints=(100, 500, 1000)):
@@ -283,10 +306,11 @@ STRESS_BYTES = {  # algorithmic bytes per launch (N all, N_f fluid, C cells)
 }
 
 
-def stress(reps):
+def stress(reps, emit=True):
     """Pair-kernel stress: 4,000,000 jittered particles (2000 x 2000, dx 1e-3)
     in a closed box; per kernel and precision, the neighbour update and the
-    continuity, momentum and fused pair passes timed apart (HIP events)."""
+    continuity, momentum and fused pair passes timed apart (HIP events, mean
+    over `reps` launches, each stage after a neighbour update of its own)."""
     import numpy as np
     from hipsph import scenarios
     from hipsph.device import Engine, EngineParams, soa_from_particles
@@ -301,12 +325,14 @@ def stress(reps):
             e.nn_update()
             pairs = int(np.sum(np.maximum(e.compute_counts(), 0)))
             cells = e.grid()['n_cells']
-            row = {'kernel': kernel, 'precision': prec, 'pairs': pairs, 'cells': cells}
+            row = {'kernel': kernel, 'precision': prec, 'pair_form': e.pair_form(), 'pairs': pairs,
+                   'cells': cells}
             e.timing_enable(True)
             for _ in range(reps):
                 e.nn_update()
             row['neighbour_update_ms'] = e.timing_read()['nn_ms']
             for stage, fn in (('continuity', e.continuity), ('momentum', e.momentum), ('fused', e.compute)):
+                e.nn_update()
                 fn()
                 e.timing_read()
                 for _ in range(reps):
@@ -320,36 +346,42 @@ def stress(reps):
             e.timing_enable(False)
             e.close()
             rows.append(row)
-            print(f'# stress {kernel} {prec}: nn {row["neighbour_update_ms"]:.3f} ms, continuity '
-                  f'{row["continuity_ms"]:.3f}, momentum {row["momentum_ms"]:.3f}, fused {row["fused_ms"]:.3f} ms',
-                  file=sys.stderr, flush=True)
-    print(json.dumps({'metric': 'pair-kernel stress stage times (4M particles)', 'unit': 'ms', 'n_gpus': 1,
-                      'higher_is_better': False, 'reps': reps, 'data': 'synthetic',
-                      'config': {'workload': sc.name, 'n_particles': n, 'n_fluid': nf, 'h': sc.h},
-                      'stages': rows}), flush=True)
+            print(f'# stress {kernel} {prec} ({row["pair_form"]}): nn {row["neighbour_update_ms"]:.3f} ms, '
+                  f'continuity {row["continuity_ms"]:.3f}, momentum {row["momentum_ms"]:.3f}, fused '
+                  f'{row["fused_ms"]:.3f} ms', file=sys.stderr, flush=True)
+    out = {'workload': sc.name, 'n_particles': n, 'n_fluid': nf, 'h': sc.h, 'reps': reps, 'stages': rows}
+    if emit:
+        print(json.dumps({'metric': 'pair-kernel stress stage times (4M particles)', 'unit': 'ms', 'n_gpus': 1,
+                          'higher_is_better': False, 'data': 'synthetic', **out}), flush=True)
+    return out
 
 
-def _profile_field(name, precision, field):
+def _profile(name):
     p = os.path.join(ROOT, 'profiles', name)
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            d = json.load(f)
-        return d.get(precision, {}).get(field)
+            return json.load(f)
     except Exception:
         return None
 
 
-def traffic_from_profiles(precision):
-    """HBM bytes per pair launch from the calibrated FETCH_SIZE/WRITE_SIZE runs."""
-    return _profile_field('pair_traffic.json', precision, 'hbm_bytes_per_launch')
+def profile_for_build(name, build):
+    """A committed counter summary, if it was made with this library build
+    (the source hash sph_version() reports); else None and the reason."""
+    d = _profile(name)
+    if d is None:
+        return None, f'profiles/{name} is missing'
+    if d.get('build') != build:
+        return None, f'profiles/{name} is from build {d.get("build")}, this library is {build}'
+    return d, None
 
 
-def valu_from_profiles(precision):
-    """Fraction of the SIMD cycles the pair pass issues VALU work (SQ counters;
-    the pass is VALU-bound, the HBM fraction beside it is small by design)."""
-    return _profile_field('pair_valu.json', precision, 'valu_issue_frac')
+def build_id():
+    from hipsph import _lib
+    v = _lib.load().sph_version().decode()
+    return v.split(' source ')[1].split()[0] if ' source ' in v else v
 
 
 def main():
@@ -357,16 +389,21 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=1000)
     ap.add_argument('--warmup', type=int, default=20)
-    ap.add_argument('--graph-steps', type=int, default=200)
+    ap.add_argument('--timing-steps', type=int, default=50,
+                    help='eager steps with HIP events around each stage, after the timed region')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=12.0)
     ap.add_argument('--small', action='store_true', help='C1 (dx 0.02) instead of the 1M workload')
     ap.add_argument('--cpu-threads', type=int, default=16,
                     help='threads of the second (OpenMP) CPU baseline; 0 skips it')
     ap.add_argument('--profile-pair', type=int, default=0, metavar='K',
-                    help='only run K pair passes per precision and a calibration copy (for counter runs)')
+                    help='only run K pair passes per precision and the calibration kernels (for counter runs)')
     ap.add_argument('--stress', type=int, default=0, metavar='REPS',
                     help='only run the 4M-particle pair-kernel stress timings (REPS launches per stage)')
+    ap.add_argument('--stress-reps', type=int, default=10, help='launches per stage of the stress lattice in '
+                    'the default run (0 skips it)')
+    ap.add_argument('--ensemble-steps', type=int, default=200, help='steps of the 256-instance ensemble in the '
+                    'default run (0 skips it)')
     ap.add_argument('--no-precision', action='store_true', help='skip the FP32C-vs-FP64 error report')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward+adjoint timing')
     ap.add_argument('--adjoint', action='store_true', help='only run the forward+adjoint timing')
@@ -396,10 +433,12 @@ def main():
         profile_pair(sc, a.profile_pair)
         return
 
-    r32 = run_gpu(sc, 'fp32c', a.steps, a.warmup, a.graph_steps, host_reps=10)
-    print(f'# fp32c {r32["ms_per_step"]:.4f} ms/step, pair {r32["pair_ms"]:.4f} ms', file=sys.stderr, flush=True)
-    r64 = run_gpu(sc, 'fp64', a.steps, a.warmup, a.graph_steps)
-    print(f'# fp64 {r64["ms_per_step"]:.4f} ms/step, pair {r64["pair_ms"]:.4f} ms', file=sys.stderr, flush=True)
+    r32 = run_gpu(sc, 'fp32c', a.steps, a.warmup, a.timing_steps, host_reps=10)
+    print(f'# fp32c {r32["ms_per_step"]:.4f} ms/step ({r32["pair_form"]}), pair {r32["pair_ms"]:.4f} ms, '
+          f'neighbour update {r32["nn_ms"]:.4f} ms', file=sys.stderr, flush=True)
+    r64 = run_gpu(sc, 'fp64', a.steps, a.warmup, a.timing_steps)
+    print(f'# fp64 {r64["ms_per_step"]:.4f} ms/step ({r64["pair_form"]}), pair {r64["pair_ms"]:.4f} ms',
+          file=sys.stderr, flush=True)
 
     n, nf = sc.n, sc.n_fluid
     cells = r32['grid']['n_cells']
@@ -410,8 +449,17 @@ def main():
     ach64 = b64 / (r64['pair_ms'] * 1e-3) / 1e9
     pairs = r64['pairs_per_step']
 
+    build = build_id()
+    traffic, traffic_note = profile_for_build('pair_traffic.json', build)
+    valu, valu_note = profile_for_build('pair_valu.json', build)
+
+    def prof(d, prec, field):
+        return d.get(prec, {}).get(field) if d else None
+
     prec_report = None if a.no_precision else precision_report(sc)
     adj = None if a.no_adjoint else adjoint_timing()
+    st4m = stress(a.stress_reps, emit=False) if a.stress_reps else None
+    ens = ensemble(steps=a.ensemble_steps, seq_instances=16) if a.ensemble_steps else None
 
     cpu = cpu_omp = None
     if not a.no_cpu:
@@ -428,6 +476,8 @@ def main():
             cpu_omp = {'value': cm['value'], 'unit': 'particle-steps/s', 'cores': a.cpu_threads, 'kind': 'port',
                        'sample': f'{cm["steps"]} steps ({cm["elapsed_s"]:.1f} s), OpenMP pair loop of the same port'}
 
+    kname = {'fp32c-uniform': 'k_pair_uniform<cubic>', 'fp32c': 'k_pair<float, cubic>',
+             'fp64-uniform': 'k_pair_uniform64<cubic>', 'fp64': 'k_pair<double, cubic>'}
     line = {
         'metric': METRIC,
         'value': value,
@@ -442,13 +492,19 @@ def main():
         'dtype': 'f32',
         'data': 'synthetic',
         'config': {'workload': sc.name, 'n_particles': n, 'n_fluid': nf, 'dx': sc.dx, 'kernel': 'cubic',
-                   'integrator': 'pec', 'pair_precision': 'fp32c (float pairs, TwoSum-compensated sums)',
-                   'integrator_dtype': 'f64', 'h': sc.h, 'cells': cells, 'parallelism': 'single GPU'},
+                   'integrator': 'pec', 'pair_precision': 'fp32c (float pairs, compensated sums)',
+                   'pair_form': r32['pair_form'], 'integrator_dtype': 'f64', 'h': sc.h, 'cells': cells,
+                   'timed_region': 'K steps replayed from hipGraphs, no per-launch timing',
+                   'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': ach32, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': ach32 / HBM_PEAK_GBS, 'traffic': traffic_from_profiles('fp32c'),
-                     'kernel': 'k_pair_fp32c<cubic>', 'algorithmic_bytes_per_launch': b32,
-                     'avg_launch_ms': r32['pair_ms'], 'launches': r32['timed_launches'],
-                     'valu_issue_frac': valu_from_profiles('fp32c')},
+                     'frac': ach32 / HBM_PEAK_GBS, 'traffic': prof(traffic, 'fp32c', 'hbm_bytes_per_launch'),
+                     'traffic_note': traffic_note, 'kernel': kname[r32['pair_form']],
+                     'algorithmic_bytes_per_launch': b32, 'avg_launch_ms': r32['pair_ms'],
+                     'launches': r32['timed_launches'],
+                     'timing': f'HIP events around each pair launch over {a.timing_steps} eager steps after the '
+                               f'timed region',
+                     'valu_issue_frac': prof(valu, 'fp32c', 'valu_issue_frac'), 'valu_note': valu_note,
+                     'build': build},
         'cpu_baseline': cpu,
         'speedup_vs_cpu': (value / cpu['value']) if cpu else None,
         'cpu_baseline_omp': cpu_omp,
@@ -456,23 +512,26 @@ def main():
         'pairs_per_step': pairs,
         'pair_interactions_per_s': pairs * a.steps / r32['elapsed_s'],
         'stage_ms': {'pair': r32['pair_ms'], 'neighbour_update': r32['nn_ms'],
-                     'rest': r32['ms_per_step'] - r32['pair_ms'] - r32['nn_ms']},
-        'graph_ms_per_step': r32.get('graph_ms_per_step'),
+                     'rest': r32['eager_events_ms_per_step'] - r32['pair_ms'] - r32['nn_ms']},
+        'eager_events_ms_per_step': r32['eager_events_ms_per_step'],
         'pcie_inclusive': ({'ms_per_step': r32['host_roundtrip_ms_per_step'],
                             'particle_steps_per_s': nf / (r32['host_roundtrip_ms_per_step'] * 1e-3),
                             'note': 'state uploaded from host numpy arrays (pageable) and downloaded every step; '
                                     'not the headline value'}
                            if 'host_roundtrip_ms_per_step' in r32 else None),
         'fp64': {'value': nf * a.steps / r64['elapsed_s'], 'ms_per_step': r64['ms_per_step'],
-                 'graph_ms_per_step': r64.get('graph_ms_per_step'),
+                 'pair_form': r64['pair_form'],
                  'roofline': {'bound': 'hbm', 'achieved': ach64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                              'frac': ach64 / HBM_PEAK_GBS, 'traffic': traffic_from_profiles('fp64'),
-                              'kernel': 'k_pair_fp64<cubic>', 'algorithmic_bytes_per_launch': b64,
-                              'avg_launch_ms': r64['pair_ms'], 'valu_issue_frac': valu_from_profiles('fp64')},
+                              'frac': ach64 / HBM_PEAK_GBS, 'traffic': prof(traffic, 'fp64', 'hbm_bytes_per_launch'),
+                              'kernel': kname[r64['pair_form']], 'algorithmic_bytes_per_launch': b64,
+                              'avg_launch_ms': r64['pair_ms'],
+                              'valu_issue_frac': prof(valu, 'fp64', 'valu_issue_frac')},
                  'stage_ms': {'pair': r64['pair_ms'], 'neighbour_update': r64['nn_ms']}},
         'precision': prec_report,
         'fwd_adjoint_over_fwd': adj['ratio'] if adj else None,
         'adjoint': adj,
+        'stress_4M': st4m,
+        'ensemble_256': ens,
         'sim_time_s': r32['status']['t'],
     }
     print(json.dumps(line), flush=True)

@@ -335,6 +335,15 @@ int sph_tait_eos(int64_t n, double gamma, double B, double rho0, double Pb, cons
  * autotuner (offshore-sph_amd/hipsph/autotune.py). */
 int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads);
 int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads);
+/* No reference counterpart: which form of the fused pair pass the context
+ * runs, chosen from the state at each upload, restore and scatter:
+ * SPH_PAIR_FP64 / SPH_PAIR_FP32C (general), or the forms for one h for every
+ * particle and one mass for every fluid particle (the cubic spline or
+ * Wendland, no XSPH, summation density or dynamic h), SPH_PAIR_FP32C_UNIFORM
+ * (float terms summed in double) and SPH_PAIR_FP64_UNIFORM (bit-identical to
+ * SPH_PAIR_FP64). */
+enum { SPH_PAIR_FP64 = 0, SPH_PAIR_FP32C = 1, SPH_PAIR_FP32C_UNIFORM = 2, SPH_PAIR_FP64_UNIFORM = 3 };
+int sph_pair_form(sph_ctx_t *ctx, int32_t *form);
 
 /* ---- nn.update + _loop with no allocation ----
  * No counterpart in the reference (whose _loop allocates per particle). For

@@ -47,9 +47,24 @@ def hipcc():
     raise RuntimeError('hipcc not found')
 
 
+def source_id():
+    """A hash of every source file the library is built from (sph_version()
+    reports it, so measurements can be tied to the build they came from)."""
+    import hashlib
+    h = hashlib.sha256()
+    inc = os.path.join(HERE, '..', 'include')
+    for d in (CSRC, inc):
+        for f in sorted(os.listdir(d)):
+            if f.endswith(('.hip', '.h')):
+                with open(os.path.join(d, f), 'rb') as fh:
+                    h.update(f.encode() + b'\0' + fh.read())
+    return h.hexdigest()[:12]
+
+
 def _compile(src, flags):
     obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + '.o')
-    cmd = [hipcc(), *COMMON, *flags, '-c', os.path.join(CSRC, src), '-o', obj]
+    cmd = [hipcc(), *COMMON, *flags, f'-DHIPSPH_BUILD_ID="{source_id()}"', '-c', os.path.join(CSRC, src),
+           '-o', obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f'compile failed: {" ".join(cmd)}\n{r.stdout}\n{r.stderr}')

@@ -40,6 +40,9 @@ struct sph_ctx {
     // pass then takes its uniform form, with kRecUniform records
     bool uni_hm = false;
     double uni_h = 0.0, uni_m = 0.0;
+    // one h and one mass for every particle (walls included) and no dynamic
+    // h: both buffers hold them in every slot and nothing permutes them
+    bool static_hm = false;
     int rec_fmt = 0;  // format of the records the last reorder wrote
     // timing of the pair pass and of the neighbour update
     bool timing = false;

@@ -206,7 +206,7 @@ __global__ void k_pec_predict(const Ctrl *ctrl, State s, int n, int strict, int 
 // (NewmarkBeta.py:54-66, src/Solver.py:397-398) with the accelerations the
 // state holds now (the coupling hook's).
 __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, int integrator, int strict, int xs,
-                          double *part) {
+                          double *part, HmConst hc) {
     if (!ctrl->active) return;
     __shared__ double lds[kDtQ * 16];
     const double dt = ctrl->dt, damping = ctrl->damping;
@@ -225,7 +225,7 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
             continue;
         }
         const double ax = s.ax[j], ay = s.ay[j];
-        v.add(s.h[j], ax, ay);
+        v.add(hc.on ? hc.h : s.h[j], ax, ay);
         if (integrator == SPH_INTEGRATOR_PEC) {
             double x0, y0, vx0, vy0, rho0, vxh, vyh;
             if (perm) {
@@ -305,7 +305,7 @@ __device__ inline void upd_pos(const Ctrl *ctrl, const State &a, int i, int pred
 // slower, the per-workgroup fold paid in more rounds; 2048 workgroups with
 // one particle per round 3 % slower).
 constexpr int kBoundsBlocks = 1024;
-__global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *part) {
+__global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *part, HmConst hc) {
     if (!ctrl->active) return;
     __shared__ double lds[kBoundsQ * 16];
     double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, hmx = -INFINITY;
@@ -319,7 +319,7 @@ __global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *par
             if (i < n) {
                 double xi, yi;
                 upd_pos(ctrl, a, i, pred, dt, xi, yi);
-                const double hi = a.h[i];
+                const double hi = hc.on ? hc.h : a.h[i];
                 if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;  // NaN or Inf
                 xmn = xi < xmn ? xi : xmn;
                 xmx = xi > xmx ? xi : xmx;
@@ -496,7 +496,7 @@ __device__ inline double write_pair_records(int t, double x, double y, double vx
 }
 
 __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fmt,
-                          void *recA, void *recB, int pred, int strict, int lean, int xs, int dyn) {
+                          void *recA, void *recB, int pred, int strict, int lean, int xs, int dyn, HmConst hc) {
     // An inactive step (halted, or past the duration) still flips the
     // buffers on the host, so copy the state unpermuted.
     const int act = ctrl->active;
@@ -504,7 +504,8 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
     if (t >= n) return;
     if (t == 0 && act) ctrl->nn_valid = 1;
     int s = act ? perm[t] : t;
-    double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
+    double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s];
+    double m = hc.on ? hc.m : a.m[s], h = hc.on ? hc.h : a.h[s];
     const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
     int8_t lab = a.label[s];
     const bool skip = lean && act && pred && lab == kFluid;
@@ -550,8 +551,10 @@ __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *c
         b.vy[t] = vy;
         b.rho[t] = rho;
     }
-    b.m[t] = m;
-    b.h[t] = h;
+    if (!hc.on) {  // else both buffers hold the one m and h in every slot
+        b.m[t] = m;
+        b.h[t] = h;
+    }
     b.label[t] = lab;
     b.id[t] = a.id[s];
     if (xs || lab != kFluid) {  // without XSPH the fluid's is 0 after every loop (WCSPH.py:187-189)
@@ -579,25 +582,35 @@ __global__ void k_records(const Ctrl *ctrl, State st, Phys ph, int n, int fmt, v
                        recA, recB);
 }
 
-// min and max h over all particles and min and max m over the fluid, in
-// out[0..3] (one workgroup; upload, restore and scatter only).
+// min and max h over all particles, min and max m over the fluid and over
+// all particles, in out[0..5] (one workgroup; upload, restore and scatter
+// only).
 __global__ void k_hm_range(State st, int n, double *out) {
-    __shared__ double lds[4 * 16];
-    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    __shared__ double lds[6 * 16];
+    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double h = st.h[i];
+        const double h = st.h[i], m = st.m[i];
         v[0] = h < v[0] ? h : v[0];
         v[1] = h > v[1] ? h : v[1];
         if (st.label[i] == kFluid) {
-            const double m = st.m[i];
             v[2] = m < v[2] ? m : v[2];
             v[3] = m > v[3] ? m : v[3];
         }
+        v[4] = m < v[4] ? m : v[4];
+        v[5] = m > v[5] ? m : v[5];
     }
-    const int kind[4] = {0, 1, 0, 1};
-    block_fold<4>(v, kind, lds);
+    const int kind[6] = {0, 1, 0, 1, 0, 1};
+    block_fold<6>(v, kind, lds);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; ++q) out[q] = v[q];
+        for (int q = 0; q < 6; ++q) out[q] = v[q];
+}
+
+// The one h and m into every slot of a buffer.
+__global__ void k_fill_hm(State st, int n, double h, double m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    st.h[i] = h;
+    st.m[i] = m;
 }
 
 // ---------------------------------------------------------------------------
@@ -1008,9 +1021,12 @@ extern "C" {
 
 const char *sph_last_error(void) { return g_err.c_str(); }
 
+#ifndef HIPSPH_BUILD_ID
+#define HIPSPH_BUILD_ID "unknown"
+#endif
 const char *sph_version(void) {
-    return "hipsph 1.0 gfx950 (engine: -ffp-contract=off; pair fp64: -ffp-contract=off; pair fp32c: "
-           "-ffp-contract=fast-honor-pragmas)";
+    return "hipsph 1.1 gfx950 source " HIPSPH_BUILD_ID " (engine: -ffp-contract=off; pair fp64: -ffp-contract=off; "
+           "pair fp32c: -ffp-contract=fast-honor-pragmas)";
 }
 
 int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
@@ -1256,16 +1272,34 @@ static void drop_graphs(sph_ctx *c) {
 // drops the captured steps.
 static int refresh_uniform(sph_ctx *c, hipStream_t s) {
     const int before = rec_format(c);
+    const bool static_before = c->static_hm;
+    const double h_before = c->uni_h, m_before = c->uni_m;
     k_hm_range<<<1, 1024, 0, s>>>(c->buf[c->cur], c->n, c->part_k);
     HIPCHK(hipGetLastError());
-    double v[4];
+    double v[6];
     HIPCHK(hipMemcpyAsync(v, c->part_k, sizeof v, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     c->uni_hm = v[0] == v[1] && v[2] == v[3];
     c->uni_h = v[0];
     c->uni_m = v[2];
-    if (rec_format(c) != before) drop_graphs(c);
+    c->static_hm = !c->P.dynamic_h && v[0] == v[1] && v[4] == v[5];
+    if (c->static_hm) {  // both buffers, every slot: the reorder will not write them
+        for (int k = 0; k < 2; ++k) k_fill_hm<<<nblk(c->n), kBlock, 0, s>>>(c->buf[k], c->n, v[0], v[4]);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (rec_format(c) != before || c->static_hm != static_before ||
+        (c->static_hm && (c->uni_h != h_before || v[4] != m_before)))
+        drop_graphs(c);
     return 0;
+}
+
+static HmConst hm_const(const sph_ctx *c) {
+    HmConst hc{};
+    hc.on = c->static_hm && !c->P.dynamic_h;
+    hc.h = c->uni_h;
+    hc.m = c->uni_m;
+    return hc;
 }
 
 // pred != 0: the PEC predictor runs inside the update (bounds and keys see
@@ -1280,7 +1314,7 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
         if (rc) return rc;
     }
     const int R = redblk(n) < kBoundsBlocks ? redblk(n) : kBoundsBlocks;
-    k_bounds<<<R, kBlock, 0, s>>>(c->ctrl, a, n, pred, c->part_b);
+    k_bounds<<<R, kBlock, 0, s>>>(c->ctrl, a, n, pred, c->part_b, hm_const(c));
     k_grid<<<1, 1024, 0, s>>>(c->ctrl, c->part_b, R, c->P.scale, c->cap_cells);
     k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, n, pred, c->counts, c->bin, c->rank, c->binflag);
     long long tiles_cap = (c->cap_cells + 1 + kScanTile - 1) / kScanTile;
@@ -1293,7 +1327,8 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
                                                            c->binflag, c->bin, c->rank);
     c->rec_fmt = rec_format(c);
     k_reorder<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, b, c->perm, c->cxy_s, make_phys(c->P), n, c->rec_fmt,
-                                         c->recA, c->recB, pred, c->P.strict, lean, c->P.use_xsph, c->P.dynamic_h);
+                                         c->recA, c->recB, pred, c->P.strict, lean, c->P.use_xsph, c->P.dynamic_h,
+                                         hm_const(c));
     HIPCHK(hipGetLastError());
     if (!c->P.use_xsph) c->xs_fluid_zero = true;
     if (c->timing) {
@@ -1386,7 +1421,7 @@ static int predict(sph_ctx *c, hipStream_t s) {
 static int correct(sph_ctx *c, hipStream_t s, bool lean = false) {
     k_correct<<<redblk(c->n), kBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->buf[1 - c->cur],
                                               lean ? c->perm : nullptr, c->n, c->P.integrator, c->P.strict,
-                                              c->P.use_xsph, c->part_d);
+                                              c->P.use_xsph, c->part_d, hm_const(c));
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1920,6 +1955,12 @@ int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads) {
     return 0;
 }
 
+int sph_pair_form(sph_ctx_t *ctx, int32_t *form) {
+    if (!ctx || !form) return fail(SPH_EINVAL, "NULL argument");
+    *form = rec_format(ctx);  // kRecFp64 .. kRecUniform64 are SPH_PAIR_* in the same order
+    return 0;
+}
+
 int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads) {
     if (!ctx || !pair_threads) return fail(SPH_EINVAL, "NULL argument");
     *pair_threads = ctx->pair_threads;
@@ -2108,7 +2149,7 @@ extern "C" int sph_loop_noalloc(const sph_params_t *params, const sph_state_t *s
     const long long ninit = (kNaCells + 1 > n ? kNaCells + 1 : n);
     k_na_init<<<nblk(ninit), kBlock, 0, s>>>(ctrl, counts, binflag, id, nn, kNaCells);
     const int R = redblk(n) < kBoundsBlocks ? redblk(n) : kBoundsBlocks;
-    k_bounds<<<R, kBlock, 0, s>>>(ctrl, a, nn, 0, part);
+    k_bounds<<<R, kBlock, 0, s>>>(ctrl, a, nn, 0, part, HmConst{});
     k_grid<<<1, 1024, 0, s>>>(ctrl, part, R, params->scale, kNaCells);
     k_keys<<<nblk(n), kBlock, 0, s>>>(ctrl, a, nn, 0, counts, bin, rank, binflag);
     const int tiles = (int)((kNaCells + 1 + kScanTile - 1) / kScanTile);

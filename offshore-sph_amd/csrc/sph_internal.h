@@ -85,6 +85,14 @@ struct Rec<float> {
     struct __align__(8) B { float rho, prs; };
 };
 
+// One h and one mass for every particle of the state (and no dynamic h):
+// both buffers then hold them in every slot, so the reorder need not permute
+// them and the bounds and dt reductions need not read them.
+struct HmConst {
+    int on;
+    double h, m;
+};
+
 // Record formats written by the reorder: the double pass's, the general
 // float pass's, and the float pass's for one h and one fluid mass
 // (A = double2 (x, y), B = float4 (vx, vy, rho, p / rho^2), p / rho^2 NaN for
@@ -93,6 +101,9 @@ constexpr int kRecFp64 = 0, kRecFp32 = 1, kRecUniform = 2;
 // the double pass's for one h and one fluid mass: A = double2 (x, y),
 // B = double4 (vx, vy, rho, p / rho^2), p / rho^2 NaN when not fluid
 constexpr int kRecUniform64 = 3;
+static_assert(kRecFp64 == SPH_PAIR_FP64 && kRecFp32 == SPH_PAIR_FP32C && kRecUniform == SPH_PAIR_FP32C_UNIFORM &&
+                  kRecUniform64 == SPH_PAIR_FP64_UNIFORM,
+              "record formats are the pair forms of the C ABI");
 
 // Constants of the uniform float pass (one h, one fluid mass), in double.
 struct UniformParams {

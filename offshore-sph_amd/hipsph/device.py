@@ -365,6 +365,14 @@ class Engine:
             out['r'] = r[:total.value].cpu().numpy()
         return out
 
+    def pair_form(self):
+        """Which form of the fused pair pass the context runs: 'fp64', 'fp32c'
+        (general) or 'fp64-uniform', 'fp32c-uniform' (one h for every
+        particle and one fluid mass)."""
+        f = C.c_int32()
+        call('sph_pair_form', self._ctx, C.byref(f))
+        return {0: 'fp64', 1: 'fp32c', 2: 'fp32c-uniform', 3: 'fp64-uniform'}[f.value]
+
     # -- timing --------------------------------------------------------------
     def timing_enable(self, on=True):
         call('sph_timing_enable', self._ctx, int(bool(on)))
