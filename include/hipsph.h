@@ -344,6 +344,9 @@ int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads);
  * SPH_PAIR_FP64). */
 enum { SPH_PAIR_FP64 = 0, SPH_PAIR_FP32C = 1, SPH_PAIR_FP32C_UNIFORM = 2, SPH_PAIR_FP64_UNIFORM = 3 };
 int sph_pair_form(sph_ctx_t *ctx, int32_t *form);
+/* The same for a batch, chosen at each sph_batch_upload: a uniform form when
+ * every instance has one h and one fluid mass (each its own) and allows it. */
+int sph_batch_pair_form(sph_batch_t *batch, int32_t *form);
 
 /* ---- nn.update + _loop with no allocation ----
  * No counterpart in the reference (whose _loop allocates per particle). For

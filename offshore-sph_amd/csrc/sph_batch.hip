@@ -300,7 +300,7 @@ __global__ void kb_keys(BInst *inst, const int *instOf, State a, int n, int pec,
 // overwrites the rest), no rates of a particle inside its instance's grid
 // (written by the pair pass).
 __global__ void kb_reorder(BInst *inst, const int *instOf, State a, State b, const int *perm, int2 *cxy_s,
-                           int n, int fp32, void *recA, void *recB, int pec) {
+                           int n, int fmt, void *recA, void *recB, int pec) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int k = instOf[t];  // slots never change instance
@@ -339,24 +339,30 @@ __global__ void kb_reorder(BInst *inst, const int *instOf, State a, State b, con
         b.drho[t] = drho;
     }
     cxy_s[t] = cc;
-    const double p = tait_p(rho, lab, I.gamma, I.B, I.rho0, I.Pb);
-    const double pr = p / (rho * rho);
-    const double w = lab == kFluid ? m : -(double)lab;
+    const double pr = write_pair_records(t, x, y, vx, vy, rho, m, h, lab, inst_phys(I), fmt, recA, recB);
     if (I.active && lab == kFluid && !isfinite(pr)) atomicOr(&inst[k].flags, (unsigned)SPH_FLAG_ZERO_DENSITY);
-    if (fp32) {
-        Rec<float>::A ra;
-        ra.x = x;
-        ra.y = y;
-        ra.h = (float)h;
-        ra.w = (float)w;
-        ra.vx = (float)vx;
-        ra.vy = (float)vy;
-        ((Rec<float>::A *)recA)[t] = ra;
-        ((Rec<float>::B *)recB)[t] = Rec<float>::B{(float)rho, (float)pr};
-    } else {
-        ((Rec<double>::A *)recA)[t] = Rec<double>::A{x, y, h, w};
-        ((Rec<double>::B *)recB)[t] = Rec<double>::B{vx, vy, rho, pr};
+}
+
+// Per instance (one workgroup each): min and max h over its particles and
+// min and max m over its fluid, in out[4k .. 4k+3].
+__global__ void kb_hm_range(const BInst *inst, State st, double *out) {
+    __shared__ double lds[4 * 16];
+    const int k = blockIdx.x, s0 = inst[k].off, s1 = s0 + inst[k].n;
+    double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int i = s0 + threadIdx.x; i < s1; i += blockDim.x) {
+        const double h = st.h[i];
+        v[0] = h < v[0] ? h : v[0];
+        v[1] = h > v[1] ? h : v[1];
+        if (st.label[i] == kFluid) {
+            const double m = st.m[i];
+            v[2] = m < v[2] ? m : v[2];
+            v[3] = m > v[3] ? m : v[3];
+        }
     }
+    const int kind[4] = {0, 1, 0, 1};
+    block_fold<4>(v, kind, lds);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) out[4 * k + q] = v[q];
 }
 
 // PEC.correct / Euler (fluid of active instances) + the next dt partials.
@@ -474,6 +480,12 @@ struct sph_batch {
     int pair_threads = 256;
     int list_len = 0;  // 0: the single-simulation pass's length
     int persistent = 0;    // the pair pass as a persistent work-queue kernel
+    // every instance has one h for all its particles and one fluid mass and
+    // allows the uniform pass (found at upload): records kRecUniform(64) and
+    // the uniform pair pass, each instance's constants in uniK
+    bool uni = false;
+    void *uniK = nullptr;
+    double *hm = nullptr;  // per-instance h and fluid-mass ranges (4 per instance)
     int *queue = nullptr;  // its counters (9 x 32 ints), zero between launches
     int cus = 0;           // compute units of the batch's device
     int queue_grid = 0;    // its grid (0: to be sized before the next launch)
@@ -493,7 +505,7 @@ void free_batch(sph_batch *b) {
     }
     void *ptrs[] = {b->inst, b->view, b->instOf, b->cs, b->chunkInst, b->chunk0, b->bin, b->rank, b->perm,
                     b->permKey, b->cxy_s, b->counts, b->cellStart, b->tileSums, b->binflag,
-                    b->part_b, b->part_d, b->recA, b->recB, b->queue};
+                    b->part_b, b->part_d, b->recA, b->recB, b->queue, b->uniK, b->hm};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : b->ev) hipEventDestroy(e);
@@ -506,7 +518,9 @@ void free_batch(sph_batch *b) {
 // sized before a step is launched or captured.
 void size_queue(sph_batch *b) {
     if (!b->persistent || b->queue_grid > 0) return;
-    const int per = b->precision == SPH_PREC_FP32C
+    const int per = b->uni ? (b->precision == SPH_PREC_FP32C ? pair_batch_uniform_occupancy_fp32(b->kernel)
+                                                             : pair_batch_uniform_occupancy_fp64(b->kernel))
+                    : b->precision == SPH_PREC_FP32C
                         ? pair_batch_queue_occupancy_fp32c(b->kernel, b->list_len, b->pair_threads)
                         : pair_batch_queue_occupancy_fp64(b->kernel, b->list_len, b->pair_threads);
     b->queue_grid = (b->cus > 0 ? b->cus : kXcds) * per;
@@ -536,8 +550,10 @@ int b_nn(sph_batch *b, hipStream_t s) {
     long long cb = (b->capBins + kBlock - 1) / kBlock;
     k_cellsort<<<(int)(cb < kRedBlocks ? (cb < 1 ? 1 : cb) : kRedBlocks), kBlock, 0, s>>>(
         b->view, b->cellStart, b->perm, b->permKey, b->binflag, b->bin, b->rank);
-    kb_reorder<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, c, b->perm, b->cxy_s, n,
-                                            b->precision == SPH_PREC_FP32C, b->recA, b->recB, pec);
+    const int fmt = b->precision == SPH_PREC_FP32C ? (b->uni ? kRecUniform : kRecFp32)
+                                                   : (b->uni ? kRecUniform64 : kRecFp64);
+    kb_reorder<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, c, b->perm, b->cxy_s, n, fmt, b->recA, b->recB,
+                                            pec);
     HIPCHK(hipGetLastError());
     b->cur = 1 - b->cur;
     return 0;
@@ -556,11 +572,17 @@ int b_pair(sph_batch *b, hipStream_t s) {
         HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
     }
     int *q = b->persistent ? b->queue : nullptr;
-    hipError_t e = b->precision == SPH_PREC_FP32C
-                       ? launch_pair_batch_fp32c(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads,
-                                                 b->list_len, q, b->queue_grid, s)
-                       : launch_pair_batch_fp64(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads,
-                                                b->list_len, q, b->queue_grid, s);
+    hipError_t e;
+    if (b->uni)  // the uniform pass: its own workgroup size (256) and list length
+        e = b->precision == SPH_PREC_FP32C
+                ? launch_pair_batch_fp32u(b->inst, b->instOf, in, out, b->uniK, b->kernel, b->n, q, b->queue_grid, s)
+                : launch_pair_batch_fp64u(b->inst, b->instOf, in, out, b->uniK, b->kernel, b->n, q, b->queue_grid, s);
+    else
+        e = b->precision == SPH_PREC_FP32C
+                ? launch_pair_batch_fp32c(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads, b->list_len,
+                                          q, b->queue_grid, s)
+                : launch_pair_batch_fp64(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads, b->list_len,
+                                         q, b->queue_grid, s);
     if (e != hipSuccess) return fail(SPH_EHIP, "batch pair launch: %s", hipGetErrorString(e));
     if (b->timing) {
         if (b->n_ev >= b->ev.size()) {
@@ -697,7 +719,12 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
     }
     const long long tiles = (capTot + kScanTile - 1) / kScanTile;
     const size_t ra = b->precision == SPH_PREC_FP32C ? sizeof(Rec<float>::A) : sizeof(Rec<double>::A);
-    const size_t rb = b->precision == SPH_PREC_FP32C ? sizeof(Rec<float>::B) : sizeof(Rec<double>::B);
+    // the uniform forms' records (16-B A, float4 / double4 B) fit in these
+    const size_t rb = b->precision == SPH_PREC_FP32C ? sizeof(float4) : sizeof(Rec<double>::B);
+    static_assert(sizeof(Rec<float>::A) >= sizeof(double2) && sizeof(Rec<double>::A) >= sizeof(double2) &&
+                      sizeof(Rec<double>::B) >= sizeof(double4) && sizeof(float4) >= sizeof(Rec<float>::B),
+                  "record buffers hold every format");
+    const size_t uk = b->precision == SPH_PREC_FP32C ? uniform_const_size_fp32() : uniform_const_size_fp64();
     ok = ok && hipMalloc(&b->inst, sizeof(BInst) * ninst) == hipSuccess &&
          hipMalloc(&b->view, sizeof(Ctrl)) == hipSuccess && hipMalloc(&b->instOf, sizeof(int) * n) == hipSuccess &&
          hipMalloc(&b->cs, sizeof(int) * cs.size()) == hipSuccess &&
@@ -713,7 +740,8 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMalloc(&b->part_b, sizeof(double) * kBoundsQ * b->NC) == hipSuccess &&
          hipMalloc(&b->part_d, sizeof(double) * kDtQ * b->NC) == hipSuccess &&
          hipMalloc(&b->recA, ra * n) == hipSuccess && hipMalloc(&b->recB, rb * n) == hipSuccess &&
-         hipMalloc(&b->queue, sizeof(int) * kQueueInts) == hipSuccess;
+         hipMalloc(&b->queue, sizeof(int) * kQueueInts) == hipSuccess &&
+         hipMalloc(&b->uniK, uk * ninst) == hipSuccess && hipMalloc(&b->hm, sizeof(double) * 4 * ninst) == hipSuccess;
     if (!ok) {
         free_batch(b);
         return fail(SPH_ENOMEM, "device allocation for a batch of %lld particles failed", ntot);
@@ -772,6 +800,44 @@ int sph_batch_upload(sph_batch_t *b, const sph_state_t *st, void *stream) {
         I.active = 1;
     }
     HIPCHK(hipMemcpy(b->inst, hI.data(), sizeof(BInst) * b->K, hipMemcpyHostToDevice));
+    // the uniform pass when every instance has one h and one fluid mass and
+    // its parameters allow it; a batch has no dynamic h, so this holds until
+    // the next upload
+    kb_hm_range<<<b->K, kBlock, 0, s>>>(b->inst, b->buf[0], b->hm);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hm(4 * (size_t)b->K);
+    HIPCHK(hipMemcpyAsync(hm.data(), b->hm, sizeof(double) * hm.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    bool uni = true;
+    for (int k = 0; k < b->K && uni; ++k) {
+        const double *v = &hm[4 * (size_t)k];
+        uni = v[0] == v[1] && v[2] == v[3] && uniform_eligible(b->P[k], v[0]);
+    }
+    if (uni) {
+        const bool f32 = b->precision == SPH_PREC_FP32C;
+        const size_t uk = f32 ? uniform_const_size_fp32() : uniform_const_size_fp64();
+        std::vector<unsigned char> blk(uk * (size_t)b->K);
+        for (int k = 0; k < b->K; ++k) {
+            const UniformParams u = make_uniform_params(b->P[k], hm[4 * (size_t)k], hm[4 * (size_t)k + 2]);
+            if (f32) uniform_const_fp32(u, blk.data() + uk * k);
+            else uniform_const_fp64(u, blk.data() + uk * k);
+        }
+        HIPCHK(hipMemcpy(b->uniK, blk.data(), blk.size(), hipMemcpyHostToDevice));
+    }
+    if (uni != b->uni) {  // another record format and pair kernel: recapture
+        b->uni = uni;
+        b->queue_grid = 0;
+        for (auto &g : b->gexec)
+            if (g) hipGraphExecDestroy(g), g = nullptr;
+    }  // same form: the graphs read the constants from uniK, rewritten above
+    return 0;
+}
+
+int sph_batch_pair_form(sph_batch_t *b, int32_t *form) {
+    if (!b || !form) return fail(SPH_EINVAL, "NULL argument");
+    if (!b->loaded) return fail(SPH_ESTATE, "no state: call sph_batch_upload first");
+    *form = b->precision == SPH_PREC_FP32C ? (b->uni ? SPH_PAIR_FP32C_UNIFORM : SPH_PAIR_FP32C)
+                                           : (b->uni ? SPH_PAIR_FP64_UNIFORM : SPH_PAIR_FP64);
     return 0;
 }
 

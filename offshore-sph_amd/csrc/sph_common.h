@@ -320,6 +320,38 @@ __device__ inline double tait_p(double rho, int8_t label, double gamma, double B
     return p + Pb;
 }
 
+// The packed pair-pass records of slot t in format fmt (kRecFp64,
+// kRecFp32, kRecUniform): w = m for fluid and -label otherwise; p / rho^2
+// with p the Tait pressure + Pb (src/Tools/SolverTools.py:122-127,
+// src/Equations/Momentum.py:28,36). Returns p / rho^2.
+__device__ inline double write_pair_records(int t, double x, double y, double vx, double vy, double rho, double m,
+                                            double h, int8_t lab, const Phys &ph, int fmt, void *recA, void *recB) {
+    const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
+    const double pr = p / (rho * rho);
+    const double w = lab == kFluid ? m : -(double)lab;
+    if (fmt == kRecUniform) {
+        ((double2 *)recA)[t] = make_double2(x, y);
+        ((float4 *)recB)[t] = make_float4((float)vx, (float)vy, (float)rho,
+                                          lab == kFluid ? (float)pr : __int_as_float(0x7fc00000));
+    } else if (fmt == kRecUniform64) {
+        ((double2 *)recA)[t] = make_double2(x, y);
+        ((double4 *)recB)[t] = make_double4(vx, vy, rho, lab == kFluid ? pr : __longlong_as_double(0x7ff8000000000000ll));
+    } else if (fmt == kRecFp32) {
+        Rec<float>::A ra;
+        ra.x = x;
+        ra.y = y;
+        ra.h = (float)h;
+        ra.w = (float)w;
+        ra.vx = (float)vx;
+        ra.vy = (float)vy;
+        ((Rec<float>::A *)recA)[t] = ra;
+        ((Rec<float>::B *)recB)[t] = Rec<float>::B{(float)rho, (float)pr};
+    } else {
+        ((Rec<double>::A *)recA)[t] = Rec<double>::A{x, y, h, w};
+        ((Rec<double>::B *)recB)[t] = Rec<double>::B{vx, vy, rho, pr};
+    }
+    return pr;
+}
 
 }  // namespace
 }  // namespace hipsph

@@ -462,39 +462,6 @@ __global__ void k_keys(Ctrl *ctrl, State a, int n, int pred, int *counts, int *b
 // dynamic h for the fluid, formed here, after the keys were binned with the
 // previous h (Solver._compute: nn.update :238, then computeH :242-246) and
 // from the density the loop sees (the predicted one with PEC).
-// The packed pair-pass records of slot t in format fmt (kRecFp64,
-// kRecFp32, kRecUniform): w = m for fluid and -label otherwise; p / rho^2
-// with p the Tait pressure + Pb (src/Tools/SolverTools.py:122-127,
-// src/Equations/Momentum.py:28,36). Returns p / rho^2.
-__device__ inline double write_pair_records(int t, double x, double y, double vx, double vy, double rho, double m,
-                                            double h, int8_t lab, const Phys &ph, int fmt, void *recA, void *recB) {
-    const double p = tait_p(rho, lab, ph.gamma, ph.B, ph.rho0, ph.Pb);
-    const double pr = p / (rho * rho);
-    const double w = lab == kFluid ? m : -(double)lab;
-    if (fmt == kRecUniform) {
-        ((double2 *)recA)[t] = make_double2(x, y);
-        ((float4 *)recB)[t] = make_float4((float)vx, (float)vy, (float)rho,
-                                          lab == kFluid ? (float)pr : __int_as_float(0x7fc00000));
-    } else if (fmt == kRecUniform64) {
-        ((double2 *)recA)[t] = make_double2(x, y);
-        ((double4 *)recB)[t] = make_double4(vx, vy, rho, lab == kFluid ? pr : __longlong_as_double(0x7ff8000000000000ll));
-    } else if (fmt == kRecFp32) {
-        Rec<float>::A ra;
-        ra.x = x;
-        ra.y = y;
-        ra.h = (float)h;
-        ra.w = (float)w;
-        ra.vx = (float)vx;
-        ra.vy = (float)vy;
-        ((Rec<float>::A *)recA)[t] = ra;
-        ((Rec<float>::B *)recB)[t] = Rec<float>::B{(float)rho, (float)pr};
-    } else {
-        ((Rec<double>::A *)recA)[t] = Rec<double>::A{x, y, h, w};
-        ((Rec<double>::B *)recB)[t] = Rec<double>::B{vx, vy, rho, pr};
-    }
-    return pr;
-}
-
 __global__ void k_reorder(Ctrl *ctrl, State a, State b, const int *perm, int2 *cxy_s, Phys ph, int n, int fmt,
                           void *recA, void *recB, int pred, int strict, int lean, int xs, int dyn, HmConst hc) {
     // An inactive step (halted, or past the duration) still flips the
@@ -1014,6 +981,43 @@ static int closed(sph_ctx *c) {
     return c->mid ? fail(SPH_ESTATE, "a split step is open: call sph_step_end first") : 0;
 }
 
+// Whether the fused pass takes its uniform form (sph_pair_uniform.h for
+// FP32C, sph_pair_uniform64.h for FP64): the cubic spline or Wendland, the
+// plain loop (no XSPH, summation density or dynamic h), one h > 0 for every
+// particle and one fluid mass in the state, and both the support 2h and the
+// wall force's r0 inside the neighbour cut-off qmax h.
+bool hipsph::uniform_eligible(const sph_params_t &P, double h) {
+    if (P.kernel != SPH_KERNEL_CUBIC && P.kernel != SPH_KERNEL_WENDLAND) return false;
+    if (P.use_xsph || P.use_summation_density || P.dynamic_h) return false;
+    const double cut = 0.99 * P.qmax * h;
+    return h > 0.0 && 2.0 * h < cut && P.r0 < cut;
+}
+
+UniformParams hipsph::make_uniform_params(const sph_params_t &P, double h, double m) {
+    const double pi = 3.141592653589793;
+    UniformParams u;
+    const double lim = 2.0 * h > P.r0 ? 2.0 * h : P.r0;
+    u.lim2 = lim * lim * (P.precision == SPH_PREC_FP32C ? 1.0 + 1e-5 : 1.0 + 1e-12);
+    u.h = h;
+    u.m = m;
+    u.qmax = P.qmax;
+    u.g = P.gravity;
+    u.ck = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h * h);
+    u.ac = P.alpha * (0.5 * P.co);
+    // the double pass's constants with the reference's operations
+    // (CubicSpline.py:55 / Wendland.py:34, Momentum.py:43-46)
+    u.alpha_k = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h);
+    u.c01 = 0.01 * h * h;
+    u.cij = 0.5 * (P.co + 0.0);
+    u.alpha = P.alpha;
+    u.beta = P.beta;
+    u.r0 = P.r0;
+    u.D = P.D;
+    u.p1 = P.p1;
+    u.p2 = P.p2;
+    return u;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1212,18 +1216,7 @@ static int ev_record(sph_ctx *c, std::vector<hipEvent_t> &v, size_t &used, hipSt
 }
 
 
-// Whether the fused pass takes its uniform form (sph_pair_uniform.h for
-// FP32C, sph_pair_uniform64.h for FP64): the cubic spline or Wendland, the
-// plain loop (no XSPH, summation density or dynamic h), one h > 0 for every
-// particle and one fluid mass in the state, and both the support 2h and the
-// wall force's r0 inside the neighbour cut-off qmax h.
-static bool uniform_pass(const sph_ctx *c) {
-    const sph_params_t &P = c->P;
-    if (P.kernel != SPH_KERNEL_CUBIC && P.kernel != SPH_KERNEL_WENDLAND) return false;
-    if (P.use_xsph || P.use_summation_density || P.dynamic_h || !c->uni_hm) return false;
-    const double cut = 0.99 * P.qmax * c->uni_h;
-    return c->uni_h > 0.0 && 2.0 * c->uni_h < cut && P.r0 < cut;
-}
+static bool uniform_pass(const sph_ctx *c) { return c->uni_hm && uniform_eligible(c->P, c->uni_h); }
 
 static int rec_format(const sph_ctx *c) {
     const bool uni = uniform_pass(c);
@@ -1231,32 +1224,7 @@ static int rec_format(const sph_ctx *c) {
     return uni ? kRecUniform64 : kRecFp64;
 }
 
-static UniformParams uniform_params(const sph_ctx *c) {
-    const sph_params_t &P = c->P;
-    const double h = c->uni_h, pi = 3.141592653589793;
-    UniformParams u;
-    const double lim = 2.0 * h > P.r0 ? 2.0 * h : P.r0;
-    u.lim2 = lim * lim * (1.0 + 1e-5);
-    u.h = h;
-    u.m = c->uni_m;
-    u.qmax = P.qmax;
-    u.g = P.gravity;
-    u.ck = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h * h);
-    u.ac = P.alpha * (0.5 * P.co);
-    // the double pass's constants with the reference's operations
-    // (CubicSpline.py:55 / Wendland.py:34, Momentum.py:43-46)
-    u.alpha_k = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h);
-    u.c01 = 0.01 * h * h;
-    u.cij = 0.5 * (P.co + 0.0);
-    u.alpha = P.alpha;
-    if (P.precision != SPH_PREC_FP32C) u.lim2 = lim * lim * (1.0 + 1e-12);
-    u.beta = P.beta;
-    u.r0 = P.r0;
-    u.D = P.D;
-    u.p1 = P.p1;
-    u.p2 = P.p2;
-    return u;
-}
+static UniformParams uniform_params(const sph_ctx *c) { return make_uniform_params(c->P, c->uni_h, c->uni_m); }
 
 static void drop_graphs(sph_ctx *c) {
     for (int k = 0; k < 2; ++k)

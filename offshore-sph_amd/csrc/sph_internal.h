@@ -149,6 +149,8 @@ struct Phys {
     double eps;  // XSPH factor
 };
 
+struct BInst;
+
 // Launchers implemented in the pair translation units (threads: workgroup
 // size of the pair pass, 64, 128 or 256; chosen by the autotuner).
 hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, int n, int threads, hipStream_t s);
@@ -161,6 +163,25 @@ hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
 // cubic spline or Wendland), bit-identical to launch_pair_fp64's.
 hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
                              hipStream_t s);
+// Whether parameters P with one h for every particle allow the uniform
+// passes (the cubic spline or Wendland, the plain loop, 2h and r0 inside the
+// cut-off), and their constants for that h and fluid mass m.
+bool uniform_eligible(const sph_params_t &P, double h);
+UniformParams make_uniform_params(const sph_params_t &P, double h, double m);
+// The uniform passes of a batch: each instance's constants are a block of
+// uniform_const_size_*() bytes in the device array Ks (instance order),
+// written on the host by uniform_const_*. queue / grid as for
+// launch_pair_batch_*.
+size_t uniform_const_size_fp32();
+size_t uniform_const_size_fp64();
+void uniform_const_fp32(const UniformParams &u, void *out);
+void uniform_const_fp64(const UniformParams &u, void *out);
+hipError_t launch_pair_batch_fp32u(const BInst *inst, const int *instOf, PairIn in, PairOut out, const void *Ks,
+                                   int kernel, int n, int *queue, int grid, hipStream_t s);
+hipError_t launch_pair_batch_fp64u(const BInst *inst, const int *instOf, PairIn in, PairOut out, const void *Ks,
+                                   int kernel, int n, int *queue, int grid, hipStream_t s);
+int pair_batch_uniform_occupancy_fp32(int kernel);
+int pair_batch_uniform_occupancy_fp64(int kernel);
 // Summation density before the pair pass (src/Tools/SolverTools.py:129-140).
 hipError_t launch_sumden_fp64(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s);
 hipError_t launch_sumden_fp32c(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s);

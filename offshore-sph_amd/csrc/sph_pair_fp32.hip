@@ -34,17 +34,9 @@ hipError_t launch_pair_batch_fp32c(const BInst *inst, const int *instOf, PairIn 
                                            grid, s);
 }
 
-hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
-                             hipStream_t s) {
-    UniformArgs P;
-    P.A = (const double2 *)in.A;
-    P.B = (const float4 *)in.B;
-    P.cxy = in.cxy;
-    P.cellStart = in.cellStart;
-    P.ax = out.ax;
-    P.ay = out.ay;
-    P.drho = out.drho;
-    P.n = n;
+size_t uniform_const_size_fp32() { return sizeof(UniformConst); }
+
+void uniform_const_fp32(const UniformParams &u, void *out) {
     UniformConst K;
     K.lim2 = u.lim2;
     K.hd = u.h;
@@ -61,12 +53,64 @@ hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
     K.D = (float)u.D;
     K.p1 = (float)u.p1;
     K.p2 = (float)u.p2;
+    *(UniformConst *)out = K;
+}
+
+static UniformArgs uniform_args(const PairIn &in, const PairOut &out, int n) {
+    UniformArgs P;
+    P.A = (const double2 *)in.A;
+    P.B = (const float4 *)in.B;
+    P.cxy = in.cxy;
+    P.cellStart = in.cellStart;
+    P.ax = out.ax;
+    P.ay = out.ay;
+    P.drho = out.drho;
+    P.n = n;
+    return P;
+}
+
+hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                             hipStream_t s) {
+    const UniformArgs P = uniform_args(in, out, n);
+    UniformConst K;
+    uniform_const_fp32(u, &K);
     const int nb = xcd_blocks(n, 256);
     if (kernel == SPH_KERNEL_WENDLAND)
         k_pair_uniform<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K);
     else
         k_pair_uniform<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K);
     return hipGetLastError();
+}
+
+hipError_t launch_pair_batch_fp32u(const BInst *inst, const int *instOf, PairIn in, PairOut out, const void *Ks,
+                                   int kernel, int n, int *queue, int grid, hipStream_t s) {
+    const UniformArgs P = uniform_args(in, out, n);
+    const UniformConst *K = (const UniformConst *)Ks;
+    const int nb = xcd_blocks(n, 256);
+    if (queue) {  // a multiple of the XCD count, no more workgroups than tiles
+        if (grid > nb) grid = nb;
+        grid = grid / kXcds * kXcds;
+        if (grid < kXcds) grid = kXcds;
+        if (kernel == SPH_KERNEL_WENDLAND)
+            k_pair_batch_uniform_queue<SPH_KERNEL_WENDLAND><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
+        else
+            k_pair_batch_uniform_queue<SPH_KERNEL_CUBIC><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
+    } else if (kernel == SPH_KERNEL_WENDLAND) {
+        k_pair_batch_uniform<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+    } else {
+        k_pair_batch_uniform<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+    }
+    return hipGetLastError();
+}
+
+int pair_batch_uniform_occupancy_fp32(int kernel) {
+    int per = 0;
+    hipError_t e = kernel == SPH_KERNEL_WENDLAND
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, k_pair_batch_uniform_queue<SPH_KERNEL_WENDLAND>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, k_pair_batch_uniform_queue<SPH_KERNEL_CUBIC>, 256, 0);
+    return e == hipSuccess && per > 0 ? per : 1;
 }
 
 int pair_batch_queue_occupancy_fp32c(int kernel, int list, int threads) {

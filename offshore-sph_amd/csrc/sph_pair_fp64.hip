@@ -24,17 +24,9 @@ hipError_t launch_pair_fp64(const Ctrl *ctrl, PairIn in, PairOut out, Phys ph, i
     return launch_pair<double, true>(ctrl, pair_args<double>(in, &out, n), ph, threads, out.mode, s);
 }
 
-hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
-                             hipStream_t s) {
-    Uniform64Args P;
-    P.A = (const double2 *)in.A;
-    P.B = (const double4 *)in.B;
-    P.cxy = in.cxy;
-    P.cellStart = in.cellStart;
-    P.ax = out.ax;
-    P.ay = out.ay;
-    P.drho = out.drho;
-    P.n = n;
+size_t uniform_const_size_fp64() { return sizeof(Uniform64Const); }
+
+void uniform_const_fp64(const UniformParams &u, void *out) {
     Uniform64Const K;
     K.lim2 = u.lim2;
     K.h = u.h;
@@ -50,12 +42,64 @@ hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
     K.p2 = u.p2;
     K.g = u.g;
     K.qmax = u.qmax;
+    *(Uniform64Const *)out = K;
+}
+
+static Uniform64Args uniform64_args(const PairIn &in, const PairOut &out, int n) {
+    Uniform64Args P;
+    P.A = (const double2 *)in.A;
+    P.B = (const double4 *)in.B;
+    P.cxy = in.cxy;
+    P.cellStart = in.cellStart;
+    P.ax = out.ax;
+    P.ay = out.ay;
+    P.drho = out.drho;
+    P.n = n;
+    return P;
+}
+
+hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                             hipStream_t s) {
+    const Uniform64Args P = uniform64_args(in, out, n);
+    Uniform64Const K;
+    uniform_const_fp64(u, &K);
     const int nb = xcd_blocks(n, 256);
     if (kernel == SPH_KERNEL_WENDLAND)
         k_pair_uniform64<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K);
     else
         k_pair_uniform64<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K);
     return hipGetLastError();
+}
+
+hipError_t launch_pair_batch_fp64u(const BInst *inst, const int *instOf, PairIn in, PairOut out, const void *Ks,
+                                   int kernel, int n, int *queue, int grid, hipStream_t s) {
+    const Uniform64Args P = uniform64_args(in, out, n);
+    const Uniform64Const *K = (const Uniform64Const *)Ks;
+    const int nb = xcd_blocks(n, 256);
+    if (queue) {  // a multiple of the XCD count, no more workgroups than tiles
+        if (grid > nb) grid = nb;
+        grid = grid / kXcds * kXcds;
+        if (grid < kXcds) grid = kXcds;
+        if (kernel == SPH_KERNEL_WENDLAND)
+            k_pair_batch_uniform64_queue<SPH_KERNEL_WENDLAND><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
+        else
+            k_pair_batch_uniform64_queue<SPH_KERNEL_CUBIC><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
+    } else if (kernel == SPH_KERNEL_WENDLAND) {
+        k_pair_batch_uniform64<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+    } else {
+        k_pair_batch_uniform64<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+    }
+    return hipGetLastError();
+}
+
+int pair_batch_uniform_occupancy_fp64(int kernel) {
+    int per = 0;
+    hipError_t e = kernel == SPH_KERNEL_WENDLAND
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, k_pair_batch_uniform64_queue<SPH_KERNEL_WENDLAND>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per, k_pair_batch_uniform64_queue<SPH_KERNEL_CUBIC>, 256, 0);
+    return e == hipSuccess && per > 0 ? per : 1;
 }
 
 hipError_t launch_sumden_fp64(const Ctrl *ctrl, PairIn in, SdOut out, Phys ph, int n, int threads, hipStream_t s) {

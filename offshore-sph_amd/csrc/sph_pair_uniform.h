@@ -116,7 +116,7 @@ __device__ inline float3 uniform_terms(const UniformConst &K, float dx, float dy
 // The reference's neighbour count over the 3 x 3 cells of an out-of-grid
 // target (q = r / h <= qmax, the exact double test in the rounding band).
 __device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K, double X, double Y, int2 c, int ncx,
-                                    int ncy) {
+                                    int ncy, long long base) {
     const int lo = c.x - 1 > 0 ? c.x - 1 : 0;
     const int hi = c.x + 1 < ncx - 1 ? c.x + 1 : ncx - 1;
     const double lim = (K.qmax * K.hd) * (K.qmax * K.hd);
@@ -124,7 +124,8 @@ __device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K,
     for (int db = -1; db <= 1 && lo <= hi; ++db) {
         const int cy = c.y + db;
         if (!(ncy > cy && cy > -1)) continue;
-        const int s = P.cellStart[ncx * cy + lo], e = P.cellStart[ncx * cy + hi + 1];
+        const long long row = base + (long long)ncx * cy;
+        const int s = P.cellStart[row + lo], e = P.cellStart[row + hi + 1];
         for (int k = s; k < e; ++k) {
             const double2 a = P.A[k];
             const double dx = X - a.x, dy = Y - a.y;
@@ -137,18 +138,15 @@ __device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K,
     return cnt;
 }
 
+// One target t of the pass over a grid of ncx x ncy cells starting at bin
+// `base` (0 for a single simulation, the instance's first bin in a batch).
 template <int KERNEL>
-__global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformArgs P, UniformConst K) {
-    __shared__ unsigned s_list[kUniformList * 256];
-    if (!ctrl->active) return;
-    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= P.n) return;
+__device__ inline void uniform_target(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
+                                      int t, unsigned *s_list) {
     const float4 bt = P.B[t];
     if (not_fluid(bt.w)) return;  // walls are not targets
     const double2 at = P.A[t];
     const double X = at.x, Y = at.y;
-    const Grid &g = ctrl->grid;
-    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
     const int2 c = P.cxy[t];
     const unsigned stride = blockDim.x;
     unsigned *const list0 = s_list + threadIdx.x;
@@ -182,7 +180,8 @@ __global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformA
     for (int db = -1; db <= 1 && lo <= hi; ++db) {
         const int cy = c.y + db;
         if (!(ncy > cy && cy > -1)) continue;
-        const int s = P.cellStart[ncx * cy + lo], e = P.cellStart[ncx * cy + hi + 1];
+        const long long row = base + (long long)ncx * cy;
+        const int s = P.cellStart[row + lo], e = P.cellStart[row + hi + 1];
         int k = s;
         for (; k + U <= e; k += U) {
             if (lp > full) flush();
@@ -199,10 +198,82 @@ __global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformA
     }
     flush();
     const bool inside = c.x >= 0 && c.x < ncx && c.y >= 0 && c.y < ncy;  // it finds itself: a neighbour
-    if (!inside && uniform_count(P, K, X, Y, c, ncx, ncy) == 0) return;  // SolverTools.py:166-169
+    if (!inside && uniform_count(P, K, X, Y, c, ncx, ncy, base) == 0) return;  // SolverTools.py:166-169
     P.drho[t] = srho;
     P.ax[t] = sax;
     P.ay[t] = say - K.g;  // WCSPH.py:169
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformArgs P, UniformConst K) {
+    __shared__ unsigned s_list[kUniformList * 256];
+    if (!ctrl->active) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const Grid &g = ctrl->grid;
+    uniform_target<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, s_list);
+}
+
+// A batch target t: its instance's grid and constants. The lanes of a wave
+// may hold targets of two (rarely more) instances; each pass of the loop
+// takes the first remaining lane's instance, whose grid and constants are
+// then the same for the whole pass (scalar registers, not one copy per
+// lane), and runs the lanes of that instance. Every pass retires at least
+// the lane it was chosen from.
+template <int KERNEL>
+__device__ inline void uniform_target_batch(const BInst *inst, const int *instOf, const UniformArgs &P, const UniformConst *Ks, int t,
+                                   unsigned *s_list) {
+    const int k = instOf[t];
+    for (;;) {
+        const int k0 = __builtin_amdgcn_readfirstlane(k);
+        if (k == k0) {
+            const BInst &I = inst[k0];
+            if (I.active) uniform_target<KERNEL>(P, Ks[k0], (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, s_list);
+            break;
+        }
+    }
+}
+
+// The batched ensemble: each target with its own instance's grid and
+// constants (Ks[k] for instance k); targets of finished or halted instances
+// do nothing.
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_batch_uniform(const BInst *inst, const int *instOf, UniformArgs P,
+                                                            const UniformConst *Ks) {
+    __shared__ unsigned s_list[kUniformList * 256];
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    uniform_target_batch<KERNEL>(inst, instOf, P, Ks, t, s_list);
+}
+
+// The same as a persistent kernel draining a per-XCD work queue of tiles of
+// blockDim.x slots (as k_pair_batch_queue; 9 counters kQueueStride ints
+// apart, zero before the first launch, cleared by the last workgroup out).
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_batch_uniform_queue(const BInst *inst, const int *instOf, UniformArgs P,
+                                                                  const UniformConst *Ks, int *queue) {
+    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ int s_tile;
+    const int ntiles = (int)((P.n + (long long)blockDim.x - 1) / blockDim.x);
+    const int xcd = (int)(blockIdx.x % kXcds);
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int lo = xcd * per;
+    const int hi = lo + per < ntiles ? lo + per : ntiles;
+    int *ctr = queue + xcd * kQueueStride;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = lo + atomicAdd(ctr, 1);
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads();  // every lane has read s_tile before lane 0 writes the next
+        if (tile >= hi) break;
+        const int t = tile * (int)blockDim.x + (int)threadIdx.x;
+        if (t < P.n) uniform_target_batch<KERNEL>(inst, instOf, P, Ks, t, s_list);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(queue + kXcds * kQueueStride, 1) == (int)gridDim.x - 1)
+            for (int k = 0; k <= kXcds; ++k) atomicExch(queue + k * kQueueStride, 0);
+    }
 }
 
 }  // namespace

@@ -49,18 +49,15 @@ struct Uniform64Const {
 
 __device__ inline bool not_fluid64(double prs) { return prs != prs; }
 
+// One target t over a grid of ncx x ncy cells starting at bin `base` (0
+// for a single simulation, the instance's first bin in a batch).
 template <int KERNEL>
-__global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Uniform64Args P, Uniform64Const K) {
-    __shared__ unsigned s_list[kUniform64List * 256];
-    if (!ctrl->active) return;
-    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= P.n) return;
+__device__ inline void uniform64_target(const Uniform64Args &P, const Uniform64Const &K, int ncx, int ncy,
+                                        long long base, int t, unsigned *s_list) {
     const double4 bt = P.B[t];
     if (not_fluid64(bt.w)) return;  // walls are not targets
     const double2 at = P.A[t];
     const double X = at.x, Y = at.y;
-    const Grid &g = ctrl->grid;
-    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
     const int2 c = P.cxy[t];
     const unsigned stride = blockDim.x;
     unsigned *const list0 = s_list + threadIdx.x;
@@ -133,7 +130,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Unifor
         for (int db = -1; db <= 1; ++db) {
             const int cy = c.y + db;
             if (!(ncy > cy && cy > -1)) continue;
-            const int cell = cx + ncx * cy;
+            const long long cell = base + cx + (long long)ncx * cy;
             const int s = P.cellStart[cell], e = P.cellStart[cell + 1];
             int k = e - 1;
             for (; k - U + 1 >= s; k -= U) {
@@ -162,7 +159,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Unifor
             for (int db = -1; db <= 1; ++db) {
                 const int cy = c.y + db;
                 if (!(ncy > cy && cy > -1)) continue;
-                const int cell = cx + ncx * cy;
+                const long long cell = base + cx + (long long)ncx * cy;
                 for (int k = P.cellStart[cell]; k < P.cellStart[cell + 1]; ++k) {
                     const double2 a = P.A[k];
                     const double ddx = X - a.x, ddy = Y - a.y;
@@ -178,6 +175,75 @@ __global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Unifor
     P.drho[t] = arho;
     P.ax[t] = a0 + f0;
     P.ay[t] = (a1 - K.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Uniform64Args P, Uniform64Const K) {
+    __shared__ unsigned s_list[kUniform64List * 256];
+    if (!ctrl->active) return;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const Grid &g = ctrl->grid;
+    uniform64_target<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, s_list);
+}
+
+// A batch target t: its instance's grid and constants. The lanes of a wave
+// may hold targets of two (rarely more) instances; each pass of the loop
+// takes the first remaining lane's instance, whose grid and constants are
+// then the same for the whole pass (scalar registers, not one copy per
+// lane), and runs the lanes of that instance. Every pass retires at least
+// the lane it was chosen from.
+template <int KERNEL>
+__device__ inline void uniform64_target_batch(const BInst *inst, const int *instOf, const Uniform64Args &P, const Uniform64Const *Ks, int t,
+                                   unsigned *s_list) {
+    const int k = instOf[t];
+    for (;;) {
+        const int k0 = __builtin_amdgcn_readfirstlane(k);
+        if (k == k0) {
+            const BInst &I = inst[k0];
+            if (I.active) uniform64_target<KERNEL>(P, Ks[k0], (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, s_list);
+            break;
+        }
+    }
+}
+
+// The batched ensemble (one thread per slot, and the persistent work-queue
+// form), as the float pass's k_pair_batch_uniform[_queue].
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_batch_uniform64(const BInst *inst, const int *instOf, Uniform64Args P,
+                                                              const Uniform64Const *Ks) {
+    __shared__ unsigned s_list[kUniform64List * 256];
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    uniform64_target_batch<KERNEL>(inst, instOf, P, Ks, t, s_list);
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_batch_uniform64_queue(const BInst *inst, const int *instOf,
+                                                                    Uniform64Args P, const Uniform64Const *Ks,
+                                                                    int *queue) {
+    __shared__ unsigned s_list[kUniform64List * 256];
+    __shared__ int s_tile;
+    const int ntiles = (int)((P.n + (long long)blockDim.x - 1) / blockDim.x);
+    const int xcd = (int)(blockIdx.x % kXcds);
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int lo = xcd * per;
+    const int hi = lo + per < ntiles ? lo + per : ntiles;
+    int *ctr = queue + xcd * kQueueStride;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = lo + atomicAdd(ctr, 1);
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads();  // every lane has read s_tile before lane 0 writes the next
+        if (tile >= hi) break;
+        const int t = tile * (int)blockDim.x + (int)threadIdx.x;
+        if (t < P.n) uniform64_target_batch<KERNEL>(inst, instOf, P, Ks, t, s_list);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(queue + kXcds * kQueueStride, 1) == (int)gridDim.x - 1)
+            for (int k = 0; k <= kXcds; ++k) atomicExch(queue + k * kQueueStride, 0);
+    }
 }
 
 }  // namespace

@@ -128,6 +128,7 @@ _PROTOS = {
     'sph_timing_enable': (C.c_int, [_P, _I32]),
     'sph_timing_read': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I64), _I32]),
     'sph_pair_form': (C.c_int, [_P, C.POINTER(_I32)]),
+    'sph_batch_pair_form': (C.c_int, [_P, C.POINTER(_I32)]),
     'sph_loop_noalloc': (C.c_int, [C.POINTER(sph_params_t), C.POINTER(sph_state_t), _P, _P, _P, _P, _P]),
 }
 

@@ -90,6 +90,15 @@ class BatchEngine:
         return [{'t': d[4 * k], 'dt': d[4 * k + 1], 'dt_c': d[4 * k + 2], 'dt_f': d[4 * k + 3],
                  'steps': int(i[2 * k]), 'flags': int(i[2 * k + 1])} for k in range(self.K)]
 
+    def pair_form(self):
+        """The pair pass the batch runs, chosen at upload: 'fp64', 'fp32c', or
+        'fp64-uniform', 'fp32c-uniform' when every instance has one h and one
+        fluid mass (the uniform pass has its own workgroup size and list, so
+        only the schedule of set_tuning applies to it)."""
+        f = C.c_int32()
+        call('sph_batch_pair_form', self._b, C.byref(f))
+        return {0: 'fp64', 1: 'fp32c', 2: 'fp32c-uniform', 3: 'fp64-uniform'}[f.value]
+
     def set_tuning(self, pair_threads, list_len=None, persistent=None):
         call('sph_batch_set_tuning', self._b, int(pair_threads))
         if list_len is not None:
@@ -126,6 +135,8 @@ def autotune(batch, candidates=(64, 128, 256), lists=(16, 24, 32), schedules=(Fa
     choice changes a result); returns ((threads, list_len, persistent), table
     of (threads, list_len, persistent, ms))."""
     table = []
+    if batch.pair_form().endswith('uniform'):  # one launch shape: time the schedules only
+        candidates, lists = candidates[-1:], lists[-1:]
     for threads in candidates:
         for ll in lists:
             for per in schedules:

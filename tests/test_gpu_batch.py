@@ -7,9 +7,10 @@ oracle) on the same initial state:
   * FP64 pair pass: bit-identical state and clock after 40 steps, eager and
     hipGraph replay, with the pair pass one thread per slot and as the
     persistent work-queue kernel;
-  * FP32C pair pass: within 1e-6 of the state's largest magnitude and 1e-4
-    of the rates' (the single engine's float pass, in its uniform form, rounds
-    the same pairs differently);
+  * FP32C pair pass: bit-identical as well, batch and single engine taking
+    the same form of the float pass;
+  * both in the uniform form of the pair pass (every instance has one h and
+    one fluid mass) and in the general one (a second fluid mass);
   * an instance whose duration ends early stops while the others go on,
     exactly as a single engine with that duration.
 """
@@ -34,55 +35,59 @@ def _members():
     return out
 
 
-def _single(sc, alpha, steps, precision, duration=np.inf):
+def _two_masses(pA):
+    """A copy with a second fluid mass (the first fluid particle's 0.1 %
+    heavier): the general form of the pair pass."""
+    pA = pA.copy()
+    i = int(np.flatnonzero(pA['label'] == 0)[0])
+    pA['m'][i] *= 1.001
+    return pA
+
+
+def _single(sc, alpha, steps, precision, duration=np.inf, pA=None):
     m = sc.method()
     m.alpha = alpha
-    e = make_engine(sc.pA, m, precision=precision, duration=duration)
+    e = make_engine(sc.pA if pA is None else pA, m, precision=precision, duration=duration)
     e.step(steps)
     d = e.download()
     st = e.status()
+    form = e.pair_form()
     e.close()
-    return d, st
+    return d, st, form
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
 @pytest.mark.parametrize('graph', [False, True])
 @pytest.mark.parametrize('persistent', [False, True])
-def test_batch_instances_match_single_engines(precision, graph, persistent):
+@pytest.mark.parametrize('uniform', [True, False])
+def test_batch_instances_match_single_engines(precision, graph, persistent, uniform):
     mem = _members()
     K = 40
+    states = [sc.pA if uniform else _two_masses(sc.pA) for sc, _ in mem]
     b = BatchEngine([params_for(sc, alpha, precision=precision) for sc, alpha in mem])
     b.set_tuning(256, 0, persistent)
-    b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+    b.upload([soa_from_particles(pA) for pA in states])
+    form = b.pair_form()
+    assert form == precision + ('-uniform' if uniform else '')
     b.step(K, graph=graph)
     got = b.download()
     st = b.status()
     b.close()
-    say('batch', precision, 'graph', graph, 'sizes', [sc.n for sc, _ in mem])
+    say('batch', form, 'graph', graph, 'persistent', persistent, 'sizes', [sc.n for sc, _ in mem])
     for k, (sc, alpha) in enumerate(mem):
-        want, wst = _single(sc, alpha, K, precision)
+        want, wst, wform = _single(sc, alpha, K, precision, pA=states[k])
+        assert wform == form
         assert st[k]['steps'] == wst['steps'] == K
-        f = sc.pA['label'] == 0
-        if precision == 'fp64':
-            assert st[k]['t'] == wst['t'] and st[k]['dt'] == wst['dt']
-            for fld in FIELDS:
-                np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
-        else:
-            # the single engine takes the float pass's uniform form (one h, one
-            # fluid mass: double accumulation of the float terms), the batch the
-            # general one (TwoSum): the same pairs, rounded differently
-            assert st[k]['t'] == pytest.approx(wst['t'], rel=1e-6)
-            for fld in FIELDS:
-                scale = max(1.0, float(np.max(np.abs(want[fld][f]))))
-                tol = 1e-4 if fld in ('ax', 'ay', 'drho') else 1e-6
-                assert np.max(np.abs(got[k][fld] - want[fld])) <= tol * scale, (k, fld)
+        assert st[k]['t'] == wst['t'] and st[k]['dt'] == wst['dt']
+        for fld in FIELDS:
+            np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
 
 
 def test_batch_duration_per_instance():
     mem = _members()[:3]
     # instance 1 stops after about 10 steps' time
     sc1, a1 = mem[1]
-    _, probe = _single(sc1, a1, 10, 'fp64')
+    _, probe, _ = _single(sc1, a1, 10, 'fp64')
     durations = [np.inf, probe['t'] * 0.999, np.inf]
     ps = []
     for (sc, alpha), dur in zip(mem, durations):
@@ -98,7 +103,7 @@ def test_batch_duration_per_instance():
     assert st[0]['steps'] == 25 and st[2]['steps'] == 25
     assert st[1]['steps'] == 10
     for k, ((sc, alpha), dur) in enumerate(zip(mem, durations)):
-        want, wst = _single(sc, alpha, 25, 'fp64', duration=dur)
+        want, wst, _ = _single(sc, alpha, 25, 'fp64', duration=dur)
         assert st[k]['steps'] == wst['steps'] and st[k]['t'] == wst['t']
         for fld in FIELDS:
             np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
@@ -111,11 +116,12 @@ def test_batch_autotune_keeps_results():
     workgroup takes its tile. The persistent runs replay their graph 12
     times, so the queue's self-reset is exercised too."""
     mem = _members()
+    states = [_two_masses(sc.pA) for sc, _ in mem]  # the general pass: every launch shape applies
     b = BatchEngine([params_for(sc, alpha, precision='fp32c') for sc, alpha in mem])
     runs = []
     for threads, ll, per in ((256, 24, False), (64, 16, False), (128, 32, False), (256, 24, True),
                              (64, 32, True), (128, 16, True)):
-        b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+        b.upload([soa_from_particles(pA) for pA in states])
         b.set_tuning(threads, ll, per)
         b.step(12, graph=True)
         runs.append(b.download())
@@ -123,9 +129,23 @@ def test_batch_autotune_keeps_results():
         for k, (got, want) in enumerate(zip(other, runs[0])):
             for fld in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho'):
                 np.testing.assert_array_equal(got[fld], want[fld], err_msg=f'{k} {fld}')
-    b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+    b.upload([soa_from_particles(pA) for pA in states])
     best, table = autotune(b, steps=2)
     say('batch autotune', best, table)
     assert best[0] in (64, 128, 256) and best[1] in (16, 24, 32) and best[2] in (False, True)
     assert len(table) == 18
+    # the uniform pass has one launch shape: only the schedule is timed, and
+    # the two schedules agree
+    outs = []
+    for per in (False, True):
+        b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+        assert b.pair_form() == 'fp32c-uniform'
+        b.set_tuning(256, 0, per)
+        b.step(12, graph=True)
+        outs.append(b.download())
+    for k, (got, want) in enumerate(zip(*outs)):
+        for fld in ('x', 'y', 'vx', 'vy', 'rho', 'ax', 'ay', 'drho'):
+            np.testing.assert_array_equal(got[fld], want[fld], err_msg=f'{k} {fld}')
+    best, table = autotune(b, steps=2)
+    assert len(table) == 2 and best[:2] == (256, 32)
     b.close()

@@ -57,6 +57,7 @@ def _batch(precision, persistent, graph=True):
     b = BatchEngine([params_for(sc, a, precision=precision) for sc, a in zip(scs, alphas)])
     b.set_tuning(256, 0, persistent)
     b.upload([soa_from_particles(sc.pA) for sc in scs])
+    assert b.pair_form() == precision + '-uniform'  # each member: one h, one fluid mass
     b.step(STEPS, graph=graph)
     got = b.download()
     st = b.status()
