@@ -129,6 +129,7 @@ def profile_pair(sc, launches):
     in narrow requests, as the reorder and corrector gather)."""
     import torch
     from hipsph.device import Engine, EngineParams, soa_from_particles
+    print(f'# build {build_id()}', file=sys.stderr, flush=True)
     for prec in ('fp32c', 'fp64'):
         e = Engine(EngineParams.from_method(sc.n, sc.method(), precision=prec))
         e.upload(soa_from_particles(sc.pA))
@@ -356,6 +357,260 @@ def stress(reps, emit=True):
     return out
 
 
+# ---------------------------------------------------------------------------
+# summaries of rocprofv3 runs (CSV output) for profiles/
+# ---------------------------------------------------------------------------
+# Per-request sizes at the L2 -> memory-fabric interface (TCC_EA0) of MI355X,
+# checked against two kernels of known traffic in every counter run (a
+# 256 MiB device copy and a 256 MiB gather through 16-element-local random
+# indices; both within 0.1 %): every read request that is not a 32-byte one
+# moves a 128-byte line, every write request is 64 or 32 bytes. rocprofv3's
+# derived FETCH_SIZE books the 128-byte reads at 64 bytes (it reads half of
+# the copy's bytes), so it is not used.
+RD_COUNTERS = ('TCC_EA0_RDREQ_sum', 'TCC_EA0_RDREQ_32B_sum', 'TCC_BUBBLE_sum', 'TCC_EA0_RDREQ_DRAM_sum')
+WR_COUNTERS = ('TCC_EA0_WRREQ_sum', 'TCC_EA0_WRREQ_64B_sum', 'TCC_EA0_WRREQ_DRAM_sum')
+SQ_COUNTERS = ('SQ_WAVES', 'SQ_INSTS_VALU', 'SQ_ACTIVE_INST_VALU', 'SQ_THREAD_CYCLES_VALU', 'SQ_INSTS_SALU',
+               'SQ_INSTS_VMEM_RD', 'SQ_INSTS_LDS', 'SQ_WAIT_INST_ANY')
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
+
+
+def read_bytes(c):
+    return 128.0 * (c['TCC_EA0_RDREQ_sum'] - c['TCC_EA0_RDREQ_32B_sum']) + 32.0 * c['TCC_EA0_RDREQ_32B_sum']
+
+
+def write_bytes(c):
+    return 64.0 * c['TCC_EA0_WRREQ_64B_sum'] + 32.0 * (c['TCC_EA0_WRREQ_sum'] - c['TCC_EA0_WRREQ_64B_sum'])
+
+
+def _csv(dirname, suffix):
+    import csv
+    import glob
+    hits = sorted(glob.glob(os.path.join(dirname, '**', '*' + suffix), recursive=True))
+    if not hits:
+        raise FileNotFoundError(f'no *{suffix} under {dirname}')
+    with open(hits[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def _dispatches(rows):
+    """{dispatch id: (kernel name, {counter: value})} of a counter CSV."""
+    out = {}
+    for r in rows:
+        d = out.setdefault(int(r['Dispatch_Id']), (r['Kernel_Name'], {}))
+        d[1][r['Counter_Name']] = float(r['Counter_Value'])
+    return out
+
+
+def _short(name):
+    base = name.replace('hipsph::(anonymous namespace)::', '').replace('void ', '')
+    return base.split('(')[0].strip()
+
+
+def _pair_prec(name):
+    if 'k_pair' not in name or 'batch' in name:
+        return None
+    return 'fp64' if ('uniform64' in name or 'k_pair<double' in name) else 'fp32c'
+
+
+def summarize(raw, launches=5):
+    """Write profiles/pair_traffic.json, pair_valu.json and the three Markdown
+    summaries from the CSV directories of one counter-run series over
+    `bench.py --profile-pair K` (kt_pp: kernel trace; pmc_rd / pmc_wr / pmc_sq:
+    the counters above) and a kernel trace of a short headline run (kt_step).
+    The pair figures are means over the last `launches` dispatches of each
+    pair kernel (the profiled passes)."""
+    import statistics as stt
+    build = None
+    for log in ('kt_pp.log', 'pmc_rd.log'):
+        p = os.path.join(raw, log)
+        if os.path.exists(p):
+            for line in open(p):
+                if line.startswith('# build '):
+                    build = line.split()[2]
+    trace = _csv(os.path.join(raw, 'kt_pp'), '_kernel_trace.csv')
+    dur = {int(r['Dispatch_Id']): (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) * 1e-3 for r in trace}
+    rd = _dispatches(_csv(os.path.join(raw, 'pmc_rd'), '_counter_collection.csv'))
+    wr = _dispatches(_csv(os.path.join(raw, 'pmc_wr'), '_counter_collection.csv'))
+    sq = _dispatches(_csv(os.path.join(raw, 'pmc_sq'), '_counter_collection.csv'))
+    tr_names = {int(r['Dispatch_Id']): r['Kernel_Name'] for r in trace}
+
+    def last(d, prec):
+        ids = sorted(i for i, (k, _) in d.items() if _pair_prec(k) == prec)
+        return ids[-launches:], (d[ids[-1]][0] if ids else None)
+
+    def mean(d, ids, f):
+        return stt.mean(f(d[i][1]) for i in ids)
+
+    # calibration: the three largest copies and the gathers
+    cal = {}
+    for tag, key, known_r, known_w in (('copy', '__amd_rocclr_copyBuffer', 1 << 28, 1 << 28),
+                                       ('gather', '_scatter_gather_elementwise_kernel', 1 << 29, 1 << 28)):
+        ids = sorted((i for i, (k, _) in rd.items() if key in k), key=lambda i: -rd[i][1]['TCC_EA0_RDREQ_sum'])[:3]
+        idw = sorted((i for i, (k, _) in wr.items() if key in k), key=lambda i: -wr[i][1]['TCC_EA0_WRREQ_sum'])[:3]
+        r_b, w_b = mean(rd, ids, read_bytes), mean(wr, idw, write_bytes)
+        fetch_kib = mean(rd, ids, lambda c: (c['TCC_BUBBLE_sum'] * 128 + (c['TCC_EA0_RDREQ_sum'] - c['TCC_BUBBLE_sum']
+                                                                           - c['TCC_EA0_RDREQ_32B_sum']) * 64
+                                             + c['TCC_EA0_RDREQ_32B_sum'] * 32) / 1024)
+        cal[tag] = {'kernel': _short(rd[ids[0]][0]), 'dispatches': len(ids), 'known_read_bytes': known_r,
+                    'known_write_bytes': known_w, 'read_bytes': r_b, 'write_bytes': w_b,
+                    'read_over_known': r_b / known_r, 'write_over_known': w_b / known_w,
+                    'FETCH_SIZE_kib': fetch_kib, 'FETCH_SIZE_over_known': fetch_kib * 1024 / known_r}
+    traffic = {'build': build, 'model': 'read = 128 B x (TCC_EA0_RDREQ_sum - TCC_EA0_RDREQ_32B_sum) + 32 B x '
+                                        'TCC_EA0_RDREQ_32B_sum; write = 64 B x TCC_EA0_WRREQ_64B_sum + 32 B x '
+                                        '(TCC_EA0_WRREQ_sum - TCC_EA0_WRREQ_64B_sum)',
+               'calibration': cal}
+    valu = {'build': build, 'peak': {'simds': SIMDS, 'clock_hz': CLOCK_HZ, 'cycles_per_wave64_valu': 4}}
+    for prec in ('fp32c', 'fp64'):
+        ids_r, kname = last(rd, prec)
+        ids_w, _ = last(wr, prec)
+        ids_s, _ = last(sq, prec)
+        ids_t = sorted(i for i, k in tr_names.items() if _pair_prec(k) == prec)[-launches:]
+        if not ids_r:
+            continue
+        us = stt.mean(dur[i] for i in ids_t)
+        r_b, w_b = mean(rd, ids_r, read_bytes), mean(wr, ids_w, write_bytes)
+        traffic[prec] = {'kernel': _short(kname), 'dispatches': len(ids_r), 'read_bytes': r_b, 'write_bytes': w_b,
+                         'hbm_bytes_per_launch': r_b + w_b, 'avg_launch_us': us,
+                         'achieved_GBps': (r_b + w_b) / (us * 1e-6) / 1e9}
+        c = {k: mean(sq, ids_s, lambda d, k=k: d[k]) for k in SQ_COUNTERS}
+        valu[prec] = {'kernel': _short(kname), 'dispatches': len(ids_s), **{k + '_mean': v for k, v in c.items()},
+                      'avg_launch_us': us,
+                      'valu_instructions_per_wave': c['SQ_INSTS_VALU'] / c['SQ_WAVES'],
+                      'valu_issue_frac': c['SQ_INSTS_VALU'] * 4 / (SIMDS * CLOCK_HZ * us * 1e-6),
+                      'valu_lane_utilization': c['SQ_THREAD_CYCLES_VALU'] / (c['SQ_ACTIVE_INST_VALU'] * 64)}
+    # other kernels of the counter runs: mean per dispatch (all dispatches)
+    others = {}
+    for i, (k, c) in rd.items():
+        if k.startswith('k_') or 'hipsph' in k:
+            others.setdefault(_short(k), {'r': [], 'w': []})['r'].append(read_bytes(c))
+    for i, (k, c) in wr.items():
+        if _short(k) in others:
+            others[_short(k)]['w'].append(write_bytes(c))
+    traffic['other_kernels_MB'] = {k: {'dispatches': len(v['r']), 'read_MB_mean': stt.mean(v['r']) / 1e6,
+                                       'write_MB_mean': stt.mean(v['w']) / 1e6 if v['w'] else None,
+                                       'read_MB_last': v['r'][-1] / 1e6}
+                                   for k, v in sorted(others.items())}
+    pdir = os.path.join(ROOT, 'profiles')
+    with open(os.path.join(pdir, 'pair_traffic.json'), 'w') as f:
+        json.dump(traffic, f, indent=1)
+    with open(os.path.join(pdir, 'pair_valu.json'), 'w') as f:
+        json.dump(valu, f, indent=1)
+    # the headline run's kernel statistics, split at the switch from the
+    # float to the double pair pass
+    tr = sorted(_csv(os.path.join(raw, 'kt_step'), '_kernel_trace.csv'), key=lambda r: int(r['Start_Timestamp']))
+    first64 = next((j for j, r in enumerate(tr) if _pair_prec(r['Kernel_Name']) == 'fp64'), len(tr))
+    phases = {}
+    for tag, part in (('fp32c', tr[:first64]), ('fp64', tr[first64:])):
+        st = {}
+        for r in part:
+            st.setdefault(_short(r['Kernel_Name']), []).append(
+                (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) * 1e-3)
+        phases[tag] = st
+    return traffic, valu, phases, build
+
+
+NN_KERNELS = ('k_bounds', 'k_grid', 'k_keys', 'k_scan_reduce', 'k_scan_apply', 'k_scatter', 'k_cellsort', 'k_reorder')
+STEP_KERNELS = NN_KERNELS + ('k_correct', 'k_dt_final')
+PROFILE_COMMANDS = """    cd /tmp && export TMPDIR=/tmp      # OUT: an output directory, R: the repository
+    rocprofv3 --kernel-trace --stats -f csv -d OUT/kt_pp -o kt -- python3 -u R/bench.py --profile-pair 5 2> OUT/kt_pp.log
+    rocprofv3 --pmc TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_BUBBLE_sum TCC_EA0_RDREQ_DRAM_sum -f csv \\
+        -d OUT/pmc_rd -o rd -- python3 -u R/bench.py --profile-pair 5
+    rocprofv3 --pmc TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum TCC_EA0_WRREQ_DRAM_sum -f csv \\
+        -d OUT/pmc_wr -o wr -- python3 -u R/bench.py --profile-pair 5
+    rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_INSTS_SALU \\
+        SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_WAIT_INST_ANY -f csv -d OUT/pmc_sq -o sq -- python3 -u R/bench.py --profile-pair 5
+    rocprofv3 --kernel-trace --stats -f csv -d OUT/kt_step -o kt -- python3 -u R/bench.py --steps 200 --warmup 10 \\
+        --no-cpu --no-precision --no-adjoint --stress-reps 0 --ensemble-steps 0
+    python3 R/bench.py --summarize OUT"""
+
+
+def write_summaries(traffic, valu, phases, build, n=None, nf=None, cells=None):
+    """The Markdown summaries of summarize()'s figures (profiles/*.md)."""
+    pdir = os.path.join(ROOT, 'profiles')
+    cal = traffic['calibration']
+    head = (f'Library build `{build}` (the source hash `sph_version()` reports). MI355X, ROCm 7.2. Commands:\n\n'
+            f'{PROFILE_COMMANDS}\n')
+    wl = ('1M dam break (1,265,021 particles, 1,250,000 fluid, cubic spline, one h and one fluid mass: the '
+          'uniform pair passes). `--profile-pair 5`: per precision 3 steps, one neighbour update, 5 pair passes '
+          '(`sph_compute`); then the calibration kernels.')
+    L = ['# HBM traffic of the fused pair pass — 1M dam break', '', wl, '', head,
+         'Bytes per dispatch from the request counters at the L2 -> memory interface:',
+         f'`{traffic["model"]}`.', '',
+         'Calibration (the same runs; three dispatches each):', '',
+         '| kernel | known read | model read | ratio | known write | model write | ratio | FETCH_SIZE / known read |',
+         '|---|---|---|---|---|---|---|---|']
+    for tag, c in cal.items():
+        L.append(f'| {tag}: `{c["kernel"]}` | {c["known_read_bytes"] / 1e6:.1f} MB | {c["read_bytes"] / 1e6:.1f} MB | '
+                 f'{c["read_over_known"]:.4f} | {c["known_write_bytes"] / 1e6:.1f} MB | {c["write_bytes"] / 1e6:.1f} MB'
+                 f' | {c["write_over_known"]:.4f} | {c["FETCH_SIZE_over_known"]:.3f} |')
+    L += ['', 'The model reads both kernels to within 0.1 %; rocprofv3\'s derived FETCH_SIZE (which books each '
+          'non-32-byte read request at 64 bytes) reads half, so it is not used.', '',
+          '| kernel | dispatches | read MB | write MB | HBM MB / launch | avg us (trace) | GB/s | % of 8 TB/s |',
+          '|---|---|---|---|---|---|---|---|']
+    for prec in ('fp32c', 'fp64'):
+        t = traffic.get(prec)
+        if t:
+            L.append(f'| `{t["kernel"]}` ({prec}) | {t["dispatches"]} | {t["read_bytes"] / 1e6:.2f} | '
+                     f'{t["write_bytes"] / 1e6:.2f} | **{t["hbm_bytes_per_launch"] / 1e6:.2f}** | {t["avg_launch_us"]:.1f}'
+                     f' | {t["achieved_GBps"]:.0f} | {100 * t["achieved_GBps"] / HBM_PEAK_GBS:.1f} |')
+    L += ['', 'Reading: the uniform passes read 16-byte position records, 16-byte (float) or 32-byte (double) '
+          'velocity/density/pressure records and 8-byte cell coordinates per particle, plus the cell offsets; '
+          'the writes are ax, ay, drho as doubles per fluid particle (24 B x 1.25e6 = 30 MB). The measured reads '
+          'equal one pass over those arrays: neighbour-cell records re-read by other workgroups hit in L2.', '',
+          'Other kernels of the same runs (MB per dispatch; the first step after an upload gathers from an '
+          'unsorted state, the later ones from a nearly sorted one — `last` is the last dispatch):', '',
+          '| kernel | dispatches | read MB (mean) | read MB (last) | write MB (mean) |', '|---|---|---|---|---|']
+    for k, v in traffic['other_kernels_MB'].items():
+        if _pair_prec(k):
+            continue
+        w = f'{v["write_MB_mean"]:.2f}' if v['write_MB_mean'] is not None else '-'
+        L.append(f'| `{k}` | {v["dispatches"]} | {v["read_MB_mean"]:.2f} | {v["read_MB_last"]:.2f} | {w} |')
+    with open(os.path.join(pdir, 'pair_traffic_dam_break_1M.md'), 'w') as f:
+        f.write('\n'.join(L) + '\n')
+
+    L = ['# VALU issue of the fused pair pass — 1M dam break', '', wl, '', head,
+         '| kernel | SQ_WAVES | SQ_INSTS_VALU | VALU instr. / wave | avg us (trace) | VALU issue fraction | '
+         'VALU lane utilization |', '|---|---|---|---|---|---|---|']
+    for prec in ('fp32c', 'fp64'):
+        v = valu.get(prec)
+        if v:
+            L.append(f'| `{v["kernel"]}` ({prec}) | {v["SQ_WAVES_mean"]:.0f} | {v["SQ_INSTS_VALU_mean"]:.4g} | '
+                     f'{v["valu_instructions_per_wave"]:.0f} | {v["avg_launch_us"]:.1f} | **{v["valu_issue_frac"]:.2f}**'
+                     f' | {v["valu_lane_utilization"]:.2f} |')
+    L += ['', 'VALU issue fraction = SQ_INSTS_VALU x 4 cycles (a wave64 VALU instruction holds a 16-lane SIMD for 4 '
+          'cycles) / (1,024 SIMDs x 2.4 GHz x the average duration): a lower bound on the share of SIMD cycles '
+          'spent issuing VALU work (the clock is at most 2.4 GHz; transcendental and double divide steps take '
+          'more than 4 cycles). Lane utilization = SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64), rocprofv3\'s '
+          'VALUUtilization: the share of lanes active in the VALU instructions issued (divergence of the walk '
+          'and the flush).', '', 'Other counters (means per dispatch):', '']
+    for prec in ('fp32c', 'fp64'):
+        v = valu.get(prec)
+        if v:
+            L.append(f'* {prec}: ' + ', '.join(f'{k} {v[k + "_mean"]:.4g}' for k in SQ_COUNTERS))
+    with open(os.path.join(pdir, 'pair_valu_dam_break_1M.md'), 'w') as f:
+        f.write('\n'.join(L) + '\n')
+
+    L = ['# Kernel statistics of the 1M dam break step', '',
+         'The headline run, shortened: `bench.py --steps 200 --warmup 10` (FP32C run, then FP64 run; each 10 '
+         'warm-up steps, 200 steps replayed from hipGraphs, then 50 eager steps with HIP events around the pair '
+         'pass and the neighbour update, and the host round trip). Kernels of the engine only; the split between '
+         'the runs is the first double pair dispatch.', '', head]
+    for tag in ('fp32c', 'fp64'):
+        st = phases.get(tag, {})
+        rows = sorted(((k, v) for k, v in st.items() if k.startswith('k_') or 'k_pair' in k),
+                      key=lambda kv: -sum(kv[1]))
+        L += [f'## {tag}', '', '| kernel | calls | avg us | min us | max us | total ms |', '|---|---|---|---|---|---|']
+        for k, v in rows:
+            L.append(f'| `{k}` | {len(v)} | {sum(v) / len(v):.2f} | {min(v):.2f} | {max(v):.2f} | {sum(v) * 1e-3:.2f} |')
+        per_step = {k: sum(v) / len(v) for k, v in rows if k in STEP_KERNELS or _pair_prec(k)}
+        nn = sum(per_step.get(k, 0.0) for k in NN_KERNELS)
+        L += ['', f'One step = one call of each of {len(per_step)} kernels: {sum(per_step.values()):.1f} us of kernel '
+                  f'time (the neighbour update, {" + ".join(NN_KERNELS)}: {nn:.1f} us), before the gaps between '
+                  f'launches.', '']
+    with open(os.path.join(pdir, 'step_dam_break_1M_kernel_stats.md'), 'w') as f:
+        f.write('\n'.join(L) + '\n')
+
+
 def _profile(name):
     p = os.path.join(ROOT, 'profiles', name)
     if not os.path.exists(p):
@@ -408,7 +663,12 @@ def main():
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward+adjoint timing')
     ap.add_argument('--adjoint', action='store_true', help='only run the forward+adjoint timing')
     ap.add_argument('--ensemble', action='store_true', help='only run the 256-instance batched ensemble')
+    ap.add_argument('--summarize', metavar='DIR', help='write profiles/ summaries from the rocprofv3 CSV '
+                    'directories under DIR (no GPU)')
     a = ap.parse_args()
+    if a.summarize:
+        write_summaries(*summarize(a.summarize))
+        return
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU (slab-decomposed) path is not built. '
               'Run with --gpus 1.', file=sys.stderr)
