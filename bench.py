@@ -41,10 +41,18 @@ HBM_PEAK_GBS = 8000.0
 METRIC = 'WCSPH particle-steps/sec at 1M particles (FP32 vs FP64); fwd+adjoint/fwd time'
 
 
-def algorithmic_pair_bytes(precision, n, nf, cells):
+def algorithmic_pair_bytes(precision, n, nf, cells, corrector=False):
+    """Bytes the pair pass must move once per launch: per particle its
+    position, velocity, density and label (float: 8 + 8 + 4 + 1, double:
+    16 + 16 + 8 + 1), per fluid particle the three rates it writes (float 12,
+    double 24), per cell its offset (8). With the corrector fused in, per
+    fluid particle also the start-of-step x, y, vx, vy, rho and previous ax,
+    ay it reads (56, double) and the x, y, vx, vy, rho it writes (40)."""
     if precision == 'fp32c':
-        return 21 * n + 12 * nf + 8 * cells
-    return 41 * n + 24 * nf + 8 * cells
+        b = 21 * n + 12 * nf + 8 * cells
+    else:
+        b = 41 * n + 24 * nf + 8 * cells
+    return b + (96 * nf if corrector else 0)
 
 
 def run_gpu(sc, precision, steps, warmup, timing_steps=50, host_reps=0):
@@ -120,25 +128,27 @@ def run_cpu(sc, budget_s=12.0, max_steps=20, nthreads=1):
 
 
 def profile_pair(sc, launches):
-    """Counter-collection driver: `launches` pair passes per precision on the
-    1M workload (after 3 steps and a neighbour update), then two kernels of
+    """Counter-collection driver: per precision 3 + `launches` eager steps of
+    the 1M workload (the last `launches` are the profiled ones: the step's
+    pair pass with the corrector fused in), then `launches` plain pair passes
+    (`sph_compute`, no corrector) on the state they leave; then two kernels of
     known bytes that calibrate the read and write counters: a plain 256 MiB
     device copy (torch copy_, streaming requests) and a gather of 32M doubles
     through a permutation that is random inside each block of 16 elements
-    (torch index_select: every 128-byte line of the source is read whole,
-    in narrow requests, as the reorder and corrector gather)."""
+    (torch index_select: every 128-byte line of the source is read whole, in
+    narrow requests, as the reorder and corrector gather)."""
     import torch
     from hipsph.device import Engine, EngineParams, soa_from_particles
     print(f'# build {build_id()}', file=sys.stderr, flush=True)
     for prec in ('fp32c', 'fp64'):
         e = Engine(EngineParams.from_method(sc.n, sc.method(), precision=prec))
         e.upload(soa_from_particles(sc.pA))
-        e.step(3)
-        e.nn_update()
+        e.step(3 + launches, graph=False)
         for _ in range(launches):
             e.compute()
         torch.cuda.synchronize()
-        print(f'# profiled {launches} pair passes, {prec} ({e.pair_form()})', file=sys.stderr, flush=True)
+        print(f'# profiled {launches} steps and {launches} pair passes, {prec} ({e.pair_form()})', file=sys.stderr,
+              flush=True)
         e.close()
     n = 32 * 1024 * 1024
     src = torch.empty(n, dtype=torch.float64, device='cuda')
@@ -434,9 +444,9 @@ def summarize(raw, launches=5):
     sq = _dispatches(_csv(os.path.join(raw, 'pmc_sq'), '_counter_collection.csv'))
     tr_names = {int(r['Dispatch_Id']): r['Kernel_Name'] for r in trace}
 
-    def last(d, prec):
-        ids = sorted(i for i, (k, _) in d.items() if _pair_prec(k) == prec)
-        return ids[-launches:], (d[ids[-1]][0] if ids else None)
+    def last(d, name):
+        ids = sorted(i for i, (k, _) in d.items() if _short(k) == name)
+        return ids[-launches:]
 
     def mean(d, ids, f):
         return stt.mean(f(d[i][1]) for i in ids)
@@ -460,24 +470,32 @@ def summarize(raw, launches=5):
                                         '(TCC_EA0_WRREQ_sum - TCC_EA0_WRREQ_64B_sum)',
                'calibration': cal}
     valu = {'build': build, 'peak': {'simds': SIMDS, 'clock_hz': CLOCK_HZ, 'cycles_per_wave64_valu': 4}}
-    for prec in ('fp32c', 'fp64'):
-        ids_r, kname = last(rd, prec)
-        ids_w, _ = last(wr, prec)
-        ids_s, _ = last(sq, prec)
-        ids_t = sorted(i for i, k in tr_names.items() if _pair_prec(k) == prec)[-launches:]
-        if not ids_r:
+    # every single-simulation pair kernel of the runs; 'fp32c' / 'fp64' name
+    # the step's (the pass with the corrector fused in when it ran)
+    names = sorted({_short(k) for k, _ in rd.values() if _pair_prec(k)})
+    traffic['kernels'], valu['kernels'] = {}, {}
+    for name in names:
+        ids_r, ids_w, ids_s = last(rd, name), last(wr, name), last(sq, name)
+        ids_t = sorted(i for i, k in tr_names.items() if _short(k) == name)[-launches:]
+        if not (ids_r and ids_w and ids_s and ids_t):
             continue
         us = stt.mean(dur[i] for i in ids_t)
         r_b, w_b = mean(rd, ids_r, read_bytes), mean(wr, ids_w, write_bytes)
-        traffic[prec] = {'kernel': _short(kname), 'dispatches': len(ids_r), 'read_bytes': r_b, 'write_bytes': w_b,
-                         'hbm_bytes_per_launch': r_b + w_b, 'avg_launch_us': us,
-                         'achieved_GBps': (r_b + w_b) / (us * 1e-6) / 1e9}
+        traffic['kernels'][name] = {'kernel': name, 'dispatches': len(ids_r), 'read_bytes': r_b, 'write_bytes': w_b,
+                                    'hbm_bytes_per_launch': r_b + w_b, 'avg_launch_us': us,
+                                    'achieved_GBps': (r_b + w_b) / (us * 1e-6) / 1e9}
         c = {k: mean(sq, ids_s, lambda d, k=k: d[k]) for k in SQ_COUNTERS}
-        valu[prec] = {'kernel': _short(kname), 'dispatches': len(ids_s), **{k + '_mean': v for k, v in c.items()},
-                      'avg_launch_us': us,
-                      'valu_instructions_per_wave': c['SQ_INSTS_VALU'] / c['SQ_WAVES'],
-                      'valu_issue_frac': c['SQ_INSTS_VALU'] * 4 / (SIMDS * CLOCK_HZ * us * 1e-6),
-                      'valu_lane_utilization': c['SQ_THREAD_CYCLES_VALU'] / (c['SQ_ACTIVE_INST_VALU'] * 64)}
+        valu['kernels'][name] = {'kernel': name, 'dispatches': len(ids_s),
+                                 **{k + '_mean': v for k, v in c.items()}, 'avg_launch_us': us,
+                                 'valu_instructions_per_wave': c['SQ_INSTS_VALU'] / c['SQ_WAVES'],
+                                 'valu_issue_frac': c['SQ_INSTS_VALU'] * 4 / (SIMDS * CLOCK_HZ * us * 1e-6),
+                                 'valu_lane_utilization': c['SQ_THREAD_CYCLES_VALU'] / (c['SQ_ACTIVE_INST_VALU'] * 64)}
+    for prec in ('fp32c', 'fp64'):
+        mine = [k for k in traffic['kernels'] if _pair_prec(k) == prec]
+        step = [k for k in mine if '_pec' in k] or mine
+        if step:
+            traffic[prec] = traffic['kernels'][step[0]]
+            valu[prec] = valu['kernels'][step[0]]
     # other kernels of the counter runs: mean per dispatch (all dispatches)
     others = {}
     for i, (k, c) in rd.items():
@@ -531,8 +549,9 @@ def write_summaries(traffic, valu, phases, build, n=None, nf=None, cells=None):
     head = (f'Library build `{build}` (the source hash `sph_version()` reports). MI355X, ROCm 7.2. Commands:\n\n'
             f'{PROFILE_COMMANDS}\n')
     wl = ('1M dam break (1,265,021 particles, 1,250,000 fluid, cubic spline, one h and one fluid mass: the '
-          'uniform pair passes). `--profile-pair 5`: per precision 3 steps, one neighbour update, 5 pair passes '
-          '(`sph_compute`); then the calibration kernels.')
+          'uniform pair passes). `--profile-pair 5`: per precision 8 eager steps (the last 5 profiled: the '
+          'step\'s pair pass with the PEC corrector fused in, `k_pair_uniform*_pec`), then 5 plain pair passes '
+          '(`sph_compute`, `k_pair_uniform*`) on the state they leave; then the calibration kernels.')
     L = ['# HBM traffic of the fused pair pass — 1M dam break', '', wl, '', head,
          'Bytes per dispatch from the request counters at the L2 -> memory interface:',
          f'`{traffic["model"]}`.', '',
@@ -547,16 +566,19 @@ def write_summaries(traffic, valu, phases, build, n=None, nf=None, cells=None):
           'non-32-byte read request at 64 bytes) reads half, so it is not used.', '',
           '| kernel | dispatches | read MB | write MB | HBM MB / launch | avg us (trace) | GB/s | % of 8 TB/s |',
           '|---|---|---|---|---|---|---|---|']
-    for prec in ('fp32c', 'fp64'):
-        t = traffic.get(prec)
+    for name, t in traffic['kernels'].items():
+        prec = _pair_prec(name)
         if t:
             L.append(f'| `{t["kernel"]}` ({prec}) | {t["dispatches"]} | {t["read_bytes"] / 1e6:.2f} | '
                      f'{t["write_bytes"] / 1e6:.2f} | **{t["hbm_bytes_per_launch"] / 1e6:.2f}** | {t["avg_launch_us"]:.1f}'
                      f' | {t["achieved_GBps"]:.0f} | {100 * t["achieved_GBps"] / HBM_PEAK_GBS:.1f} |')
-    L += ['', 'Reading: the uniform passes read 16-byte position records, 16-byte (float) or 32-byte (double) '
-          'velocity/density/pressure records and 8-byte cell coordinates per particle, plus the cell offsets; '
-          'the writes are ax, ay, drho as doubles per fluid particle (24 B x 1.25e6 = 30 MB). The measured reads '
-          'equal one pass over those arrays: neighbour-cell records re-read by other workgroups hit in L2.', '',
+    L += ['', 'Reading: the plain uniform passes read 16-byte position records, 16-byte (float) or 32-byte '
+          '(double) velocity/density/pressure records and 8-byte cell coordinates per particle, plus the cell '
+          'offsets; the writes are ax, ay, drho as doubles per fluid particle (24 B x 1.25e6 = 30 MB). The '
+          'measured reads equal one pass over those arrays: neighbour-cell records re-read by other workgroups '
+          'hit in L2. The fused passes also gather the start-of-step x, y, vx, vy, rho and the previous ax, ay '
+          '(56 B per fluid particle, through the reorder\'s permutation) and write the corrected x, y, vx, vy, '
+          'rho (40 B).', '',
           'Other kernels of the same runs (MB per dispatch; the first step after an upload gathers from an '
           'unsorted state, the later ones from a nearly sorted one — `last` is the last dispatch):', '',
           '| kernel | dispatches | read MB (mean) | read MB (last) | write MB (mean) |', '|---|---|---|---|---|']
@@ -571,8 +593,8 @@ def write_summaries(traffic, valu, phases, build, n=None, nf=None, cells=None):
     L = ['# VALU issue of the fused pair pass — 1M dam break', '', wl, '', head,
          '| kernel | SQ_WAVES | SQ_INSTS_VALU | VALU instr. / wave | avg us (trace) | VALU issue fraction | '
          'VALU lane utilization |', '|---|---|---|---|---|---|---|']
-    for prec in ('fp32c', 'fp64'):
-        v = valu.get(prec)
+    for name, v in valu['kernels'].items():
+        prec = _pair_prec(name)
         if v:
             L.append(f'| `{v["kernel"]}` ({prec}) | {v["SQ_WAVES_mean"]:.0f} | {v["SQ_INSTS_VALU_mean"]:.4g} | '
                      f'{v["valu_instructions_per_wave"]:.0f} | {v["avg_launch_us"]:.1f} | **{v["valu_issue_frac"]:.2f}**'
@@ -583,10 +605,8 @@ def write_summaries(traffic, valu, phases, build, n=None, nf=None, cells=None):
           'more than 4 cycles). Lane utilization = SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64), rocprofv3\'s '
           'VALUUtilization: the share of lanes active in the VALU instructions issued (divergence of the walk '
           'and the flush).', '', 'Other counters (means per dispatch):', '']
-    for prec in ('fp32c', 'fp64'):
-        v = valu.get(prec)
-        if v:
-            L.append(f'* {prec}: ' + ', '.join(f'{k} {v[k + "_mean"]:.4g}' for k in SQ_COUNTERS))
+    for name, v in valu['kernels'].items():
+        L.append(f'* `{name}`: ' + ', '.join(f'{k} {v[k + "_mean"]:.4g}' for k in SQ_COUNTERS))
     with open(os.path.join(pdir, 'pair_valu_dam_break_1M.md'), 'w') as f:
         f.write('\n'.join(L) + '\n')
 
@@ -703,8 +723,11 @@ def main():
     n, nf = sc.n, sc.n_fluid
     cells = r32['grid']['n_cells']
     value = nf * a.steps / r32['elapsed_s']
-    b32 = algorithmic_pair_bytes('fp32c', n, nf, cells)
-    b64 = algorithmic_pair_bytes('fp64', n, nf, cells)
+    # one h and one fluid mass, PEC without XSPH: the step runs the uniform
+    # pair pass with the corrector fused in
+    fused32, fused64 = r32['pair_form'].endswith('uniform'), r64['pair_form'].endswith('uniform')
+    b32 = algorithmic_pair_bytes('fp32c', n, nf, cells, corrector=fused32)
+    b64 = algorithmic_pair_bytes('fp64', n, nf, cells, corrector=fused64)
     ach32 = b32 / (r32['pair_ms'] * 1e-3) / 1e9
     ach64 = b64 / (r64['pair_ms'] * 1e-3) / 1e9
     pairs = r64['pairs_per_step']
@@ -736,8 +759,8 @@ def main():
             cpu_omp = {'value': cm['value'], 'unit': 'particle-steps/s', 'cores': a.cpu_threads, 'kind': 'port',
                        'sample': f'{cm["steps"]} steps ({cm["elapsed_s"]:.1f} s), OpenMP pair loop of the same port'}
 
-    kname = {'fp32c-uniform': 'k_pair_uniform<cubic>', 'fp32c': 'k_pair<float, cubic>',
-             'fp64-uniform': 'k_pair_uniform64<cubic>', 'fp64': 'k_pair<double, cubic>'}
+    kname = {'fp32c-uniform': 'k_pair_uniform_pec<cubic> (pair pass + PEC corrector)', 'fp32c': 'k_pair<float, cubic>',
+             'fp64-uniform': 'k_pair_uniform64_pec<cubic> (pair pass + PEC corrector)', 'fp64': 'k_pair<double, cubic>'}
     line = {
         'metric': METRIC,
         'value': value,
@@ -771,7 +794,8 @@ def main():
         'speedup_vs_cpu_omp': (value / cpu_omp['value']) if cpu_omp else None,
         'pairs_per_step': pairs,
         'pair_interactions_per_s': pairs * a.steps / r32['elapsed_s'],
-        'stage_ms': {'pair': r32['pair_ms'], 'neighbour_update': r32['nn_ms'],
+        'stage_ms': {('pair_and_corrector' if fused32 else 'pair'): r32['pair_ms'],
+                     'neighbour_update': r32['nn_ms'],
                      'rest': r32['eager_events_ms_per_step'] - r32['pair_ms'] - r32['nn_ms']},
         'eager_events_ms_per_step': r32['eager_events_ms_per_step'],
         'pcie_inclusive': ({'ms_per_step': r32['host_roundtrip_ms_per_step'],
@@ -786,7 +810,8 @@ def main():
                               'kernel': kname[r64['pair_form']], 'algorithmic_bytes_per_launch': b64,
                               'avg_launch_ms': r64['pair_ms'],
                               'valu_issue_frac': prof(valu, 'fp64', 'valu_issue_frac')},
-                 'stage_ms': {'pair': r64['pair_ms'], 'neighbour_update': r64['nn_ms']}},
+                 'stage_ms': {('pair_and_corrector' if fused64 else 'pair'): r64['pair_ms'],
+                              'neighbour_update': r64['nn_ms']}},
         'precision': prec_report,
         'fwd_adjoint_over_fwd': adj['ratio'] if adj else None,
         'adjoint': adj,

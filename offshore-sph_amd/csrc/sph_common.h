@@ -100,8 +100,77 @@ __device__ inline void block_partials(double (&v)[NQ], const int (&kind)[NQ], do
 // fused step evaluates the same expressions inside the neighbour update.
 __device__ inline double pec_half_pos(double x, double v, double dt) { return x + 0.5 * dt * v; }
 __device__ inline double pec_half_vel(double v, double a, double dt, double damping) {
+#pragma clang fp contract(off)  // the integrator's arithmetic in every unit that inlines it
     return (v + 0.5 * dt * a) / (1 + 0.5 * damping);
 }
+
+// src/Integrators/PEC.py:61-89, one fluid particle from its start-of-step
+// state (x0 ... rho0), the velocity that moved it in the predictor (mvx, mvy:
+// the half-step velocity, or the XSPH velocity with useXSPH) and this step's
+// rates: writes x, y, vx, vy, rho of slot j.
+__device__ inline void pec_correct(const State &s, int j, double x0, double y0, double vx0, double vy0, double rho0,
+                                   double mvx, double mvy, double ax, double ay, double drho, double dt,
+                                   double damping, int strict) {
+#pragma clang fp contract(off)
+    double mid_x = x0 + 0.5 * dt * mvx;
+    double mid_y = y0 + 0.5 * dt * mvy;
+    double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
+    double mid_vy = (vy0 + 0.5 * dt * ay) / (1 + 0.5 * damping);
+    s.x[j] = 2 * mid_x - x0;
+    s.y[j] = 2 * mid_y - y0;
+    s.vx[j] = 2 * mid_vx - vx0;
+    s.vy[j] = 2 * mid_vy - vy0;
+    double mid_rho = rho0 + 0.5 * dt * drho;
+    double r = 2 * mid_rho - rho0;
+    if (strict && r < 0.0) r = 0.0;
+    s.rho[j] = r;
+}
+
+// The corrector of fluid slot j after a lean reorder: the start-of-step state
+// from the previous buffer at the particle's previous slot, the half-step
+// velocity formed again there (the predictor's expression). Loaded by
+// pec_start (before the pair walk, so that the gathers are in flight under
+// it), applied by pec_finish.
+struct PecStart {
+    double x0, y0, vx0, vy0, rho0, vxh, vyh;
+};
+__device__ inline PecStart pec_start(const PecArgs &C, int j, double dt, double damping) {
+    const int q = C.perm[j];
+    PecStart p;
+    p.x0 = C.o.x[q];
+    p.y0 = C.o.y[q];
+    p.vx0 = C.o.vx[q];
+    p.vy0 = C.o.vy[q];
+    p.rho0 = C.o.rho[q];
+    p.vxh = pec_half_vel(p.vx0, C.o.ax[q], dt, damping);
+    p.vyh = pec_half_vel(p.vy0, C.o.ay[q], dt, damping);
+    return p;
+}
+__device__ inline void pec_finish(const PecArgs &C, int j, const PecStart &p, double ax, double ay, double drho,
+                                  double dt, double damping) {
+    pec_correct(C.s, j, p.x0, p.y0, p.vx0, p.vy0, p.rho0, p.vxh, p.vyh, ax, ay, drho, dt, damping, C.strict);
+}
+
+// time step: src/Equations/TimeStep.py:58-90 computeVars over the fluid
+// (min h, max |a|^2, count, non-finite count), one workgroup's partials.
+struct DtVars {
+    double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
+    __device__ void add(double h, double ax, double ay) {
+#pragma clang fp contract(off)
+        double a2 = ax * ax + ay * ay;
+        if (h < mn) mn = h;
+        if (a2 > mx) mx = a2;
+        if (!(a2 == a2)) bad += 1.0;
+        cnt += 1.0;
+    }
+    // this workgroup's partials (one LDS round, plain stores); lds holds
+    // kDtQ * 16 doubles
+    __device__ void publish(double *part, double *lds) {
+        const int kind[kDtQ] = {0, 1, 2, 2};
+        double v[kDtQ] = {mn, mx, cnt, bad};
+        block_partials<kDtQ>(v, kind, lds, part, gridDim.x, blockIdx.x);
+    }
+};
 __device__ inline double pec_half_rho(double rho, double drho, double dt, int strict) {
     double r = rho + 0.5 * dt * drho;
     if (strict && r < 0.0) r = 0.0;

@@ -58,30 +58,14 @@ Phys hipsph::make_phys(const sph_params_t &P) {
 
 // ---------------------------------------------------------------------------
 // time step: src/Equations/TimeStep.py:58-90 computeVars over the fluid
+// (per-workgroup partials: DtVars), then Solver._minTimeStep
 // ---------------------------------------------------------------------------
-struct DtVars {
-    double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
-    __device__ void add(double h, double ax, double ay) {
-        double a2 = ax * ax + ay * ay;
-        if (h < mn) mn = h;
-        if (a2 > mx) mx = a2;
-        if (!(a2 == a2)) bad += 1.0;
-        cnt += 1.0;
-    }
-    // this workgroup's partials (one LDS round, plain stores); lds holds
-    // kDtQ * 16 doubles
-    __device__ void publish(double *part, double *lds) {
-        const int kind[kDtQ] = {0, 1, 2, 2};
-        double v[kDtQ] = {mn, mx, cnt, bad};
-        block_partials<kDtQ>(v, kind, lds, part, gridDim.x, blockIdx.x);
-    }
-};
-
 // Dt reduction over the fluid of the current state (used when no corrector
 // has left the reduction of this state behind). Grid-stride over R workgroups.
-__global__ void k_dt_partial(const int8_t *label, const double *h, const double *ax, const double *ay, int n,
-                             double *part) {
+__global__ void k_dt_partial(Ctrl *ctrl, const int8_t *label, const double *h, const double *ax, const double *ay,
+                             int n, double *part) {
     __shared__ double lds[kDtQ * 16];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
     DtVars v;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         if (label[i] == kFluid) v.add(h[i], ax[i], ay[i]);
@@ -89,12 +73,13 @@ __global__ void k_dt_partial(const int8_t *label, const double *h, const double 
 }
 
 // Solver._minTimeStep (src/Solver.py:210-230): courant (TimeStep.py:47-49),
-// force (:51-56), min(c, f) (:45), from the R partials of the fluid
-// reduction. Also opens the step: the reference's loop condition
+// force (:51-56), min(c, f) (:45), from the partials of the fluid reduction
+// (ctrl->dt_parts of them, as the kernel that wrote them recorded). Also opens the step: the reference's loop condition
 // `while self.t < self.duration` (src/Solver.py:366) and `self.t += self.dt`
 // (:409) move here, so a captured step needs no host. One workgroup.
-__global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *hist, const double *part, int R) {
+__global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *hist, const double *part) {
     __shared__ double lds[kDtQ * 16];
+    const int R = ctrl->dt_parts;
     // control fields are loaded with the partials (used after the fold)
     unsigned flags = ctrl->flags;
     const double t0 = ctrl->t, duration = ctrl->duration, fixed_dt = ctrl->fixed_dt, co = ctrl->co;
@@ -209,6 +194,7 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
                           double *part, HmConst hc) {
     if (!ctrl->active) return;
     __shared__ double lds[kDtQ * 16];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
     const double dt = ctrl->dt, damping = ctrl->damping;
     const int nm = ctrl->nm;
     DtVars v;
@@ -248,18 +234,8 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
                 vxh = s.vx[j];
                 vyh = s.vy[j];
             }
-            double mid_x = x0 + 0.5 * dt * (xs ? s.xsphx[j] : vxh);
-            double mid_y = y0 + 0.5 * dt * (xs ? s.xsphy[j] : vyh);
-            double mid_vx = (vx0 + 0.5 * dt * ax) / (1 + 0.5 * damping);
-            double mid_vy = (vy0 + 0.5 * dt * ay) / (1 + 0.5 * damping);
-            s.x[j] = 2 * mid_x - x0;
-            s.y[j] = 2 * mid_y - y0;
-            s.vx[j] = 2 * mid_vx - vx0;
-            s.vy[j] = 2 * mid_vy - vy0;
-            double mid_rho = rho0 + 0.5 * dt * s.drho[j];
-            double r = 2 * mid_rho - rho0;
-            if (strict && r < 0.0) r = 0.0;
-            s.rho[j] = r;
+            pec_correct(s, j, x0, y0, vx0, vy0, rho0, xs ? s.xsphx[j] : vxh, xs ? s.xsphy[j] : vyh, ax, ay, s.drho[j],
+                        dt, damping, strict);
         } else {
             double vx = s.vx[j], vy = s.vy[j];
             s.x[j] = s.x[j] + dt * vx + 0.5 * dt * dt * ax;
@@ -1076,7 +1052,10 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
               hipMalloc(&c->cxy_s, sizeof(int2) * n) == hipSuccess &&
 
               hipMalloc(&c->part_b, sizeof(double) * kBoundsQ * kRedBlocks) == hipSuccess &&
-              hipMalloc(&c->part_d, sizeof(double) * kDtQ * kRedBlocks) == hipSuccess &&
+              // dt partials: a reduction's workgroups or the fused pair pass's
+              hipMalloc(&c->part_d, sizeof(double) * kDtQ * (kRedBlocks > xcd_blocks(n, 256) ? kRedBlocks
+                                                                                             : xcd_blocks(n, 256))) ==
+                  hipSuccess &&
               hipMalloc(&c->part_k, sizeof(double) * (kRedBlocks + 1)) == hipSuccess &&
               hipMalloc(&c->slot_of, sizeof(int) * n) == hipSuccess;
     if (ok) {
@@ -1370,9 +1349,9 @@ static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr, int mode = 0
 static int dt_step(sph_ctx *c, hipStream_t s) {
     State st = c->buf[c->cur];
     if (!c->dt_fresh) {
-        k_dt_partial<<<redblk(c->n), kBlock, 0, s>>>(st.label, st.h, st.ax, st.ay, c->n, c->part_d);
+        k_dt_partial<<<redblk(c->n), kBlock, 0, s>>>(c->ctrl, st.label, st.h, st.ax, st.ay, c->n, c->part_d);
     }
-    k_dt_final<<<1, 1024, 0, s>>>(c->ctrl, c->P.gamma_c, c->P.gamma_f, c->dt_hist, c->part_d, redblk(c->n));
+    k_dt_final<<<1, 1024, 0, s>>>(c->ctrl, c->P.gamma_c, c->P.gamma_f, c->dt_hist, c->part_d);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1394,8 +1373,32 @@ static int correct(sph_ctx *c, hipStream_t s, bool lean = false) {
     return 0;
 }
 
+// The uniform pair pass with PEC's corrector and the next dt's partials
+// fused in, after a lean reorder (no Newmark-beta particles, whose
+// corrector k_correct applies). Timed as the pair pass.
+static int compute_pec(sph_ctx *c, hipStream_t s) {
+    State st = c->buf[c->cur];
+    PairIn in{c->recA, c->recB, st.h, c->cxy_s, c->cellStart};
+    PairOut out{st.ax, st.ay, st.drho, nullptr, 0};
+    const PecArgs pec{st, c->buf[1 - c->cur], c->perm, c->part_d, hm_const(c), c->P.strict};
+    if (c->timing) {
+        int rc = ev_record(c, c->ev_pair, c->n_pair, s);
+        if (rc) return rc;
+    }
+    hipError_t e = c->P.precision == SPH_PREC_FP32C
+                       ? launch_pair_fp32u_pec(c->ctrl, in, out, uniform_params(c), c->P.kernel, c->n, pec, s)
+                       : launch_pair_fp64u_pec(c->ctrl, in, out, uniform_params(c), c->P.kernel, c->n, pec, s);
+    if (e != hipSuccess) return fail(SPH_EHIP, "pair launch: %s", hipGetErrorString(e));
+    if (c->timing) {
+        int rc = ev_record(c, c->ev_pair, c->n_pair, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 // One Solver.run loop body (src/Solver.py:366-410) with the predictor fused
-// into the neighbour update and the next dt reduction into the corrector.
+// into the neighbour update and the next dt reduction into the corrector
+// (for one h and one fluid mass, the corrector into the pair pass).
 static int one_step(sph_ctx *c, hipStream_t s) {
     int rc;
     const bool pec = c->P.integrator == SPH_INTEGRATOR_PEC;
@@ -1404,8 +1407,12 @@ static int one_step(sph_ctx *c, hipStream_t s) {
     const bool lean = pec && !c->P.use_xsph && !c->P.use_summation_density;
     if ((rc = dt_step(c, s))) return rc;
     if ((rc = nn_update(c, s, pec, lean))) return rc;
-    if ((rc = compute(c, s))) return rc;
-    if ((rc = correct(c, s, lean))) return rc;
+    if (lean && c->P.coupled_integrator != SPH_COUPLED_NEWMARK && uniform_pass(c)) {
+        if ((rc = compute_pec(c, s))) return rc;
+    } else {
+        if ((rc = compute(c, s))) return rc;
+        if ((rc = correct(c, s, lean))) return rc;
+    }
     c->dt_fresh = true;
     return 0;
 }

@@ -47,6 +47,7 @@ struct Ctrl {
     int nn_valid;        // cell order and offsets match the positions
     int nm;              // Newmark-beta on the coupled particles
     double nm_beta, nm_gamma;
+    int dt_parts;        // workgroups whose dt partials the partials buffer holds (set by their kernel)
 };
 
 // Grid-wide reductions (bounds, dt) run over at most kRedBlocks workgroups in
@@ -91,6 +92,18 @@ struct Rec<float> {
 struct HmConst {
     int on;
     double h, m;
+};
+
+// PEC's corrector fused into the uniform pair passes (a step whose reorder
+// ran lean): slot j of s (this step's cell order) came from slot perm[j] of
+// o (the start-of-step buffer), which holds the start-of-step state and the
+// previous rates; part receives the next dt's partials (kDtQ per workgroup).
+struct PecArgs {
+    State s, o;
+    const int *perm;
+    double *part;
+    HmConst hc;
+    int strict;
 };
 
 // Record formats written by the reorder: the double pass's, the general
@@ -163,6 +176,13 @@ hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
 // cubic spline or Wendland), bit-identical to launch_pair_fp64's.
 hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
                              hipStream_t s);
+// The same passes with PEC's corrector and the next dt's partials fused in
+// (no Newmark-beta particles): they write ax, ay, drho and the corrected x,
+// y, vx, vy, rho of every fluid slot.
+hipError_t launch_pair_fp32u_pec(Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                                 const PecArgs &pec, hipStream_t s);
+hipError_t launch_pair_fp64u_pec(Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                                 const PecArgs &pec, hipStream_t s);
 // Whether parameters P with one h for every particle allow the uniform
 // passes (the cubic spline or Wendland, the plain loop, 2h and r0 inside the
 // cut-off), and their constants for that h and fluid mass m.

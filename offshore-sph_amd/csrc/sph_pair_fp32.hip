@@ -82,6 +82,19 @@ hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
     return hipGetLastError();
 }
 
+hipError_t launch_pair_fp32u_pec(Ctrl *ctrl, PairIn in, PairOut out, const UniformParams &u, int kernel, int n,
+                                 const PecArgs &pec, hipStream_t s) {
+    const UniformArgs P = uniform_args(in, out, n);
+    UniformConst K;
+    uniform_const_fp32(u, &K);
+    const int nb = xcd_blocks(n, 256);  // its dt partials: one per workgroup
+    if (kernel == SPH_KERNEL_WENDLAND)
+        k_pair_uniform_pec<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
+    else
+        k_pair_uniform_pec<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
+    return hipGetLastError();
+}
+
 hipError_t launch_pair_batch_fp32u(const BInst *inst, const int *instOf, PairIn in, PairOut out, const void *Ks,
                                    int kernel, int n, int *queue, int grid, hipStream_t s) {
     const UniformArgs P = uniform_args(in, out, n);

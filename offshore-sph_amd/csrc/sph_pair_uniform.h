@@ -30,17 +30,24 @@
 //   * The walk loads UNROLL records before testing them (no rotating
 //     registers), and the list of contributors advances by pointer.
 #pragma once
-#include "sph_internal.h"
+#include "sph_common.h"
 
 namespace hipsph {
 namespace {
+
+#ifndef HIPSPH_PEC_EARLY
+#define HIPSPH_PEC_EARLY 1  // the corrector's gathers issued before the walk (1 % faster than after it)
+#endif
 
 #ifndef HIPSPH_UNIFORM_UNROLL
 #define HIPSPH_UNIFORM_UNROLL 4
 #endif
 // Contributor-list entries per thread (28 KB of LDS per workgroup); a list is
 // flushed before a group of UNROLL candidates would overflow it.
-constexpr int kUniformList = 28;
+#ifndef HIPSPH_UNIFORM_LIST
+#define HIPSPH_UNIFORM_LIST 28
+#endif
+constexpr int kUniformList = HIPSPH_UNIFORM_LIST;
 
 struct UniformArgs {
     const double2 *A;  // (x, y)
@@ -138,13 +145,13 @@ __device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K,
     return cnt;
 }
 
-// One target t of the pass over a grid of ncx x ncy cells starting at bin
-// `base` (0 for a single simulation, the instance's first bin in a batch).
+// The rates of fluid target t (bt: its B record) over a grid of ncx x ncy
+// cells starting at bin `base` (0 for a single simulation, the instance's
+// first bin in a batch); false when t has no neighbour and keeps its
+// previous rates.
 template <int KERNEL>
-__device__ inline void uniform_target(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
-                                      int t, unsigned *s_list) {
-    const float4 bt = P.B[t];
-    if (not_fluid(bt.w)) return;  // walls are not targets
+__device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
+                                     int t, const float4 &bt, unsigned *s_list, double &drho, double &ax, double &ay) {
     const double2 at = P.A[t];
     const double X = at.x, Y = at.y;
     const int2 c = P.cxy[t];
@@ -198,10 +205,25 @@ __device__ inline void uniform_target(const UniformArgs &P, const UniformConst &
     }
     flush();
     const bool inside = c.x >= 0 && c.x < ncx && c.y >= 0 && c.y < ncy;  // it finds itself: a neighbour
-    if (!inside && uniform_count(P, K, X, Y, c, ncx, ncy, base) == 0) return;  // SolverTools.py:166-169
-    P.drho[t] = srho;
-    P.ax[t] = sax;
-    P.ay[t] = say - K.g;  // WCSPH.py:169
+    if (!inside && uniform_count(P, K, X, Y, c, ncx, ncy, base) == 0) return false;  // SolverTools.py:166-169
+    drho = srho;
+    ax = sax;
+    ay = say - K.g;  // WCSPH.py:169
+    return true;
+}
+
+// One target t of the pass: its rates, or nothing for a wall or a target
+// that keeps its previous rates.
+template <int KERNEL>
+__device__ inline void uniform_target(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
+                                      int t, unsigned *s_list) {
+    const float4 bt = P.B[t];
+    if (not_fluid(bt.w)) return;  // walls are not targets
+    double drho, ax, ay;
+    if (!uniform_rates<KERNEL>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay)) return;
+    P.drho[t] = drho;
+    P.ax[t] = ax;
+    P.ay[t] = ay;
 }
 
 template <int KERNEL>
@@ -232,6 +254,49 @@ __device__ inline void uniform_target_batch(const BInst *inst, const int *instOf
             break;
         }
     }
+}
+
+// The pass with PEC's corrector and the next dt's partials fused in (a PEC
+// step whose reorder ran lean, no Newmark-beta particles): each fluid target
+// takes its rates (or keeps the previous ones, which the reorder left in its
+// slot), then the corrector's update of its slot (src/Integrators/PEC.py:61-
+// 89, double arithmetic without contraction) and the time-step reduction's
+// terms of the new rates (TimeStep.py:58-90). The workgroup's dt partials go
+// to C.part[q * gridDim.x + blockIdx.x]; ctrl->dt_parts records the count.
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArgs P, UniformConst K, PecArgs C) {
+    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ double lds[kDtQ * 16];
+    if (!ctrl->active) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    DtVars v;
+    if (t < P.n) {
+        const float4 bt = P.B[t];
+        if (!not_fluid(bt.w)) {
+            const Grid &g = ctrl->grid;
+            const double dt = ctrl->dt, damping = ctrl->damping;
+#if HIPSPH_PEC_EARLY
+            const PecStart p0 = pec_start(C, t, dt, damping);
+#endif
+            double drho, ax, ay;
+            if (uniform_rates<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt, s_list, drho, ax, ay)) {
+                P.drho[t] = drho;
+                P.ax[t] = ax;
+                P.ay[t] = ay;
+            } else {
+                drho = P.drho[t];
+                ax = P.ax[t];
+                ay = P.ay[t];
+            }
+            v.add(C.hc.on ? C.hc.h : C.s.h[t], ax, ay);
+#if !HIPSPH_PEC_EARLY
+            const PecStart p0 = pec_start(C, t, dt, damping);
+#endif
+            pec_finish(C, t, p0, ax, ay, drho, dt, damping);
+        }
+    }
+    v.publish(C.part, lds);
 }
 
 // The batched ensemble: each target with its own instance's grid and

@@ -20,15 +20,22 @@
 //     y shift inner, each cell from its highest slot down), UNROLL records
 //     loaded before they are tested.
 #pragma once
-#include "sph_internal.h"
+#include "sph_common.h"
 
 namespace hipsph {
 namespace {
 
+#ifndef HIPSPH_PEC64_EARLY
+#define HIPSPH_PEC64_EARLY 0  // 1: the corrector's gathers before the walk (137 VGPRs, 3 waves: 10 % slower)
+#endif
+
 #ifndef HIPSPH_UNIFORM64_UNROLL
 #define HIPSPH_UNIFORM64_UNROLL 4
 #endif
-constexpr int kUniform64List = 32;  // contributor-list entries per thread (32 KB of LDS per workgroup)
+#ifndef HIPSPH_UNIFORM64_LIST
+#define HIPSPH_UNIFORM64_LIST 32
+#endif
+constexpr int kUniform64List = HIPSPH_UNIFORM64_LIST;  // contributor-list entries per thread (4 B each)
 
 struct Uniform64Args {
     const double2 *A;  // (x, y)
@@ -49,13 +56,14 @@ struct Uniform64Const {
 
 __device__ inline bool not_fluid64(double prs) { return prs != prs; }
 
-// One target t over a grid of ncx x ncy cells starting at bin `base` (0
-// for a single simulation, the instance's first bin in a batch).
+// The rates of fluid target t (bt: its B record) over a grid of ncx x ncy
+// cells starting at bin `base` (0 for a single simulation, the instance's
+// first bin in a batch); false when t has no neighbour and keeps its
+// previous rates.
 template <int KERNEL>
-__device__ inline void uniform64_target(const Uniform64Args &P, const Uniform64Const &K, int ncx, int ncy,
-                                        long long base, int t, unsigned *s_list) {
-    const double4 bt = P.B[t];
-    if (not_fluid64(bt.w)) return;  // walls are not targets
+__device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Const &K, int ncx, int ncy,
+                                       long long base, int t, const double4 &bt, unsigned *s_list, double &drho,
+                                       double &ax, double &ay) {
     const double2 at = P.A[t];
     const double X = at.x, Y = at.y;
     const int2 c = P.cxy[t];
@@ -170,11 +178,26 @@ __device__ inline void uniform64_target(const Uniform64Args &P, const Uniform64C
                 }
             }
         }
-        if (cnt == 0) return;
+        if (cnt == 0) return false;
     }
-    P.drho[t] = arho;
-    P.ax[t] = a0 + f0;
-    P.ay[t] = (a1 - K.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+    drho = arho;
+    ax = a0 + f0;
+    ay = (a1 - K.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+    return true;
+}
+
+// One target t of the pass: its rates, or nothing for a wall or a target
+// that keeps its previous rates.
+template <int KERNEL>
+__device__ inline void uniform64_target(const Uniform64Args &P, const Uniform64Const &K, int ncx, int ncy,
+                                        long long base, int t, unsigned *s_list) {
+    const double4 bt = P.B[t];
+    if (not_fluid64(bt.w)) return;  // walls are not targets
+    double drho, ax, ay;
+    if (!uniform64_rates<KERNEL>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay)) return;
+    P.drho[t] = drho;
+    P.ax[t] = ax;
+    P.ay[t] = ay;
 }
 
 template <int KERNEL>
@@ -205,6 +228,46 @@ __device__ inline void uniform64_target_batch(const BInst *inst, const int *inst
             break;
         }
     }
+}
+
+// The pass with PEC's corrector and the next dt's partials fused in, as the
+// float pass's k_pair_uniform_pec (the corrector's arithmetic is the
+// engine's, bit for bit).
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_uniform64_pec(Ctrl *ctrl, Uniform64Args P, Uniform64Const K,
+                                                            PecArgs C) {
+    __shared__ unsigned s_list[kUniform64List * 256];
+    __shared__ double lds[kDtQ * 16];
+    if (!ctrl->active) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    DtVars v;
+    if (t < P.n) {
+        const double4 bt = P.B[t];
+        if (!not_fluid64(bt.w)) {
+            const Grid &g = ctrl->grid;
+            const double dt = ctrl->dt, damping = ctrl->damping;
+#if HIPSPH_PEC64_EARLY
+            const PecStart p0 = pec_start(C, t, dt, damping);
+#endif
+            double drho, ax, ay;
+            if (uniform64_rates<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt, s_list, drho, ax, ay)) {
+                P.drho[t] = drho;
+                P.ax[t] = ax;
+                P.ay[t] = ay;
+            } else {
+                drho = P.drho[t];
+                ax = P.ax[t];
+                ay = P.ay[t];
+            }
+            v.add(C.hc.on ? C.hc.h : C.s.h[t], ax, ay);
+#if !HIPSPH_PEC64_EARLY
+            const PecStart p0 = pec_start(C, t, dt, damping);
+#endif
+            pec_finish(C, t, p0, ax, ay, drho, dt, damping);
+        }
+    }
+    v.publish(C.part, lds);
 }
 
 // The batched ensemble (one thread per slot, and the persistent work-queue
