@@ -35,7 +35,7 @@ UNITS = [
     ('sph_engine.hip', ['-ffp-contract=off']),
     ('sph_pair_fp64.hip', ['-ffp-contract=off']),
     ('sph_pair_fp32.hip', ['-ffp-contract=fast-honor-pragmas']),
-    ('sph_adjoint.hip', ['-ffp-contract=fast']),
+    ('sph_adjoint.hip', ['-ffp-contract=fast-honor-pragmas']),
     ('sph_batch.hip', ['-ffp-contract=off']),
 ]
 

@@ -36,6 +36,7 @@
 #include <cstring>
 
 #include "sph_ctx.h"
+#include "sph_common.h"
 
 using namespace hipsph;
 
@@ -270,9 +271,13 @@ __global__ void k_tape_amax_final(const double *pv, const int *pi, int R, TapeHd
 // grid's ncells + 2 entries); dt; and per-workgroup partials of the argmax of
 // |a_prev|^2 over the fluid with the engine's expression (contraction off).
 // An inactive step is recorded as such only.
+// lean: the step's reorder ran lean and the corrector was fused into the
+// pair pass, so the fluid's half-step state is not in s: it is formed again
+// from the start-of-step values at the particle's previous slot with the
+// predictor's expressions (as the reorder formed it for the pair records).
 __global__ void k_tape_pair(const Ctrl *ctrl, State s, State o, const int *perm, const int2 *cxy,
                             const int *cellStart, int n, TapeSlot T, TapeHdr *hdr, long long cap, Phys ph,
-                            double *pv, int *pi) {
+                            double *pv, int *pi, int lean, int strict) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int act = ctrl->active;
@@ -298,8 +303,23 @@ __global__ void k_tape_pair(const Ctrl *ctrl, State s, State o, const int *perm,
             T.axp[i] = ax;
             T.ayp[i] = ay;
             T.drhop[i] = o.drho[q];
-            const double vx = s.vx[t], vy = s.vy[t], rho = s.rho[t], m = s.m[t], h = s.h[t];
             const int8_t lab = s.label[t];
+            const double m = s.m[t], h = s.h[t];
+            double x, y, vx, vy, rho;
+            if (lean && lab == 0) {
+                const double dt = ctrl->dt, damping = ctrl->damping;
+                x = pec_half_pos(o.x[q], o.vx[q], dt);
+                y = pec_half_pos(o.y[q], o.vy[q], dt);
+                vx = pec_half_vel(o.vx[q], ax, dt, damping);
+                vy = pec_half_vel(o.vy[q], ay, dt, damping);
+                rho = pec_half_rho(o.rho[q], o.drho[q], dt, strict);
+            } else {
+                x = s.x[t];
+                y = s.y[t];
+                vx = s.vx[t];
+                vy = s.vy[t];
+                rho = s.rho[t];
+            }
             T.vxh[i] = vx;
             T.vyh[i] = vy;
             T.ax[i] = s.ax[t];
@@ -308,7 +328,7 @@ __global__ void k_tape_pair(const Ctrl *ctrl, State s, State o, const int *perm,
             T.slot[i] = t;
             T.id[t] = i;
             T.cxy[t] = cxy[t];
-            T.pa[t] = make_double4(s.x[t], s.y[t], h, lab == 0 ? m : -(double)lab);
+            T.pa[t] = make_double4(x, y, h, lab == 0 ? m : -(double)lab);
             T.pb[t] = make_double4(vx, vy, rho, m);
             if (lab == 0) {
                 const Side sd = make_side(h, rho, m, ph);
@@ -392,10 +412,15 @@ __global__ void k_adj_correct(const TapeHdr *hdr, TapeSlot T, const int8_t *labe
 // evaluations.)
 constexpr int kAdjLanes = 8;
 
+// The group's first lane then reverses the predictor for its particle (PEC;
+// for Euler the pair inputs are the start state), which needs nothing of
+// any other particle: the half-step adjoint goes straight into A, and the
+// predictor's dt term into this workgroup's partial 0.
 template <int KERNEL>
 __global__ __launch_bounds__(kAdjBlock) void k_adj_pair(const TapeHdr *hdr, TapeSlot T, int n, Phys ph,
-                                                         const double4 *lk, double *gx, double *gy, double *gvx,
-                                                         double *gvy, double *grho, double *part) {
+                                                         const double4 *lk, Adj A, const double *vbx,
+                                                         const double *vby, double damping, int euler,
+                                                         double *part) {
     double red[kAdjQ] = {0.0, 0.0, 0.0, 0.0};
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = (int)(gt / kAdjLanes);  // uniform within a lane group
@@ -453,68 +478,46 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_pair(const TapeHdr *hdr, Tape
     for (int o = kAdjLanes / 2; o > 0; o >>= 1)
 #pragma unroll
         for (int q = 0; q < 5; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
-    if (live && l == 0) {
-        if (ak.w >= 0.0) {
-            gx[k] = acc[0];
-            gy[k] = acc[1];
-            gvx[k] = acc[2];
-            gvy[k] = acc[3];
-            grho[k] = acc[4];
+    if (live && l == 0 && ak.w >= 0.0) {
+        const double dt = hdr->dt, e = 1.0 + 0.5 * damping;
+        const double xhb = acc[0], yhb = acc[1], rhb = acc[4];
+        const double vhx = vbx[k] + acc[2], vhy = vby[k] + acc[3];
+        if (euler) {
+            A.x[k] += xhb;
+            A.y[k] += yhb;
+            A.vx[k] += vhx;
+            A.vy[k] += vhy;
+            A.rho[k] += rhb;
+            A.ax[k] = 0.0;
+            A.ay[k] = 0.0;
+            A.drho[k] = 0.0;
         } else {
-            gx[k] = gy[k] = gvx[k] = gvy[k] = grho[k] = 0.0;
+            A.x[k] += xhb;
+            A.y[k] += yhb;
+            A.vx[k] += 0.5 * dt * xhb + vhx / e;
+            A.vy[k] += 0.5 * dt * yhb + vhy / e;
+            A.rho[k] += rhb;
+            A.ax[k] = 0.5 * dt * vhx / e;
+            A.ay[k] = 0.5 * dt * vhy / e;
+            A.drho[k] = 0.5 * dt * rhb;
+            red[0] = 0.5 * (xhb * T.vx0[k] + yhb * T.vy0[k] + (vhx * T.axp[k] + vhy * T.ayp[k]) / e +
+                            rhb * T.drhop[k]);
         }
     }
     block_partials<kAdjQ>(red, part, gridDim.x);
 }
 
-// Predictor (PEC) reversed; for Euler the pair inputs are the start state.
-__global__ void k_adj_predict(const TapeHdr *hdr, TapeSlot T, const int8_t *label, int n, Adj A, double damping,
-                              int euler, const double *vbx, const double *vby, const double *gx,
-                              const double *gy, const double *gvx, const double *gvy, const double *grho,
-                              double *part) {
-    if (!hdr->active) return;
-    const double dt = hdr->dt, e = 1.0 + 0.5 * damping;
-    double v[kAdjQ] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (label[i] != 0) continue;
-        const double xhb = gx[i], yhb = gy[i], rhb = grho[i];
-        const double vhx = vbx[i] + gvx[i], vhy = vby[i] + gvy[i];
-        if (euler) {
-            A.x[i] += xhb;
-            A.y[i] += yhb;
-            A.vx[i] += vhx;
-            A.vy[i] += vhy;
-            A.rho[i] += rhb;
-            A.ax[i] = 0.0;
-            A.ay[i] = 0.0;
-            A.drho[i] = 0.0;
-        } else {
-            A.x[i] += xhb;
-            A.y[i] += yhb;
-            A.vx[i] += 0.5 * dt * xhb + vhx / e;
-            A.vy[i] += 0.5 * dt * yhb + vhy / e;
-            A.rho[i] += rhb;
-            A.ax[i] = 0.5 * dt * vhx / e;
-            A.ay[i] = 0.5 * dt * vhy / e;
-            A.drho[i] = 0.5 * dt * rhb;
-            v[0] += 0.5 * (xhb * T.vx0[i] + yhb * T.vy0[i] + (vhx * T.axp[i] + vhy * T.ayp[i]) / e +
-                           rhb * T.drhop[i]);
-        }
-    }
-    block_partials<kAdjQ>(v, part, gridDim.x);
-}
-
-// Fold the partials of the three kernels; add this step's dL/dalpha and
-// dL/dc0. One workgroup.
+// Fold the partials of the two kernels (the corrector's dt term; the pair
+// pass's dL/dalpha, dL/dc0 and the predictor's dt term); add this step's
+// dL/dalpha and dL/dc0. One workgroup.
 __global__ void k_adj_final(const TapeHdr *hdr, TapeSlot T, Adj A, const double *p1, int R1, const double *p2,
-                            int R2, const double *p3, int R3, double co, double fixed_dt, double *g_alpha,
-                            double *g_co, int *force_steps) {
+                            int R2, double co, double fixed_dt, double *g_alpha, double *g_co, int *force_steps) {
     if (!hdr->active) return;
     __shared__ double lds[3 * 16];
     double dtb = 0.0, ga = 0.0, gc = 0.0;
     for (int b = threadIdx.x; b < R1; b += blockDim.x) dtb += p1[b];
-    for (int b = threadIdx.x; b < R3; b += blockDim.x) dtb += p3[b];
     for (int b = threadIdx.x; b < R2; b += blockDim.x) {
+        dtb += p2[b];
         ga += p2[1 * R2 + b];
         gc += p2[2 * R2 + b];
     }
@@ -567,11 +570,11 @@ inline int redblk_(long long n) {
 // ---------------------------------------------------------------------------
 // hooks for sph_step_record
 // ---------------------------------------------------------------------------
-int hipsph::tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s) {
+int hipsph::tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s, bool lean) {
     const int R = nblk_(c->n, kAdjBlock);
     k_tape_pair<<<R, kAdjBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->buf[1 - c->cur], c->perm, c->cxy_s,
                                         c->cellStart, c->n, t->slot[slot], t->hdr + slot, t->cells,
-                                        make_phys(c->P), t->amax_v, t->amax_i);
+                                        make_phys(c->P), t->amax_v, t->amax_i, lean, c->P.strict);
     k_tape_amax_final<<<1, 1024, 0, s>>>(t->amax_v, t->amax_i, R, t->hdr + slot);
     HIPCHK(hipGetLastError());
     return 0;
@@ -616,8 +619,8 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     // (each carved array is rounded up to 256 bytes: 20 per slot, 20 fixed)
     const size_t per = n * (10 * sizeof(double) + 3 * sizeof(double4) + 2 * sizeof(int) + sizeof(int2)) +
                        (size_t)t->cells * sizeof(int) + 20 * 256;
-    const size_t npart = (size_t)kAdjQ * (2 * kRedBlocks + nblk_((long long)ctx->n * kAdjLanes, kAdjBlock));
-    const size_t fixed = n + n * 7 * sizeof(double) + n * sizeof(double4) + npart * sizeof(double) +
+    const size_t npart = (size_t)kAdjQ * (kRedBlocks + nblk_((long long)ctx->n * kAdjLanes, kAdjBlock));
+    const size_t fixed = n + n * 2 * sizeof(double) + n * sizeof(double4) + npart * sizeof(double) +
                          sizeof(TapeHdr) * nslots + nblk_(ctx->n, kAdjBlock) * (sizeof(double) + sizeof(int)) + 20 * 256 + 4096;
     if (hipMalloc(&t->arena, per * nslots + fixed) != hipSuccess) {
         delete t;
@@ -636,7 +639,7 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     t->amax_v = (double *)take(nblk_(ctx->n, kAdjBlock) * sizeof(double));
     t->amax_i = (int *)take(nblk_(ctx->n, kAdjBlock) * sizeof(int));
     t->label = (int8_t *)take(n);
-    double **scr[] = {&t->vbx, &t->vby, &t->gx, &t->gy, &t->gvx, &t->gvy, &t->grho};
+    double **scr[] = {&t->vbx, &t->vby};
     for (double **d : scr) *d = (double *)take(n * sizeof(double));
     t->lk = (double4 *)take(n * sizeof(double4));
     t->slot.resize(nslots);
@@ -719,7 +722,7 @@ int hipsph::adjoint_launch(sph_ctx *ctx, sph_tape *tape, int slot, const sph_adj
     const int n = ctx->n;
     const int R = redblk_(n);
     const int Rp = nblk_((long long)n * kAdjLanes, kAdjBlock);  // the pair kernel: one partial per workgroup
-    double *p1 = tape->part, *p3 = tape->part + kAdjQ * kRedBlocks, *p2 = tape->part + 2 * kAdjQ * kRedBlocks;
+    double *p1 = tape->part, *p2 = tape->part + kAdjQ * kRedBlocks;
     const TapeSlot &T = tape->slot[slot];
     const TapeHdr *H = tape->hdr + slot;
     Adj A{adj->x, adj->y, adj->vx, adj->vy, adj->rho, adj->ax, adj->ay, adj->drho};
@@ -727,23 +730,22 @@ int hipsph::adjoint_launch(sph_ctx *ctx, sph_tape *tape, int slot, const sph_adj
     const int euler = ctx->P.integrator == SPH_INTEGRATOR_EULER;
     k_adj_correct<<<R, kAdjBlock, 0, s>>>(H, T, tape->label, n, A, ctx->P.damping, euler, tape->lk, tape->vbx,
                                            tape->vby, p1);
+    const double dmp = ctx->P.damping;
     switch (ctx->P.kernel) {
         case SPH_KERNEL_GAUSSIAN:
-            k_adj_pair<SPH_KERNEL_GAUSSIAN><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, tape->gx, tape->gy,
-                                                                     tape->gvx, tape->gvy, tape->grho, p2);
+            k_adj_pair<SPH_KERNEL_GAUSSIAN><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, A, tape->vbx, tape->vby,
+                                                                     dmp, euler, p2);
             break;
         case SPH_KERNEL_WENDLAND:
-            k_adj_pair<SPH_KERNEL_WENDLAND><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, tape->gx, tape->gy,
-                                                                     tape->gvx, tape->gvy, tape->grho, p2);
+            k_adj_pair<SPH_KERNEL_WENDLAND><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, A, tape->vbx, tape->vby,
+                                                                     dmp, euler, p2);
             break;
         default:
-            k_adj_pair<SPH_KERNEL_CUBIC><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, tape->gx, tape->gy,
-                                                                  tape->gvx, tape->gvy, tape->grho, p2);
+            k_adj_pair<SPH_KERNEL_CUBIC><<<Rp, kAdjBlock, 0, s>>>(H, T, n, ph, tape->lk, A, tape->vbx, tape->vby,
+                                                                  dmp, euler, p2);
     }
-    k_adj_predict<<<R, kAdjBlock, 0, s>>>(H, T, tape->label, n, A, ctx->P.damping, euler, tape->vbx, tape->vby,
-                                           tape->gx, tape->gy, tape->gvx, tape->gvy, tape->grho, p3);
-    k_adj_final<<<1, 1024, 0, s>>>(H, T, A, p1, R, p2, Rp, p3, R, ctx->P.co, ctx->P.fixed_dt, adj->g_alpha,
-                                   adj->g_co, tape->force_steps);
+    k_adj_final<<<1, 1024, 0, s>>>(H, T, A, p1, R, p2, Rp, ctx->P.co, ctx->P.fixed_dt, adj->g_alpha, adj->g_co,
+                                   tape->force_steps);
     HIPCHK(hipGetLastError());
     return 0;
 }

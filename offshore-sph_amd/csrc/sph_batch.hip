@@ -808,7 +808,7 @@ int sph_batch_upload(sph_batch_t *b, const sph_state_t *st, void *stream) {
     std::vector<double> hm(4 * (size_t)b->K);
     HIPCHK(hipMemcpyAsync(hm.data(), b->hm, sizeof(double) * hm.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    bool uni = true;
+    bool uni = b->n <= kUniformMaxN;  // the records of all instances, addressed with 32-bit byte offsets
     for (int k = 0; k < b->K && uni; ++k) {
         const double *v = &hm[4 * (size_t)k];
         uni = v[0] == v[1] && v[2] == v[3] && uniform_eligible(b->P[k], v[0]);

@@ -94,7 +94,6 @@ struct sph_tape {
     // reverse-pass scratch: id order, and (lam_x, lam_y, kap, 0) in the cell
     // slot order of the step being reversed
     double *vbx = nullptr, *vby = nullptr;
-    double *gx = nullptr, *gy = nullptr, *gvx = nullptr, *gvy = nullptr, *grho = nullptr;
     double4 *lk = nullptr;
     double *part = nullptr;  // reduction partials
     double *amax_v = nullptr;  // argmax partials (value, id) per workgroup
@@ -114,7 +113,7 @@ int fail(int code, const char *fmt, ...);
 Phys make_phys(const sph_params_t &P);
 
 // Tape hooks called by sph_step_record (implemented with the adjoint).
-int tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s);
+int tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s, bool lean);
 // The reverse of one recorded step (the kernels of sph_adjoint_step).
 int adjoint_launch(sph_ctx *c, sph_tape *t, int slot, const sph_adjoint_t *adj, hipStream_t s);
 void tape_drop_graphs(sph_tape *t);

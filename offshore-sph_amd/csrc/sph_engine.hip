@@ -960,10 +960,12 @@ static int closed(sph_ctx *c) {
 // Whether the fused pass takes its uniform form (sph_pair_uniform.h for
 // FP32C, sph_pair_uniform64.h for FP64): the cubic spline or Wendland, the
 // plain loop (no XSPH, summation density or dynamic h), one h > 0 for every
-// particle and one fluid mass in the state, and both the support 2h and the
-// wall force's r0 inside the neighbour cut-off qmax h.
+// particle and one fluid mass in the state, both the support 2h and the
+// wall force's r0 inside the neighbour cut-off qmax h, and at most
+// kUniformMaxN particles.
 bool hipsph::uniform_eligible(const sph_params_t &P, double h) {
     if (P.kernel != SPH_KERNEL_CUBIC && P.kernel != SPH_KERNEL_WENDLAND) return false;
+    if (P.n > kUniformMaxN) return false;  // records addressed with 32-bit byte offsets
     if (P.use_xsph || P.use_summation_density || P.dynamic_h) return false;
     const double cut = 0.99 * P.qmax * h;
     return h > 0.0 && 2.0 * h < cut && P.r0 < cut;
@@ -1417,6 +1419,8 @@ static int one_step(sph_ctx *c, hipStream_t s) {
     return 0;
 }
 
+static int record_step(sph_ctx *ctx, sph_tape *tape, int slot, hipStream_t s);
+
 int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream) {
     if (!ctx || !tape) return fail(SPH_EINVAL, "NULL argument");
     if (tape->ctx != ctx) return fail(SPH_EINVAL, "tape belongs to another context");
@@ -1426,23 +1430,27 @@ int sph_step_record(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, void *stream
     int rc;
     if ((rc = tape_supported(ctx))) return rc;
     if (slot == 0 && (rc = tape_refresh_labels(ctx, tape, s))) return rc;
-    if ((rc = dt_step(ctx, s))) return rc;
-    if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
-    if ((rc = compute(ctx, s))) return rc;
-    if ((rc = tape_record_pair(ctx, tape, slot, s))) return rc;
-    if ((rc = correct(ctx, s))) return rc;
-    ctx->dt_fresh = true;
-    return 0;
+    return record_step(ctx, tape, slot, s);
 }
 
-// One recorded step (as sph_step_record, without the argument checks).
+// One recorded step (as sph_step_record, without the argument checks): the
+// ordinary step (the same kernels as one_step, so the same trajectory) with
+// the tape written after the pair pass, from the half-step state the pass
+// saw and its rates.
 static int record_step(sph_ctx *ctx, sph_tape *tape, int slot, hipStream_t s) {
     int rc;
+    const bool pec = ctx->P.integrator == SPH_INTEGRATOR_PEC;
+    const bool fused = pec && uniform_pass(ctx);  // the tape covers no XSPH, summation density or coupling
     if ((rc = dt_step(ctx, s))) return rc;
-    if ((rc = nn_update(ctx, s, ctx->P.integrator == SPH_INTEGRATOR_PEC))) return rc;
-    if ((rc = compute(ctx, s))) return rc;
-    if ((rc = tape_record_pair(ctx, tape, slot, s))) return rc;
-    if ((rc = correct(ctx, s))) return rc;
+    if ((rc = nn_update(ctx, s, pec, fused))) return rc;
+    if (fused) {
+        if ((rc = compute_pec(ctx, s))) return rc;
+        if ((rc = tape_record_pair(ctx, tape, slot, s, true))) return rc;
+    } else {
+        if ((rc = compute(ctx, s))) return rc;
+        if ((rc = tape_record_pair(ctx, tape, slot, s, false))) return rc;
+        if ((rc = correct(ctx, s))) return rc;
+    }
     ctx->dt_fresh = true;
     return 0;
 }

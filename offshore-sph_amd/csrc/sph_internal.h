@@ -187,6 +187,7 @@ hipError_t launch_pair_fp64u_pec(Ctrl *ctrl, PairIn in, PairOut out, const Unifo
 // passes (the cubic spline or Wendland, the plain loop, 2h and r0 inside the
 // cut-off), and their constants for that h and fluid mass m.
 bool uniform_eligible(const sph_params_t &P, double h);
+constexpr long long kUniformMaxN = 1LL << 27;  // 32-B records of this many slots stay below 4 GiB
 UniformParams make_uniform_params(const sph_params_t &P, double h, double m);
 // The uniform passes of a batch: each instance's constants are a block of
 // uniform_const_size_*() bytes in the device array Ks (instance order),
@@ -281,6 +282,21 @@ inline int xcd_blocks(long long n, int bs) {
 __device__ inline int xcd_remap(int b, int nb) {
     int per = nb / kXcds;
     return (b % kXcds) * per + (b / kXcds);
+}
+
+// Element i of a device array as the array's base plus a 32-bit byte offset
+// (for arrays below 4 GiB): the base stays in scalar registers and each load
+// takes one 32-bit vector offset instead of 64-bit address arithmetic per lane.
+#ifndef HIPSPH_AT32
+#define HIPSPH_AT32 1
+#endif
+template <typename T>
+__device__ inline const T &at32(const T *base, unsigned i) {
+#if HIPSPH_AT32
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(T));
+#else
+    return base[i];
+#endif
 }
 
 // Integer power as a left-to-right product for small integer exponents

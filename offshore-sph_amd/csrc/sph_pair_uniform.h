@@ -134,7 +134,7 @@ __device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K,
         const long long row = base + (long long)ncx * cy;
         const int s = P.cellStart[row + lo], e = P.cellStart[row + hi + 1];
         for (int k = s; k < e; ++k) {
-            const double2 a = P.A[k];
+            const double2 a = at32(P.A, (unsigned)k);
             const double dx = X - a.x, dy = Y - a.y;
             const double r2 = dx * dx + dy * dy;
             bool in = r2 < lim * (1.0 - 1e-12);
@@ -164,8 +164,8 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
     auto flush = [&]() {
         for (unsigned *q = list0; q != lp; q += stride) {
             const unsigned u = *q;
-            const double2 a = P.A[u];
-            const float4 b = P.B[u];
+            const double2 a = at32(P.A, u);
+            const float4 b = at32(P.B, u);
             const float3 f = uniform_terms<KERNEL>(K, (float)(X - a.x), (float)(Y - a.y), bt.x, bt.y, bt.z, bt.w, b);
             srho += (double)f.x;
             sax += (double)f.y;
@@ -194,13 +194,13 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
             if (lp > full) flush();
             double2 a[U];
 #pragma unroll
-            for (int j = 0; j < U; ++j) a[j] = P.A[k + j];
+            for (int j = 0; j < U; ++j) a[j] = at32(P.A, (unsigned)(k + j));
 #pragma unroll
             for (int j = 0; j < U; ++j) test(a[j], k + j);
         }
         if (k < e) {
             if (lp > full) flush();
-            for (; k < e; ++k) test(P.A[k], k);  // fewer than U
+            for (; k < e; ++k) test(at32(P.A, (unsigned)k), k);  // fewer than U
         }
     }
     flush();

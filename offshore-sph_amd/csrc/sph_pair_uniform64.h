@@ -77,8 +77,8 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
     auto flush = [&]() {
         for (unsigned *q = list0; q != lp; q += stride) {
             const unsigned u = *q;
-            const double2 a = P.A[u];
-            const double4 b = P.B[u];
+            const double2 a = at32(P.A, u);
+            const double4 b = at32(P.B, u);
             const double dx = X - a.x, dy = Y - a.y;
             const double xij2 = dx * dx + dy * dy;
             const double r = sqrt(xij2);
@@ -145,13 +145,13 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
                 if (lp > full) flush();
                 double2 a[U];
 #pragma unroll
-                for (int j = 0; j < U; ++j) a[j] = P.A[k - j];
+                for (int j = 0; j < U; ++j) a[j] = at32(P.A, (unsigned)(k - j));
 #pragma unroll
                 for (int j = 0; j < U; ++j) test(a[j], k - j);
             }
             if (k >= s) {
                 if (lp > full) flush();
-                for (; k >= s; --k) test(P.A[k], k);  // fewer than U
+                for (; k >= s; --k) test(at32(P.A, (unsigned)k), k);  // fewer than U
             }
         }
     }
@@ -169,7 +169,7 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
                 if (!(ncy > cy && cy > -1)) continue;
                 const long long cell = base + cx + (long long)ncx * cy;
                 for (int k = P.cellStart[cell]; k < P.cellStart[cell + 1]; ++k) {
-                    const double2 a = P.A[k];
+                    const double2 a = at32(P.A, (unsigned)k);
                     const double ddx = X - a.x, ddy = Y - a.y;
                     const double r2 = ddx * ddx + ddy * ddy;
                     bool in = r2 < lim * (1.0 - 1e-12);
