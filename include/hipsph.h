@@ -312,6 +312,9 @@ int sph_batch_set_list_len(sph_batch_t *batch, int32_t list_len);
  * workgroups take tiles from a per-XCD work queue until it is drained.
  * Results are identical either way. Chosen by the autotuner. */
 int sph_batch_set_persistent(sph_batch_t *batch, int32_t on);
+/* Eager steps with timing on record HIP events around each step's pair pass
+ * and corrector (one kernel when the corrector is fused into the uniform
+ * pass); timing_read returns their mean in ms. */
 int sph_batch_timing_enable(sph_batch_t *batch, int32_t on);
 int sph_batch_timing_read(sph_batch_t *batch, double *avg_ms_pair, int64_t *count, int32_t reset);
 

@@ -33,7 +33,13 @@ using namespace hipsph;
 
 namespace {
 
-constexpr int kChunk = 2048;  // particles per reduction chunk (one instance each)
+// Two segmentations of the slots, each segment inside one instance: chunks
+// of kChunk for the bounds reduction, tiles of kTile for the dt reduction,
+// the corrector and the uniform pair pass with the corrector fused in (one
+// 256-thread workgroup per tile).
+constexpr int kChunk = 2048;
+constexpr int kTile = 256;
+static_assert(kTile == 256, "the fused batch pair pass runs one 256-thread workgroup per tile");
 
 struct BState {  // the concatenated cell-ordered state (as State)
     State s;
@@ -42,28 +48,11 @@ struct BState {  // the concatenated cell-ordered state (as State)
 // ---------------------------------------------------------------------------
 // per-instance dt (src/Solver.py:210-230, src/Equations/TimeStep.py:40-90)
 // ---------------------------------------------------------------------------
-struct DtAcc {
-    double mn = INFINITY, mx = -INFINITY, cnt = 0.0, bad = 0.0;
-    __device__ void add(double h, double ax, double ay) {
-        double a2 = ax * ax + ay * ay;
-        if (h < mn) mn = h;
-        if (a2 > mx) mx = a2;
-        if (!(a2 == a2)) bad += 1.0;
-        cnt += 1.0;
-    }
-    // lds holds kDtQ * 16 doubles
-    __device__ void store(double *part, int NC, int c, double *lds) {
-        const int kind[kDtQ] = {0, 1, 2, 2};
-        double v[kDtQ] = {mn, mx, cnt, bad};
-        block_partials<kDtQ>(v, kind, lds, part, NC, c);
-    }
-};
-
-// chunk c covers [cs[c], cs[c + 1]) inside one instance
+// segment c (a tile) covers [cs[c], cs[c + 1]) inside one instance
 __global__ void kb_dt_partial(const int *cs, State s, double *part) {
     __shared__ double lds[kDtQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
-    DtAcc v;
+    DtVars v;
     for (int t = cs[c] + threadIdx.x; t < cs[c + 1]; t += blockDim.x)
         if (s.label[t] == kFluid) v.add(s.h[t], s.ax[t], s.ay[t]);
     v.store(part, NC, c, lds);
@@ -374,7 +363,7 @@ __global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *ins
     const int c = blockIdx.x, NC = gridDim.x;
     const BInst &I = inst[chunkInst[c]];
     const double dt = I.dt, damping = I.damping;
-    DtAcc v;
+    DtVars v;
     for (int j = cs[c] + threadIdx.x; j < cs[c + 1]; j += blockDim.x) {
         if (s.label[j] != kFluid) continue;
         const double ax = s.ax[j], ay = s.ay[j];
@@ -467,6 +456,8 @@ struct sph_batch {
     BInst *inst = nullptr;
     Ctrl *view = nullptr;
     int *instOf = nullptr, *cs = nullptr, *chunkInst = nullptr, *chunk0 = nullptr;
+    int NT = 0;  // tiles: starts ts (NT + 1), instance of each, first tile of each instance (K + 1)
+    int *ts = nullptr, *tileInst = nullptr, *tile0 = nullptr;
     int *bin = nullptr, *rank = nullptr, *perm = nullptr, *permKey = nullptr;
     int2 *cxy_s = nullptr;
     int *counts = nullptr, *cellStart = nullptr, *tileSums = nullptr, *binflag = nullptr;
@@ -503,7 +494,8 @@ void free_batch(sph_batch *b) {
         if (b->arena[k]) hipFree(b->arena[k]);
         if (b->gexec[k]) hipGraphExecDestroy(b->gexec[k]);
     }
-    void *ptrs[] = {b->inst, b->view, b->instOf, b->cs, b->chunkInst, b->chunk0, b->bin, b->rank, b->perm,
+    void *ptrs[] = {b->inst, b->view, b->instOf, b->cs, b->chunkInst, b->chunk0, b->ts, b->tileInst, b->tile0,
+                    b->bin, b->rank, b->perm,
                     b->permKey, b->cxy_s, b->counts, b->cellStart, b->tileSums, b->binflag,
                     b->part_b, b->part_d, b->recA, b->recB, b->queue, b->uniK, b->hm};
     for (void *p : ptrs)
@@ -528,8 +520,8 @@ void size_queue(sph_batch *b) {
 
 int b_dt(sph_batch *b, hipStream_t s) {
     if (!b->dt_fresh)
-        kb_dt_partial<<<b->NC, kBlock, 0, s>>>(b->cs, b->buf[b->cur], b->part_d);
-    kb_dt_final<<<b->K, kBlock, 0, s>>>(b->inst, b->chunk0, b->part_d, b->NC);
+        kb_dt_partial<<<b->NT, kBlock, 0, s>>>(b->ts, b->buf[b->cur], b->part_d);
+    kb_dt_final<<<b->K, kBlock, 0, s>>>(b->inst, b->tile0, b->part_d, b->NT);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -559,18 +551,21 @@ int b_nn(sph_batch *b, hipStream_t s) {
     return 0;
 }
 
+// An event around the pair pass (timing on).
+int b_mark(sph_batch *b, hipStream_t s) {
+    if (b->n_ev >= b->ev.size()) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        b->ev.push_back(e);
+    }
+    HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
+    return 0;
+}
+
 int b_pair(sph_batch *b, hipStream_t s) {
     State st = b->buf[b->cur];
     PairIn in{b->recA, b->recB, st.h, b->cxy_s, b->cellStart};
     PairOut out{st.ax, st.ay, st.drho, nullptr, 0};
-    if (b->timing) {
-        if (b->n_ev >= b->ev.size()) {
-            hipEvent_t e;
-            HIPCHK(hipEventCreate(&e));
-            b->ev.push_back(e);
-        }
-        HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
-    }
     int *q = b->persistent ? b->queue : nullptr;
     hipError_t e;
     if (b->uni)  // the uniform pass: its own workgroup size (256) and list length
@@ -584,14 +579,22 @@ int b_pair(sph_batch *b, hipStream_t s) {
                 : launch_pair_batch_fp64(b->inst, b->instOf, in, out, b->kernel, b->n, b->pair_threads, b->list_len,
                                          q, b->queue_grid, s);
     if (e != hipSuccess) return fail(SPH_EHIP, "batch pair launch: %s", hipGetErrorString(e));
-    if (b->timing) {
-        if (b->n_ev >= b->ev.size()) {
-            hipEvent_t e2;
-            HIPCHK(hipEventCreate(&e2));
-            b->ev.push_back(e2);
-        }
-        HIPCHK(hipEventRecord(b->ev[b->n_ev++], s));
-    }
+    return 0;
+}
+
+// The uniform pair pass with PEC's corrector and the tiles' dt partials
+// fused in: one workgroup per tile (one instance each).
+int b_pair_pec(sph_batch *b, hipStream_t s) {
+    State st = b->buf[b->cur];
+    PairIn in{b->recA, b->recB, st.h, b->cxy_s, b->cellStart};
+    PairOut out{st.ax, st.ay, st.drho, nullptr, 0};
+    const PecArgs pec{st, b->buf[1 - b->cur], b->perm, b->part_d, HmConst{}, 0};
+    hipError_t e = b->precision == SPH_PREC_FP32C
+                       ? launch_pair_batch_fp32u_pec(b->inst, b->ts, b->tileInst, b->NT, in, out, b->uniK, b->kernel,
+                                                     b->n, pec, s)
+                       : launch_pair_batch_fp64u_pec(b->inst, b->ts, b->tileInst, b->NT, in, out, b->uniK, b->kernel,
+                                                     b->n, pec, s);
+    if (e != hipSuccess) return fail(SPH_EHIP, "batch pair launch: %s", hipGetErrorString(e));
     return 0;
 }
 
@@ -599,10 +602,17 @@ int b_step(sph_batch *b, hipStream_t s) {
     int rc;
     if ((rc = b_dt(b, s))) return rc;
     if ((rc = b_nn(b, s))) return rc;
-    if ((rc = b_pair(b, s))) return rc;
-    kb_correct<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->inst, b->buf[b->cur], b->buf[1 - b->cur], b->perm,
-                                        b->integrator == SPH_INTEGRATOR_EULER, b->part_d);
-    HIPCHK(hipGetLastError());
+    // timed: the pair pass and the corrector, fused or apart
+    if (b->timing && (rc = b_mark(b, s))) return rc;
+    if (b->uni && !b->persistent && b->integrator == SPH_INTEGRATOR_PEC) {
+        if ((rc = b_pair_pec(b, s))) return rc;
+    } else {
+        if ((rc = b_pair(b, s))) return rc;
+        kb_correct<<<b->NT, kBlock, 0, s>>>(b->ts, b->tileInst, b->inst, b->buf[b->cur], b->buf[1 - b->cur],
+                                            b->perm, b->integrator == SPH_INTEGRATOR_EULER, b->part_d);
+        HIPCHK(hipGetLastError());
+    }
+    if (b->timing && (rc = b_mark(b, s))) return rc;
     b->dt_fresh = true;
     return 0;
 }
@@ -650,8 +660,8 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
     b->P.assign(params, params + ninst);
     b->capBins = capTot;
     const size_t n = (size_t)ntot;
-    // chunk table
-    std::vector<int> cs, chunkInst, chunk0;
+    // chunk and tile tables
+    std::vector<int> cs, chunkInst, chunk0, ts, tileInst, tile0;
     std::vector<BInst> hI(ninst);
     long long off = 0;
     for (int k = 0; k < ninst; ++k) {
@@ -660,6 +670,11 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
         for (long long c = 0; c < P.n; c += kChunk) {
             cs.push_back((int)(off + c));
             chunkInst.push_back(k);
+        }
+        tile0.push_back((int)ts.size());
+        for (long long c = 0; c < P.n; c += kTile) {
+            ts.push_back((int)(off + c));
+            tileInst.push_back(k);
         }
         BInst &I = hI[k];
         std::memset(&I, 0, sizeof I);
@@ -696,6 +711,9 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
     cs.push_back((int)off);
     chunk0.push_back((int)chunkInst.size());
     b->NC = (int)chunkInst.size();
+    ts.push_back((int)off);
+    tile0.push_back((int)tileInst.size());
+    b->NT = (int)tileInst.size();
     std::vector<int> instOf(n);
     for (int k = 0; k < ninst; ++k)
         for (int i = b->offs[k]; i < b->offs[k + 1]; ++i) instOf[i] = k;
@@ -730,6 +748,9 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMalloc(&b->cs, sizeof(int) * cs.size()) == hipSuccess &&
          hipMalloc(&b->chunkInst, sizeof(int) * chunkInst.size()) == hipSuccess &&
          hipMalloc(&b->chunk0, sizeof(int) * chunk0.size()) == hipSuccess &&
+         hipMalloc(&b->ts, sizeof(int) * ts.size()) == hipSuccess &&
+         hipMalloc(&b->tileInst, sizeof(int) * tileInst.size()) == hipSuccess &&
+         hipMalloc(&b->tile0, sizeof(int) * tile0.size()) == hipSuccess &&
          hipMalloc(&b->bin, sizeof(int) * n) == hipSuccess && hipMalloc(&b->rank, sizeof(int) * n) == hipSuccess &&
          hipMalloc(&b->perm, sizeof(int) * n) == hipSuccess && hipMalloc(&b->permKey, sizeof(int) * n) == hipSuccess &&
          hipMalloc(&b->cxy_s, sizeof(int2) * n) == hipSuccess &&
@@ -738,7 +759,7 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMalloc(&b->tileSums, sizeof(int) * (tiles + 1)) == hipSuccess &&
          hipMalloc(&b->binflag, sizeof(int) * (capTot + 1)) == hipSuccess &&
          hipMalloc(&b->part_b, sizeof(double) * kBoundsQ * b->NC) == hipSuccess &&
-         hipMalloc(&b->part_d, sizeof(double) * kDtQ * b->NC) == hipSuccess &&
+         hipMalloc(&b->part_d, sizeof(double) * kDtQ * b->NT) == hipSuccess &&
          hipMalloc(&b->recA, ra * n) == hipSuccess && hipMalloc(&b->recB, rb * n) == hipSuccess &&
          hipMalloc(&b->queue, sizeof(int) * kQueueInts) == hipSuccess &&
          hipMalloc(&b->uniK, uk * ninst) == hipSuccess && hipMalloc(&b->hm, sizeof(double) * 4 * ninst) == hipSuccess;
@@ -756,6 +777,10 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMemcpy(b->chunkInst, chunkInst.data(), sizeof(int) * chunkInst.size(), hipMemcpyHostToDevice) ==
              hipSuccess &&
          hipMemcpy(b->chunk0, chunk0.data(), sizeof(int) * chunk0.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->ts, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b->tileInst, tileInst.data(), sizeof(int) * tileInst.size(), hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(b->tile0, tile0.data(), sizeof(int) * tile0.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemset(b->counts, 0, sizeof(int) * (capTot + 1)) == hipSuccess &&
          hipMemset(b->cellStart, 0, sizeof(int) * (capTot + 2)) == hipSuccess &&
          hipMemset(b->binflag, 0, sizeof(int) * (capTot + 1)) == hipSuccess &&

@@ -163,13 +163,14 @@ struct DtVars {
         if (!(a2 == a2)) bad += 1.0;
         cnt += 1.0;
     }
-    // this workgroup's partials (one LDS round, plain stores); lds holds
-    // kDtQ * 16 doubles
-    __device__ void publish(double *part, double *lds) {
+    // this workgroup's partials as entry k of R (one LDS round, plain
+    // stores: part[q * R + k]); lds holds kDtQ * 16 doubles
+    __device__ void store(double *part, int R, int k, double *lds) {
         const int kind[kDtQ] = {0, 1, 2, 2};
         double v[kDtQ] = {mn, mx, cnt, bad};
-        block_partials<kDtQ>(v, kind, lds, part, gridDim.x, blockIdx.x);
+        block_partials<kDtQ>(v, kind, lds, part, R, k);
     }
+    __device__ void publish(double *part, double *lds) { store(part, gridDim.x, blockIdx.x, lds); }
 };
 __device__ inline double pec_half_rho(double rho, double drho, double dt, int strict) {
     double r = rho + 0.5 * dt * drho;

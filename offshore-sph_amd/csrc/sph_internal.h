@@ -201,6 +201,17 @@ hipError_t launch_pair_batch_fp32u(const BInst *inst, const int *instOf, PairIn 
                                    int kernel, int n, int *queue, int grid, hipStream_t s);
 hipError_t launch_pair_batch_fp64u(const BInst *inst, const int *instOf, PairIn in, PairOut out, const void *Ks,
                                    int kernel, int n, int *queue, int grid, hipStream_t s);
+// The batch's uniform passes with PEC's corrector and the dt partials
+// fused in: one 256-thread workgroup per chunk of one instance (slots
+// cs[c] .. cs[c + 1], c < NC, at most 256 each; chunkInst: the chunk's
+// instance); the partials go to pec.part[q * NC + c] (pec.strict is unused:
+// each instance's own).
+hipError_t launch_pair_batch_fp32u_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC, PairIn in,
+                                       PairOut out, const void *Ks, int kernel, int n, const PecArgs &pec,
+                                       hipStream_t s);
+hipError_t launch_pair_batch_fp64u_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC, PairIn in,
+                                       PairOut out, const void *Ks, int kernel, int n, const PecArgs &pec,
+                                       hipStream_t s);
 int pair_batch_uniform_occupancy_fp32(int kernel);
 int pair_batch_uniform_occupancy_fp64(int kernel);
 // Summation density before the pair pass (src/Tools/SolverTools.py:129-140).

@@ -130,4 +130,16 @@ int pair_batch_queue_occupancy_fp32c(int kernel, int list, int threads) {
     return queue_occupancy<float, false>(kernel, list, threads);
 }
 
+hipError_t launch_pair_batch_fp32u_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC, PairIn in, PairOut out,
+                                       const void *Ks, int kernel, int n, const PecArgs &pec, hipStream_t s) {
+    const UniformArgs P = uniform_args(in, out, n);
+    const UniformConst *K = (const UniformConst *)Ks;
+    const int nb = (NC + kXcds - 1) / kXcds * kXcds;  // xcd_remap maps [0, nb) onto itself
+    if (kernel == SPH_KERNEL_WENDLAND)
+        k_pair_batch_uniform_pec<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    else
+        k_pair_batch_uniform_pec<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    return hipGetLastError();
+}
+
 }  // namespace hipsph

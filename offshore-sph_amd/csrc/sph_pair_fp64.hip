@@ -129,4 +129,16 @@ int pair_batch_queue_occupancy_fp64(int kernel, int list, int threads) {
     return queue_occupancy<double, true>(kernel, list, threads);
 }
 
+hipError_t launch_pair_batch_fp64u_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC, PairIn in, PairOut out,
+                                       const void *Ks, int kernel, int n, const PecArgs &pec, hipStream_t s) {
+    const Uniform64Args P = uniform64_args(in, out, n);
+    const Uniform64Const *K = (const Uniform64Const *)Ks;
+    const int nb = (NC + kXcds - 1) / kXcds * kXcds;  // xcd_remap maps [0, nb) onto itself
+    if (kernel == SPH_KERNEL_WENDLAND)
+        k_pair_batch_uniform64_pec<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    else
+        k_pair_batch_uniform64_pec<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    return hipGetLastError();
+}
+
 }  // namespace hipsph

@@ -341,5 +341,58 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_queue(const BInst *i
     }
 }
 
+
+// The batched ensemble with PEC's corrector and the dt partials fused in:
+// one workgroup per chunk (slots cs[c] .. cs[c + 1], at most 256 of one
+// instance), so the instance's grid, constants and integrator fields are the
+// same for the whole workgroup. The chunk's dt partials go to
+// C.part[q * NC + c], where the batch's corrector stores them. An instance
+// that is not active keeps its state; its fluid still enters the partials.
+template <int KERNEL>
+__global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC,
+                                                       UniformArgs P, const UniformConst *Ks, PecArgs C) {
+    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ double lds[kDtQ * 16];
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    if (c >= NC) return;  // the whole workgroup
+    const int k = chunkInst[c];
+    const BInst &I = inst[k];
+    DtVars v;
+    const int t = cs[c] + (int)threadIdx.x;
+    if (t < cs[c + 1]) {
+        const float4 bt = P.B[t];
+        if (!not_fluid(bt.w)) {
+            double drho, ax, ay;
+            if (I.active) {
+                const double dt = I.dt, damping = I.damping;
+                PecArgs Ci = C;
+                Ci.strict = I.strict;
+#if HIPSPH_PEC_EARLY
+                const PecStart p0 = pec_start(Ci, t, dt, damping);
+#endif
+                if (uniform_rates<KERNEL>(P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, bt, s_list, drho, ax,
+                                ay)) {
+                    P.drho[t] = drho;
+                    P.ax[t] = ax;
+                    P.ay[t] = ay;
+                } else {
+                    drho = P.drho[t];
+                    ax = P.ax[t];
+                    ay = P.ay[t];
+                }
+#if !HIPSPH_PEC_EARLY
+                const PecStart p0 = pec_start(Ci, t, dt, damping);
+#endif
+                pec_finish(Ci, t, p0, ax, ay, drho, dt, damping);
+            } else {
+                ax = P.ax[t];
+                ay = P.ay[t];
+            }
+            v.add(C.s.h[t], ax, ay);
+        }
+    }
+    v.store(C.part, NC, c, lds);
+}
+
 }  // namespace
 }  // namespace hipsph

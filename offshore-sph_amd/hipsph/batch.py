@@ -110,6 +110,8 @@ class BatchEngine:
         call('sph_batch_timing_enable', self._b, int(bool(on)))
 
     def timing_read(self, reset=True):
+        """{'pair_ms': mean ms of the pair pass + corrector over the timed
+        eager steps, 'count': how many}."""
         p = C.c_double()
         k = C.c_int64()
         call('sph_batch_timing_read', self._b, C.byref(p), C.byref(k), int(bool(reset)))
