@@ -49,12 +49,13 @@ struct BState {  // the concatenated cell-ordered state (as State)
 // per-instance dt (src/Solver.py:210-230, src/Equations/TimeStep.py:40-90)
 // ---------------------------------------------------------------------------
 // segment c (a tile) covers [cs[c], cs[c + 1]) inside one instance
-__global__ void kb_dt_partial(const int *cs, State s, double *part) {
+__global__ void kb_dt_partial(const int *cs, const int *segInst, const BInst *inst, State s, double *part) {
     __shared__ double lds[kDtQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
+    const BInst &I = inst[segInst[c]];
     DtVars v;
     for (int t = cs[c] + threadIdx.x; t < cs[c + 1]; t += blockDim.x)
-        if (s.label[t] == kFluid) v.add(s.h[t], s.ax[t], s.ay[t]);
+        if (s.label[t] == kFluid) v.add(I.uh_on ? I.uh : s.h[t], s.ax[t], s.ay[t]);
     v.store(part, NC, c, lds);
 }
 
@@ -130,7 +131,7 @@ __global__ void kb_bounds(const int *cs, const int *chunkInst, const BInst *inst
     for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x) {
         double xi, yi;
         bupd_pos(a, i, pred, I.dt, xi, yi);
-        const double hi = a.h[i];
+        const double hi = I.uh_on ? I.uh : a.h[i];
         if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;
         xmn = xi < xmn ? xi : xmn;
         xmx = xi > xmx ? xi : xmx;
@@ -295,7 +296,9 @@ __global__ void kb_reorder(BInst *inst, const int *instOf, State a, State b, con
     const int k = instOf[t];  // slots never change instance
     const BInst &I = inst[k];
     const int s = perm[t];
-    double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s], m = a.m[s], h = a.h[s];
+    const bool hs = I.uh_on;  // m and h stay in place (and m is not read: the uniform records have none)
+    double x = a.x[s], y = a.y[s], vx = a.vx[s], vy = a.vy[s], rho = a.rho[s];
+    const double m = hs ? 0.0 : a.m[s], h = hs ? I.uh : a.h[s];
     const double ax = a.ax[s], ay = a.ay[s], drho = a.drho[s];
     const int8_t lab = a.label[s];
     const bool skip = I.active && pec && lab == kFluid;
@@ -318,8 +321,10 @@ __global__ void kb_reorder(BInst *inst, const int *instOf, State a, State b, con
         b.vy[t] = vy;
         b.rho[t] = rho;
     }
-    b.m[t] = m;
-    b.h[t] = h;
+    if (!hs) {
+        b.m[t] = m;
+        b.h[t] = h;
+    }
     b.label[t] = lab;
     b.id[t] = a.id[s];
     if (!skip || !(cc.x >= 0 && cc.x < g.ncx && cc.y >= 0 && cc.y < g.ncy && h > 0.0 && I.qmax >= 0.0)) {
@@ -367,7 +372,7 @@ __global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *ins
     for (int j = cs[c] + threadIdx.x; j < cs[c + 1]; j += blockDim.x) {
         if (s.label[j] != kFluid) continue;
         const double ax = s.ax[j], ay = s.ay[j];
-        v.add(s.h[j], ax, ay);
+        v.add(I.uh_on ? I.uh : s.h[j], ax, ay);
         if (!I.active) continue;
         if (!euler) {
             const int q = perm[j];
@@ -415,10 +420,14 @@ __global__ void kb_upload(State d, sph_state_t in, int n) {
     d.id[i] = i;
 }
 
-__global__ void kb_download(State s, sph_state_t out, const BInst *inst, const int *instOf, int n) {
+// m0, h0: the upload's buffer, where an instance with uh_on keeps m and h at
+// slot = id.
+__global__ void kb_download(State s, sph_state_t out, const BInst *inst, const int *instOf, int n,
+                            const double *m0, const double *h0) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = s.id[t];
+    const BInst &I = inst[instOf[t]];
     if (out.x) out.x[i] = s.x[t];
     if (out.y) out.y[i] = s.y[t];
     if (out.vx) out.vx[i] = s.vx[t];
@@ -427,13 +436,10 @@ __global__ void kb_download(State s, sph_state_t out, const BInst *inst, const i
     if (out.ax) out.ax[i] = s.ax[t];
     if (out.ay) out.ay[i] = s.ay[t];
     if (out.drho) out.drho[i] = s.drho[t];
-    if (out.m) out.m[i] = s.m[t];
-    if (out.h) out.h[i] = s.h[t];
+    if (out.m) out.m[i] = I.uh_on ? m0[i] : s.m[t];
+    if (out.h) out.h[i] = I.uh_on ? h0[i] : s.h[t];
     if (out.label) out.label[i] = s.label[t];
-    if (out.p) {
-        const BInst &I = inst[instOf[t]];
-        out.p[i] = tait_p(s.rho[t], s.label[t], I.gamma, I.B, I.rho0, I.Pb);
-    }
+    if (out.p) out.p[i] = tait_p(s.rho[t], s.label[t], I.gamma, I.B, I.rho0, I.Pb);
 }
 
 inline int nblk_b(long long n, int bs = kBlock) {
@@ -520,7 +526,7 @@ void size_queue(sph_batch *b) {
 
 int b_dt(sph_batch *b, hipStream_t s) {
     if (!b->dt_fresh)
-        kb_dt_partial<<<b->NT, kBlock, 0, s>>>(b->ts, b->buf[b->cur], b->part_d);
+        kb_dt_partial<<<b->NT, kBlock, 0, s>>>(b->ts, b->tileInst, b->inst, b->buf[b->cur], b->part_d);
     kb_dt_final<<<b->K, kBlock, 0, s>>>(b->inst, b->tile0, b->part_d, b->NT);
     HIPCHK(hipGetLastError());
     return 0;
@@ -849,6 +855,13 @@ int sph_batch_upload(sph_batch_t *b, const sph_state_t *st, void *stream) {
         }
         HIPCHK(hipMemcpy(b->uniK, blk.data(), blk.size(), hipMemcpyHostToDevice));
     }
+    // m and h stay in place for the uniform pass (the upload's buffer, slot =
+    // id): every instance's one h in its control block
+    for (int k = 0; k < b->K; ++k) {
+        hI[k].uh_on = uni ? 1 : 0;
+        hI[k].uh = uni ? hm[4 * (size_t)k] : 0.0;
+    }
+    HIPCHK(hipMemcpy(b->inst, hI.data(), sizeof(BInst) * b->K, hipMemcpyHostToDevice));
     if (uni != b->uni) {  // another record format and pair kernel: recapture
         b->uni = uni;
         b->queue_grid = 0;
@@ -869,7 +882,8 @@ int sph_batch_pair_form(sph_batch_t *b, int32_t *form) {
 int sph_batch_download(sph_batch_t *b, const sph_state_t *st, void *stream) {
     if (!b || !st) return fail(SPH_EINVAL, "NULL argument");
     if (!b->loaded) return fail(SPH_ESTATE, "no state: call sph_batch_upload first");
-    kb_download<<<nblk_b(b->n), kBlock, 0, (hipStream_t)stream>>>(b->buf[b->cur], *st, b->inst, b->instOf, b->n);
+    kb_download<<<nblk_b(b->n), kBlock, 0, (hipStream_t)stream>>>(b->buf[b->cur], *st, b->inst, b->instOf, b->n,
+                                                                  b->buf[0].m, b->buf[0].h);
     HIPCHK(hipGetLastError());
     return 0;
 }

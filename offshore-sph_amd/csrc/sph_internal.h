@@ -237,6 +237,10 @@ struct BInst {
     double duration, fixed_dt, damping;
     double rho0, gamma, co, B, Pb, alpha, beta, r0, D, p1, p2, g, qmax, scale, gamma_c, gamma_f;
     int strict, kernel;
+    // one h for every particle of the instance (the uniform pass): uh; m and
+    // h then stay where the upload put them (slot = id) and are not permuted
+    int uh_on;
+    double uh;
 };
 
 __host__ __device__ inline Phys inst_phys(const BInst &I) {

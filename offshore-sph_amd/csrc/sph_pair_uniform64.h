@@ -356,7 +356,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform64_pec(const BInst *i
                 ax = P.ax[t];
                 ay = P.ay[t];
             }
-            v.add(C.s.h[t], ax, ay);
+            v.add(I.uh_on ? I.uh : C.s.h[t], ax, ay);
         }
     }
     v.store(C.part, NC, c, lds);
