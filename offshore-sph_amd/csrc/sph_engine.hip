@@ -526,12 +526,12 @@ __global__ void k_records(const Ctrl *ctrl, State st, Phys ph, int n, int fmt, v
 }
 
 // min and max h over all particles, min and max m over the fluid and over
-// all particles, in out[0..5] (one workgroup; upload, restore and scatter
-// only).
-__global__ void k_hm_range(State st, int n, double *out) {
+// all particles (upload, restore and scatter only): per-workgroup partials
+// part[q * gridDim.x + b] (grid-stride), folded by k_hm_final into out[0..5].
+__global__ void k_hm_range(State st, int n, double *part) {
     __shared__ double lds[6 * 16];
     double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double h = st.h[i], m = st.m[i];
         v[0] = h < v[0] ? h : v[0];
         v[1] = h > v[1] ? h : v[1];
@@ -543,6 +543,20 @@ __global__ void k_hm_range(State st, int n, double *out) {
         v[5] = m > v[5] ? m : v[5];
     }
     const int kind[6] = {0, 1, 0, 1, 0, 1};
+    block_partials<6>(v, kind, lds, part, gridDim.x, blockIdx.x);
+}
+
+// The R partials of k_hm_range into out[0..5]. One workgroup.
+__global__ void k_hm_final(const double *part, int R, double *out) {
+    __shared__ double lds[6 * 16];
+    const int kind[6] = {0, 1, 0, 1, 0, 1};
+    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int k = threadIdx.x; k < R; k += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double u = part[q * R + k];
+            v[q] = kind[q] == 0 ? (u < v[q] ? u : v[q]) : (u > v[q] ? u : v[q]);
+        }
     block_fold<6>(v, kind, lds);
     if (threadIdx.x == 0)
         for (int q = 0; q < 6; ++q) out[q] = v[q];
@@ -1223,7 +1237,10 @@ static int refresh_uniform(sph_ctx *c, hipStream_t s) {
     const int before = rec_format(c);
     const bool static_before = c->static_hm;
     const double h_before = c->uni_h, m_before = c->uni_m;
-    k_hm_range<<<1, 1024, 0, s>>>(c->buf[c->cur], c->n, c->part_k);
+    // partials in the bounds buffer (free between updates), result in part_k
+    const int R = redblk(c->n) < kBoundsBlocks ? redblk(c->n) : kBoundsBlocks;
+    k_hm_range<<<R, kBlock, 0, s>>>(c->buf[c->cur], c->n, c->part_b);
+    k_hm_final<<<1, 1024, 0, s>>>(c->part_b, R, c->part_k);
     HIPCHK(hipGetLastError());
     double v[6];
     HIPCHK(hipMemcpyAsync(v, c->part_k, sizeof v, hipMemcpyDeviceToHost, s));

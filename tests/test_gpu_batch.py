@@ -79,8 +79,12 @@ def test_batch_instances_match_single_engines(precision, graph, persistent, unif
         assert wform == form
         assert st[k]['steps'] == wst['steps'] == K
         assert st[k]['t'] == wst['t'] and st[k]['dt'] == wst['dt']
-        for fld in FIELDS:
+        for fld in FIELDS + ('p',):
             np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
+        # m, h and label never change: the download returns the uploaded ones
+        # in reference order (the uniform form keeps them where the upload put them)
+        for fld in ('m', 'h', 'label'):
+            np.testing.assert_array_equal(got[k][fld], states[k][fld].astype(got[k][fld].dtype), err_msg=f'{k} {fld}')
 
 
 def test_batch_duration_per_instance():
