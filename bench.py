@@ -412,7 +412,7 @@ def _dispatches(rows):
 
 
 def _short(name):
-    base = name.replace('hipsph::(anonymous namespace)::', '').replace('void ', '')
+    base = name.replace('(anonymous namespace)::', '').replace('hipsph::', '').replace('void ', '')
     return base.split('(')[0].strip()
 
 
@@ -631,6 +631,66 @@ def write_summaries(traffic, valu, phases, build, n=None, nf=None, cells=None):
         f.write('\n'.join(L) + '\n')
 
 
+STATS_RUNS = {  # kernel-trace runs summarised into profiles/ (directory, command, file, title)
+    'kt_stress': ('--stress 20', 'stress_4M_kernel_stats.md',
+                  'Kernel statistics — pair-kernel stress lattice (4,024,030 particles)'),
+    'kt_ens': ('--ensemble', 'ensemble_256_dam_breaks_kernel_stats.md',
+               'Kernel statistics — batched ensemble of 256 dam breaks (7,677,758 particles)'),
+    'kt_adj': ('--adjoint', 'adjoint_dam_break_20k_kernel_stats.md',
+               'Kernel statistics — forward + adjoint, differentiable dam break (23,021 particles)'),
+}
+STATS_NOTES = {
+    'kt_stress': 'Workload: 2000 x 2000 jittered fluid particles at dx = 1e-3 m (h = 1.3 dx, velocities '
+                 'N(0, (0.05 c0)^2)) in a closed box of 3 wall layers: 595,984 cells, 168,747,172 neighbour '
+                 'pairs. Per kernel (cubic, Gaussian) and precision, 20 launches each of the neighbour update, '
+                 'the continuity pass, the momentum pass and the fused pass (the cubic spline takes the uniform '
+                 'passes `k_pair_uniform*`; the split passes and the Gaussian the general `k_pair<...>`, whose '
+                 'last template argument is the mode: 0 fused, 1 continuity, 2 momentum).',
+    'kt_ens': 'Workload: 256 dam breaks (column width U(0.5, 1.5) m, height U(0.5, 2.5) m, dx from '
+              '{0.005, 0.01, 0.02} m, alpha U(0.005, 0.05), 4.0 x 3.0 m tank), 961 to 136,948 fluid particles '
+              'each, FP32C pairs, FP64 integrator, PEC, per-instance dt; every stage one launch over all '
+              'instances, the step replayed as a hipGraph. Every instance has one h and one fluid mass, so the '
+              'pair pass is the uniform one with the corrector fused in (`k_pair_batch_uniform_pec`, one '
+              'workgroup per 256-slot tile of one instance); the autotuner times it against the persistent '
+              'work-queue schedule (`k_pair_batch_uniform_queue` + `kb_correct`).',
+    'kt_adj': 'Workload: 0.5 m x 1.0 m column at dx = 0.005 m (20,000 fluid, 3,021 wall particles), cubic, PEC, '
+              'dynamic dt, FP64, 200 steps, checkpoints every 20; backward per segment: restore, recompute '
+              'while recording (the fused pair + corrector pass, then `k_tape_pair`), reverse '
+              '(`k_adj_correct`, `k_adj_pair` with the predictor\'s reverse folded in, `k_adj_final`). The run: '
+              'one target forward, one warm-up forward + backward, three timed pairs.',
+}
+
+
+def write_stats(raw, build):
+    """profiles/*_kernel_stats.md from the kernel-trace runs under raw (see
+    STATS_RUNS), each with the JSON line the run printed."""
+    for d, (args, fname, title) in STATS_RUNS.items():
+        if not os.path.isdir(os.path.join(raw, d)):
+            continue
+        rows = _csv(os.path.join(raw, d), '_kernel_stats.csv')
+        line = None
+        logp = os.path.join(raw, d + '.log')
+        if os.path.exists(logp):
+            for ln in open(logp):
+                if ln.startswith('{'):
+                    line = ln.strip()
+        L = [f'# {title}', '', STATS_NOTES[d], '',
+             f'Library build `{build}`. MI355X, ROCm 7.2. Command (R: the repository):', '',
+             '    cd /tmp && export TMPDIR=/tmp',
+             f'    rocprofv3 --kernel-trace --stats -f csv -d OUT/{d} -o kt -- python3 -u R/bench.py {args} '
+             f'> OUT/{d}.log 2>&1',
+             f'    python3 R/bench.py --summarize OUT', '']
+        if line:
+            L += ['The run\'s own result line (timings under the profiler):', '', '```', line[:3000], '```', '']
+        L += ['| kernel | calls | avg us | min us | max us | total ms | % |', '|---|---|---|---|---|---|---|']
+        for r in rows[:28]:
+            L.append(f'| `{_short(r["Name"])[:70]}` | {r["Calls"]} | {float(r["AverageNs"]) / 1e3:.2f} | '
+                     f'{float(r["MinNs"]) / 1e3:.2f} | {float(r["MaxNs"]) / 1e3:.2f} | '
+                     f'{float(r["TotalDurationNs"]) / 1e6:.2f} | {float(r["Percentage"]):.2f} |')
+        with open(os.path.join(ROOT, 'profiles', fname), 'w') as f:
+            f.write('\n'.join(L) + '\n')
+
+
 def _profile(name):
     p = os.path.join(ROOT, 'profiles', name)
     if not os.path.exists(p):
@@ -687,7 +747,9 @@ def main():
                     'directories under DIR (no GPU)')
     a = ap.parse_args()
     if a.summarize:
-        write_summaries(*summarize(a.summarize))
+        res = summarize(a.summarize)
+        write_summaries(*res)
+        write_stats(a.summarize, res[3])
         return
     if a.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py: this build runs on one GPU; the multi-GPU (slab-decomposed) path is not built. '
