@@ -7,8 +7,9 @@ Bars (stated per test):
   same order, no contraction on either side); Gaussian: within 1e-12 of the
   largest magnitude (its exp comes from a different math library);
 * FP32C pair pass: within 2e-5 of the largest magnitude per field;
-* multi-step trajectories: FP64 cubic bit-exact after 1,000 steps (C1) and
-  100 steps (C2, 500k), Wendland after 200.
+* multi-step trajectories: FP64 cubic bit-exact after 1,000 steps (C1),
+  100 steps (C2, 500k) and 20 steps of the 1M headline workload (with FP32C
+  pairs there within 1e-7 relative density), Wendland after 200.
 """
 import json
 import os
@@ -304,6 +305,44 @@ def test_step_c2_500k_fp64_bitexact_100():
     assert st['t'] == s.t
     for k in FIELDS:
         np.testing.assert_array_equal(out[k], s.pA[k], err_msg=k)
+
+
+def test_step_c2_1M_headline_vs_oracle():
+    """The headline workload itself (1M dam break: 2.0 x 2.5 column at dx
+    0.002, 1,265,021 particles), 20 steps replayed from the graph, both pair
+    precisions against the oracle (its pair loop on 16 host threads):
+    FP64 every field and the dt series bit-identical; FP32C pairs (FP64
+    integrator) within 1e-7 relative density, 1e-6 dx in position and 1e-7
+    c0 in velocity over the fluid, walls identical."""
+    sc = scenarios.dam_break_c2(million=True)
+    m = sc.method()
+    steps = 20
+    s = oracle.Solver(sc.pA.copy(), m, 'cubic', 'pec', h=sc.h, nthreads=16)  # it steps in place
+    rhist = s.run(steps)
+    say('c2 1M oracle done: n', sc.n)
+    f = sc.pA['label'] == 0
+    for prec in ('fp64', 'fp32c'):
+        e = make_engine(sc.pA, m, precision=prec)
+        assert e.pair_form() == prec + '-uniform'
+        e.step(steps, graph=True)
+        st = e.status()
+        out = e.download()
+        hist = e.dt_history(steps)
+        e.close()
+        assert st['steps'] == steps and st['flags'] & 3 == 0
+        if prec == 'fp64':
+            np.testing.assert_array_equal(hist, rhist)
+            assert st['t'] == s.t
+            for k in FIELDS:
+                np.testing.assert_array_equal(out[k], s.pA[k], err_msg=k)
+        else:
+            e_rho = float(np.max(np.abs(out['rho'][f] - s.pA['rho'][f]) / s.pA['rho'][f]))
+            e_x = float(np.max(np.hypot(out['x'][f] - s.pA['x'][f], out['y'][f] - s.pA['y'][f]))) / sc.dx
+            e_v = float(np.max(np.hypot(out['vx'][f] - s.pA['vx'][f], out['vy'][f] - s.pA['vy'][f]))) / m.co
+            say('c2 1M fp32c vs oracle: rho', e_rho, 'x/dx', e_x, 'v/c0', e_v)
+            assert e_rho <= 1e-7 and e_x <= 1e-6 and e_v <= 1e-7
+            for k in ('x', 'y', 'vx', 'vy', 'rho'):
+                np.testing.assert_array_equal(out[k][~f], s.pA[k][~f], err_msg=k)
 
 
 def test_step_wendland_fp64_bitexact():
