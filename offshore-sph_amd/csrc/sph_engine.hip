@@ -87,14 +87,24 @@ __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *h
     const int advance_t = ctrl->advance_t;
     const int kind[kDtQ] = {0, 1, 2, 2};
     double v[kDtQ] = {INFINITY, -INFINITY, 0.0, 0.0};
-    for (int k = threadIdx.x; k < R; k += blockDim.x) {
-        double u[kDtQ];
+    // four partials per thread per round, their loads issued together (the
+    // fused pair pass leaves one per workgroup: ~5,000 at 1M particles)
+    constexpr int U = 4;
+    for (int k0 = threadIdx.x; k0 < R; k0 += U * blockDim.x) {
+        double u[U][kDtQ];
 #pragma unroll
-        for (int q = 0; q < kDtQ; ++q) u[q] = part[q * R + k];
-        v[0] = u[0] < v[0] ? u[0] : v[0];
-        v[1] = u[1] > v[1] ? u[1] : v[1];
-        v[2] += u[2];
-        v[3] += u[3];
+        for (int j = 0; j < U; ++j) {
+            const int k = k0 + j * blockDim.x;
+#pragma unroll
+            for (int q = 0; q < kDtQ; ++q) u[j][q] = k < R ? part[q * R + k] : (q == 0 ? INFINITY : (q == 1 ? -INFINITY : 0.0));
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            v[0] = u[j][0] < v[0] ? u[j][0] : v[0];
+            v[1] = u[j][1] > v[1] ? u[j][1] : v[1];
+            v[2] += u[j][2];
+            v[3] += u[j][3];
+        }
     }
     block_fold<kDtQ>(v, kind, lds);
     if (threadIdx.x != 0) return;

@@ -42,10 +42,11 @@ namespace {
 #ifndef HIPSPH_UNIFORM_UNROLL
 #define HIPSPH_UNIFORM_UNROLL 4
 #endif
-// Contributor-list entries per thread (28 KB of LDS per workgroup); a list is
-// flushed before a group of UNROLL candidates would overflow it.
+// Contributor-list entries per thread (24 KB of LDS per workgroup: 6
+// workgroups per CU); a list is flushed before a group of UNROLL candidates
+// would overflow it.
 #ifndef HIPSPH_UNIFORM_LIST
-#define HIPSPH_UNIFORM_LIST 28
+#define HIPSPH_UNIFORM_LIST 24
 #endif
 constexpr int kUniformList = HIPSPH_UNIFORM_LIST;
 
