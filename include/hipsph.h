@@ -254,7 +254,9 @@ int sph_dt_history(sph_ctx_t *ctx, double *out3, int64_t count, void *stream);
  * *g_alpha and *g_co. Steps are reversed in the opposite order of recording;
  * checkpointing (re-recording a segment from a saved state) is up to the
  * caller. All arrays are device double[n] in reference index order; entries
- * of non-fluid particles are ignored and set to 0. PEC (strict = 0) and Euler.
+ * of non-fluid particles are ignored and set to 0. PEC and Euler; with PEC's
+ * strict clamp a density the clamp set to 0 passes no adjoint back (the
+ * derivative of max(rho, 0) on the clamped side).
  * dt is differentiated on both branches of min(courant, force)
  * (src/Equations/TimeStep.py:40-56): the Courant value through c0, the force
  * value through the carried acceleration of the particle attaining max |a|^2

@@ -320,6 +320,17 @@ __global__ void k_tape_pair(const Ctrl *ctrl, State s, State o, const int *perm,
                 vy = s.vy[t];
                 rho = s.rho[t];
             }
+            // PEC's strict clamp (src/Integrators/PEC.py:57-58, 87-88): whether it
+            // took the predicted or the corrected density (the adjoint of a
+            // clamped density is zero)
+            int8_t cl = 0;
+            if (strict && lab == 0) {
+                const double dt = ctrl->dt, r0 = o.rho[q];
+                const double rp = r0 + 0.5 * dt * o.drho[q];
+                const double rc = 2 * (r0 + 0.5 * dt * s.drho[t]) - r0;
+                cl = (int8_t)((rp < 0.0 ? 1 : 0) | (rc < 0.0 ? 2 : 0));
+            }
+            T.clamp[i] = cl;
             T.vxh[i] = vx;
             T.vyh[i] = vy;
             T.ax[i] = s.ax[t];
@@ -389,14 +400,19 @@ __global__ void k_adj_correct(const TapeHdr *hdr, TapeSlot T, const int8_t *labe
             const double vx = T.vx0[i], vy = T.vy0[i], ax = T.ax[i], ay = T.ay[i];
             v[0] += xb * (vx + dt * ax) + yb * (vy + dt * ay) + vxb * ax + vyb * ay + rb * T.drho[i];
         } else {
-            lk[T.slot[i]] = make_double4(A.ax[i] + dt / e * vxb, A.ay[i] + dt / e * vyb, A.drho[i] + dt * rb, 0.0);
+            // a density the strict clamp set to 0 passes no adjoint back
+            const bool cc = (T.clamp[i] & 2) != 0;
+            const double rbe = cc ? 0.0 : rb;
+            lk[T.slot[i]] = make_double4(A.ax[i] + dt / e * vxb, A.ay[i] + dt / e * vyb, A.drho[i] + dt * rbe, 0.0);
             vbx[i] = dt * xb;
             vby[i] = dt * yb;
             A.vx[i] = (2.0 / e - 1.0) * vxb;
             A.vy[i] = (2.0 / e - 1.0) * vyb;
-            v[0] += xb * T.vxh[i] + yb * T.vyh[i] + (vxb * T.ax[i] + vyb * T.ay[i]) / e + rb * T.drho[i];
+            v[0] += xb * T.vxh[i] + yb * T.vyh[i] + (vxb * T.ax[i] + vyb * T.ay[i]) / e + rbe * T.drho[i];
+            if (cc) A.rho[i] = 0.0;
         }
-        // x and rho pass through with coefficient 1 (A.x, A.y, A.rho unchanged)
+        // x and rho pass through with coefficient 1 (A.x, A.y, A.rho unchanged,
+        // but for a clamped density)
     }
     block_partials<kAdjQ>(v, part, gridDim.x);
 }
@@ -480,7 +496,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_pair(const TapeHdr *hdr, Tape
         for (int q = 0; q < 5; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
     if (live && l == 0 && ak.w >= 0.0) {
         const double dt = hdr->dt, e = 1.0 + 0.5 * damping;
-        const double xhb = acc[0], yhb = acc[1], rhb = acc[4];
+        // a predicted density the strict clamp set to 0 passes no adjoint back
+        const double xhb = acc[0], yhb = acc[1], rhb = (T.clamp[k] & 1) ? 0.0 : acc[4];
         const double vhx = vbx[k] + acc[2], vhy = vby[k] + acc[3];
         if (euler) {
             A.x[k] += xhb;
@@ -574,7 +591,8 @@ int hipsph::tape_record_pair(sph_ctx *c, sph_tape *t, int slot, hipStream_t s, b
     const int R = nblk_(c->n, kAdjBlock);
     k_tape_pair<<<R, kAdjBlock, 0, s>>>(c->ctrl, c->buf[c->cur], c->buf[1 - c->cur], c->perm, c->cxy_s,
                                         c->cellStart, c->n, t->slot[slot], t->hdr + slot, t->cells,
-                                        make_phys(c->P), t->amax_v, t->amax_i, lean, c->P.strict);
+                                        make_phys(c->P), t->amax_v, t->amax_i, lean,
+                                        c->P.strict && c->P.integrator == SPH_INTEGRATOR_PEC);
     k_tape_amax_final<<<1, 1024, 0, s>>>(t->amax_v, t->amax_i, R, t->hdr + slot);
     HIPCHK(hipGetLastError());
     return 0;
@@ -615,9 +633,9 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
     t->ctx = ctx;
     t->nslots = nslots;
     t->cells = ctx->cap_cells + 2;
-    // per slot: 10 double arrays, 3 double4 arrays, slot, id, cxy, cellStart
+    // per slot: 10 double arrays, 3 double4 arrays, slot, clamp, id, cxy, cellStart
     // (each carved array is rounded up to 256 bytes: 20 per slot, 20 fixed)
-    const size_t per = n * (10 * sizeof(double) + 3 * sizeof(double4) + 2 * sizeof(int) + sizeof(int2)) +
+    const size_t per = n * (10 * sizeof(double) + 3 * sizeof(double4) + 2 * sizeof(int) + sizeof(int2) + 1) +
                        (size_t)t->cells * sizeof(int) + 20 * 256;
     const size_t npart = (size_t)kAdjQ * (kRedBlocks + nblk_((long long)ctx->n * kAdjLanes, kAdjBlock));
     const size_t fixed = n + n * 2 * sizeof(double) + n * sizeof(double4) + npart * sizeof(double) +
@@ -653,6 +671,7 @@ int sph_tape_create(sph_ctx_t *ctx, int32_t nslots, sph_tape_t **out) {
         double4 **f4[] = {&S.pa, &S.pb, &S.ps};
         for (double4 **d : f4) *d = (double4 *)take(n * sizeof(double4));
         S.slot = (int *)take(n * sizeof(int));
+        S.clamp = (int8_t *)take(n);
         S.id = (int *)take(n * sizeof(int));
         S.cxy = (int2 *)take(n * sizeof(int2));
         S.cellStart = (int *)take((size_t)t->cells * sizeof(int));
@@ -705,7 +724,6 @@ int sph_adjoint_step(sph_ctx_t *ctx, sph_tape_t *tape, int32_t slot, const sph_a
     if (!adj->x || !adj->y || !adj->vx || !adj->vy || !adj->rho || !adj->ax || !adj->ay || !adj->drho ||
         !adj->g_alpha || !adj->g_co)
         return fail(SPH_EINVAL, "every sph_adjoint_t pointer is required");
-    if (ctx->P.strict) return fail(SPH_EUNSUPPORTED, "the adjoint of the strict density clamp is not implemented");
     return adjoint_launch(ctx, tape, slot, adj, (hipStream_t)stream);
 }
 

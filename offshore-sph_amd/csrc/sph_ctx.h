@@ -70,6 +70,7 @@ struct TapeSlot {
     double *vx0, *vy0;          // velocity at the start of the step
     double *axp, *ayp, *drhop;  // rates carried into the step (previous pair pass)
     int *slot;                  // id -> cell slot
+    int8_t *clamp;              // PEC's strict clamp took the predicted (bit 0) / corrected (bit 1) density
     // cell-slot order (the step's walk order, read by the pair adjoint)
     double4 *pa;     // (x, y, h, w) of the predicted state; w = m for fluid, -label otherwise
     double4 *pb;     // (vx, vy, rho, m) of the predicted state

@@ -1521,7 +1521,6 @@ int sph_adjoint_segment(sph_ctx_t *ctx, sph_tape_t *tape, int32_t nsteps, const 
     if (!adj->x || !adj->y || !adj->vx || !adj->vy || !adj->rho || !adj->ax || !adj->ay || !adj->drho ||
         !adj->g_alpha || !adj->g_co)
         return fail(SPH_EINVAL, "every sph_adjoint_t pointer is required");
-    if (ctx->P.strict) return fail(SPH_EUNSUPPORTED, "the adjoint of the strict density clamp is not implemented");
     int rc = tape_supported(ctx);
     if (rc) return rc;
     if (tape->gen != ctx->gen) {  // constants or launch shape changed since the capture: every graph is stale

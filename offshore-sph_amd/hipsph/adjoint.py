@@ -86,7 +86,7 @@ class SPHRun:
     """nsteps device steps from the particle records ``pA`` (active set)."""
 
     def __init__(self, pA, method, nsteps, checkpoint_every=20, kernel='cubic', precision='fp64',
-                 integrator='pec', device='cuda:0', fixed_dt=None):
+                 integrator='pec', device='cuda:0', fixed_dt=None, strict=False):
         import torch
         self.pA = np.ascontiguousarray(pA).copy()
         self.n = len(self.pA)
@@ -94,7 +94,8 @@ class SPHRun:
         self.nsteps = int(nsteps)
         self.k = max(1, int(checkpoint_every))
         self.params = EngineParams.from_method(self.n, method, kernel=kernel, precision=precision,
-                                               integrator=integrator, fixed_dt=float(fixed_dt or 0.0))
+                                               integrator=integrator, fixed_dt=float(fixed_dt or 0.0),
+                                               strict=bool(strict))
         self.engine = Engine(self.params, device=device)
         self.device = self.engine.device
         fl = self.pA['label'] == 0

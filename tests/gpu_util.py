@@ -43,3 +43,21 @@ def rel_err(got, want, mask=None):
         got, want = got[mask], want[mask]
     scale = np.max(np.abs(want))
     return float(np.max(np.abs(got - want)) / scale) if scale > 0 else float(np.max(np.abs(got)))
+
+
+def diverging(sc, centres, speed=5.0, rho_low=0.2):
+    """Fluid around each centre moves radially outwards at `speed` and the
+    centre particle has a density of `rho_low`: continuity then gives it
+    dt * drho/dt far below -rho_low in the first step."""
+    pA = sc.pA.copy()
+    f = np.nonzero(pA['label'] == 0)[0]
+    for c in centres:
+        d = np.hypot(pA['x'][f] - pA['x'][c], pA['y'][f] - pA['y'][c])
+        near = f[(d > 0) & (d < 3 * sc.h)]
+        rx = pA['x'][near] - pA['x'][c]
+        ry = pA['y'][near] - pA['y'][c]
+        r = np.hypot(rx, ry)
+        pA['vx'][near] = speed * rx / r
+        pA['vy'][near] = speed * ry / r
+        pA['rho'][c] = rho_low
+    return pA

@@ -6,6 +6,8 @@ Checks, all FP64 unless stated:
     velocities, positions and densities, against central finite differences
     of the device forward, across checkpoint segments;
   * the checkpoint interval does not change the gradient (bit-identical);
+  * PEC's strict clamp: bit-identical to strict=False where it never acts,
+    and against central differences on a step where it clamps densities;
   * the 2,000-particle differentiable dam break (200 steps) against the
     gradient of the FP64 dense-pair torch model (tests/torch_ref.py, itself
     tied to the CPU oracle), whose final state also matches the device's;
@@ -95,6 +97,74 @@ def test_adjoint_directional_derivatives_vs_fd():
         say('initial', k, an, fd)
         assert _rel(an, fd) < 1e-4, k
     run.close()
+
+
+def test_adjoint_strict_clamp():
+    """PEC(strict=True), the reference's default integrator setting.
+    (1) No density near zero: the clamp never acts, and the gradient (and the
+    adjoint of every initial field) is bit-identical to strict=False.
+    (2) One step whose corrector clamps two densities to exactly zero (fluid
+    streaming away from two low-density centres): the gradient of a linear
+    functional of the final densities in c0 and the initial velocities
+    against central differences (to 1e-5), with the clamp and, on the same
+    state, without it (strict=False, where those densities come out
+    negative). A clamped density is 0 whatever the inputs, so it passes no
+    adjoint back; the two cases' c0 derivatives differ 30-fold (the centres'
+    dt drho/dt term through the Courant dt), so the check sees the clamp."""
+    from gpu_util import diverging
+    sc = scenarios.dam_break(0.4, 0.5, 0.02, tank_w=1.0, tank_h=0.8, name='adj_small')
+    m = sc.method()
+    rng = np.random.default_rng(3)
+    f = sc.pA['label'] == 0
+    v0 = torch.as_tensor(rng.normal(0, 0.3, (int(f.sum()), 2)), device='cuda')
+    got = []
+    for strict in (False, True):
+        run = SPHRun(sc.pA, m, nsteps=7, checkpoint_every=3, strict=strict)
+        w = _linear_loss_setup(run, 5)
+        run.forward(m.alpha, m.co, v0)
+        g_xy = torch.stack((w['x'][run.fluid], w['y'][run.fluid]), 1)
+        ga, gc, gv = run.backward(g_xy, {'vx': w['vx'], 'vy': w['vy'], 'rho': w['rho']})
+        got.append((float(ga), float(gc), gv.cpu().numpy(), {k: v.cpu().numpy() for k, v in run.last_adjoint.items()}))
+        run.close()
+    assert got[0][0] == got[1][0] and got[0][1] == got[1][1]
+    np.testing.assert_array_equal(got[0][2], got[1][2])
+    for k in got[0][3]:
+        np.testing.assert_array_equal(got[0][3][k], got[1][3][k], err_msg=k)
+
+    fl = np.nonzero(f)[0]
+    centres = [int(fl[len(fl) // 3]), int(fl[2 * len(fl) // 3])]
+    pA = diverging(sc, centres)
+    v0 = torch.as_tensor(np.stack((pA['vx'][f], pA['vy'][f]), 1), device='cuda')
+    alpha, co = m.alpha, m.co
+    dv = torch.as_tensor(rng.normal(0, 1, tuple(v0.shape)), device='cuda')
+    res = {}
+    for strict in (True, False):
+        run = SPHRun(pA, m, nsteps=1, checkpoint_every=1, strict=strict)
+        # the final densities, the centres' weighted like the rest
+        w = _linear_loss_setup(run, 7)
+        w = {'rho': w['rho']}
+        run.forward(alpha, co, v0)
+        fin = run.engine.download()
+        clamped = bool(np.all(fin['rho'][centres] == 0.0))
+        ga, gc, gv = run.backward(torch.zeros((run.n_fluid, 2), dtype=torch.float64, device='cuda'), w)
+
+        def L(c=co, v=v0):
+            return _linear_loss(run, w, alpha, c, v)
+
+        eps = 1e-6
+        fd_c = (L(c=co * (1 + eps)) - L(c=co * (1 - eps))) / (2 * eps * co)
+        fd_v = (L(v=v0 + eps * dv) - L(v=v0 - eps * dv)) / (2 * eps)
+        an_v = float(torch.sum(gv * dv))
+        say('strict', strict, 'clamped', clamped, 'c0', float(gc), fd_c, 'v0', an_v, fd_v, 'alpha', float(ga))
+        assert clamped == strict  # the corrector's clamp took the centres
+        assert float(ga) == 0.0  # alpha does not reach a density in one step
+        assert _rel(float(gc), fd_c) < 1e-5
+        assert _rel(an_v, fd_v) < 1e-5
+        res[strict] = (float(gc), an_v)
+        run.close()
+    # the clamp matters here: without it the centres' dt drho/dt terms
+    # change both derivatives
+    assert _rel(res[True][0], res[False][0]) > 1.0 and _rel(res[True][1], res[False][1]) > 1e-3
 
 
 def _target(sc, nsteps, fixed_dt=None):

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from gpu_util import FIELDS, make_engine, oracle_loop, perturbed, rel_err, say
+from gpu_util import FIELDS, diverging as _diverging, make_engine, oracle_loop, perturbed, rel_err, say
 from hipsph import scenarios
 from hipsph._lib import SPH_FLAG_ZERO_DENSITY
 
@@ -65,24 +65,6 @@ def test_quadratic_viscosity_beta_fp64_bitexact():
         np.testing.assert_array_equal(out[k], s.pA[k], err_msg=k)
     base, _, _, _, _ = _steps_both(pA, sc.method(), 200, sc.h)
     assert not np.array_equal(base['ax'], out['ax'])
-
-
-def _diverging(sc, centres, speed=5.0, rho_low=0.2):
-    """Fluid around each centre moves radially outwards at `speed` and the
-    centre particle has a density of `rho_low`: continuity then gives it
-    dt * drho/dt far below -rho_low in the first step."""
-    pA = sc.pA.copy()
-    f = np.nonzero(pA['label'] == 0)[0]
-    for c in centres:
-        d = np.hypot(pA['x'][f] - pA['x'][c], pA['y'][f] - pA['y'][c])
-        near = f[(d > 0) & (d < 3 * sc.h)]
-        rx = pA['x'][near] - pA['x'][c]
-        ry = pA['y'][near] - pA['y'][c]
-        r = np.hypot(rx, ry)
-        pA['vx'][near] = speed * rx / r
-        pA['vy'][near] = speed * ry / r
-        pA['rho'][c] = rho_low
-    return pA
 
 
 def test_pec_strict_clamp_fp64_vs_oracle():
