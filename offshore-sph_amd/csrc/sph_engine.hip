@@ -1318,7 +1318,8 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
 // _loop (src/Tools/SolverTools.py:120-175): the summation density first when
 // it is on (then the pass writes no continuity), the pair pass with the XSPH
 // velocity when XSPH is on. The split passes (mode 1, 2) are the plain
-// equations only.
+// equations only; in the uniform form they take the uniform pass of their
+// equations.
 static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr, int mode = 0) {
     const bool sd = c->P.use_summation_density, xs = c->P.use_xsph;
     if (mode != 0 && (sd || xs))
@@ -1338,7 +1339,7 @@ static int compute(sph_ctx *c, hipStream_t s, int *count = nullptr, int mode = 0
     const bool f32 = c->P.precision == SPH_PREC_FP32C;
     const Phys ph = make_phys(c->P);
     const bool uni_fmt = c->rec_fmt == kRecUniform || c->rec_fmt == kRecUniform64;
-    const bool uni = mode == 0 && !count && uni_fmt && rec_format(c) == c->rec_fmt;
+    const bool uni = !count && uni_fmt && rec_format(c) == c->rec_fmt;
     if (!uni && uni_fmt) {  // a pass that reads the general records
         const int fmt = f32 ? kRecFp32 : kRecFp64;
         k_records<<<nblk(c->n), kBlock, 0, s>>>(c->ctrl, st, ph, c->n, fmt, c->recA, c->recB);

@@ -106,6 +106,10 @@ struct PecArgs {
     int strict;
 };
 
+// The equations of a pair pass: all of them (continuity, momentum, wall
+// force, gravity), the continuity equation alone (drho) or the rest (ax, ay).
+constexpr int kFused = 0, kContinuity = 1, kMomentum = 2;
+
 // Record formats written by the reorder: the double pass's, the general
 // float pass's, and the float pass's for one h and one fluid mass
 // (A = double2 (x, y), B = float4 (vx, vy, rho, p / rho^2), p / rho^2 NaN for

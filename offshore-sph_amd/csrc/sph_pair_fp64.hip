@@ -64,10 +64,16 @@ hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
     Uniform64Const K;
     uniform_const_fp64(u, &K);
     const int nb = xcd_blocks(n, 256);
-    if (kernel == SPH_KERNEL_WENDLAND)
-        k_pair_uniform64<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K);
+    const bool w = kernel == SPH_KERNEL_WENDLAND;
+    if (out.mode == kContinuity)
+        w ? k_pair_uniform64<SPH_KERNEL_WENDLAND, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K)
+          : k_pair_uniform64<SPH_KERNEL_CUBIC, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K);
+    else if (out.mode == kMomentum)
+        w ? k_pair_uniform64<SPH_KERNEL_WENDLAND, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K)
+          : k_pair_uniform64<SPH_KERNEL_CUBIC, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K);
     else
-        k_pair_uniform64<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K);
+        w ? k_pair_uniform64<SPH_KERNEL_WENDLAND, kFused><<<nb, 256, 0, s>>>(ctrl, P, K)
+          : k_pair_uniform64<SPH_KERNEL_CUBIC, kFused><<<nb, 256, 0, s>>>(ctrl, P, K);
     return hipGetLastError();
 }
 

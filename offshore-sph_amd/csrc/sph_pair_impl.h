@@ -53,7 +53,6 @@ __host__ __device__ constexpr int list_len() {
     return (sizeof(R) == 4 && KERNEL != SPH_KERNEL_GAUSSIAN) ? 24 : kListLen;
 }
 constexpr double kPI = 3.141592653589793;
-constexpr int kFused = 0, kContinuity = 1, kMomentum = 2;
 
 __device__ inline double rec_vx(const Rec<double>::A &, const Rec<double>::B &b) { return b.vx; }
 __device__ inline double rec_vy(const Rec<double>::A &, const Rec<double>::B &b) { return b.vy; }

@@ -42,13 +42,21 @@ namespace {
 #ifndef HIPSPH_UNIFORM_UNROLL
 #define HIPSPH_UNIFORM_UNROLL 4
 #endif
-// Contributor-list entries per thread (24 KB of LDS per workgroup: 6
-// workgroups per CU); a list is flushed before a group of UNROLL candidates
-// would overflow it.
+// Contributor-list entries per thread; a list is flushed before a group of
+// UNROLL candidates would overflow it, so a thread with more than LIST -
+// UNROLL candidates flushes mid-walk while the rest of its wave waits.
+// 26 (26 KB of LDS per workgroup, 6 workgroups per CU) for the rate passes:
+// on a jittered lattice (18-25 candidates) 24 costs 8 % more. 24 for the
+// passes with the corrector fused in (their reduction adds 512 B): 1 % less
+// on a regular lattice (20 candidates), measured on the 1M dam break.
 #ifndef HIPSPH_UNIFORM_LIST
-#define HIPSPH_UNIFORM_LIST 24
+#define HIPSPH_UNIFORM_LIST 26
+#endif
+#ifndef HIPSPH_UNIFORM_PEC_LIST
+#define HIPSPH_UNIFORM_PEC_LIST 24
 #endif
 constexpr int kUniformList = HIPSPH_UNIFORM_LIST;
+constexpr int kUniformPecList = HIPSPH_UNIFORM_PEC_LIST;
 
 struct UniformArgs {
     const double2 *A;  // (x, y)
@@ -149,8 +157,9 @@ __device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K,
 // The rates of fluid target t (bt: its B record) over a grid of ncx x ncy
 // cells starting at bin `base` (0 for a single simulation, the instance's
 // first bin in a batch); false when t has no neighbour and keeps its
-// previous rates.
-template <int KERNEL>
+// previous rates. MODE kContinuity sums drho only, kMomentum ax and ay only
+// (the other outputs are then zero).
+template <int KERNEL, int MODE = kFused, int LIST = kUniformList>
 __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
                                      int t, const float4 &bt, unsigned *s_list, double &drho, double &ax, double &ay) {
     const double2 at = P.A[t];
@@ -168,15 +177,17 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
             const double2 a = at32(P.A, u);
             const float4 b = at32(P.B, u);
             const float3 f = uniform_terms<KERNEL>(K, (float)(X - a.x), (float)(Y - a.y), bt.x, bt.y, bt.z, bt.w, b);
-            srho += (double)f.x;
-            sax += (double)f.y;
-            say += (double)f.z;
+            if constexpr (MODE != kMomentum) srho += (double)f.x;
+            if constexpr (MODE != kContinuity) {
+                sax += (double)f.y;
+                say += (double)f.z;
+            }
         }
         lp = list0;
     };
     constexpr int U = HIPSPH_UNIFORM_UNROLL;
-    static_assert(U >= 1 && U < kUniformList, "unroll within the list");
-    unsigned *const full = list0 + (kUniformList - U) * stride;  // past this, U appends could overflow
+    static_assert(U >= 1 && U < LIST, "unroll within the list");
+    unsigned *const full = list0 + (LIST - U) * stride;  // past this, U appends could overflow
     auto test = [&](double2 a, int k) {
         const double dx = X - a.x, dy = Y - a.y;
         const double r2 = dx * dx + dy * dy;
@@ -209,32 +220,34 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
     if (!inside && uniform_count(P, K, X, Y, c, ncx, ncy, base) == 0) return false;  // SolverTools.py:166-169
     drho = srho;
     ax = sax;
-    ay = say - K.g;  // WCSPH.py:169
+    ay = MODE == kContinuity ? 0.0 : say - K.g;  // WCSPH.py:169
     return true;
 }
 
 // One target t of the pass: its rates, or nothing for a wall or a target
-// that keeps its previous rates.
-template <int KERNEL>
+// that keeps its previous rates; MODE selects the rates written.
+template <int KERNEL, int MODE = kFused>
 __device__ inline void uniform_target(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
                                       int t, unsigned *s_list) {
     const float4 bt = P.B[t];
     if (not_fluid(bt.w)) return;  // walls are not targets
     double drho, ax, ay;
-    if (!uniform_rates<KERNEL>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay)) return;
-    P.drho[t] = drho;
-    P.ax[t] = ax;
-    P.ay[t] = ay;
+    if (!uniform_rates<KERNEL, MODE>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay)) return;
+    if constexpr (MODE != kMomentum) P.drho[t] = drho;
+    if constexpr (MODE != kContinuity) {
+        P.ax[t] = ax;
+        P.ay[t] = ay;
+    }
 }
 
-template <int KERNEL>
+template <int KERNEL, int MODE>
 __global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformArgs P, UniformConst K) {
     __shared__ unsigned s_list[kUniformList * 256];
     if (!ctrl->active) return;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const Grid &g = ctrl->grid;
-    uniform_target<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, s_list);
+    uniform_target<KERNEL, MODE>(P, K, (int)g.ncx, (int)g.ncy, 0, t, s_list);
 }
 
 // A batch target t: its instance's grid and constants. The lanes of a wave
@@ -266,7 +279,7 @@ __device__ inline void uniform_target_batch(const BInst *inst, const int *instOf
 // to C.part[q * gridDim.x + blockIdx.x]; ctrl->dt_parts records the count.
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArgs P, UniformConst K, PecArgs C) {
-    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ unsigned s_list[kUniformPecList * 256];
     __shared__ double lds[kDtQ * 16];
     if (!ctrl->active) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
@@ -281,7 +294,8 @@ __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArg
             const PecStart p0 = pec_start(C, t, dt, damping);
 #endif
             double drho, ax, ay;
-            if (uniform_rates<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt, s_list, drho, ax, ay)) {
+            if (uniform_rates<KERNEL, kFused, kUniformPecList>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt, s_list, drho, ax,
+                                                               ay)) {
                 P.drho[t] = drho;
                 P.ax[t] = ax;
                 P.ay[t] = ay;
@@ -352,7 +366,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_queue(const BInst *i
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC,
                                                        UniformArgs P, const UniformConst *Ks, PecArgs C) {
-    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ unsigned s_list[kUniformPecList * 256];
     __shared__ double lds[kDtQ * 16];
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= NC) return;  // the whole workgroup
@@ -371,8 +385,8 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *ins
 #if HIPSPH_PEC_EARLY
                 const PecStart p0 = pec_start(Ci, t, dt, damping);
 #endif
-                if (uniform_rates<KERNEL>(P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, bt, s_list, drho, ax,
-                                ay)) {
+                if (uniform_rates<KERNEL, kFused, kUniformPecList>(P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy,
+                                                                   I.cellBase, t, bt, s_list, drho, ax, ay)) {
                     P.drho[t] = drho;
                     P.ax[t] = ax;
                     P.ay[t] = ay;

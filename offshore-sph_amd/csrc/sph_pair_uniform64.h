@@ -59,8 +59,10 @@ __device__ inline bool not_fluid64(double prs) { return prs != prs; }
 // The rates of fluid target t (bt: its B record) over a grid of ncx x ncy
 // cells starting at bin `base` (0 for a single simulation, the instance's
 // first bin in a batch); false when t has no neighbour and keeps its
-// previous rates.
-template <int KERNEL>
+// previous rates. MODE kContinuity sums drho only, kMomentum ax and ay only
+// (the other outputs are then zero); each sum keeps the order of the fused
+// pass, so the split passes give its bits.
+template <int KERNEL, int MODE = kFused>
 __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Const &K, int ncx, int ncy,
                                        long long base, int t, const double4 &bt, unsigned *s_list, double &drho,
                                        double &ax, double &ay) {
@@ -97,7 +99,8 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
                 const double dwx = norm * dx, dwy = norm * dy;
                 const double vx = bt.x - b.x, vy = bt.y - b.y;
                 const double dot = vx * dwx + vy * dwy;
-                arho += K.m * dot;
+                if constexpr (MODE != kMomentum) arho += K.m * dot;
+                if constexpr (MODE == kContinuity) continue;
                 const double dv = vx * dx + vy * dy;
                 double PI_ij = 0.0;
                 if (dv < 0) {
@@ -109,7 +112,7 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
                 const double factor = bt.w + b.w + PI_ij;
                 a0 += -K.m * factor * dwx;
                 a1 += -K.m * factor * dwy;
-            } else if (!(r > K.r0) && r > 1e-12) {
+            } else if (MODE != kContinuity && !(r > K.r0) && r > 1e-12) {
                 const double frac = K.r0 / r;
                 const double tmp = ipow(frac, K.p1) - ipow(frac, K.p2);
                 const double fac = K.D * tmp;
@@ -182,32 +185,34 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
     }
     drho = arho;
     ax = a0 + f0;
-    ay = (a1 - K.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
+    ay = MODE == kContinuity ? 0.0 : (a1 - K.g) + f1;  // WCSPH.py:169, then SolverTools.py:170-171
     return true;
 }
 
 // One target t of the pass: its rates, or nothing for a wall or a target
-// that keeps its previous rates.
-template <int KERNEL>
+// that keeps its previous rates; MODE selects the rates written.
+template <int KERNEL, int MODE = kFused>
 __device__ inline void uniform64_target(const Uniform64Args &P, const Uniform64Const &K, int ncx, int ncy,
                                         long long base, int t, unsigned *s_list) {
     const double4 bt = P.B[t];
     if (not_fluid64(bt.w)) return;  // walls are not targets
     double drho, ax, ay;
-    if (!uniform64_rates<KERNEL>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay)) return;
-    P.drho[t] = drho;
-    P.ax[t] = ax;
-    P.ay[t] = ay;
+    if (!uniform64_rates<KERNEL, MODE>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay)) return;
+    if constexpr (MODE != kMomentum) P.drho[t] = drho;
+    if constexpr (MODE != kContinuity) {
+        P.ax[t] = ax;
+        P.ay[t] = ay;
+    }
 }
 
-template <int KERNEL>
+template <int KERNEL, int MODE>
 __global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Uniform64Args P, Uniform64Const K) {
     __shared__ unsigned s_list[kUniform64List * 256];
     if (!ctrl->active) return;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const Grid &g = ctrl->grid;
-    uniform64_target<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, s_list);
+    uniform64_target<KERNEL, MODE>(P, K, (int)g.ncx, (int)g.ncy, 0, t, s_list);
 }
 
 // A batch target t: its instance's grid and constants. The lanes of a wave
