@@ -4,10 +4,23 @@
 // scatter, per-bin rank sort). Included by more than one translation unit:
 // everything is in an anonymous namespace, so each gets its own copy.
 #pragma once
+#include <type_traits>
+
 #include "sph_internal.h"
 
 namespace hipsph {
 namespace {
+
+// Host: calls f with the kernel id as a compile-time constant
+// (std::integral_constant<int, id>), for launchers instantiated per kernel.
+template <typename F>
+inline void with_kernel(int kernel, F &&f) {
+    switch (kernel) {
+        case SPH_KERNEL_WENDLAND: f(std::integral_constant<int, SPH_KERNEL_WENDLAND>{}); break;
+        case SPH_KERNEL_GAUSSIAN: f(std::integral_constant<int, SPH_KERNEL_GAUSSIAN>{}); break;
+        default: f(std::integral_constant<int, SPH_KERNEL_CUBIC>{}); break;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // reductions (wavefront shuffles, then LDS across the 4 waves of a block)

@@ -988,27 +988,37 @@ static int closed(sph_ctx *c) {
 // wall force's r0 inside the neighbour cut-off qmax h, and at most
 // kUniformMaxN particles.
 bool hipsph::uniform_eligible(const sph_params_t &P, double h) {
-    if (P.kernel != SPH_KERNEL_CUBIC && P.kernel != SPH_KERNEL_WENDLAND) return false;
     if (P.n > kUniformMaxN) return false;  // records addressed with 32-bit byte offsets
     if (P.use_xsph || P.use_summation_density || P.dynamic_h) return false;
+    if (!(h > 0.0)) return false;
     const double cut = 0.99 * P.qmax * h;
-    return h > 0.0 && 2.0 * h < cut && P.r0 < cut;
+    if (P.kernel == SPH_KERNEL_GAUSSIAN)  // support 3h, cut by the cut-off unless q_max >= 3
+        return P.qmax >= 3.0 && P.r0 < 0.99 * 3.0 * h;
+    return 2.0 * h < cut && P.r0 < cut;
 }
 
 UniformParams hipsph::make_uniform_params(const sph_params_t &P, double h, double m) {
     const double pi = 3.141592653589793;
     UniformParams u;
-    const double lim = 2.0 * h > P.r0 ? 2.0 * h : P.r0;
+    const bool gauss = P.kernel == SPH_KERNEL_GAUSSIAN;
+    const double sup = (gauss ? 3.0 : 2.0) * h;
+    const double lim = sup > P.r0 ? sup : P.r0;
     u.lim2 = lim * lim * (P.precision == SPH_PREC_FP32C ? 1.0 + 1e-5 : 1.0 + 1e-12);
+    u.lo = u.lim2;
+    if (gauss && P.precision == SPH_PREC_FP32C) {  // the float pass lists exactly q <= 3 (r0 < 3h)
+        u.lim2 = sup * sup * (1.0 + 1e-12);
+        u.lo = sup * sup * (1.0 - 1e-12);
+    }
     u.h = h;
     u.m = m;
     u.qmax = P.qmax;
     u.g = P.gravity;
-    u.ck = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h * h);
+    const double norm = gauss ? 1.0 / pi : (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi));
+    u.ck = norm / (h * h * h);
     u.ac = P.alpha * (0.5 * P.co);
     // the double pass's constants with the reference's operations
-    // (CubicSpline.py:55 / Wendland.py:34, Momentum.py:43-46)
-    u.alpha_k = (P.kernel == SPH_KERNEL_WENDLAND ? 9.0 / (4.0 * pi) : 10.0 / (7.0 * pi)) / (h * h);
+    // (CubicSpline.py:55 / Gaussian.py:47,53 / Wendland.py:34, Momentum.py:43-46)
+    u.alpha_k = norm / (h * h);
     u.c01 = 0.01 * h * h;
     u.cij = 0.5 * (P.co + 0.0);
     u.alpha = P.alpha;

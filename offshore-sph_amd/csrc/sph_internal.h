@@ -124,13 +124,16 @@ static_assert(kRecFp64 == SPH_PAIR_FP64 && kRecFp32 == SPH_PAIR_FP32C && kRecUni
 
 // Constants of the uniform float pass (one h, one fluid mass), in double.
 struct UniformParams {
-    double lim2;   // max((2h)^2, r0^2) (1 + 1e-5; 1 + 1e-12 for the double pass)
+    // max(support^2, r0^2) (1 + 1e-5; 1 + 1e-12 for the double pass); the
+    // float Gaussian pass: (3h)^2 (1 + 1e-12), with lo = (3h)^2 (1 - 1e-12)
+    // the start of the band of its exact q <= 3 test (elsewhere lo = lim2)
+    double lim2, lo;
     double h, m, qmax, g;
     double ck;     // the kernel's normalisation / h^3
     double ac;     // alpha c0 / 2
     double beta, r0, D, p1, p2;
     // the double pass's constants, formed with the reference's operations
-    double alpha_k;  // 10 / (7 pi) / (h h), Wendland 9 / (4 pi) / (h h)
+    double alpha_k;  // 10 / (7 pi) / (h h), Wendland 9 / (4 pi) / (h h), Gaussian (1 / pi) / (h h)
     double c01;      // (0.01 h) h
     double cij;      // 0.5 (c0 + 0)
     double alpha;

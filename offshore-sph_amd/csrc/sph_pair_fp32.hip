@@ -39,6 +39,7 @@ size_t uniform_const_size_fp32() { return sizeof(UniformConst); }
 void uniform_const_fp32(const UniformParams &u, void *out) {
     UniformConst K;
     K.lim2 = u.lim2;
+    K.lo = u.lo;
     K.hd = u.h;
     K.qmax = u.qmax;
     K.g = u.g;
@@ -75,16 +76,15 @@ hipError_t launch_pair_fp32u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
     UniformConst K;
     uniform_const_fp32(u, &K);
     const int nb = xcd_blocks(n, 256);
-    const bool w = kernel == SPH_KERNEL_WENDLAND;
-    if (out.mode == kContinuity)
-        w ? k_pair_uniform<SPH_KERNEL_WENDLAND, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K)
-          : k_pair_uniform<SPH_KERNEL_CUBIC, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K);
-    else if (out.mode == kMomentum)
-        w ? k_pair_uniform<SPH_KERNEL_WENDLAND, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K)
-          : k_pair_uniform<SPH_KERNEL_CUBIC, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K);
-    else
-        w ? k_pair_uniform<SPH_KERNEL_WENDLAND, kFused><<<nb, 256, 0, s>>>(ctrl, P, K)
-          : k_pair_uniform<SPH_KERNEL_CUBIC, kFused><<<nb, 256, 0, s>>>(ctrl, P, K);
+    with_kernel(kernel, [&](auto kk) {
+        constexpr int KN = decltype(kk)::value;
+        if (out.mode == kContinuity)
+            k_pair_uniform<KN, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K);
+        else if (out.mode == kMomentum)
+            k_pair_uniform<KN, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K);
+        else
+            k_pair_uniform<KN, kFused><<<nb, 256, 0, s>>>(ctrl, P, K);
+    });
     return hipGetLastError();
 }
 
@@ -94,10 +94,9 @@ hipError_t launch_pair_fp32u_pec(Ctrl *ctrl, PairIn in, PairOut out, const Unifo
     UniformConst K;
     uniform_const_fp32(u, &K);
     const int nb = xcd_blocks(n, 256);  // its dt partials: one per workgroup
-    if (kernel == SPH_KERNEL_WENDLAND)
-        k_pair_uniform_pec<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
-    else
-        k_pair_uniform_pec<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
+    with_kernel(kernel, [&](auto kk) {
+        k_pair_uniform_pec<decltype(kk)::value><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
+    });
     return hipGetLastError();
 }
 
@@ -110,25 +109,23 @@ hipError_t launch_pair_batch_fp32u(const BInst *inst, const int *instOf, PairIn 
         if (grid > nb) grid = nb;
         grid = grid / kXcds * kXcds;
         if (grid < kXcds) grid = kXcds;
-        if (kernel == SPH_KERNEL_WENDLAND)
-            k_pair_batch_uniform_queue<SPH_KERNEL_WENDLAND><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
-        else
-            k_pair_batch_uniform_queue<SPH_KERNEL_CUBIC><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
-    } else if (kernel == SPH_KERNEL_WENDLAND) {
-        k_pair_batch_uniform<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+        with_kernel(kernel, [&](auto kk) {
+            k_pair_batch_uniform_queue<decltype(kk)::value><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
+        });
     } else {
-        k_pair_batch_uniform<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+        with_kernel(kernel, [&](auto kk) {
+            k_pair_batch_uniform<decltype(kk)::value><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+        });
     }
     return hipGetLastError();
 }
 
 int pair_batch_uniform_occupancy_fp32(int kernel) {
     int per = 0;
-    hipError_t e = kernel == SPH_KERNEL_WENDLAND
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             &per, k_pair_batch_uniform_queue<SPH_KERNEL_WENDLAND>, 256, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             &per, k_pair_batch_uniform_queue<SPH_KERNEL_CUBIC>, 256, 0);
+    hipError_t e = hipSuccess;
+    with_kernel(kernel, [&](auto kk) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pair_batch_uniform_queue<decltype(kk)::value>, 256, 0);
+    });
     return e == hipSuccess && per > 0 ? per : 1;
 }
 
@@ -141,10 +138,9 @@ hipError_t launch_pair_batch_fp32u_pec(const BInst *inst, const int *cs, const i
     const UniformArgs P = uniform_args(in, out, n);
     const UniformConst *K = (const UniformConst *)Ks;
     const int nb = (NC + kXcds - 1) / kXcds * kXcds;  // xcd_remap maps [0, nb) onto itself
-    if (kernel == SPH_KERNEL_WENDLAND)
-        k_pair_batch_uniform_pec<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
-    else
-        k_pair_batch_uniform_pec<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    with_kernel(kernel, [&](auto kk) {
+        k_pair_batch_uniform_pec<decltype(kk)::value><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    });
     return hipGetLastError();
 }
 

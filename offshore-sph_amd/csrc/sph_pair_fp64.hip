@@ -64,16 +64,15 @@ hipError_t launch_pair_fp64u(const Ctrl *ctrl, PairIn in, PairOut out, const Uni
     Uniform64Const K;
     uniform_const_fp64(u, &K);
     const int nb = xcd_blocks(n, 256);
-    const bool w = kernel == SPH_KERNEL_WENDLAND;
-    if (out.mode == kContinuity)
-        w ? k_pair_uniform64<SPH_KERNEL_WENDLAND, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K)
-          : k_pair_uniform64<SPH_KERNEL_CUBIC, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K);
-    else if (out.mode == kMomentum)
-        w ? k_pair_uniform64<SPH_KERNEL_WENDLAND, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K)
-          : k_pair_uniform64<SPH_KERNEL_CUBIC, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K);
-    else
-        w ? k_pair_uniform64<SPH_KERNEL_WENDLAND, kFused><<<nb, 256, 0, s>>>(ctrl, P, K)
-          : k_pair_uniform64<SPH_KERNEL_CUBIC, kFused><<<nb, 256, 0, s>>>(ctrl, P, K);
+    with_kernel(kernel, [&](auto kk) {
+        constexpr int KN = decltype(kk)::value;
+        if (out.mode == kContinuity)
+            k_pair_uniform64<KN, kContinuity><<<nb, 256, 0, s>>>(ctrl, P, K);
+        else if (out.mode == kMomentum)
+            k_pair_uniform64<KN, kMomentum><<<nb, 256, 0, s>>>(ctrl, P, K);
+        else
+            k_pair_uniform64<KN, kFused><<<nb, 256, 0, s>>>(ctrl, P, K);
+    });
     return hipGetLastError();
 }
 
@@ -83,10 +82,9 @@ hipError_t launch_pair_fp64u_pec(Ctrl *ctrl, PairIn in, PairOut out, const Unifo
     Uniform64Const K;
     uniform_const_fp64(u, &K);
     const int nb = xcd_blocks(n, 256);  // its dt partials: one per workgroup
-    if (kernel == SPH_KERNEL_WENDLAND)
-        k_pair_uniform64_pec<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
-    else
-        k_pair_uniform64_pec<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
+    with_kernel(kernel, [&](auto kk) {
+        k_pair_uniform64_pec<decltype(kk)::value><<<nb, 256, 0, s>>>(ctrl, P, K, pec);
+    });
     return hipGetLastError();
 }
 
@@ -99,25 +97,24 @@ hipError_t launch_pair_batch_fp64u(const BInst *inst, const int *instOf, PairIn 
         if (grid > nb) grid = nb;
         grid = grid / kXcds * kXcds;
         if (grid < kXcds) grid = kXcds;
-        if (kernel == SPH_KERNEL_WENDLAND)
-            k_pair_batch_uniform64_queue<SPH_KERNEL_WENDLAND><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
-        else
-            k_pair_batch_uniform64_queue<SPH_KERNEL_CUBIC><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
-    } else if (kernel == SPH_KERNEL_WENDLAND) {
-        k_pair_batch_uniform64<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+        with_kernel(kernel, [&](auto kk) {
+            k_pair_batch_uniform64_queue<decltype(kk)::value><<<grid, 256, 0, s>>>(inst, instOf, P, K, queue);
+        });
     } else {
-        k_pair_batch_uniform64<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+        with_kernel(kernel, [&](auto kk) {
+            k_pair_batch_uniform64<decltype(kk)::value><<<nb, 256, 0, s>>>(inst, instOf, P, K);
+        });
     }
     return hipGetLastError();
 }
 
 int pair_batch_uniform_occupancy_fp64(int kernel) {
     int per = 0;
-    hipError_t e = kernel == SPH_KERNEL_WENDLAND
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             &per, k_pair_batch_uniform64_queue<SPH_KERNEL_WENDLAND>, 256, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             &per, k_pair_batch_uniform64_queue<SPH_KERNEL_CUBIC>, 256, 0);
+    hipError_t e = hipSuccess;
+    with_kernel(kernel, [&](auto kk) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pair_batch_uniform64_queue<decltype(kk)::value>, 256,
+                                                         0);
+    });
     return e == hipSuccess && per > 0 ? per : 1;
 }
 
@@ -140,10 +137,9 @@ hipError_t launch_pair_batch_fp64u_pec(const BInst *inst, const int *cs, const i
     const Uniform64Args P = uniform64_args(in, out, n);
     const Uniform64Const *K = (const Uniform64Const *)Ks;
     const int nb = (NC + kXcds - 1) / kXcds * kXcds;  // xcd_remap maps [0, nb) onto itself
-    if (kernel == SPH_KERNEL_WENDLAND)
-        k_pair_batch_uniform64_pec<SPH_KERNEL_WENDLAND><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
-    else
-        k_pair_batch_uniform64_pec<SPH_KERNEL_CUBIC><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    with_kernel(kernel, [&](auto kk) {
+        k_pair_batch_uniform64_pec<decltype(kk)::value><<<nb, 256, 0, s>>>(inst, cs, chunkInst, NC, P, K, pec);
+    });
     return hipGetLastError();
 }
 

@@ -55,8 +55,27 @@ namespace {
 #ifndef HIPSPH_UNIFORM_PEC_LIST
 #define HIPSPH_UNIFORM_PEC_LIST 24
 #endif
+#ifndef HIPSPH_UNIFORM_GAUSS_LIST
+#define HIPSPH_UNIFORM_GAUSS_LIST 22
+#endif
+#ifndef HIPSPH_UNIFORM_FLUSH_ALL
+#define HIPSPH_UNIFORM_FLUSH_ALL 0
+#endif
 constexpr int kUniformList = HIPSPH_UNIFORM_LIST;
 constexpr int kUniformPecList = HIPSPH_UNIFORM_PEC_LIST;
+// The Gaussian's 3h support holds about 42 contributors: every list fills
+// at least once, whatever its length, so occupancy decides: 22 entries (7
+// workgroups per CU, as many as its registers allow) take 0.65 ms on a 4M
+// jittered lattice, 18 0.74, 26 0.68, 36 0.85, 48 1.05.
+constexpr int kUniformGaussList = HIPSPH_UNIFORM_GAUSS_LIST;
+template <int KERNEL>
+__host__ __device__ constexpr int uniform_list() {
+    return KERNEL == SPH_KERNEL_GAUSSIAN ? kUniformGaussList : kUniformList;
+}
+template <int KERNEL>
+__host__ __device__ constexpr int uniform_pec_list() {
+    return KERNEL == SPH_KERNEL_GAUSSIAN ? kUniformGaussList : kUniformPecList;
+}
 
 struct UniformArgs {
     const double2 *A;  // (x, y)
@@ -68,7 +87,11 @@ struct UniformArgs {
 };
 
 struct UniformConst {
-    double lim2;         // max((2h)^2, r0^2) (1 + 1e-5): the listed pairs
+    // The listed pairs. Cubic spline and Wendland: r^2 <= lim2 = max((2h)^2,
+    // r0^2) (1 + 1e-5), a superset of the contributors (the terms test the
+    // support). Gaussian: the reference's q = r / h <= 3, the exact double
+    // test for r^2 within [lo, lim2] = (3h)^2 (1 -+ 1e-12).
+    double lim2, lo;
     double hd, qmax, g;  // h, the cut-off in units of h, gravity
     float h, hinv, m, ck;  // ck: the kernel's normalisation / h^3 (dW/dr / r = ck g(q) / r)
     float c01, ac, beta;   // 0.01 h^2, alpha c0 / 2, beta
@@ -78,10 +101,13 @@ struct UniformConst {
 
 __device__ inline bool not_fluid(float prs) { return prs != prs; }
 
-// The gradient factor g(q) of the kernel's dW/dq (zero outside the support).
+// The gradient factor g(q) of the kernel's dW/dq (zero outside the support;
+// the Gaussian's listed pairs all lie inside its support).
 template <int KERNEL>
 __device__ inline float grad_q(float q) {
-    if (KERNEL == SPH_KERNEL_WENDLAND) {
+    if (KERNEL == SPH_KERNEL_GAUSSIAN) {
+        return -2.0f * q * __expf(-q * q);  // src/Kernels/Gaussian.py:54
+    } else if (KERNEL == SPH_KERNEL_WENDLAND) {
         const float inner = 1.0f - 0.5f * q;
         const float i2 = inner * inner;
         const float gq = i2 * i2 * inner * (-14.0f / 3.0f) * q * (1.0f + 2.5f * q);
@@ -105,7 +131,8 @@ __device__ inline float3 uniform_terms(const UniformConst &K, float dx, float dy
     const float r = r2 * rinv;
     if (!not_fluid(b.w)) {
         const float q = r * K.hinv;
-        const float norm = K.ck * grad_q<KERNEL>(q) * rinv;
+        float norm = K.ck * grad_q<KERNEL>(q) * rinv;
+        if (KERNEL == SPH_KERNEL_GAUSSIAN && !(r * K.h > 1e-12f)) norm = 0.0f;  // Gaussian.py:48-49
         const float dwx = norm * dx, dwy = norm * dy;
         const float vx = vxi - b.x, vy = vyi - b.y;
         const float dv = vx * dx + vy * dy;
@@ -159,7 +186,7 @@ __device__ inline int uniform_count(const UniformArgs &P, const UniformConst &K,
 // first bin in a batch); false when t has no neighbour and keeps its
 // previous rates. MODE kContinuity sums drho only, kMomentum ax and ay only
 // (the other outputs are then zero).
-template <int KERNEL, int MODE = kFused, int LIST = kUniformList>
+template <int KERNEL, int MODE = kFused, int LIST = uniform_list<KERNEL>()>
 __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
                                      int t, const float4 &bt, unsigned *s_list, double &drho, double &ax, double &ay) {
     const double2 at = P.A[t];
@@ -191,10 +218,20 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
     auto test = [&](double2 a, int k) {
         const double dx = X - a.x, dy = Y - a.y;
         const double r2 = dx * dx + dy * dy;
-        if (r2 <= K.lim2 && r2 > 0.0) {
+        bool in = r2 <= K.lim2 && r2 > 0.0;
+        if (KERNEL == SPH_KERNEL_GAUSSIAN && in && !(r2 < K.lo)) in = exact_q_test(X, Y, K.hd, a.x, a.y, K.hd, 3.0);
+        if (in) {
             *lp = (unsigned)k;
             lp += stride;
         }
+    };
+    // A list about to overflow is flushed; with FLUSH_ALL (always for the
+    // Gaussian) every active lane of the wave flushes with it, so the wave
+    // runs one flush loop instead of one per lane that fills at its own
+    // candidate. A lane's sums keep their order either way.
+    auto overflow = [&]() {
+        if constexpr (HIPSPH_UNIFORM_FLUSH_ALL || KERNEL == SPH_KERNEL_GAUSSIAN) return __ballot(lp > full) != 0;
+        return lp > full;
     };
     for (int db = -1; db <= 1 && lo <= hi; ++db) {
         const int cy = c.y + db;
@@ -203,7 +240,7 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
         const int s = P.cellStart[row + lo], e = P.cellStart[row + hi + 1];
         int k = s;
         for (; k + U <= e; k += U) {
-            if (lp > full) flush();
+            if (overflow()) flush();
             double2 a[U];
 #pragma unroll
             for (int j = 0; j < U; ++j) a[j] = at32(P.A, (unsigned)(k + j));
@@ -211,7 +248,7 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
             for (int j = 0; j < U; ++j) test(a[j], k + j);
         }
         if (k < e) {
-            if (lp > full) flush();
+            if (overflow()) flush();
             for (; k < e; ++k) test(at32(P.A, (unsigned)k), k);  // fewer than U
         }
     }
@@ -242,7 +279,7 @@ __device__ inline void uniform_target(const UniformArgs &P, const UniformConst &
 
 template <int KERNEL, int MODE>
 __global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformArgs P, UniformConst K) {
-    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ unsigned s_list[uniform_list<KERNEL>() * 256];
     if (!ctrl->active) return;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
@@ -279,7 +316,7 @@ __device__ inline void uniform_target_batch(const BInst *inst, const int *instOf
 // to C.part[q * gridDim.x + blockIdx.x]; ctrl->dt_parts records the count.
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArgs P, UniformConst K, PecArgs C) {
-    __shared__ unsigned s_list[kUniformPecList * 256];
+    __shared__ unsigned s_list[uniform_pec_list<KERNEL>() * 256];
     __shared__ double lds[kDtQ * 16];
     if (!ctrl->active) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
@@ -294,8 +331,8 @@ __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArg
             const PecStart p0 = pec_start(C, t, dt, damping);
 #endif
             double drho, ax, ay;
-            if (uniform_rates<KERNEL, kFused, kUniformPecList>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt, s_list, drho, ax,
-                                                               ay)) {
+            if (uniform_rates<KERNEL, kFused, uniform_pec_list<KERNEL>()>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt,
+                                                                          s_list, drho, ax, ay)) {
                 P.drho[t] = drho;
                 P.ax[t] = ax;
                 P.ay[t] = ay;
@@ -320,7 +357,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArg
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform(const BInst *inst, const int *instOf, UniformArgs P,
                                                             const UniformConst *Ks) {
-    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ unsigned s_list[uniform_list<KERNEL>() * 256];
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     uniform_target_batch<KERNEL>(inst, instOf, P, Ks, t, s_list);
@@ -332,7 +369,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform(const BInst *inst, c
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform_queue(const BInst *inst, const int *instOf, UniformArgs P,
                                                                   const UniformConst *Ks, int *queue) {
-    __shared__ unsigned s_list[kUniformList * 256];
+    __shared__ unsigned s_list[uniform_list<KERNEL>() * 256];
     __shared__ int s_tile;
     const int ntiles = (int)((P.n + (long long)blockDim.x - 1) / blockDim.x);
     const int xcd = (int)(blockIdx.x % kXcds);
@@ -366,7 +403,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_queue(const BInst *i
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC,
                                                        UniformArgs P, const UniformConst *Ks, PecArgs C) {
-    __shared__ unsigned s_list[kUniformPecList * 256];
+    __shared__ unsigned s_list[uniform_pec_list<KERNEL>() * 256];
     __shared__ double lds[kDtQ * 16];
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= NC) return;  // the whole workgroup
@@ -385,8 +422,8 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *ins
 #if HIPSPH_PEC_EARLY
                 const PecStart p0 = pec_start(Ci, t, dt, damping);
 #endif
-                if (uniform_rates<KERNEL, kFused, kUniformPecList>(P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy,
-                                                                   I.cellBase, t, bt, s_list, drho, ax, ay)) {
+                if (uniform_rates<KERNEL, kFused, uniform_pec_list<KERNEL>()>(P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy,
+                                                                              I.cellBase, t, bt, s_list, drho, ax, ay)) {
                     P.drho[t] = drho;
                     P.ax[t] = ax;
                     P.ay[t] = ay;

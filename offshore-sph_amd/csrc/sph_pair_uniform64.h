@@ -7,13 +7,16 @@
 // Every contribution keeps the reference's operation order; what the uniform
 // case changes is everything around it:
 //   * h_ij = 0.5 (h + h) = h, the re-averaged h of the viscosity likewise,
-//     m_j = m: the kernel normalisation 10 / (7 pi) / (h h) (or Wendland's),
-//     0.01 h h and c_ij = c0 / 2 are formed once on the host with the same
-//     operations, so their bits are the ones the reference forms per pair;
-//   * both the support 2h and the wall range r0 lie inside the cut-off 3h:
-//     the walk lists every particle within max(2h, r0) (a double r^2 test,
-//     no cut-off test, no square root), and a listed pair outside the support
-//     or the wall range adds nothing, as in the reference;
+//     m_j = m: the kernel normalisation 10 / (7 pi) / (h h) (Wendland's,
+//     the Gaussian's (1 / pi) / (h h)), 0.01 h h and c_ij = c0 / 2 are formed
+//     once on the host with the same operations, so their bits are the ones
+//     the reference forms per pair;
+//   * the kernel support (2h; 3h for the Gaussian) lies within the cut-off
+//     q_max h and the wall range r0 inside it: the walk lists every particle
+//     within max(support, r0) (a double r^2 test, no cut-off test, no square
+//     root), and a listed pair outside the support or the wall range adds
+//     nothing, as in the reference (a Gaussian pair at q <= 3 is a neighbour
+//     whenever q_max >= 3: the same double q decides both);
 //   * records: A = (x, y), B = (vx, vy, rho, p / rho^2) with a NaN in place of
 //     p / rho^2 for particles that are not fluid;
 //   * the walk visits the reference's nine cells in its order (x shift outer,
@@ -35,7 +38,25 @@ namespace {
 #ifndef HIPSPH_UNIFORM64_LIST
 #define HIPSPH_UNIFORM64_LIST 32
 #endif
+// The Gaussian's lists (about 42 contributors) fill at least once: the
+// longest list that keeps the 4 workgroups per CU its registers allow (40 KB
+// of LDS each; 38 beside the corrector's 512 B) flushes least often (on a 4M
+// jittered lattice 40 entries take 1.59 ms, 32 1.84 ms, 24 1.96 ms).
+#ifndef HIPSPH_UNIFORM64_GAUSS_LIST
+#define HIPSPH_UNIFORM64_GAUSS_LIST 40
+#endif
+#ifndef HIPSPH_UNIFORM64_GAUSS_PEC_LIST
+#define HIPSPH_UNIFORM64_GAUSS_PEC_LIST 38
+#endif
 constexpr int kUniform64List = HIPSPH_UNIFORM64_LIST;  // contributor-list entries per thread (4 B each)
+template <int KERNEL>
+__host__ __device__ constexpr int uniform64_list() {
+    return KERNEL == SPH_KERNEL_GAUSSIAN ? HIPSPH_UNIFORM64_GAUSS_LIST : kUniform64List;
+}
+template <int KERNEL>
+__host__ __device__ constexpr int uniform64_pec_list() {
+    return KERNEL == SPH_KERNEL_GAUSSIAN ? HIPSPH_UNIFORM64_GAUSS_PEC_LIST : kUniform64List;
+}
 
 struct Uniform64Args {
     const double2 *A;  // (x, y)
@@ -62,7 +83,7 @@ __device__ inline bool not_fluid64(double prs) { return prs != prs; }
 // previous rates. MODE kContinuity sums drho only, kMomentum ax and ay only
 // (the other outputs are then zero); each sum keeps the order of the fused
 // pass, so the split passes give its bits.
-template <int KERNEL, int MODE = kFused>
+template <int KERNEL, int MODE = kFused, int LIST = uniform64_list<KERNEL>()>
 __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Const &K, int ncx, int ncy,
                                        long long base, int t, const double4 &bt, unsigned *s_list, double &drho,
                                        double &ax, double &ay) {
@@ -74,8 +95,8 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
     unsigned *lp = list0;
     double arho = 0.0, a0 = 0.0, a1 = 0.0, f0 = 0.0, f1 = 0.0;
     // The reference's expressions (Continuity.py:13-16, Momentum.py:36-56,
-    // BoundaryForce.py:31-41, CubicSpline.py:53-70 / Wendland.py:32-64) for
-    // each listed pair, in listing order.
+    // BoundaryForce.py:31-41, CubicSpline.py:53-70 / Gaussian.py:41-60 /
+    // Wendland.py:32-64) for each listed pair, in listing order.
     auto flush = [&]() {
         for (unsigned *q = list0; q != lp; q += stride) {
             const unsigned u = *q;
@@ -86,17 +107,28 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
             const double r = sqrt(xij2);
             if (!not_fluid64(b.w)) {
                 const double qq = r / K.h;
-                double grad;
-                if (KERNEL == SPH_KERNEL_WENDLAND) {
-                    if ((qq >= 2.0) || (r < 1e-10)) continue;
-                    const double inner = 1.0 - 0.5 * qq;
-                    grad = ipow(inner, 5.0) * (-14.0 / 3.0) * qq * (1 + 2.5 * qq);
+                double dwx, dwy;
+                if (KERNEL == SPH_KERNEL_GAUSSIAN) {
+                    const double tmp = r * K.h;
+                    if (!(tmp > 1e-12 && qq <= 3)) continue;
+                    const double dwdq = -2 * qq * K.alpha_k * exp(-qq * qq);
+                    const double f = dwdq / tmp;
+                    dwx = f * dx;
+                    dwy = f * dy;
                 } else {
-                    if ((qq > 2.0) || (r < 1e-10)) continue;
-                    grad = (qq > 1.0) ? -0.75 * ((2.0 - qq) * (2.0 - qq)) : -3.0 * qq * (1.0 - 0.75 * qq);
+                    double grad;
+                    if (KERNEL == SPH_KERNEL_WENDLAND) {
+                        if ((qq >= 2.0) || (r < 1e-10)) continue;
+                        const double inner = 1.0 - 0.5 * qq;
+                        grad = ipow(inner, 5.0) * (-14.0 / 3.0) * qq * (1 + 2.5 * qq);
+                    } else {
+                        if ((qq > 2.0) || (r < 1e-10)) continue;
+                        grad = (qq > 1.0) ? -0.75 * ((2.0 - qq) * (2.0 - qq)) : -3.0 * qq * (1.0 - 0.75 * qq);
+                    }
+                    const double norm = K.alpha_k * grad / (K.h * r);
+                    dwx = norm * dx;
+                    dwy = norm * dy;
                 }
-                const double norm = K.alpha_k * grad / (K.h * r);
-                const double dwx = norm * dx, dwy = norm * dy;
                 const double vx = bt.x - b.x, vy = bt.y - b.y;
                 const double dot = vx * dwx + vy * dwy;
                 if constexpr (MODE != kMomentum) arho += K.m * dot;
@@ -123,8 +155,14 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
         lp = list0;
     };
     constexpr int U = HIPSPH_UNIFORM64_UNROLL;
-    static_assert(U >= 1 && U < kUniform64List, "unroll within the list");
-    unsigned *const full = list0 + (kUniform64List - U) * stride;  // past this, U appends could overflow
+    static_assert(U >= 1 && U < LIST, "unroll within the list");
+    unsigned *const full = list0 + (LIST - U) * stride;  // past this, U appends could overflow
+    // the Gaussian's lists all fill: every active lane flushes when one must,
+    // one flush loop for the wave instead of one per lane
+    auto overflow = [&]() {
+        if constexpr (KERNEL == SPH_KERNEL_GAUSSIAN) return __ballot(lp > full) != 0;
+        return lp > full;
+    };
     auto test = [&](double2 a, int k) {
         const double dx = X - a.x, dy = Y - a.y;
         const double r2 = dx * dx + dy * dy;
@@ -145,7 +183,7 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
             const int s = P.cellStart[cell], e = P.cellStart[cell + 1];
             int k = e - 1;
             for (; k - U + 1 >= s; k -= U) {
-                if (lp > full) flush();
+                if (overflow()) flush();
                 double2 a[U];
 #pragma unroll
                 for (int j = 0; j < U; ++j) a[j] = at32(P.A, (unsigned)(k - j));
@@ -153,7 +191,7 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
                 for (int j = 0; j < U; ++j) test(a[j], k - j);
             }
             if (k >= s) {
-                if (lp > full) flush();
+                if (overflow()) flush();
                 for (; k >= s; --k) test(at32(P.A, (unsigned)k), k);  // fewer than U
             }
         }
@@ -207,7 +245,7 @@ __device__ inline void uniform64_target(const Uniform64Args &P, const Uniform64C
 
 template <int KERNEL, int MODE>
 __global__ __launch_bounds__(256) void k_pair_uniform64(const Ctrl *ctrl, Uniform64Args P, Uniform64Const K) {
-    __shared__ unsigned s_list[kUniform64List * 256];
+    __shared__ unsigned s_list[uniform64_list<KERNEL>() * 256];
     if (!ctrl->active) return;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
@@ -241,7 +279,7 @@ __device__ inline void uniform64_target_batch(const BInst *inst, const int *inst
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_uniform64_pec(Ctrl *ctrl, Uniform64Args P, Uniform64Const K,
                                                             PecArgs C) {
-    __shared__ unsigned s_list[kUniform64List * 256];
+    __shared__ unsigned s_list[uniform64_pec_list<KERNEL>() * 256];
     __shared__ double lds[kDtQ * 16];
     if (!ctrl->active) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
@@ -256,7 +294,8 @@ __global__ __launch_bounds__(256) void k_pair_uniform64_pec(Ctrl *ctrl, Uniform6
             const PecStart p0 = pec_start(C, t, dt, damping);
 #endif
             double drho, ax, ay;
-            if (uniform64_rates<KERNEL>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt, s_list, drho, ax, ay)) {
+            if (uniform64_rates<KERNEL, kFused, uniform64_pec_list<KERNEL>()>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt,
+                                                                              s_list, drho, ax, ay)) {
                 P.drho[t] = drho;
                 P.ax[t] = ax;
                 P.ay[t] = ay;
@@ -280,7 +319,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform64_pec(Ctrl *ctrl, Uniform6
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform64(const BInst *inst, const int *instOf, Uniform64Args P,
                                                               const Uniform64Const *Ks) {
-    __shared__ unsigned s_list[kUniform64List * 256];
+    __shared__ unsigned s_list[uniform64_list<KERNEL>() * 256];
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     uniform64_target_batch<KERNEL>(inst, instOf, P, Ks, t, s_list);
@@ -290,7 +329,7 @@ template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform64_queue(const BInst *inst, const int *instOf,
                                                                     Uniform64Args P, const Uniform64Const *Ks,
                                                                     int *queue) {
-    __shared__ unsigned s_list[kUniform64List * 256];
+    __shared__ unsigned s_list[uniform64_list<KERNEL>() * 256];
     __shared__ int s_tile;
     const int ntiles = (int)((P.n + (long long)blockDim.x - 1) / blockDim.x);
     const int xcd = (int)(blockIdx.x % kXcds);
@@ -324,7 +363,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform64_queue(const BInst 
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform64_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC,
                                                        Uniform64Args P, const Uniform64Const *Ks, PecArgs C) {
-    __shared__ unsigned s_list[kUniform64List * 256];
+    __shared__ unsigned s_list[uniform64_pec_list<KERNEL>() * 256];
     __shared__ double lds[kDtQ * 16];
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= NC) return;  // the whole workgroup
@@ -343,8 +382,8 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform64_pec(const BInst *i
 #if HIPSPH_PEC64_EARLY
                 const PecStart p0 = pec_start(Ci, t, dt, damping);
 #endif
-                if (uniform64_rates<KERNEL>(P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, bt, s_list, drho, ax,
-                                ay)) {
+                if (uniform64_rates<KERNEL, kFused, uniform64_pec_list<KERNEL>()>(
+                        P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy, I.cellBase, t, bt, s_list, drho, ax, ay)) {
                     P.drho[t] = drho;
                     P.ax[t] = ax;
                     P.ay[t] = ay;

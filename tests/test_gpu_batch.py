@@ -87,6 +87,34 @@ def test_batch_instances_match_single_engines(precision, graph, persistent, unif
             np.testing.assert_array_equal(got[k][fld], states[k][fld].astype(got[k][fld].dtype), err_msg=f'{k} {fld}')
 
 
+@pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
+@pytest.mark.parametrize('persistent', [False, True])
+def test_batch_gaussian_matches_single_engines(precision, persistent):
+    """The Gaussian kernel (3h support, the fullest contributor lists) in
+    the uniform form of the batch: every instance bit-identical to its
+    single engine after 40 eager steps."""
+    mem = _members()[:3]
+    K = 40
+    b = BatchEngine([params_for(sc, alpha, precision=precision, kernel='gaussian') for sc, alpha in mem])
+    b.set_tuning(256, 0, persistent)
+    b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+    assert b.pair_form() == precision + '-uniform'
+    b.step(K)
+    got = b.download()
+    b.close()
+    say('batch gaussian', precision, 'persistent', persistent)
+    for k, (sc, alpha) in enumerate(mem):
+        m = sc.method()
+        m.alpha = alpha
+        e = make_engine(sc.pA, m, kernel='gaussian', precision=precision)
+        assert e.pair_form() == precision + '-uniform'
+        e.step(K)
+        want = e.download()
+        e.close()
+        for fld in FIELDS + ('p',):
+            np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
+
+
 def test_batch_duration_per_instance():
     mem = _members()[:3]
     # instance 1 stops after about 10 steps' time

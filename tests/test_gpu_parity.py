@@ -184,7 +184,38 @@ def test_compute_fp64_vs_oracle(kernel):
     e.close()
 
 
-@pytest.mark.parametrize('kernel', ['cubic', 'gaussian'])
+@pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
+def test_uniform_pass_matches_general(kernel, precision):
+    """One h and one fluid mass: the uniform pass (sph_compute) against the
+    general pass on the same records (sph_compute_counts takes the general
+    one): FP64 bit-identical (same expressions, same order; for the Gaussian
+    also the exact q <= 3 cut at the cut-off q_max = 3), FP32C within 2e-6 of
+    the largest value (the general pass sums in float with TwoSum, the
+    uniform one in double)."""
+    sc = scenarios.dam_break_c1()
+    pA = perturbed(sc.pA, 11)
+    m = sc.method()
+    say('uniform vs general', kernel, precision)
+    e = make_engine(pA, m, kernel=kernel, precision=precision)
+    e.nn_update()
+    assert e.pair_form() == precision + '-uniform'
+    e.compute()
+    uni = e.download()
+    e.compute_counts()
+    gen = e.download()
+    e.close()
+    f = pA['label'] == 0
+    for k in ('drho', 'ax', 'ay'):
+        if precision == 'fp64':
+            np.testing.assert_array_equal(uni[k][f], gen[k][f], err_msg=k)
+        else:
+            err = rel_err(uni[k], gen[k], f)
+            say(k, err)
+            assert err < 2e-6, (k, err)
+
+
+@pytest.mark.parametrize('kernel', ['cubic', 'gaussian', 'wendland'])
 def test_compute_fp32c_vs_oracle(kernel):
     sc = scenarios.dam_break_c1()
     pA = perturbed(sc.pA, 4)
@@ -208,8 +239,8 @@ def test_compute_fp32c_vs_oracle(kernel):
 def test_split_passes_match_fused_and_oracle(kernel, precision):
     """sph_continuity writes drho only and sph_momentum ax, ay only; each
     equals the fused pass (same expressions, same order) and so the oracle.
-    One h and one fluid mass: the cubic and Wendland passes take the uniform
-    form (split and fused alike), the Gaussian the general one."""
+    One h and one fluid mass: every kernel takes the uniform form (split and
+    fused alike)."""
     sc = scenarios.dam_break_c1()
     pA = perturbed(sc.pA, 5)
     m = sc.method()
@@ -221,7 +252,7 @@ def test_split_passes_match_fused_and_oracle(kernel, precision):
     fused.close()
     e = make_engine(pA, m, kernel=kernel, precision=precision)
     e.nn_update()
-    assert e.pair_form() == (precision if kernel == 'gaussian' else precision + '-uniform')
+    assert e.pair_form() == precision + '-uniform'
     e.continuity()
     c = e.download()
     e.momentum()
