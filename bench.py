@@ -643,9 +643,9 @@ STATS_NOTES = {
     'kt_stress': 'Workload: 2000 x 2000 jittered fluid particles at dx = 1e-3 m (h = 1.3 dx, velocities '
                  'N(0, (0.05 c0)^2)) in a closed box of 3 wall layers: 595,984 cells, 168,747,172 neighbour '
                  'pairs. Per kernel (cubic, Gaussian) and precision, 20 launches each of the neighbour update, '
-                 'the continuity pass, the momentum pass and the fused pass (the cubic spline takes the uniform '
-                 'passes `k_pair_uniform*`, the Gaussian the general `k_pair<...>`; the last template argument '
-                 'of either is the mode: 0 fused, 1 continuity, 2 momentum).',
+                 'the continuity pass, the momentum pass and the fused pass, all in the uniform form '
+                 '`k_pair_uniform<kernel, mode>` / `k_pair_uniform64<kernel, mode>` (kernel 0 cubic, 1 Gaussian; '
+                 'mode 0 fused, 1 continuity, 2 momentum).',
     'kt_ens': 'Workload: 256 dam breaks (column width U(0.5, 1.5) m, height U(0.5, 2.5) m, dx from '
               '{0.005, 0.01, 0.02} m, alpha U(0.005, 0.05), 4.0 x 3.0 m tank), 961 to 136,948 fluid particles '
               'each, FP32C pairs, FP64 integrator, PEC, per-instance dt; every stage one launch over all '

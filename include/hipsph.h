@@ -340,6 +340,12 @@ int sph_tait_eos(int64_t n, double gamma, double B, double rho0, double Pb, cons
  * autotuner (offshore-sph_amd/hipsph/autotune.py). */
 int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads);
 int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads);
+/* No reference counterpart: whether the uniform float pair passes (FP32C,
+ * one h, one fluid mass; sph_compute, the split passes and the step's fused
+ * pass) stage each workgroup's neighbour records in LDS before the walk (1,
+ * the default) or read them from global memory (0). Results are
+ * bit-identical either way; for tests and tuning. */
+int sph_ctx_set_pair_staging(sph_ctx_t *ctx, int32_t on);
 /* No reference counterpart: which form of the fused pair pass the context
  * runs, chosen from the state at each upload, restore and scatter:
  * SPH_PAIR_FP64 / SPH_PAIR_FP32C (general), or the forms for one h for every

@@ -436,5 +436,102 @@ __device__ inline double write_pair_records(int t, double x, double y, double vx
     return pr;
 }
 
+// ---------------------------------------------------------------------------
+// staged walks of the uniform pair passes
+// ---------------------------------------------------------------------------
+// Staged walk: the A records of the slot ranges a workgroup's targets walk,
+// one range per row offset, are copied to LDS once (coalesced) and the walk
+// and the flush read them there (the scattered 16-byte loads of the walk
+// kept the vector memory path of the float pass 80 % busy); a workgroup
+// whose ranges hold more than STAGE_CAP records walks global memory instead.
+// 1024 (16 KB; the 1M dam break's workgroups need about 800): 5 workgroups
+// per CU, as many as the fused float pass's 81 registers allow. Staging B as
+// well (3 workgroups per CU) or a shorter list (a flush mid-walk in most
+// threads) cost more than the walk saves. Workgroups of at most 256 threads.
+#ifndef HIPSPH_STAGE_CAP
+#define HIPSPH_STAGE_CAP 1024
+#endif
+constexpr int kStageCap = HIPSPH_STAGE_CAP;
+static_assert(kStageCap > 0 && kStageCap < 65536, "LDS indices are 16-bit");
+
+// The runs of slots target c walks in rows c.y - 1, c.y, c.y + 1 (index d =
+// 0, 1, 2): [rs[d], re[d]), empty for a row outside the grid.
+template <typename Args>
+__device__ inline void row_runs(const Args &P, int ncx, int ncy, long long base, int2 c, int rs[3], int re[3]) {
+    const int lo = c.x - 1 > 0 ? c.x - 1 : 0;
+    const int hi = c.x + 1 < ncx - 1 ? c.x + 1 : ncx - 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int cy = c.y + d - 1;
+        rs[d] = 0;
+        re[d] = 0;
+        if (lo <= hi && ncy > cy && cy > -1) {
+            const long long row = base + (long long)ncx * cy;
+            rs[d] = P.cellStart[row + lo];
+            re[d] = P.cellStart[row + hi + 1];
+        }
+    }
+}
+
+// The workgroup's staging plan: row range d is [smin[d], smin[d] + len[d])
+// of the slots, held from LDS record off[d] on.
+struct Stage {
+    int smin[3], len[3], off[3], total;
+};
+
+// Folds every thread's row runs (empty for a thread without a walk) into
+// the workgroup's ranges. Ends with a barrier; every thread returns the same
+// plan.
+__device__ inline Stage stage_plan(const int rs[3], const int re[3], int *s_red) {
+    int v[6];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const bool any = rs[d] < re[d];
+        v[d] = any ? rs[d] : 0x7fffffff;
+        v[3 + d] = any ? re[d] : -0x7fffffff;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int a = __shfl_xor(v[d], o, 64), b = __shfl_xor(v[3 + d], o, 64);
+            v[d] = a < v[d] ? a : v[d];
+            v[3 + d] = b > v[3 + d] ? b : v[3 + d];
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s_red[q * 4 + w] = v[q];
+    __syncthreads();
+    Stage S;
+    int off = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int mn = 0x7fffffff, mx = -0x7fffffff;
+        for (int k = 0; k < nw; ++k) {
+            const int a = s_red[d * 4 + k], b = s_red[(3 + d) * 4 + k];
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+        }
+        S.smin[d] = mn;
+        S.len[d] = mx > mn ? mx - mn : 0;
+        S.off[d] = off;
+        off += S.len[d];
+    }
+    S.total = off;
+    return S;
+}
+
+// Copies the planned ranges of A into LDS. Ends with a barrier.
+template <typename Args>
+__device__ inline void stage_copy(const Args &P, const Stage &S, double2 *sA) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        for (int i = threadIdx.x; i < S.len[d]; i += blockDim.x)
+            sA[S.off[d] + i] = at32(P.A, (unsigned)(S.smin[d] + i));
+    __syncthreads();
+}
+
 }  // namespace
 }  // namespace hipsph

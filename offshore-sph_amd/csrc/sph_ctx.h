@@ -35,6 +35,7 @@ struct sph_ctx {
     // are stale
     unsigned long long gen = 0;
     int pair_threads = 256;
+    int pair_staging = 1;  // the uniform float pair passes stage each workgroup's records in LDS
     // one h for every particle and one mass for every fluid particle in the
     // current state (found at upload, restore and scatter): the fused float
     // pass then takes its uniform form, with kRecUniform records

@@ -1239,7 +1239,11 @@ static int rec_format(const sph_ctx *c) {
     return uni ? kRecUniform64 : kRecFp64;
 }
 
-static UniformParams uniform_params(const sph_ctx *c) { return make_uniform_params(c->P, c->uni_h, c->uni_m); }
+static UniformParams uniform_params(const sph_ctx *c) {
+    UniformParams u = make_uniform_params(c->P, c->uni_h, c->uni_m);
+    u.stage = c->pair_staging;
+    return u;
+}
 
 static void drop_graphs(sph_ctx *c) {
     for (int k = 0; k < 2; ++k)
@@ -1972,6 +1976,13 @@ int sph_ctx_set_tuning(sph_ctx_t *ctx, int32_t pair_threads) {
             ctx->gexec[k] = nullptr;
         }
     ++ctx->gen;
+    return 0;
+}
+
+int sph_ctx_set_pair_staging(sph_ctx_t *ctx, int32_t on) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    ctx->pair_staging = on ? 1 : 0;
+    drop_graphs(ctx);
     return 0;
 }
 

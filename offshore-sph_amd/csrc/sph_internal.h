@@ -137,6 +137,7 @@ struct UniformParams {
     double c01;      // (0.01 h) h
     double cij;      // 0.5 (c0 + 0)
     double alpha;
+    int stage = 1;   // the single-simulation float passes stage each workgroup's records in LDS
 };
 
 // Inputs of the pair pass, cell ordered.
@@ -319,6 +320,14 @@ __device__ inline const T &at32(const T *base, unsigned i) {
 #else
     return base[i];
 #endif
+}
+
+// base[i + j] for a constant j: the 32-bit byte offset of i, then j records
+// further in 64-bit address space, so that j folds into the load's
+// immediate offset (one offset computation for a run of loads).
+template <typename T>
+__device__ inline const T &at32j(const T *base, unsigned i, int j) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(T))[j];
 }
 
 // Integer power as a left-to-right product for small integer exponents

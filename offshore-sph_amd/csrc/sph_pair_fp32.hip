@@ -54,6 +54,7 @@ void uniform_const_fp32(const UniformParams &u, void *out) {
     K.D = (float)u.D;
     K.p1 = (float)u.p1;
     K.p2 = (float)u.p2;
+    K.stage = u.stage;
     *(UniformConst *)out = K;
 }
 

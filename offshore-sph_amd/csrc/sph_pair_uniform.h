@@ -97,6 +97,7 @@ struct UniformConst {
     float c01, ac, beta;   // 0.01 h^2, alpha c0 / 2, beta
     float r0, D;
     float p1, p2;
+    int stage;  // the single-simulation passes stage each workgroup's records in LDS
 };
 
 __device__ inline bool not_fluid(float prs) { return prs != prs; }
@@ -136,13 +137,23 @@ __device__ inline float3 uniform_terms(const UniformConst &K, float dx, float dy
         const float dwx = norm * dx, dwy = norm * dy;
         const float vx = vxi - b.x, vy = vyi - b.y;
         const float dv = vx * dx + vy * dy;
-        float pi = 0.0f;
-        if (dv < 0.0f) {
-            // mu (beta mu - alpha c_ij) / rho_ij, mu = h dv / (r^2 + 0.01 h^2), one reciprocal
+        // mu (beta mu - alpha c_ij) / rho_ij, mu = h dv / (r^2 + 0.01 h^2), one reciprocal
+        auto av = [&]() {
             const float rhoij = 0.5f * (rhoi + b.z);
             const float dn = r2 + K.c01;
             const float hv = K.h * dv;
-            pi = hv * (K.beta * hv - K.ac * dn) * __builtin_amdgcn_rcpf(dn * dn * rhoij);
+            return hv * (K.beta * hv - K.ac * dn) * __builtin_amdgcn_rcpf(dn * dn * rhoij);
+        };
+        float pi = 0.0f;
+        // The Gaussian (no branch in its gradient) takes the term for every
+        // pair and keeps it for the approaching ones (3-5 % faster there);
+        // the cubic spline and Wendland branch (selects here and in the
+        // support test made the cubic pass 20 % slower).
+        if constexpr (KERNEL == SPH_KERNEL_GAUSSIAN) {
+            const float v = av();
+            pi = dv < 0.0f ? v : 0.0f;
+        } else if (dv < 0.0f) {
+            pi = av();
         }
         const float fm = -K.m * (slf + b.w + pi);
         return make_float3(K.m * (vx * dwx + vy * dwy), fm * dwx, fm * dwy);
@@ -220,10 +231,10 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
         const double r2 = dx * dx + dy * dy;
         bool in = r2 <= K.lim2 && r2 > 0.0;
         if (KERNEL == SPH_KERNEL_GAUSSIAN && in && !(r2 < K.lo)) in = exact_q_test(X, Y, K.hd, a.x, a.y, K.hd, 3.0);
-        if (in) {
-            *lp = (unsigned)k;
-            lp += stride;
-        }
+        // written whether or not it is kept (within the list's bounds either
+        // way): no branch per candidate
+        *lp = (unsigned)k;
+        lp += in ? stride : 0u;
     };
     // A list about to overflow is flushed; with FLUSH_ALL (always for the
     // Gaussian) every active lane of the wave flushes with it, so the wave
@@ -243,7 +254,7 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
             if (overflow()) flush();
             double2 a[U];
 #pragma unroll
-            for (int j = 0; j < U; ++j) a[j] = at32(P.A, (unsigned)(k + j));
+            for (int j = 0; j < U; ++j) a[j] = at32j(P.A, (unsigned)k, j);
 #pragma unroll
             for (int j = 0; j < U; ++j) test(a[j], k + j);
         }
@@ -260,6 +271,126 @@ __device__ inline bool uniform_rates(const UniformArgs &P, const UniformConst &K
     ay = MODE == kContinuity ? 0.0 : say - K.g;  // WCSPH.py:169
     return true;
 }
+
+// ---------------------------------------------------------------------------
+// staged walk
+// ---------------------------------------------------------------------------
+// uniform_rates over the staged records: the same walk (rows c.y - 1 ..
+// c.y + 1, each run of slots ascending, UNROLL candidates loaded before they
+// are tested) and the same flushes in the same order, so the same sums. The
+// list holds LDS record indices (16-bit).
+template <int KERNEL, int MODE, int LIST>
+__device__ inline bool uniform_rates_staged(const UniformArgs &P, const UniformConst &K, int ncx, int ncy,
+                                            long long base, int t, const float4 &bt, const int rs[3], const int re[3],
+                                            const Stage &S, const double2 *sA, unsigned short *s_list,
+                                            double &drho, double &ax, double &ay) {
+    const double2 at = P.A[t];
+    const double X = at.x, Y = at.y;
+    const int2 c = P.cxy[t];
+    const unsigned stride = blockDim.x;
+    unsigned short *const list0 = s_list + threadIdx.x;
+    unsigned short *lp = list0;
+    double srho = 0.0, sax = 0.0, say = 0.0;
+    // LDS record i holds slot i - sh[d] of the range d it lies in
+    const int sh0 = S.off[0] - S.smin[0], sh1 = S.off[1] - S.smin[1], sh2 = S.off[2] - S.smin[2];
+    auto flush = [&]() {
+        for (unsigned short *q = list0; q != lp; q += stride) {
+            const int i = *q;
+            const double2 a = sA[i];
+            const int shi = i >= S.off[2] ? sh2 : (i >= S.off[1] ? sh1 : sh0);
+            const float4 b = at32(P.B, (unsigned)(i - shi));
+            const float3 f = uniform_terms<KERNEL>(K, (float)(X - a.x), (float)(Y - a.y), bt.x, bt.y, bt.z, bt.w, b);
+            if constexpr (MODE != kMomentum) srho += (double)f.x;
+            if constexpr (MODE != kContinuity) {
+                sax += (double)f.y;
+                say += (double)f.z;
+            }
+        }
+        lp = list0;
+    };
+    constexpr int U = HIPSPH_UNIFORM_UNROLL;
+    static_assert(U >= 1 && U < LIST, "unroll within the list");
+    unsigned short *const full = list0 + (LIST - U) * stride;  // past this, U appends could overflow
+    auto test = [&](double2 a, int i) {
+        const double dx = X - a.x, dy = Y - a.y;
+        const double r2 = dx * dx + dy * dy;
+        bool in = r2 <= K.lim2 && r2 > 0.0;
+        if (KERNEL == SPH_KERNEL_GAUSSIAN && in && !(r2 < K.lo)) in = exact_q_test(X, Y, K.hd, a.x, a.y, K.hd, 3.0);
+        *lp = (unsigned short)i;  // written whether or not it is kept (within the list's bounds)
+        lp += in ? stride : 0u;
+    };
+    auto overflow = [&]() {
+        if constexpr (HIPSPH_UNIFORM_FLUSH_ALL || KERNEL == SPH_KERNEL_GAUSSIAN) return __ballot(lp > full) != 0;
+        return lp > full;
+    };
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!(rs[d] < re[d])) continue;
+        const int shd = d == 0 ? sh0 : (d == 1 ? sh1 : sh2);
+        const int e = re[d] + shd;  // the run in LDS record indices
+        int i = rs[d] + shd;
+        for (; i + U <= e; i += U) {
+            if (overflow()) flush();
+            double2 a[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) a[j] = sA[i + j];
+#pragma unroll
+            for (int j = 0; j < U; ++j) test(a[j], i + j);
+        }
+        if (i < e) {
+            if (overflow()) flush();
+            for (; i < e; ++i) test(sA[i], i);  // fewer than U
+        }
+    }
+    flush();
+    const bool inside = c.x >= 0 && c.x < ncx && c.y >= 0 && c.y < ncy;  // it finds itself: a neighbour
+    if (!inside && uniform_count(P, K, X, Y, c, ncx, ncy, base) == 0) return false;  // SolverTools.py:166-169
+    drho = srho;
+    ax = sax;
+    ay = MODE == kContinuity ? 0.0 : say - K.g;  // WCSPH.py:169
+    return true;
+}
+
+// LDS of a staged pass with a LIST-entry list: the staged records and a
+// 16-bit list, or (a workgroup that walks global memory) a 32-bit list.
+template <int LIST>
+constexpr int stage_bytes() {
+    constexpr int staged = kStageCap * 16 + LIST * 256 * 2;
+    constexpr int global = LIST * 256 * 4;
+    return staged > global ? staged : global;
+}
+
+// One workgroup's walk, staged or not: every thread of the workgroup
+// constructs it (the plan and the copy have barriers); rates() then gives a
+// walking thread's rates as uniform_rates does.
+template <int KERNEL, int MODE, int LIST>
+struct StagedWalk {
+    unsigned char *mem;
+    int rs[3] = {0, 0, 0}, re[3] = {0, 0, 0};
+    Stage S{};
+    bool staged = false;
+
+    __device__ StagedWalk(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, int t, bool walks,
+                          unsigned char *s_mem, int *s_red)
+        : mem(s_mem) {
+        if (!K.stage) return;  // uniform over the launch
+        if (walks) row_runs(P, ncx, ncy, 0, P.cxy[t], rs, re);
+        S = stage_plan(rs, re, s_red);
+        staged = S.total <= kStageCap;
+        if (staged) stage_copy(P, S, reinterpret_cast<double2 *>(mem));
+    }
+
+    __device__ bool rates(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, int t, const float4 &bt,
+                          double &drho, double &ax, double &ay) {
+        if (staged)
+            return uniform_rates_staged<KERNEL, MODE, LIST>(P, K, ncx, ncy, 0, t, bt, rs, re, S,
+                                                            reinterpret_cast<const double2 *>(mem),
+                                                            reinterpret_cast<unsigned short *>(mem + kStageCap * 16),
+                                                            drho, ax, ay);
+        return uniform_rates<KERNEL, MODE, LIST>(P, K, ncx, ncy, 0, t, bt, reinterpret_cast<unsigned *>(mem), drho,
+                                                 ax, ay);
+    }
+};
 
 // One target t of the pass: its rates, or nothing for a wall or a target
 // that keeps its previous rates; MODE selects the rates written.
@@ -279,12 +410,24 @@ __device__ inline void uniform_target(const UniformArgs &P, const UniformConst &
 
 template <int KERNEL, int MODE>
 __global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformArgs P, UniformConst K) {
-    __shared__ unsigned s_list[uniform_list<KERNEL>() * 256];
+    constexpr int LIST = uniform_list<KERNEL>();
+    __shared__ __align__(16) unsigned char s_mem[stage_bytes<LIST>()];
+    __shared__ int s_red[6 * 4];
     if (!ctrl->active) return;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= P.n) return;
     const Grid &g = ctrl->grid;
-    uniform_target<KERNEL, MODE>(P, K, (int)g.ncx, (int)g.ncy, 0, t, s_list);
+    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
+    float4 bt = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
+    if (t < P.n) bt = P.B[t];
+    const bool walks = t < P.n && !not_fluid(bt.w);  // walls are not targets
+    StagedWalk<KERNEL, MODE, LIST> W(P, K, ncx, ncy, t, walks, s_mem, s_red);
+    double drho, ax, ay;
+    if (!walks || !W.rates(P, K, ncx, ncy, t, bt, drho, ax, ay)) return;
+    if constexpr (MODE != kMomentum) P.drho[t] = drho;
+    if constexpr (MODE != kContinuity) {
+        P.ax[t] = ax;
+        P.ay[t] = ay;
+    }
 }
 
 // A batch target t: its instance's grid and constants. The lanes of a wave
@@ -314,25 +457,33 @@ __device__ inline void uniform_target_batch(const BInst *inst, const int *instOf
 // 89, double arithmetic without contraction) and the time-step reduction's
 // terms of the new rates (TimeStep.py:58-90). The workgroup's dt partials go
 // to C.part[q * gridDim.x + blockIdx.x]; ctrl->dt_parts records the count.
+// With K.stage the workgroup stages its walk's records in LDS first (see
+// stage_plan); a workgroup whose ranges exceed the stage, or K.stage = 0,
+// walks global memory. The sums are the same either way.
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArgs P, UniformConst K, PecArgs C) {
-    __shared__ unsigned s_list[uniform_pec_list<KERNEL>() * 256];
+    constexpr int LIST = uniform_pec_list<KERNEL>();
+    __shared__ __align__(16) unsigned char s_mem[stage_bytes<LIST>()];
     __shared__ double lds[kDtQ * 16];
+    __shared__ int s_red[6 * 4];
     if (!ctrl->active) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const Grid &g = ctrl->grid;
+    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
+    float4 bt = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
+    if (t < P.n) bt = P.B[t];
+    const bool walks = t < P.n && !not_fluid(bt.w);
+    StagedWalk<KERNEL, kFused, LIST> W(P, K, ncx, ncy, t, walks, s_mem, s_red);
     DtVars v;
     if (t < P.n) {
-        const float4 bt = P.B[t];
-        if (!not_fluid(bt.w)) {
-            const Grid &g = ctrl->grid;
+        if (walks) {
             const double dt = ctrl->dt, damping = ctrl->damping;
 #if HIPSPH_PEC_EARLY
             const PecStart p0 = pec_start(C, t, dt, damping);
 #endif
             double drho, ax, ay;
-            if (uniform_rates<KERNEL, kFused, uniform_pec_list<KERNEL>()>(P, K, (int)g.ncx, (int)g.ncy, 0, t, bt,
-                                                                          s_list, drho, ax, ay)) {
+            if (W.rates(P, K, ncx, ncy, t, bt, drho, ax, ay)) {
                 P.drho[t] = drho;
                 P.ax[t] = ax;
                 P.ay[t] = ay;

@@ -166,10 +166,10 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
     auto test = [&](double2 a, int k) {
         const double dx = X - a.x, dy = Y - a.y;
         const double r2 = dx * dx + dy * dy;
-        if (r2 <= K.lim2 && r2 > 0.0) {
-            *lp = (unsigned)k;
-            lp += stride;
-        }
+        // written whether or not it is kept (within the list's bounds either
+        // way): no branch per candidate
+        *lp = (unsigned)k;
+        lp += (r2 <= K.lim2 && r2 > 0.0) ? stride : 0u;
     };
     // src/Tools/NNLinkedList.py:50-56: x shift outer, y shift inner, cells at
     // the grid's edge dropped; each cell from its highest slot down (:129-141)
@@ -186,7 +186,7 @@ __device__ inline bool uniform64_rates(const Uniform64Args &P, const Uniform64Co
                 if (overflow()) flush();
                 double2 a[U];
 #pragma unroll
-                for (int j = 0; j < U; ++j) a[j] = at32(P.A, (unsigned)(k - j));
+                for (int j = 0; j < U; ++j) a[j] = at32j(P.A, (unsigned)k, -j);
 #pragma unroll
                 for (int j = 0; j < U; ++j) test(a[j], k - j);
             }

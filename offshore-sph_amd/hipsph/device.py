@@ -365,6 +365,12 @@ class Engine:
             out['r'] = r[:total.value].cpu().numpy()
         return out
 
+    def set_pair_staging(self, on):
+        """Whether the uniform float pair passes stage each workgroup's
+        neighbour records in LDS (the default) or walk global memory; the
+        results are bit-identical either way."""
+        call('sph_ctx_set_pair_staging', self._ctx, int(bool(on)))
+
     def pair_form(self):
         """Which form of the fused pair pass the context runs: 'fp64', 'fp32c'
         (general) or 'fp64-uniform', 'fp32c-uniform' (one h for every
