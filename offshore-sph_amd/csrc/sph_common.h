@@ -454,10 +454,33 @@ __device__ inline double write_pair_records(int t, double x, double y, double vx
 constexpr int kStageCap = HIPSPH_STAGE_CAP;
 static_assert(kStageCap > 0 && kStageCap < 65536, "LDS indices are 16-bit");
 
+// Which diagonal neighbour cells of a target at (X, Y) in cell c lie wholly
+// beyond the listing radius (r^2 > lim2 for every point of the cell): bit 0
+// below left, 1 below right, 2 above left, 3 above right. With cells 2h wide
+// and the radius 2h, about a fifth of them (the target near its cell's
+// centre); their particles could only be candidates that fail the test, so
+// skipping them leaves every sum as it was. Distances to the cell edges are
+// taken 1e-9 cs short of the computed ones, far more than the rounding of
+// cell_of's floor((x - xmin) / cs) can move a particle. (The staged float
+// walk skips them; in the double walk the extra registers cost more than
+// the skipped candidates save.)
+__device__ inline unsigned far_corners(double X, double Y, int2 c, const Grid &g, double lim2) {
+    const double e = 1e-9 * g.cs;
+    const double x0 = g.xmin + g.cs * c.x, y0 = g.ymin + g.cs * c.y;
+    double l = X - x0 - e, r = x0 + g.cs - X - e, b = Y - y0 - e, a = y0 + g.cs - Y - e;
+    l = l > 0.0 ? l * l : 0.0;
+    r = r > 0.0 ? r * r : 0.0;
+    b = b > 0.0 ? b * b : 0.0;
+    a = a > 0.0 ? a * a : 0.0;
+    return (l + b > lim2 ? 1u : 0u) | (r + b > lim2 ? 2u : 0u) | (l + a > lim2 ? 4u : 0u) | (r + a > lim2 ? 8u : 0u);
+}
+
 // The runs of slots target c walks in rows c.y - 1, c.y, c.y + 1 (index d =
-// 0, 1, 2): [rs[d], re[d]), empty for a row outside the grid.
+// 0, 1, 2): [rs[d], re[d]), empty for a row outside the grid; the diagonal
+// cells in `far` (far_corners) are left out of the runs of rows 0 and 2.
 template <typename Args>
-__device__ inline void row_runs(const Args &P, int ncx, int ncy, long long base, int2 c, int rs[3], int re[3]) {
+__device__ inline void row_runs(const Args &P, int ncx, int ncy, long long base, int2 c, unsigned far, int rs[3],
+                                int re[3]) {
     const int lo = c.x - 1 > 0 ? c.x - 1 : 0;
     const int hi = c.x + 1 < ncx - 1 ? c.x + 1 : ncx - 1;
 #pragma unroll
@@ -466,9 +489,12 @@ __device__ inline void row_runs(const Args &P, int ncx, int ncy, long long base,
         rs[d] = 0;
         re[d] = 0;
         if (lo <= hi && ncy > cy && cy > -1) {
+            const unsigned f = d == 0 ? far : (d == 2 ? far >> 2 : 0u);
+            const int l = (f & 1u) && lo == c.x - 1 ? lo + 1 : lo;
+            const int h = (f & 2u) && hi == c.x + 1 ? hi - 1 : hi;
             const long long row = base + (long long)ncx * cy;
-            rs[d] = P.cellStart[row + lo];
-            re[d] = P.cellStart[row + hi + 1];
+            rs[d] = P.cellStart[row + l];
+            re[d] = P.cellStart[row + h + 1];
         }
     }
 }

@@ -370,11 +370,15 @@ struct StagedWalk {
     Stage S{};
     bool staged = false;
 
-    __device__ StagedWalk(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, int t, bool walks,
-                          unsigned char *s_mem, int *s_red)
+    __device__ StagedWalk(const UniformArgs &P, const UniformConst &K, const Grid &g, int ncx, int ncy, int t,
+                          bool walks, unsigned char *s_mem, int *s_red)
         : mem(s_mem) {
         if (!K.stage) return;  // uniform over the launch
-        if (walks) row_runs(P, ncx, ncy, 0, P.cxy[t], rs, re);
+        if (walks) {
+            const double2 at = P.A[t];
+            const int2 c = P.cxy[t];
+            row_runs(P, ncx, ncy, 0, c, far_corners(at.x, at.y, c, g, K.lim2), rs, re);
+        }
         S = stage_plan(rs, re, s_red);
         staged = S.total <= kStageCap;
         if (staged) stage_copy(P, S, reinterpret_cast<double2 *>(mem));
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformA
     float4 bt = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
     if (t < P.n) bt = P.B[t];
     const bool walks = t < P.n && !not_fluid(bt.w);  // walls are not targets
-    StagedWalk<KERNEL, MODE, LIST> W(P, K, ncx, ncy, t, walks, s_mem, s_red);
+    StagedWalk<KERNEL, MODE, LIST> W(P, K, g, ncx, ncy, t, walks, s_mem, s_red);
     double drho, ax, ay;
     if (!walks || !W.rates(P, K, ncx, ncy, t, bt, drho, ax, ay)) return;
     if constexpr (MODE != kMomentum) P.drho[t] = drho;
@@ -474,7 +478,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArg
     float4 bt = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
     if (t < P.n) bt = P.B[t];
     const bool walks = t < P.n && !not_fluid(bt.w);
-    StagedWalk<KERNEL, kFused, LIST> W(P, K, ncx, ncy, t, walks, s_mem, s_red);
+    StagedWalk<KERNEL, kFused, LIST> W(P, K, g, ncx, ncy, t, walks, s_mem, s_red);
     DtVars v;
     if (t < P.n) {
         if (walks) {
