@@ -44,16 +44,18 @@ namespace {
 #endif
 // Contributor-list entries per thread; a list is flushed before a group of
 // UNROLL candidates would overflow it, so a thread with more than LIST -
-// UNROLL candidates flushes mid-walk while the rest of its wave waits.
-// 26 (26 KB of LDS per workgroup, 6 workgroups per CU) for the rate passes:
-// on a jittered lattice (18-25 candidates) 24 costs 8 % more. 24 for the
-// passes with the corrector fused in (their reduction adds 512 B): 1 % less
-// on a regular lattice (20 candidates), measured on the 1M dam break.
+// UNROLL candidates flushes mid-walk while the rest of its wave waits: the
+// list must hold the contributors of most targets (20 on the initial
+// lattice, 18-25 once the flow has disordered it). 28 for the staged passes
+// (16-bit entries beside the staged records, 5 workgroups per CU either
+// way): on the 1M dam break after 4,000 steps 24 took 133 us against 102,
+// the same at the start. 26 for the batch's unstaged rate passes (26 KB of
+// LDS, 6 workgroups per CU).
 #ifndef HIPSPH_UNIFORM_LIST
-#define HIPSPH_UNIFORM_LIST 26
+#define HIPSPH_UNIFORM_LIST 28
 #endif
-#ifndef HIPSPH_UNIFORM_PEC_LIST
-#define HIPSPH_UNIFORM_PEC_LIST 24
+#ifndef HIPSPH_UNIFORM_BATCH_LIST
+#define HIPSPH_UNIFORM_BATCH_LIST 26
 #endif
 #ifndef HIPSPH_UNIFORM_GAUSS_LIST
 #define HIPSPH_UNIFORM_GAUSS_LIST 22
@@ -62,7 +64,6 @@ namespace {
 #define HIPSPH_UNIFORM_FLUSH_ALL 0
 #endif
 constexpr int kUniformList = HIPSPH_UNIFORM_LIST;
-constexpr int kUniformPecList = HIPSPH_UNIFORM_PEC_LIST;
 // The Gaussian's 3h support holds about 42 contributors: every list fills
 // at least once, whatever its length, so occupancy decides: 22 entries (7
 // workgroups per CU, as many as its registers allow) take 0.65 ms on a 4M
@@ -73,8 +74,8 @@ __host__ __device__ constexpr int uniform_list() {
     return KERNEL == SPH_KERNEL_GAUSSIAN ? kUniformGaussList : kUniformList;
 }
 template <int KERNEL>
-__host__ __device__ constexpr int uniform_pec_list() {
-    return KERNEL == SPH_KERNEL_GAUSSIAN ? kUniformGaussList : kUniformPecList;
+__host__ __device__ constexpr int uniform_batch_list() {
+    return KERNEL == SPH_KERNEL_GAUSSIAN ? kUniformGaussList : HIPSPH_UNIFORM_BATCH_LIST;
 }
 
 struct UniformArgs {
@@ -370,14 +371,16 @@ struct StagedWalk {
     Stage S{};
     bool staged = false;
 
-    __device__ StagedWalk(const UniformArgs &P, const UniformConst &K, const Grid &g, int ncx, int ncy, int t,
-                          bool walks, unsigned char *s_mem, int *s_red)
-        : mem(s_mem) {
-        if (!K.stage) return;  // uniform over the launch
+    long long base;  // the grid's first bin (a batch instance's cellBase)
+
+    __device__ StagedWalk(const UniformArgs &P, const UniformConst &K, const Grid &g, int ncx, int ncy,
+                          long long base_, int t, bool walks, unsigned char *s_mem, int *s_red)
+        : mem(s_mem), base(base_) {
+        if (!K.stage) return;  // uniform over the workgroup
         if (walks) {
             const double2 at = P.A[t];
             const int2 c = P.cxy[t];
-            row_runs(P, ncx, ncy, 0, c, far_corners(at.x, at.y, c, g, K.lim2), rs, re);
+            row_runs(P, ncx, ncy, base, c, far_corners(at.x, at.y, c, g, K.lim2), rs, re);
         }
         S = stage_plan(rs, re, s_red);
         staged = S.total <= kStageCap;
@@ -387,24 +390,26 @@ struct StagedWalk {
     __device__ bool rates(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, int t, const float4 &bt,
                           double &drho, double &ax, double &ay) {
         if (staged)
-            return uniform_rates_staged<KERNEL, MODE, LIST>(P, K, ncx, ncy, 0, t, bt, rs, re, S,
+            return uniform_rates_staged<KERNEL, MODE, LIST>(P, K, ncx, ncy, base, t, bt, rs, re, S,
                                                             reinterpret_cast<const double2 *>(mem),
                                                             reinterpret_cast<unsigned short *>(mem + kStageCap * 16),
                                                             drho, ax, ay);
-        return uniform_rates<KERNEL, MODE, LIST>(P, K, ncx, ncy, 0, t, bt, reinterpret_cast<unsigned *>(mem), drho,
-                                                 ax, ay);
+        return uniform_rates<KERNEL, MODE, LIST>(P, K, ncx, ncy, base, t, bt, reinterpret_cast<unsigned *>(mem),
+                                                 drho, ax, ay);
     }
 };
 
-// One target t of the pass: its rates, or nothing for a wall or a target
-// that keeps its previous rates; MODE selects the rates written.
+// One target t of the batch's unstaged rate pass: its rates, or nothing for
+// a wall or a target that keeps its previous rates; MODE selects the rates
+// written.
 template <int KERNEL, int MODE = kFused>
 __device__ inline void uniform_target(const UniformArgs &P, const UniformConst &K, int ncx, int ncy, long long base,
                                       int t, unsigned *s_list) {
     const float4 bt = P.B[t];
     if (not_fluid(bt.w)) return;  // walls are not targets
     double drho, ax, ay;
-    if (!uniform_rates<KERNEL, MODE>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay)) return;
+    if (!uniform_rates<KERNEL, MODE, uniform_batch_list<KERNEL>()>(P, K, ncx, ncy, base, t, bt, s_list, drho, ax, ay))
+        return;
     if constexpr (MODE != kMomentum) P.drho[t] = drho;
     if constexpr (MODE != kContinuity) {
         P.ax[t] = ax;
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform(const Ctrl *ctrl, UniformA
     float4 bt = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
     if (t < P.n) bt = P.B[t];
     const bool walks = t < P.n && !not_fluid(bt.w);  // walls are not targets
-    StagedWalk<KERNEL, MODE, LIST> W(P, K, g, ncx, ncy, t, walks, s_mem, s_red);
+    StagedWalk<KERNEL, MODE, LIST> W(P, K, g, ncx, ncy, 0, t, walks, s_mem, s_red);
     double drho, ax, ay;
     if (!walks || !W.rates(P, K, ncx, ncy, t, bt, drho, ax, ay)) return;
     if constexpr (MODE != kMomentum) P.drho[t] = drho;
@@ -466,7 +471,7 @@ __device__ inline void uniform_target_batch(const BInst *inst, const int *instOf
 // walks global memory. The sums are the same either way.
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArgs P, UniformConst K, PecArgs C) {
-    constexpr int LIST = uniform_pec_list<KERNEL>();
+    constexpr int LIST = uniform_list<KERNEL>();
     __shared__ __align__(16) unsigned char s_mem[stage_bytes<LIST>()];
     __shared__ double lds[kDtQ * 16];
     __shared__ int s_red[6 * 4];
@@ -478,7 +483,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArg
     float4 bt = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
     if (t < P.n) bt = P.B[t];
     const bool walks = t < P.n && !not_fluid(bt.w);
-    StagedWalk<KERNEL, kFused, LIST> W(P, K, g, ncx, ncy, t, walks, s_mem, s_red);
+    StagedWalk<KERNEL, kFused, LIST> W(P, K, g, ncx, ncy, 0, t, walks, s_mem, s_red);
     DtVars v;
     if (t < P.n) {
         if (walks) {
@@ -512,7 +517,7 @@ __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArg
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform(const BInst *inst, const int *instOf, UniformArgs P,
                                                             const UniformConst *Ks) {
-    __shared__ unsigned s_list[uniform_list<KERNEL>() * 256];
+    __shared__ unsigned s_list[uniform_batch_list<KERNEL>() * 256];
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     uniform_target_batch<KERNEL>(inst, instOf, P, Ks, t, s_list);
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform(const BInst *inst, c
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform_queue(const BInst *inst, const int *instOf, UniformArgs P,
                                                                   const UniformConst *Ks, int *queue) {
-    __shared__ unsigned s_list[uniform_list<KERNEL>() * 256];
+    __shared__ unsigned s_list[uniform_batch_list<KERNEL>() * 256];
     __shared__ int s_tile;
     const int ntiles = (int)((P.n + (long long)blockDim.x - 1) / blockDim.x);
     const int xcd = (int)(blockIdx.x % kXcds);
@@ -555,19 +560,29 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_queue(const BInst *i
 // same for the whole workgroup. The chunk's dt partials go to
 // C.part[q * NC + c], where the batch's corrector stores them. An instance
 // that is not active keeps its state; its fluid still enters the partials.
+// The tile's walk is staged in LDS as the single-simulation pass's (one
+// instance per tile: its grid and bins).
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *inst, const int *cs, const int *chunkInst, int NC,
                                                        UniformArgs P, const UniformConst *Ks, PecArgs C) {
-    __shared__ unsigned s_list[uniform_pec_list<KERNEL>() * 256];
+    constexpr int LIST = uniform_list<KERNEL>();
+    __shared__ __align__(16) unsigned char s_mem[stage_bytes<LIST>()];
     __shared__ double lds[kDtQ * 16];
+    __shared__ int s_red[6 * 4];
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= NC) return;  // the whole workgroup
     const int k = chunkInst[c];
     const BInst &I = inst[k];
-    DtVars v;
+    const Grid &g = I.grid;
+    const int ncx = (int)g.ncx, ncy = (int)g.ncy;
     const int t = cs[c] + (int)threadIdx.x;
-    if (t < cs[c + 1]) {
-        const float4 bt = P.B[t];
+    const bool in = t < cs[c + 1];
+    float4 bt = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
+    if (in) bt = P.B[t];
+    StagedWalk<KERNEL, kFused, LIST> W(P, Ks[k], g, ncx, ncy, I.cellBase, t, in && I.active && !not_fluid(bt.w),
+                                       s_mem, s_red);
+    DtVars v;
+    if (in) {
         if (!not_fluid(bt.w)) {
             double drho, ax, ay;
             if (I.active) {
@@ -577,8 +592,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *ins
 #if HIPSPH_PEC_EARLY
                 const PecStart p0 = pec_start(Ci, t, dt, damping);
 #endif
-                if (uniform_rates<KERNEL, kFused, uniform_pec_list<KERNEL>()>(P, Ks[k], (int)I.grid.ncx, (int)I.grid.ncy,
-                                                                              I.cellBase, t, bt, s_list, drho, ax, ay)) {
+                if (W.rates(P, Ks[k], ncx, ncy, t, bt, drho, ax, ay)) {
                     P.drho[t] = drho;
                     P.ax[t] = ax;
                     P.ay[t] = ay;
