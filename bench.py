@@ -112,6 +112,35 @@ def run_gpu(sc, precision, steps, warmup, timing_steps=50, host_reps=0):
     return res
 
 
+def evolved_timing(sc, before=4000, steps=200, timing_steps=50):
+    """The FP32C step on a flow that has run `before` steps from the initial
+    lattice (the flow disorders it: more candidates per target, uneven
+    contributor counts): K steps replayed from the graphs, then eager steps
+    with HIP events around the pair pass and the neighbour update."""
+    import torch
+    from hipsph.device import Engine, EngineParams, soa_from_particles
+    e = Engine(EngineParams.from_method(sc.n, sc.method(), kernel='cubic', precision='fp32c', integrator='pec'))
+    e.upload(soa_from_particles(sc.pA))
+    done = 0
+    while done < before:
+        k = min(1000, before - done)
+        e.step(k, graph=True)
+        torch.cuda.synchronize()
+        done += k
+    t0 = time.perf_counter()
+    e.step(steps, graph=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    e.timing_enable(True)
+    e.step(timing_steps)
+    tm = e.timing_read()
+    e.timing_enable(False)
+    st = e.status()
+    e.close()
+    return {'steps_before': before, 'sim_time_s': st['t'], 'steps': steps, 'ms_per_step': (t1 - t0) / steps * 1e3,
+            'pair_ms': tm['pair_ms'], 'nn_ms': tm['nn_ms'], 'flags': st['flags']}
+
+
 def run_cpu(sc, budget_s=12.0, max_steps=20, nthreads=1):
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle
@@ -538,7 +567,7 @@ PROFILE_COMMANDS = """    cd /tmp && export TMPDIR=/tmp      # OUT: an output di
     rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_INSTS_SALU \\
         SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_WAIT_INST_ANY -f csv -d OUT/pmc_sq -o sq -- python3 -u R/bench.py --profile-pair 5
     rocprofv3 --kernel-trace --stats -f csv -d OUT/kt_step -o kt -- python3 -u R/bench.py --steps 200 --warmup 10 \\
-        --no-cpu --no-precision --no-adjoint --stress-reps 0 --ensemble-steps 0
+        --no-cpu --no-precision --no-adjoint --stress-reps 0 --ensemble-steps 0 --evolved-steps 0
     python3 R/bench.py --summarize OUT"""
 
 
@@ -740,6 +769,8 @@ def main():
     ap.add_argument('--ensemble-steps', type=int, default=200, help='steps of the 256-instance ensemble in the '
                     'default run (0 skips it)')
     ap.add_argument('--no-precision', action='store_true', help='skip the FP32C-vs-FP64 error report')
+    ap.add_argument('--evolved-steps', type=int, default=4000, help='steps run before timing the FP32C step on '
+                    'the evolved flow (0: skip)')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward+adjoint timing')
     ap.add_argument('--adjoint', action='store_true', help='only run the forward+adjoint timing')
     ap.add_argument('--ensemble', action='store_true', help='only run the 256-instance batched ensemble')
@@ -802,6 +833,11 @@ def main():
         return d.get(prec, {}).get(field) if d else None
 
     prec_report = None if a.no_precision else precision_report(sc)
+    evolved = evolved_timing(sc, a.evolved_steps) if a.evolved_steps else None
+    if evolved:
+        print(f'# fp32c after {evolved["steps_before"]} steps (t = {evolved["sim_time_s"]:.4f} s): '
+              f'{evolved["ms_per_step"]:.4f} ms/step, pair {evolved["pair_ms"]:.4f} ms, neighbour update '
+              f'{evolved["nn_ms"]:.4f} ms', file=sys.stderr, flush=True)
     adj = None if a.no_adjoint else adjoint_timing()
     st4m = stress(a.stress_reps, emit=False) if a.stress_reps else None
     ens = ensemble(steps=a.ensemble_steps, seq_instances=16) if a.ensemble_steps else None
@@ -875,6 +911,7 @@ def main():
                  'stage_ms': {('pair_and_corrector' if fused64 else 'pair'): r64['pair_ms'],
                               'neighbour_update': r64['nn_ms']}},
         'precision': prec_report,
+        'evolved': evolved,
         'fwd_adjoint_over_fwd': adj['ratio'] if adj else None,
         'adjoint': adj,
         'stress_4M': st4m,
