@@ -309,30 +309,40 @@ def _run_both(sc, steps, kernel='cubic', integrator='pec', precision='fp64', gra
 
 
 @pytest.mark.parametrize('case', ['dam_break_c2', 'lattice_h3'])
-def test_staged_walk_matches_global_walk(case):
-    """The fused float pass with each workgroup's neighbour records staged
-    in LDS (the default) against the same pass reading them from global
-    memory: every field bit-identical after 30 PEC steps (graph replay). The
-    dam break stages every workgroup; the lattice with h = 3 dx (about 36
-    particles per cell) puts some workgroups' ranges over the stage, which
-    then walk global memory."""
+def test_staged_walk_matches_global_walk(case, precision='fp32c'):
+    """The float passes with each workgroup's neighbour records staged in
+    LDS (the default) against the same passes reading them from global
+    memory: every field bit-identical after 30 PEC steps (graph replay), then
+    one plain and one split pass. The dam break stages every workgroup; the
+    lattice with h = 3 dx (about 36 particles per cell) puts some
+    workgroups' ranges over the stage, which then walk global memory."""
     sc = (scenarios.dam_break_c2() if case == 'dam_break_c2'
           else scenarios.stress_lattice(nside=160, hfac=3.0))
     m = sc.method()
     out = []
     for staging in (True, False):
-        say('staged walk', case, staging)
-        e = make_engine(sc.pA, m, precision='fp32c')
-        assert e.pair_form() == 'fp32c-uniform'
+        say('staged walk', case, precision, staging)
+        e = make_engine(sc.pA, m, precision=precision)
+        assert e.pair_form() == precision + '-uniform'
         e.set_pair_staging(staging)
         e.step(30, graph=True)
         st = e.status()
         assert st['steps'] == 30 and st['flags'] & 3 == 0
-        out.append((e.download(), e.dt_history(30)))
+        d = e.download()
+        hist = e.dt_history(30)
+        e.nn_update()
+        e.compute()
+        dc = e.download()
+        e.momentum()
+        dm = e.download()
+        out.append((d, hist, dc, dm))
         e.close()
     np.testing.assert_array_equal(out[0][1], out[1][1])
     for k in FIELDS:
         np.testing.assert_array_equal(out[0][0][k], out[1][0][k], err_msg=k)
+    for k in ('drho', 'ax', 'ay'):
+        np.testing.assert_array_equal(out[0][2][k], out[1][2][k], err_msg='compute ' + k)
+        np.testing.assert_array_equal(out[0][3][k], out[1][3][k], err_msg='momentum ' + k)
 
 
 def test_step_c1_fp64_bitexact_1000():
