@@ -136,9 +136,11 @@ def evolved_timing(sc, before=4000, steps=200, timing_steps=50):
     tm = e.timing_read()
     e.timing_enable(False)
     st = e.status()
+    border = e.border_count()
     e.close()
     return {'steps_before': before, 'sim_time_s': st['t'], 'steps': steps, 'ms_per_step': (t1 - t0) / steps * 1e3,
-            'pair_ms': tm['pair_ms'], 'nn_ms': tm['nn_ms'], 'flags': st['flags']}
+            'pair_ms': tm['pair_ms'], 'nn_ms': tm['nn_ms'], 'flags': st['flags'],
+            'border_bounds_updates': border, 'updates': before + steps + timing_steps}
 
 
 def run_cpu(sc, budget_s=12.0, max_steps=20, nthreads=1):

@@ -346,6 +346,16 @@ int sph_ctx_get_tuning(sph_ctx_t *ctx, int32_t *pair_threads);
  * the default) or read them from global memory (0). Results are
  * bit-identical either way; for tests and tuning. */
 int sph_ctx_set_pair_staging(sph_ctx_t *ctx, int32_t on);
+/* No reference counterpart (NNLinkedList._init, src/Tools/NNLinkedList.py:
+ * 86-101, scans every particle): whether the neighbour update of a fused PEC
+ * step (one h) takes the grid's bounds from the particles of the border cells
+ * alone when the previous corrector's displacement bound shows that no other
+ * particle can hold an extreme (1, the default), or scans every particle (0).
+ * The grid is the same either way; for tests. */
+int sph_ctx_set_border_bounds(sph_ctx_t *ctx, int32_t on);
+/* No reference counterpart: how many neighbour updates of this context took
+ * their bounds from the border cells. Synchronises `stream`. */
+int sph_nn_border_count(sph_ctx_t *ctx, int64_t *count, void *stream);
 /* No reference counterpart: which form of the fused pair pass the context
  * runs, chosen from the state at each upload, restore and scatter:
  * SPH_PAIR_FP64 / SPH_PAIR_FP32C (general), or the forms for one h for every

@@ -36,6 +36,7 @@ struct sph_ctx {
     unsigned long long gen = 0;
     int pair_threads = 256;
     int pair_staging = 1;  // the uniform float pair passes stage each workgroup's records in LDS
+    int border_bounds = 1;  // the fused PEC step's bounds from the border cells when the displacement allows
     // one h for every particle and one mass for every fluid particle in the
     // current state (found at upload, restore and scatter): the fused float
     // pass then takes its uniform form, with kRecUniform records

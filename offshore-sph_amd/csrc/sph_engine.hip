@@ -65,7 +65,10 @@ Phys hipsph::make_phys(const sph_params_t &P) {
 __global__ void k_dt_partial(Ctrl *ctrl, const int8_t *label, const double *h, const double *ax, const double *ay,
                              int n, double *part) {
     __shared__ double lds[kDtQ * 16];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->dt_parts = gridDim.x;
+        ctrl->pec_parts = 0;
+    }
     DtVars v;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         if (label[i] == kFluid) v.add(h[i], ax[i], ay[i]);
@@ -76,7 +79,18 @@ __global__ void k_dt_partial(Ctrl *ctrl, const int8_t *label, const double *h, c
 // force (:51-56), min(c, f) (:45), from the partials of the fluid reduction
 // (ctrl->dt_parts of them, as the kernel that wrote them recorded). Also opens the step: the reference's loop condition
 // `while self.t < self.duration` (src/Solver.py:366) and `self.t += self.dt`
-// (:409) move here, so a captured step needs no host. One workgroup.
+// (:409) move here, so a captured step needs no host. It also bounds how far
+// this step's predictor puts any particle from where the last update binned
+// it (ctrl->disp, for k_bounds): after the fused PEC pass of the last step
+// (ctrl->pec_parts; dt0 its dt, V = ctrl->vbound its velocity bound at the
+// start, A0 = ctrl->abound its predictor's rates, A1 this fold's), a fluid
+// particle binned at x + dt0 v / 2 was corrected to x + dt0 vh, with |vh| <=
+// V + dt0 A0 / 2 (damping only shrinks it), and its new velocity v' has
+// |v'| <= V + dt0 A1, so per axis it now lies within dt0 (V + dt0 A0 / 2) +
+// dt0 V / 2 of where it was binned and this step's predictor moves it dt |v'|
+// / 2 further; V + dt0 A1 bounds the next step's start. Anything else (other
+// partials, non-finite rates, a halted step, a second call on the same
+// partials) leaves +inf: the next update scans every particle. One workgroup.
 __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *hist, const double *part) {
     __shared__ double lds[kDtQ * 16];
     const int R = ctrl->dt_parts;
@@ -108,6 +122,12 @@ __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *h
     }
     block_fold<kDtQ>(v, kind, lds);
     if (threadIdx.x != 0) return;
+    const int pec = ctrl->pec_parts;
+    const double dt0 = ctrl->dt, V = ctrl->vbound, A0 = ctrl->abound;
+    ctrl->pec_parts = 0;  // these partials are used once
+    ctrl->disp = INFINITY;
+    ctrl->vbound = INFINITY;
+    ctrl->abound = INFINITY;
     const double mn = v[0], mx = v[1], cnt = v[2], bad = v[3];
     const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID | SPH_FLAG_ZERO_DENSITY;
     if ((flags & halt) || !(t0 < duration)) {
@@ -143,6 +163,16 @@ __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *h
     if (advance_t) ctrl->t = t0 + dt;
     ctrl->step = step + 1;
     ctrl->active = 1;
+    if (bad == 0) {
+        const double A1 = sqrt(mx);
+        ctrl->abound = A1;
+        if (pec) {
+            const double V1 = V + dt0 * A1;
+            const double d = dt0 * (V + 0.5 * dt0 * A0) + 0.5 * dt0 * V + 0.5 * dt * V1;
+            if (d == d) ctrl->disp = d;  // NaN: +inf
+            ctrl->vbound = V1;
+        }
+    }
 }
 
 // KineticEnergy (src/Equations/KineticEnergy.py:5-12) over the fluid:
@@ -204,7 +234,10 @@ __global__ void k_correct(Ctrl *ctrl, State s, State o, const int *perm, int n, 
                           double *part, HmConst hc) {
     if (!ctrl->active) return;
     __shared__ double lds[kDtQ * 16];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->dt_parts = gridDim.x;
+        ctrl->pec_parts = 0;
+    }
     const double dt = ctrl->dt, damping = ctrl->damping;
     const int nm = ctrl->nm;
     DtVars v;
@@ -284,41 +317,108 @@ __device__ inline void upd_pos(const Ctrl *ctrl, const State &a, int i, int pred
     }
 }
 
+// Whether this update's bounds can come from the border cells alone (the
+// fused PEC step's update, with one h): the state is in the cell order of
+// the grid it was last binned on (ctrl->grid, the offsets in cellStart) and
+// no particle's predicted position lies more than ctrl->disp from where it
+// was binned, per axis, with 2 disp under the cell size by a margin far
+// above the rounding of the positions and of the cell arithmetic. Then the
+// particle binned at the old minimum x (column 0) is now at most disp to its
+// right, and one of a column >= 1 was binned at least a cell further right
+// and is at most disp to its left: the minimum lies in column 0. The one
+// binned at the old maximum x lies beyond ncx - 1 cells and one of a column
+// <= ncx - 3 within ncx - 2, so the maximum lies in columns ncx - 2 or
+// ncx - 1; y likewise with rows 0, ncy - 2 and ncy - 1.
+__device__ inline bool border_bounds_ok(const Ctrl *ctrl, const int *cellStart, int pred, const HmConst &hc) {
+    if (!cellStart || pred != 1 || !hc.on || ctrl->nm || !ctrl->nn_valid) return false;
+    const Grid &g = ctrl->grid;
+    if (g.ncx < 3 || g.ncy < 3) return false;
+    const double slack = 1e-12 * (fabs(g.xmin) + fabs(g.xmax) + fabs(g.ymin) + fabs(g.ymax));
+    return 2.0 * ctrl->disp * (1.0 + 1e-6) + slack < g.cs * (1.0 - 1e-6);  // false for +inf and NaN
+}
+
 // _init :86-101 bounds and _get_cell_size :103-110 min h, over all active
 // particles: per-workgroup partials, folded by k_grid. Grid-stride over at
 // most kBoundsBlocks workgroups, four particles per thread per round with
 // their loads issued together (one workgroup per 256 particles measured 40 %
 // slower, the per-workgroup fold paid in more rounds; 2048 workgroups with
-// one particle per round 3 % slower).
+// one particle per round 3 % slower). With cellStart (the fused PEC step)
+// and border_bounds_ok, only the particles of the border cells: rows 0,
+// ncy - 2 and ncy - 1 and the out-of-grid bin (which also holds what a flat
+// key carries past the last row), one slot range, grid-stride; and for each
+// row r <= ncy - 3 the run of columns ncx - 2, ncx - 1 of row r and column 0
+// of row r + 1 (consecutive keys: one slot range, one wavefront; it also
+// holds what a flat key carries past the last column into the next row),
+// over the first kBorderBlocks workgroups.
+// The extremes are the same values the full scan finds. With PEC's
+// predictor the full scan also finds the largest |vx|, |vy| (k_grid keeps it
+// as ctrl->vbound; +inf if one is not finite).
 constexpr int kBoundsBlocks = 1024;
-__global__ void k_bounds(const Ctrl *ctrl, State a, int n, int pred, double *part, HmConst hc) {
+// workgroups that walk the border cells (one per CU: 8.4 us on the 1M dam
+// break, against 9.6 with all 1,024 and 10.1 with 64)
+constexpr int kBorderBlocks = 256;
+__global__ void k_bounds(Ctrl *ctrl, State a, int n, int pred, double *part, HmConst hc, const int *cellStart) {
     if (!ctrl->active) return;
-    __shared__ double lds[kBoundsQ * 16];
+    __shared__ double lds[kBoundsVQ * 16];
     double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, hmx = -INFINITY;
-    double bad = 0.0;
+    double bad = 0.0, vmx = -INFINITY;
     const double dt = ctrl->dt;
+    auto take = [&](int i) {
+        double xi, yi;
+        upd_pos(ctrl, a, i, pred, dt, xi, yi);
+        const double hi = hc.on ? hc.h : a.h[i];
+        if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;  // NaN or Inf
+        xmn = xi < xmn ? xi : xmn;
+        xmx = xi > xmx ? xi : xmx;
+        ymn = yi < ymn ? yi : ymn;
+        ymx = yi > ymx ? yi : ymx;
+        hmn = hi < hmn ? hi : hmn;
+        hmx = hi > hmx ? hi : hmx;
+    };
     const int stride = gridDim.x * blockDim.x;
-    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (border_bounds_ok(ctrl, cellStart, pred, hc)) {
+        if (blockIdx.x >= kBorderBlocks) {  // the rest store empty partials
+            if (threadIdx.x == 0) {
+                const double e[kBoundsVQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, hc.h, hc.h, 0.0, -INFINITY};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * stride;
-            if (i < n) {
-                double xi, yi;
-                upd_pos(ctrl, a, i, pred, dt, xi, yi);
-                const double hi = hc.on ? hc.h : a.h[i];
-                if (!(xi - xi == 0.0) || !(yi - yi == 0.0)) bad += 1.0;  // NaN or Inf
-                xmn = xi < xmn ? xi : xmn;
-                xmx = xi > xmx ? xi : xmx;
-                ymn = yi < ymn ? yi : ymn;
-                ymx = yi > ymx ? yi : ymx;
-                hmn = hi < hmn ? hi : hmn;
-                hmx = hi > hmx ? hi : hmx;
+                for (int q = 0; q < kBoundsVQ; ++q) part[q * gridDim.x + blockIdx.x] = e[q];
+            }
+            return;
+        }
+        const int fstride = (kBorderBlocks < (int)gridDim.x ? kBorderBlocks : (int)gridDim.x) * blockDim.x;
+        if (gt == 0) ctrl->border_updates += 1;
+        hmn = hmx = hc.h;  // every particle's h
+        const Grid &g = ctrl->grid;
+        const long long ncx = g.ncx, ncy = g.ncy;
+        const int a0 = cellStart[0], a1 = cellStart[ncx];
+        const int b0 = cellStart[(ncy - 2) * ncx], b1 = cellStart[g.ncells + 1];
+        const int la = a1 - a0, lab = la + (b1 - b0);
+        for (int j = gt; j < lab; j += fstride) take(j < la ? a0 + j : b0 + (j - la));
+        const int lane = threadIdx.x & 63;
+        for (long long r = gt >> 6; r <= ncy - 3; r += fstride >> 6) {
+            const int s0 = cellStart[r * ncx + ncx - 2], s1 = cellStart[(r + 1) * ncx + 1];
+            for (int i = s0 + lane; i < s1; i += 64) take(i);
+        }
+    } else {
+        for (int i0 = gt; i0 < n; i0 += 4 * stride) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * stride;
+                if (i < n) {
+                    take(i);
+                    if (pred == 1) {
+                        const double ux = fabs(a.vx[i]), uy = fabs(a.vy[i]);
+                        const double w = (isfinite(ux) && isfinite(uy)) ? (ux > uy ? ux : uy) : INFINITY;
+                        vmx = w > vmx ? w : vmx;
+                    }
+                }
             }
         }
     }
-    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
-    double v[kBoundsQ] = {xmn, xmx, ymn, ymx, hmn, hmx, bad};
-    block_partials<kBoundsQ>(v, kind, lds, part, gridDim.x, blockIdx.x);
+    const int kind[kBoundsVQ] = {0, 1, 0, 1, 0, 1, 2, 1};
+    double v[kBoundsVQ] = {xmn, xmx, ymn, ymx, hmn, hmx, bad, vmx};
+    block_partials<kBoundsVQ>(v, kind, lds, part, gridDim.x, blockIdx.x);
 }
 
 // _get_cell_size :103-110 and _get_number_of_cells :112-127 from the folded
@@ -360,23 +460,27 @@ __device__ bool derive_grid(Ctrl *ctrl, double scale, long long cap, const doubl
     return true;
 }
 
-// Fold the R bounds partials and derive the grid. One workgroup; every load,
-// the active flag's included, is issued before the fold.
+// Fold the R bounds partials and derive the grid; a velocity bound that the
+// scan found (>= 0) becomes ctrl->vbound. One workgroup; every load, the
+// active flag's included, is issued before the fold.
 __global__ void k_grid(Ctrl *ctrl, const double *part, int R, double scale, long long cap) {
     const int active = ctrl->active;
-    __shared__ double lds[kBoundsQ * 16];
-    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
-    double v[kBoundsQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
+    __shared__ double lds[kBoundsVQ * 16];
+    const int kind[kBoundsVQ] = {0, 1, 0, 1, 0, 1, 2, 1};
+    double v[kBoundsVQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0, -INFINITY};
     for (int k = threadIdx.x; k < R; k += blockDim.x) {
-        double u[kBoundsQ];
+        double u[kBoundsVQ];
 #pragma unroll
-        for (int q = 0; q < kBoundsQ; ++q) u[q] = part[q * R + k];
+        for (int q = 0; q < kBoundsVQ; ++q) u[q] = part[q * R + k];
 #pragma unroll
-        for (int q = 0; q < kBoundsQ; ++q)
+        for (int q = 0; q < kBoundsVQ; ++q)
             v[q] = kind[q] == 0 ? (u[q] < v[q] ? u[q] : v[q]) : kind[q] == 1 ? (u[q] > v[q] ? u[q] : v[q]) : v[q] + u[q];
     }
-    block_fold<kBoundsQ>(v, kind, lds);
-    if (threadIdx.x == 0 && active) derive_grid(ctrl, scale, cap, v);
+    block_fold<kBoundsVQ>(v, kind, lds);
+    if (threadIdx.x == 0 && active) {
+        if (v[7] >= 0.0) ctrl->vbound = v[7];
+        derive_grid(ctrl, scale, cap, v);
+    }
 }
 
 // _bin :129-141: cid = floor((x - xmin) / cs) per axis (:146-176), flat key
@@ -1087,7 +1191,7 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
               hipMalloc(&c->permKey, sizeof(int) * n) == hipSuccess &&
               hipMalloc(&c->cxy_s, sizeof(int2) * n) == hipSuccess &&
 
-              hipMalloc(&c->part_b, sizeof(double) * kBoundsQ * kRedBlocks) == hipSuccess &&
+              hipMalloc(&c->part_b, sizeof(double) * kBoundsVQ * kRedBlocks) == hipSuccess &&
               // dt partials: a reduction's workgroups or the fused pair pass's
               hipMalloc(&c->part_d, sizeof(double) * kDtQ * (kRedBlocks > xcd_blocks(n, 256) ? kRedBlocks
                                                                                              : xcd_blocks(n, 256))) ==
@@ -1118,6 +1222,7 @@ int sph_ctx_create(const sph_params_t *params, sph_ctx_t **out) {
     ctrl_from_params(h, c->P);
     h.advance_t = 1;
     h.active = 1;
+    h.disp = h.vbound = h.abound = INFINITY;
     if (hipMemcpy(c->ctrl, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->dt_hist, 0, sizeof(double) * 3 * SPH_DT_HISTORY) != hipSuccess) {
         free_ctx(c);
@@ -1186,6 +1291,8 @@ int sph_state_upload(sph_ctx_t *ctx, const sph_state_t *st, int32_t reset_clock,
     h.flags = 0;
     h.active = 1;
     h.nn_valid = 0;
+    h.pec_parts = 0;
+    h.disp = h.vbound = h.abound = INFINITY;
     ctrl_from_params(h, ctx->P);
     HIPCHK(hipMemcpy(ctx->ctrl, &h, sizeof h, hipMemcpyHostToDevice));
     for (int k = 0; k < 2; ++k)
@@ -1304,7 +1411,9 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
         if (rc) return rc;
     }
     const int R = redblk(n) < kBoundsBlocks ? redblk(n) : kBoundsBlocks;
-    k_bounds<<<R, kBlock, 0, s>>>(c->ctrl, a, n, pred, c->part_b, hm_const(c));
+    // the border-cell bounds only in the step's update with PEC's predictor
+    const int *border = c->border_bounds && pred == 1 ? c->cellStart : nullptr;
+    k_bounds<<<R, kBlock, 0, s>>>(c->ctrl, a, n, pred, c->part_b, hm_const(c), border);
     k_grid<<<1, 1024, 0, s>>>(c->ctrl, c->part_b, R, c->P.scale, c->cap_cells);
     k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, n, pred, c->counts, c->bin, c->rank, c->binflag);
     long long tiles_cap = (c->cap_cells + 1 + kScanTile - 1) / kScanTile;
@@ -1610,6 +1719,7 @@ int sph_dt(sph_ctx_t *ctx, void *stream) {
 int sph_predict(sph_ctx_t *ctx, void *stream) {
     if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
     if (closed(ctx)) return SPH_ESTATE;
+    ctx->dt_fresh = false;  // the positions move: the corrector's displacement terms no longer hold
     return predict(ctx, S(stream));
 }
 
@@ -1986,6 +2096,22 @@ int sph_ctx_set_pair_staging(sph_ctx_t *ctx, int32_t on) {
     return 0;
 }
 
+int sph_ctx_set_border_bounds(sph_ctx_t *ctx, int32_t on) {
+    if (!ctx) return fail(SPH_EINVAL, "ctx is NULL");
+    ctx->border_bounds = on ? 1 : 0;
+    drop_graphs(ctx);
+    return 0;
+}
+
+int sph_nn_border_count(sph_ctx_t *ctx, int64_t *count, void *stream) {
+    if (!ctx || !count) return fail(SPH_EINVAL, "NULL argument");
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    Ctrl h;
+    HIPCHK(hipMemcpy(&h, ctx->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    *count = h.border_updates;
+    return 0;
+}
+
 int sph_pair_form(sph_ctx_t *ctx, int32_t *form) {
     if (!ctx || !form) return fail(SPH_EINVAL, "NULL argument");
     *form = rec_format(ctx);  // kRecFp64 .. kRecUniform64 are SPH_PAIR_* in the same order
@@ -2050,7 +2176,7 @@ static_assert(sizeof(Rec<double>::A) == 32 && sizeof(Rec<double>::B) == 32 && si
               "pair records are sized at 32 bytes in the workspace");
 constexpr size_t kNaOffCtrl = 0;
 constexpr size_t kNaOffPart = kNaOffCtrl + al256(sizeof(Ctrl));
-constexpr size_t kNaOffId = kNaOffPart + al256(sizeof(double) * kBoundsQ * kBoundsBlocks);
+constexpr size_t kNaOffId = kNaOffPart + al256(sizeof(double) * kBoundsVQ * kBoundsBlocks);
 constexpr size_t kNaOffBin = kNaOffId + al256(4 * kNaParticles);
 constexpr size_t kNaOffRank = kNaOffBin + al256(4 * kNaParticles);
 constexpr size_t kNaOffPerm = kNaOffRank + al256(4 * kNaParticles);
@@ -2180,7 +2306,7 @@ extern "C" int sph_loop_noalloc(const sph_params_t *params, const sph_state_t *s
     const long long ninit = (kNaCells + 1 > n ? kNaCells + 1 : n);
     k_na_init<<<nblk(ninit), kBlock, 0, s>>>(ctrl, counts, binflag, id, nn, kNaCells);
     const int R = redblk(n) < kBoundsBlocks ? redblk(n) : kBoundsBlocks;
-    k_bounds<<<R, kBlock, 0, s>>>(ctrl, a, nn, 0, part, HmConst{});
+    k_bounds<<<R, kBlock, 0, s>>>(ctrl, a, nn, 0, part, HmConst{}, nullptr);
     k_grid<<<1, 1024, 0, s>>>(ctrl, part, R, params->scale, kNaCells);
     k_keys<<<nblk(n), kBlock, 0, s>>>(ctrl, a, nn, 0, counts, bin, rank, binflag);
     const int tiles = (int)((kNaCells + 1 + kScanTile - 1) / kScanTile);

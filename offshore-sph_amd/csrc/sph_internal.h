@@ -48,6 +48,12 @@ struct Ctrl {
     int nm;              // Newmark-beta on the coupled particles
     double nm_beta, nm_gamma;
     int dt_parts;        // workgroups whose dt partials the partials buffer holds (set by their kernel)
+    int pec_parts;       // 1 when those partials come from the fused PEC pass (cleared by k_dt_final)
+    double disp;         // bound on how far any particle's predicted position lies from where it was
+                         // last binned, per axis (k_dt_final; +inf when unknown)
+    double vbound;       // bound on every particle's |vx|, |vy| at the start of the step (+inf: unknown)
+    double abound;       // bound on every fluid particle's |a| in the rates the step's predictor uses
+    long long border_updates;  // neighbour updates whose bounds came from the border cells
 };
 
 // Grid-wide reductions (bounds, dt) run over at most kRedBlocks workgroups in
@@ -58,6 +64,7 @@ struct Ctrl {
 // cost ~10 ns each, far more than the second launch.
 constexpr int kBoundsQ = 7;  // min/max x, min/max y, min/max h, non-finite count
 constexpr int kDtQ = 4;      // min h, max |a|^2, fluid count, non-finite count
+constexpr int kBoundsVQ = 8;  // the single engine's bounds: kBoundsQ, then the largest |vx|, |vy|
 
 // One cell-ordered buffer set.
 struct State {

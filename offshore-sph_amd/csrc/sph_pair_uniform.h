@@ -465,10 +465,11 @@ __device__ inline void uniform_target_batch(const BInst *inst, const int *instOf
 // slot), then the corrector's update of its slot (src/Integrators/PEC.py:61-
 // 89, double arithmetic without contraction) and the time-step reduction's
 // terms of the new rates (TimeStep.py:58-90). The workgroup's dt partials go
-// to C.part[q * gridDim.x + blockIdx.x]; ctrl->dt_parts records the count.
-// With K.stage the workgroup stages its walk's records in LDS first (see
-// stage_plan); a workgroup whose ranges exceed the stage, or K.stage = 0,
-// walks global memory. The sums are the same either way.
+// to C.part[q * gridDim.x + blockIdx.x]; ctrl->dt_parts records the count
+// and ctrl->pec_parts that the corrector moved the fluid. With K.stage the
+// workgroup stages its walk's records in LDS first (see stage_plan); a
+// workgroup whose ranges exceed the stage, or K.stage = 0, walks global
+// memory. The sums are the same either way.
 template <int KERNEL>
 __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArgs P, UniformConst K, PecArgs C) {
     constexpr int LIST = uniform_list<KERNEL>();
@@ -476,7 +477,10 @@ __global__ __launch_bounds__(256) void k_pair_uniform_pec(Ctrl *ctrl, UniformArg
     __shared__ double lds[kDtQ * 16];
     __shared__ int s_red[6 * 4];
     if (!ctrl->active) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->dt_parts = gridDim.x;
+        ctrl->pec_parts = 1;
+    }
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const Grid &g = ctrl->grid;
     const int ncx = (int)g.ncx, ncy = (int)g.ncy;

@@ -282,7 +282,10 @@ __global__ __launch_bounds__(256) void k_pair_uniform64_pec(Ctrl *ctrl, Uniform6
     __shared__ unsigned s_list[uniform64_pec_list<KERNEL>() * 256];
     __shared__ double lds[kDtQ * 16];
     if (!ctrl->active) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->dt_parts = gridDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->dt_parts = gridDim.x;
+        ctrl->pec_parts = 1;
+    }
     const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     DtVars v;
     if (t < P.n) {

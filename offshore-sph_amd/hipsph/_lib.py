@@ -126,6 +126,8 @@ _PROTOS = {
     'sph_ctx_set_tuning': (C.c_int, [_P, _I32]),
     'sph_ctx_get_tuning': (C.c_int, [_P, C.POINTER(_I32)]),
     'sph_ctx_set_pair_staging': (C.c_int, [_P, _I32]),
+    'sph_ctx_set_border_bounds': (C.c_int, [_P, _I32]),
+    'sph_nn_border_count': (C.c_int, [_P, C.POINTER(_I64), _P]),
     'sph_timing_enable': (C.c_int, [_P, _I32]),
     'sph_timing_read': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I64), _I32]),
     'sph_pair_form': (C.c_int, [_P, C.POINTER(_I32)]),

@@ -371,6 +371,19 @@ class Engine:
         results are bit-identical either way."""
         call('sph_ctx_set_pair_staging', self._ctx, int(bool(on)))
 
+    def set_border_bounds(self, on):
+        """Whether the fused PEC step's neighbour update takes the grid's
+        bounds from the border cells when the previous corrector's
+        displacement bound allows (the default) or scans every particle; the
+        grid is the same either way."""
+        call('sph_ctx_set_border_bounds', self._ctx, int(bool(on)))
+
+    def border_count(self):
+        """Neighbour updates that took their bounds from the border cells."""
+        v = C.c_int64()
+        call('sph_nn_border_count', self._ctx, C.byref(v), self._stream())
+        return int(v.value)
+
     def pair_form(self):
         """Which form of the fused pair pass the context runs: 'fp64', 'fp32c'
         (general) or 'fp64-uniform', 'fp32c-uniform' (one h for every

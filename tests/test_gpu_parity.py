@@ -345,6 +345,56 @@ def test_staged_walk_matches_global_walk(case, precision='fp32c'):
         np.testing.assert_array_equal(out[0][3][k], out[1][3][k], err_msg='momentum ' + k)
 
 
+@pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
+def test_border_bounds_match_full_scan(precision):
+    """The fused PEC step's neighbour update takes the grid's bounds from the
+    border cells once the previous corrector's displacement bound allows:
+    the same trajectory, dt series and grid as scanning every particle, over
+    41 steps (the first after the upload scans everything, the other 40 take
+    the border). Then a fluid particle is moved past the right wall by the
+    host (sph_state_scatter): the next update scans everything again, and
+    the two engines still agree after 10 more steps."""
+    sc = scenarios.dam_break_c2()
+    m = sc.method()
+    fluid = np.flatnonzero(sc.pA['label'] == 0)
+    mover = int(fluid[len(fluid) // 2])
+    out = []
+    for border in (True, False):
+        say('border bounds', precision, border)
+        e = make_engine(sc.pA, m, precision=precision)
+        assert e.pair_form() == precision + '-uniform'
+        e.set_border_bounds(border)
+        e.step(41, graph=True)
+        st = e.status()
+        assert st['steps'] == 41 and st['flags'] & 3 == 0
+        n1 = e.border_count()
+        d = e.download()
+        g1 = e.grid()
+        xmax = g1['xmax']
+        e.scatter([mover], {'x': [xmax + 0.3], 'vx': [0.0], 'vy': [0.0]})
+        e.step(1, graph=True)
+        n2 = e.border_count()
+        e.step(10, graph=True)
+        n3 = e.border_count()
+        st = e.status()
+        assert st['steps'] == 52 and st['flags'] & 3 == 0
+        d2 = e.download()
+        g2 = e.grid()
+        hist = e.dt_history(52)
+        out.append((d, g1, d2, g2, hist))
+        e.close()
+        if border:
+            assert (n1, n2, n3) == (40, 40, 50), (n1, n2, n3)
+            assert g2['xmax'] >= xmax + 0.29
+        else:
+            assert (n1, n2, n3) == (0, 0, 0)
+    assert out[0][1] == out[1][1] and out[0][3] == out[1][3]
+    np.testing.assert_array_equal(out[0][4], out[1][4])
+    for k in FIELDS:
+        np.testing.assert_array_equal(out[0][0][k], out[1][0][k], err_msg=k)
+        np.testing.assert_array_equal(out[0][2][k], out[1][2][k], err_msg='after the scatter ' + k)
+
+
 def test_step_c1_fp64_bitexact_1000():
     """C1 (dx 0.02 dam break), cubic, PEC, 1,000 steps: every field bit-identical to the oracle."""
     sc = scenarios.dam_break_c1()

@@ -146,3 +146,109 @@ def test_courant_known_answers():
     assert ts.courant(0.4, 1.0, 1.0) == 0.4
     assert ts.courant(0.4, 2.0, 4.0) == 0.5 * 0.4
     assert ts.force(0.25, 1.0, 0.0) == 1e10 and ts.force(0.25, 4.0, 1.0) == 0.5
+
+
+# --------------------------------------------------------------------------
+# the border-cell bounds of the fused PEC step's neighbour update: the two
+# facts k_bounds and k_dt_final rely on, checked in numpy
+# --------------------------------------------------------------------------
+def _bin(x, y, xmin, ymin, cs, ncx, ncy):
+    """The neighbour update's cells: floor((x - xmin) / cs) per axis, flat key
+    cx + ncx cy, anything outside [0, ncx ncy) in the out-of-grid bin."""
+    cx = np.floor((x - xmin) / cs)
+    cy = np.floor((y - ymin) / cs)
+    key = cx + ncx * cy
+    return np.where((key >= 0) & (key < ncx * ncy), key, ncx * ncy).astype(np.int64)
+
+
+def _border_keys(ncx, ncy):
+    """Rows 0, ncy - 2, ncy - 1, the out-of-grid bin, and for each row r <=
+    ncy - 3 the keys of columns ncx - 2, ncx - 1 of row r and column 0 of
+    row r + 1 (one run of consecutive keys)."""
+    keys = set(range(0, ncx)) | set(range((ncy - 2) * ncx, ncy * ncx + 1))
+    for r in range(ncy - 2):
+        keys |= set(range(r * ncx + ncx - 2, (r + 1) * ncx + 1))
+    return keys
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_border_cells_hold_the_extremes(seed):
+    """Bin particles on the grid of their bounds, then move each by at most D
+    per axis with 2 D just under the cell size: the new extremes all lie among
+    the particles of the border cells. Seeds 0-2 put the old maximum on an
+    exact multiple of the cell size (so a flat key carries it past the last
+    column or row), seeds 3-5 crowd particles against cell edges."""
+    rng = np.random.default_rng(seed)
+    cs = 0.01
+    n = 20000
+    ncx0, ncy0 = int(rng.integers(3, 40)), int(rng.integers(3, 40))
+    x = rng.uniform(0.0, ncx0 * cs, n)
+    y = rng.uniform(0.0, ncy0 * cs, n)
+    if seed < 3:
+        x[0], y[0] = 0.0, 0.0
+        x[1], y[1] = ncx0 * cs, ncy0 * cs  # the maximum on a cell edge
+    else:
+        x = np.round(x / cs) * cs + rng.choice([-1e-13, 0.0, 1e-13], n)
+        y = np.round(y / cs) * cs + rng.choice([-1e-13, 0.0, 1e-13], n)
+    xmin, xmax, ymin, ymax = x.min(), x.max(), y.min(), y.max()
+    ncx = max(1, int(math.ceil((1.0 / cs) * (xmax - xmin))))
+    ncy = max(1, int(math.ceil((1.0 / cs) * (ymax - ymin))))
+    assert ncx >= 3 and ncy >= 3
+    key = _bin(x, y, xmin, ymin, cs, ncx, ncy)
+    border = np.isin(key, np.fromiter(_border_keys(ncx, ncy), np.int64))
+    D = 0.5 * cs * (1.0 - 2e-6)
+    for trial in range(20):
+        # the worst case: every interior particle moves D toward the outside
+        sx = rng.choice([-1.0, 1.0], n) if trial else np.sign(x - 0.5 * (xmin + xmax))
+        sy = rng.choice([-1.0, 1.0], n) if trial else np.sign(y - 0.5 * (ymin + ymax))
+        nx = x + D * sx * (rng.uniform(0.0, 1.0, n) if trial % 2 else 1.0)
+        ny = y + D * sy * (rng.uniform(0.0, 1.0, n) if trial % 2 else 1.0)
+        assert nx[border].min() == nx.min() and nx[border].max() == nx.max()
+        assert ny[border].min() == ny.min() and ny[border].max() == ny.max()
+
+
+def test_border_cells_need_the_last_two_columns():
+    """Plants the mistake of walking only the last column: with the old
+    maximum just past a cell edge, a particle of column ncx - 2 overtakes it,
+    so the last two columns (and rows) are needed."""
+    cs = 0.01
+    x = np.array([0.0, 3.0 * cs - 1e-9, 3.0 * cs + 1e-6, 0.5 * cs, 0.5 * cs])
+    y = np.array([0.0, 1.5 * cs, 1.5 * cs, 3.5 * cs, 0.0])
+    ncx = int(math.ceil((1.0 / cs) * (x.max() - x.min())))
+    ncy = int(math.ceil((1.0 / cs) * (y.max() - y.min())))
+    key = _bin(x, y, x.min(), y.min(), cs, ncx, ncy)
+    assert key[1] % ncx == ncx - 2 and key[2] % ncx == ncx - 1
+    nx = x + np.array([0.0, 0.4 * cs, -0.4 * cs, 0.0, 0.0])
+    assert nx.argmax() == 1  # column ncx - 2 holds the new maximum
+
+
+@pytest.mark.parametrize('seed', range(4))
+def test_pec_displacement_bound(seed):
+    """k_dt_final's bound: a fluid particle binned at x + dt0 v / 2, corrected
+    by PEC (src/Integrators/PEC.py:61-89, with damping) and predicted again
+    with dt1 lies within dt0 (V + dt0 A0 / 2) + dt0 V / 2 + dt1 (V + dt0 A1) / 2
+    of where it was binned, per axis, and |v'| <= V + dt0 A1, for |v| <= V,
+    the predictor's rates |a0| <= A0 and the new rates |a1| <= A1."""
+    rng = np.random.default_rng(seed)
+    n = 200000
+    x0 = rng.uniform(-5.0, 5.0, n)
+    v0 = rng.uniform(-8.0, 8.0, n) * rng.choice([0.0, 1.0], n, p=[0.05, 0.95])
+    a0 = rng.uniform(-1e4, 1e4, n) * rng.uniform(0.0, 1.0, n) ** 3
+    a1 = rng.uniform(-1e4, 1e4, n) * rng.uniform(0.0, 1.0, n) ** 3
+    dt0 = rng.uniform(1e-6, 1e-3, n)
+    dt1 = dt0 * rng.uniform(0.5, 2.0, n)
+    damp = rng.choice([0.0, 0.0, 0.3, 5.0], n)
+    binned = x0 + 0.5 * dt0 * v0
+    vh = (v0 + 0.5 * dt0 * a0) / (1 + 0.5 * damp)
+    x1 = 2 * (x0 + 0.5 * dt0 * vh) - x0
+    v1 = 2 * ((v0 + 0.5 * dt0 * a1) / (1 + 0.5 * damp)) - v0
+    pred = x1 + 0.5 * dt1 * v1
+    V, A0, A1 = np.abs(v0), np.abs(a0), np.abs(a1)
+    V1 = V + dt0 * A1
+    D = dt0 * (V + 0.5 * dt0 * A0) + 0.5 * dt0 * V + 0.5 * dt1 * V1
+    slack = 1e-12 * (np.abs(x0) + 1.0)
+    assert np.all(np.abs(v1) <= V1 * (1 + 1e-12) + 1e-300)
+    assert np.all(np.abs(pred - binned) <= D * (1 + 1e-6) + slack)
+    # the bound is not loose by more than the damping and the signs allow
+    tight = (damp == 0) & (np.sign(v0) == np.sign(a0)) & (np.sign(v0) == np.sign(a1))
+    assert np.max((np.abs(pred - binned) / np.maximum(D, 1e-300))[tight]) > 0.5
