@@ -314,6 +314,12 @@ int sph_batch_set_list_len(sph_batch_t *batch, int32_t list_len);
  * workgroups take tiles from a per-XCD work queue until it is drained.
  * Results are identical either way. Chosen by the autotuner. */
 int sph_batch_set_persistent(sph_batch_t *batch, int32_t on);
+/* No reference counterpart: the border-cell bounds of sph_ctx_set_border_bounds
+ * for every instance of the batch (its fused PEC step), 1 (the default) or 0;
+ * and how many neighbour updates of each instance took them (counts: one per
+ * instance). Synchronises `stream`. */
+int sph_batch_set_border_bounds(sph_batch_t *batch, int32_t on);
+int sph_batch_border_count(sph_batch_t *batch, int64_t *counts, void *stream);
 /* Eager steps with timing on record HIP events around each step's pair pass
  * and corrector (one kernel when the corrector is fused into the uniform
  * pass); timing_read returns their mean in ms. */

@@ -49,9 +49,11 @@ struct BState {  // the concatenated cell-ordered state (as State)
 // per-instance dt (src/Solver.py:210-230, src/Equations/TimeStep.py:40-90)
 // ---------------------------------------------------------------------------
 // segment c (a tile) covers [cs[c], cs[c + 1]) inside one instance
-__global__ void kb_dt_partial(const int *cs, const int *segInst, const BInst *inst, State s, double *part) {
+__global__ void kb_dt_partial(const int *cs, const int *segInst, const BInst *inst, State s, double *part,
+                              Ctrl *view) {
     __shared__ double lds[kDtQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
+    if (c == 0 && threadIdx.x == 0) view->pec_parts = 0;
     const BInst &I = inst[segInst[c]];
     DtVars v;
     for (int t = cs[c] + threadIdx.x; t < cs[c + 1]; t += blockDim.x)
@@ -59,8 +61,10 @@ __global__ void kb_dt_partial(const int *cs, const int *segInst, const BInst *in
     v.store(part, NC, c, lds);
 }
 
-// One workgroup per instance: fold its chunks, open or close its step.
-__global__ void kb_dt_final(BInst *inst, const int *chunk0, const double *part, int NC) {
+// One workgroup per instance: fold its chunks, open or close its step, and
+// carry the instance's bounds of the border-cell path as k_dt_final does
+// (view->pec_parts: the partials come from the fused PEC pass).
+__global__ void kb_dt_final(BInst *inst, const int *chunk0, const double *part, int NC, const Ctrl *view) {
     __shared__ double lds[4 * 16];
     BInst &I = inst[blockIdx.x];
     const int c0 = chunk0[blockIdx.x], c1 = chunk0[blockIdx.x + 1];
@@ -75,6 +79,9 @@ __global__ void kb_dt_final(BInst *inst, const int *chunk0, const double *part, 
     }
     block_fold<4>(v, kind, lds);
     if (threadIdx.x != 0) return;
+    const int pec = view->pec_parts;
+    const double dt0 = I.dt, V = I.vbound, A0 = I.abound;
+    I.disp = I.vbound = I.abound = INFINITY;
     const double mn = v[0], mx = v[1], cnt = v[2], bad = v[3];
     const unsigned halt = SPH_FLAG_CAPACITY | SPH_FLAG_NONFINITE | SPH_FLAG_NO_FLUID | SPH_FLAG_ZERO_DENSITY;
     if ((I.flags & halt) || !(I.t < I.duration)) {
@@ -104,6 +111,16 @@ __global__ void kb_dt_final(BInst *inst, const int *chunk0, const double *part, 
     if (I.advance_t) I.t = I.t + dt;
     I.step = I.step + 1;
     I.active = 1;
+    if (bad == 0) {
+        const double A1 = sqrt(mx);
+        I.abound = A1;
+        if (pec) {
+            const double V1 = V + dt0 * A1;
+            const double d = dt0 * (V + 0.5 * dt0 * A0) + 0.5 * dt0 * V + 0.5 * dt * V1;
+            if (d == d) I.disp = d;
+            I.vbound = V1;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -121,14 +138,22 @@ __device__ inline void bupd_pos(const State &a, int i, bool pred, double dt, dou
     }
 }
 
-__global__ void kb_bounds(const int *cs, const int *chunkInst, const BInst *inst, State a, int pec, double *part) {
-    __shared__ double lds[kBoundsQ * 16];
+// Per chunk of 2048 slots (one instance's): the bounds' partials. With PEC's
+// predictor the full scan also finds the largest |vx|, |vy| (kb_grid keeps it
+// as the instance's vbound). With cellStart (the fused PEC step) and an
+// instance whose carried bound allows it (as the single engine's k_bounds),
+// the instance's chunks share the walk of its border cells instead.
+__global__ void kb_bounds(const int *cs, const int *chunkInst, const int *chunk0, BInst *inst, State a, int pec,
+                          double *part, const int *cellStart, Ctrl *view) {
+    __shared__ double lds[kBoundsVQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
-    const BInst &I = inst[chunkInst[c]];
+    if (c == 0 && threadIdx.x == 0) view->pec_parts = 0;  // kb_dt_final has read it
+    const int k = chunkInst[c];
+    BInst &I = inst[k];
     const bool pred = pec && I.active;
     double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY, hmn = INFINITY, hmx = -INFINITY;
-    double bad = 0.0;
-    for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x) {
+    double bad = 0.0, vmx = -INFINITY;
+    auto take = [&](int i) {
         double xi, yi;
         bupd_pos(a, i, pred, I.dt, xi, yi);
         const double hi = I.uh_on ? I.uh : a.h[i];
@@ -139,30 +164,47 @@ __global__ void kb_bounds(const int *cs, const int *chunkInst, const BInst *inst
         ymx = yi > ymx ? yi : ymx;
         hmn = hi < hmn ? hi : hmn;
         hmx = hi > hmx ? hi : hmx;
+    };
+    if (cellStart && pred && I.uh_on && border_margin_ok(I.grid, I.disp)) {
+        const int j = c - chunk0[k], nck = chunk0[k + 1] - chunk0[k];
+        if (j == 0 && threadIdx.x == 0) I.border_updates += 1;
+        hmn = hmx = I.uh;
+        border_walk(cellStart, I.cellBase, I.grid, j * (int)blockDim.x + (int)threadIdx.x, nck * (int)blockDim.x,
+                    take);
+    } else {
+        for (int i = cs[c] + threadIdx.x; i < cs[c + 1]; i += blockDim.x) {
+            take(i);
+            if (pred) {
+                const double ux = fabs(a.vx[i]), uy = fabs(a.vy[i]);
+                const double w = (isfinite(ux) && isfinite(uy)) ? (ux > uy ? ux : uy) : INFINITY;
+                vmx = w > vmx ? w : vmx;
+            }
+        }
     }
-    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
-    double v[kBoundsQ] = {xmn, xmx, ymn, ymx, hmn, hmx, bad};
-    block_partials<kBoundsQ>(v, kind, lds, part, NC, c);
+    const int kind[kBoundsVQ] = {0, 1, 0, 1, 0, 1, 2, 1};
+    double v[kBoundsVQ] = {xmn, xmx, ymn, ymx, hmn, hmx, bad, vmx};
+    block_partials<kBoundsVQ>(v, kind, lds, part, NC, c);
 }
 
 // One workgroup per instance: its grid (_get_cell_size :103-110,
 // _get_number_of_cells :112-127). A finished or halted instance gets
 // ncells = 0 (its particles park in its one bin).
 __global__ void kb_grid(BInst *inst, const int *chunk0, const double *part, int NC) {
-    __shared__ double lds[kBoundsQ * 16];
+    __shared__ double lds[kBoundsVQ * 16];
     BInst &I = inst[blockIdx.x];
     const int c0 = chunk0[blockIdx.x], c1 = chunk0[blockIdx.x + 1];
-    const int kind[kBoundsQ] = {0, 1, 0, 1, 0, 1, 2};
-    double v[kBoundsQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
+    const int kind[kBoundsVQ] = {0, 1, 0, 1, 0, 1, 2, 1};
+    double v[kBoundsVQ] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0, -INFINITY};
     for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
 #pragma unroll
-        for (int q = 0; q < kBoundsQ; ++q) {
+        for (int q = 0; q < kBoundsVQ; ++q) {
             const double u = part[q * NC + c];
             v[q] = kind[q] == 0 ? (u < v[q] ? u : v[q]) : kind[q] == 1 ? (u > v[q] ? u : v[q]) : v[q] + u;
         }
     }
-    block_fold<kBoundsQ>(v, kind, lds);
+    block_fold<kBoundsVQ>(v, kind, lds);
     if (threadIdx.x != 0) return;
+    if (I.active && v[7] >= 0.0) I.vbound = v[7];  // a full scan's velocity bound
     Grid g = I.grid;
     g.ncells = 0;
     if (I.active) {
@@ -363,9 +405,10 @@ __global__ void kb_hm_range(const BInst *inst, State st, double *out) {
 // PEC's start-of-step values come from the buffer before the reorder, at the
 // particle's previous slot perm[j] (the lean reorder did not copy them).
 __global__ void kb_correct(const int *cs, const int *chunkInst, const BInst *inst, State s, State o,
-                           const int *perm, int euler, double *part) {
+                           const int *perm, int euler, double *part, Ctrl *view) {
     __shared__ double lds[kDtQ * 16];
     const int c = blockIdx.x, NC = gridDim.x;
+    if (c == 0 && threadIdx.x == 0) view->pec_parts = 0;
     const BInst &I = inst[chunkInst[c]];
     const double dt = I.dt, damping = I.damping;
     DtVars v;
@@ -477,6 +520,7 @@ struct sph_batch {
     int pair_threads = 256;
     int list_len = 0;  // 0: the single-simulation pass's length
     int persistent = 0;    // the pair pass as a persistent work-queue kernel
+    int border_bounds = 1;  // the fused PEC step's bounds from the border cells when the carried bound allows
     // every instance has one h for all its particles and one fluid mass and
     // allows the uniform pass (found at upload): records kRecUniform(64) and
     // the uniform pair pass, each instance's constants in uniK
@@ -526,8 +570,8 @@ void size_queue(sph_batch *b) {
 
 int b_dt(sph_batch *b, hipStream_t s) {
     if (!b->dt_fresh)
-        kb_dt_partial<<<b->NT, kBlock, 0, s>>>(b->ts, b->tileInst, b->inst, b->buf[b->cur], b->part_d);
-    kb_dt_final<<<b->K, kBlock, 0, s>>>(b->inst, b->tile0, b->part_d, b->NT);
+        kb_dt_partial<<<b->NT, kBlock, 0, s>>>(b->ts, b->tileInst, b->inst, b->buf[b->cur], b->part_d, b->view);
+    kb_dt_final<<<b->K, kBlock, 0, s>>>(b->inst, b->tile0, b->part_d, b->NT, b->view);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -535,7 +579,10 @@ int b_dt(sph_batch *b, hipStream_t s) {
 int b_nn(sph_batch *b, hipStream_t s) {
     const int n = b->n, pec = b->integrator == SPH_INTEGRATOR_PEC;
     State a = b->buf[b->cur], c = b->buf[1 - b->cur];
-    kb_bounds<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->inst, a, pec, b->part_b);
+    // the border-cell bounds only in the fused PEC step's update
+    const bool fused = b->uni && !b->persistent && pec;
+    kb_bounds<<<b->NC, kBlock, 0, s>>>(b->cs, b->chunkInst, b->chunk0, b->inst, a, pec, b->part_b,
+                                       fused && b->border_bounds ? b->cellStart : nullptr, b->view);
     kb_grid<<<b->K, kBlock, 0, s>>>(b->inst, b->chunk0, b->part_b, b->NC);
     kb_bases<<<1, 1024, 0, s>>>(b->inst, b->K, b->view, b->capBins);
     kb_keys<<<nblk_b(n), kBlock, 0, s>>>(b->inst, b->instOf, a, n, pec, b->counts, b->bin, b->rank, b->binflag,
@@ -594,7 +641,7 @@ int b_pair_pec(sph_batch *b, hipStream_t s) {
     State st = b->buf[b->cur];
     PairIn in{b->recA, b->recB, st.h, b->cxy_s, b->cellStart};
     PairOut out{st.ax, st.ay, st.drho, nullptr, 0};
-    const PecArgs pec{st, b->buf[1 - b->cur], b->perm, b->part_d, HmConst{}, 0};
+    const PecArgs pec{st, b->buf[1 - b->cur], b->perm, b->part_d, HmConst{}, 0, &b->view->pec_parts};
     hipError_t e = b->precision == SPH_PREC_FP32C
                        ? launch_pair_batch_fp32u_pec(b->inst, b->ts, b->tileInst, b->NT, in, out, b->uniK, b->kernel,
                                                      b->n, pec, s)
@@ -615,7 +662,7 @@ int b_step(sph_batch *b, hipStream_t s) {
     } else {
         if ((rc = b_pair(b, s))) return rc;
         kb_correct<<<b->NT, kBlock, 0, s>>>(b->ts, b->tileInst, b->inst, b->buf[b->cur], b->buf[1 - b->cur],
-                                            b->perm, b->integrator == SPH_INTEGRATOR_EULER, b->part_d);
+                                            b->perm, b->integrator == SPH_INTEGRATOR_EULER, b->part_d, b->view);
         HIPCHK(hipGetLastError());
     }
     if (b->timing && (rc = b_mark(b, s))) return rc;
@@ -684,6 +731,7 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
         }
         BInst &I = hI[k];
         std::memset(&I, 0, sizeof I);
+        I.disp = I.vbound = I.abound = INFINITY;
         I.off = (int)off;
         I.n = (int)P.n;
         I.active = 1;
@@ -764,7 +812,7 @@ int sph_batch_create(const sph_params_t *params, int32_t ninst, sph_batch_t **ou
          hipMalloc(&b->cellStart, sizeof(int) * (capTot + 2)) == hipSuccess &&
          hipMalloc(&b->tileSums, sizeof(int) * (tiles + 1)) == hipSuccess &&
          hipMalloc(&b->binflag, sizeof(int) * (capTot + 1)) == hipSuccess &&
-         hipMalloc(&b->part_b, sizeof(double) * kBoundsQ * b->NC) == hipSuccess &&
+         hipMalloc(&b->part_b, sizeof(double) * kBoundsVQ * b->NC) == hipSuccess &&
          hipMalloc(&b->part_d, sizeof(double) * kDtQ * b->NT) == hipSuccess &&
          hipMalloc(&b->recA, ra * n) == hipSuccess && hipMalloc(&b->recB, rb * n) == hipSuccess &&
          hipMalloc(&b->queue, sizeof(int) * kQueueInts) == hipSuccess &&
@@ -829,6 +877,7 @@ int sph_batch_upload(sph_batch_t *b, const sph_state_t *st, void *stream) {
         I.dt = I.dt_c = I.dt_f = 0.0;
         I.flags = 0;
         I.active = 1;
+        I.disp = I.vbound = I.abound = INFINITY;
     }
     HIPCHK(hipMemcpy(b->inst, hI.data(), sizeof(BInst) * b->K, hipMemcpyHostToDevice));
     // the uniform pass when every instance has one h and one fluid mass and
@@ -999,6 +1048,23 @@ int sph_batch_set_persistent(sph_batch_t *b, int32_t on) {
         for (auto &g : b->gexec)
             if (g) hipGraphExecDestroy(g), g = nullptr;
     }
+    return 0;
+}
+
+int sph_batch_set_border_bounds(sph_batch_t *b, int32_t on) {
+    if (!b) return fail(SPH_EINVAL, "batch is NULL");
+    b->border_bounds = on ? 1 : 0;
+    for (auto &g : b->gexec)
+        if (g) hipGraphExecDestroy(g), g = nullptr;
+    return 0;
+}
+
+int sph_batch_border_count(sph_batch_t *b, int64_t *counts, void *stream) {
+    if (!b || !counts) return fail(SPH_EINVAL, "NULL argument");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    std::vector<BInst> hI(b->K);
+    HIPCHK(hipMemcpy(hI.data(), b->inst, sizeof(BInst) * b->K, hipMemcpyDeviceToHost));
+    for (int k = 0; k < b->K; ++k) counts[k] = hI[k].border_updates;
     return 0;
 }
 

@@ -393,6 +393,39 @@ __device__ inline int2 cell_of(double x, double y, const Grid &g) {
     return make_int2((int)(long long)fx, (int)(long long)fy);
 }
 
+// The border-cell bounds (after a fused PEC step; see border_bounds_ok in
+// the engine): the margin test, 2 disp under the cell size by far more than
+// the rounding of positions and cell arithmetic (false for +inf and NaN), on
+// a grid of at least 3 x 3 cells.
+__device__ inline bool border_margin_ok(const Grid &g, double disp) {
+    if (g.ncx < 3 || g.ncy < 3) return false;
+    const double slack = 1e-12 * (fabs(g.xmin) + fabs(g.xmax) + fabs(g.ymin) + fabs(g.ymax));
+    return 2.0 * disp * (1.0 + 1e-6) + slack < g.cs * (1.0 - 1e-6);
+}
+
+// The slots of the border cells of grid g (bins from cb, offsets in
+// cellStart), shared by nthreads threads (tid in [0, nthreads), nthreads a
+// multiple of 64): rows 0, ncy - 2, ncy - 1 and the out-of-grid bin, one
+// slot range, thread-strided (it also holds what a flat key carries past the
+// last row); and for each row r <= ncy - 3 the run of columns ncx - 2,
+// ncx - 1 of row r and column 0 of row r + 1 (consecutive keys: one slot
+// range, one wavefront; it also holds what a flat key carries past the last
+// column into the next row). take(slot) for each.
+template <class F>
+__device__ inline void border_walk(const int *cellStart, long long cb, const Grid &g, int tid, int nthreads,
+                                   F &&take) {
+    const long long ncx = g.ncx, ncy = g.ncy;
+    const int a0 = cellStart[cb], a1 = cellStart[cb + ncx];
+    const int b0 = cellStart[cb + (ncy - 2) * ncx], b1 = cellStart[cb + g.ncells + 1];
+    const int la = a1 - a0, lab = la + (b1 - b0);
+    for (int j = tid; j < lab; j += nthreads) take(j < la ? a0 + j : b0 + (j - la));
+    const int lane = tid & 63;
+    for (long long r = tid >> 6; r <= ncy - 3; r += nthreads >> 6) {
+        const int s0 = cellStart[cb + r * ncx + ncx - 2], s1 = cellStart[cb + (r + 1) * ncx + 1];
+        for (int i = s0 + lane; i < s1; i += 64) take(i);
+    }
+}
+
 // Tait pressure + Pb (src/Equations/TaitEOS.py:6-31; WCSPH.py:131-149).
 __device__ inline double tait_p(double rho, int8_t label, double gamma, double B, double rho0, double Pb) {
     double p = 0.0;

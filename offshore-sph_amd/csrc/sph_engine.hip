@@ -331,10 +331,7 @@ __device__ inline void upd_pos(const Ctrl *ctrl, const State &a, int i, int pred
 // ncx - 1; y likewise with rows 0, ncy - 2 and ncy - 1.
 __device__ inline bool border_bounds_ok(const Ctrl *ctrl, const int *cellStart, int pred, const HmConst &hc) {
     if (!cellStart || pred != 1 || !hc.on || ctrl->nm || !ctrl->nn_valid) return false;
-    const Grid &g = ctrl->grid;
-    if (g.ncx < 3 || g.ncy < 3) return false;
-    const double slack = 1e-12 * (fabs(g.xmin) + fabs(g.xmax) + fabs(g.ymin) + fabs(g.ymax));
-    return 2.0 * ctrl->disp * (1.0 + 1e-6) + slack < g.cs * (1.0 - 1e-6);  // false for +inf and NaN
+    return border_margin_ok(ctrl->grid, ctrl->disp);
 }
 
 // _init :86-101 bounds and _get_cell_size :103-110 min h, over all active
@@ -343,14 +340,9 @@ __device__ inline bool border_bounds_ok(const Ctrl *ctrl, const int *cellStart, 
 // their loads issued together (one workgroup per 256 particles measured 40 %
 // slower, the per-workgroup fold paid in more rounds; 2048 workgroups with
 // one particle per round 3 % slower). With cellStart (the fused PEC step)
-// and border_bounds_ok, only the particles of the border cells: rows 0,
-// ncy - 2 and ncy - 1 and the out-of-grid bin (which also holds what a flat
-// key carries past the last row), one slot range, grid-stride; and for each
-// row r <= ncy - 3 the run of columns ncx - 2, ncx - 1 of row r and column 0
-// of row r + 1 (consecutive keys: one slot range, one wavefront; it also
-// holds what a flat key carries past the last column into the next row),
-// over the first kBorderBlocks workgroups.
-// The extremes are the same values the full scan finds. With PEC's
+// and border_bounds_ok, only the particles of the border cells
+// (border_walk), over the first kBorderBlocks workgroups. The extremes are
+// the same values the full scan finds. With PEC's
 // predictor the full scan also finds the largest |vx|, |vy| (k_grid keeps it
 // as ctrl->vbound; +inf if one is not finite).
 constexpr int kBoundsBlocks = 1024;
@@ -389,17 +381,7 @@ __global__ void k_bounds(Ctrl *ctrl, State a, int n, int pred, double *part, HmC
         const int fstride = (kBorderBlocks < (int)gridDim.x ? kBorderBlocks : (int)gridDim.x) * blockDim.x;
         if (gt == 0) ctrl->border_updates += 1;
         hmn = hmx = hc.h;  // every particle's h
-        const Grid &g = ctrl->grid;
-        const long long ncx = g.ncx, ncy = g.ncy;
-        const int a0 = cellStart[0], a1 = cellStart[ncx];
-        const int b0 = cellStart[(ncy - 2) * ncx], b1 = cellStart[g.ncells + 1];
-        const int la = a1 - a0, lab = la + (b1 - b0);
-        for (int j = gt; j < lab; j += fstride) take(j < la ? a0 + j : b0 + (j - la));
-        const int lane = threadIdx.x & 63;
-        for (long long r = gt >> 6; r <= ncy - 3; r += fstride >> 6) {
-            const int s0 = cellStart[r * ncx + ncx - 2], s1 = cellStart[(r + 1) * ncx + 1];
-            for (int i = s0 + lane; i < s1; i += 64) take(i);
-        }
+        border_walk(cellStart, 0, ctrl->grid, gt, fstride, take);
     } else {
         for (int i0 = gt; i0 < n; i0 += 4 * stride) {
 #pragma unroll

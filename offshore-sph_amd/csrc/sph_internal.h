@@ -111,6 +111,7 @@ struct PecArgs {
     double *part;
     HmConst hc;
     int strict;
+    int *mark = nullptr;  // when set, the pass stores 1 there (its partials come from a corrector that moved the fluid)
 };
 
 // The equations of a pair pass: all of them (continuity, momentum, wall
@@ -256,6 +257,9 @@ struct BInst {
     // h then stay where the upload put them (slot = id) and are not permuted
     int uh_on;
     double uh;
+    // the bounds of the border-cell path, as Ctrl's disp, vbound, abound
+    double disp, vbound, abound;
+    long long border_updates;
 };
 
 __host__ __device__ inline Phys inst_phys(const BInst &I) {

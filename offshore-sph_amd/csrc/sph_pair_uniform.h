@@ -617,6 +617,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform_pec(const BInst *ins
         }
     }
     v.store(C.part, NC, c, lds);
+    if (c == 0 && threadIdx.x == 0 && C.mark) *C.mark = 1;
 }
 
 }  // namespace

@@ -407,6 +407,7 @@ __global__ __launch_bounds__(256) void k_pair_batch_uniform64_pec(const BInst *i
         }
     }
     v.store(C.part, NC, c, lds);
+    if (c == 0 && threadIdx.x == 0 && C.mark) *C.mark = 1;
 }
 
 }  // namespace

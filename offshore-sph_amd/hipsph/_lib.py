@@ -113,6 +113,8 @@ _PROTOS = {
     'sph_batch_set_tuning': (C.c_int, [_P, _I32]),
     'sph_batch_set_list_len': (C.c_int, [_P, _I32]),
     'sph_batch_set_persistent': (C.c_int, [_P, _I32]),
+    'sph_batch_set_border_bounds': (C.c_int, [_P, _I32]),
+    'sph_batch_border_count': (C.c_int, [_P, C.POINTER(_I64), _P]),
     'sph_batch_timing_enable': (C.c_int, [_P, _I32]),
     'sph_batch_timing_read': (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), _I32]),
     'sph_momentum': (C.c_int, [_P, _P]),

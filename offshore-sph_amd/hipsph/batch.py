@@ -106,6 +106,18 @@ class BatchEngine:
         if persistent is not None:
             call('sph_batch_set_persistent', self._b, int(bool(persistent)))
 
+    def set_border_bounds(self, on):
+        """Whether each instance's fused PEC step takes the grid's bounds
+        from its border cells when its carried displacement bound allows (the
+        default) or scans all its particles; the grids are the same."""
+        call('sph_batch_set_border_bounds', self._b, int(bool(on)))
+
+    def border_count(self):
+        """Per instance: neighbour updates that took the border-cell bounds."""
+        v = (C.c_int64 * self.K)()
+        call('sph_batch_border_count', self._b, v, self._stream())
+        return [int(x) for x in v]
+
     def timing_enable(self, on=True):
         call('sph_batch_timing_enable', self._b, int(bool(on)))
 

@@ -115,6 +115,46 @@ def test_batch_gaussian_matches_single_engines(precision, persistent):
             np.testing.assert_array_equal(got[k][fld], want[fld], err_msg=f'{k} {fld}')
 
 
+@pytest.mark.parametrize('precision', ['fp64', 'fp32c'])
+def test_batch_border_bounds_match_full_scan(precision):
+    """Each instance's neighbour update after a fused PEC step takes the grid's
+    bounds from its border cells (the default) or scans all its particles:
+    the same states and clocks after 40 graph-replayed steps, one instance
+    stopping after about 10 (its updates end there); every update but each
+    instance's first takes the border path."""
+    mem = _members()
+    sc1, a1 = mem[1]
+    _, probe, _ = _single(sc1, a1, 10, precision)
+    durations = [np.inf, probe['t'] * 0.999] + [np.inf] * (len(mem) - 2)
+    out = []
+    for border in (True, False):
+        ps = []
+        for (sc, alpha), dur in zip(mem, durations):
+            p = params_for(sc, alpha, precision=precision)
+            p.duration = dur
+            ps.append(p)
+        b = BatchEngine(ps)
+        b.set_border_bounds(border)
+        b.upload([soa_from_particles(sc.pA) for sc, _ in mem])
+        assert b.pair_form() == precision + '-uniform'
+        b.step(40, graph=True)
+        got = b.download()
+        st = b.status()
+        cnt = b.border_count()
+        b.close()
+        say('batch border bounds', precision, border, cnt)
+        assert [x['steps'] for x in st] == [40, 10] + [40] * (len(mem) - 2)
+        if border:
+            assert cnt == [39, 9] + [39] * (len(mem) - 2), cnt
+        else:
+            assert cnt == [0] * len(mem)
+        out.append((got, st))
+    for k in range(len(mem)):
+        assert out[0][1][k] == out[1][1][k]
+        for fld in FIELDS + ('p',):
+            np.testing.assert_array_equal(out[0][0][k][fld], out[1][0][k][fld], err_msg=f'{k} {fld}')
+
+
 def test_batch_duration_per_instance():
     mem = _members()[:3]
     # instance 1 stops after about 10 steps' time
