@@ -354,8 +354,8 @@ def test_border_bounds_match_full_scan(precision, speed):
     the border). Then a fluid particle is moved past the right wall by the
     host (sph_state_scatter): the next update scans everything again, and
     the two engines still agree after 10 more steps. With the particle sent
-    off at 5000 m/s (dt v about five times the cell size) the bound refuses
-    the border path on every one of those steps."""
+    off at 5000 m/s (dt v more than a cell) the bound refuses the border
+    path on every one of those steps."""
     sc = scenarios.dam_break_c2()
     m = sc.method()
     fluid = np.flatnonzero(sc.pA['label'] == 0)
