@@ -355,7 +355,8 @@ def test_border_bounds_match_full_scan(precision, speed):
     host (sph_state_scatter): the next update scans everything again, and
     the two engines still agree after 10 more steps. With the particle sent
     off at 5000 m/s (dt v more than a cell) the bound refuses the border
-    path on every one of those steps."""
+    path on every one of those steps. A host predictor call (sph_predict)
+    also sends the next update to the full scan."""
     sc = scenarios.dam_break_c2()
     m = sc.method()
     fluid = np.flatnonzero(sc.pA['label'] == 0)
@@ -378,18 +379,22 @@ def test_border_bounds_match_full_scan(precision, speed):
         n2 = e.border_count()
         e.step(10, graph=True)
         n3 = e.border_count()
+        e.predict()  # moves every fluid particle: the next update scans everything
+        e.step(3, graph=True)
+        n4 = e.border_count()
         st = e.status()
-        assert st['steps'] == 52 and st['flags'] & 3 == 0
+        assert st['steps'] == 55 and st['flags'] & 3 == 0
         d2 = e.download()
         g2 = e.grid()
-        hist = e.dt_history(52)
+        hist = e.dt_history(55)
         out.append((d, g1, d2, g2, hist))
         e.close()
         if border:
-            assert (n1, n2, n3) == (40, 40, 50 if speed == 0.0 else 40), (n1, n2, n3)
+            want = (40, 40, 50, 52) if speed == 0.0 else (40, 40, 40, 40)
+            assert (n1, n2, n3, n4) == want, (n1, n2, n3, n4)
             assert g2['xmax'] >= xmax + 0.29
         else:
-            assert (n1, n2, n3) == (0, 0, 0)
+            assert (n1, n2, n3, n4) == (0, 0, 0, 0)
     assert out[0][1] == out[1][1] and out[0][3] == out[1][3]
     np.testing.assert_array_equal(out[0][4], out[1][4])
     for k in FIELDS:
