@@ -76,7 +76,9 @@ def run_gpu(sc, precision, steps, warmup, timing_steps=50, host_reps=0):
     st = e.status()
     grid = e.grid()
     res = {'elapsed_s': t1 - t0, 'ms_per_step': (t1 - t0) / steps * 1e3, 'status': st, 'grid': grid,
-           'pair_form': e.pair_form()}
+           'pair_form': e.pair_form(),
+           # neighbour updates so far (one per step) and those whose bounds came from the border cells
+           'updates': st['steps'], 'border_bounds_updates': e.border_count()}
     # per-stage times, outside the timed region
     e.timing_enable(True)
     torch.cuda.synchronize()
@@ -897,6 +899,7 @@ def main():
         'stage_ms': {('pair_and_corrector' if fused32 else 'pair'): r32['pair_ms'],
                      'neighbour_update': r32['nn_ms'],
                      'rest': r32['eager_events_ms_per_step'] - r32['pair_ms'] - r32['nn_ms']},
+        'neighbour_updates': {'steps': r32['updates'], 'border_bounds': r32['border_bounds_updates']},
         'eager_events_ms_per_step': r32['eager_events_ms_per_step'],
         'pcie_inclusive': ({'ms_per_step': r32['host_roundtrip_ms_per_step'],
                             'particle_steps_per_s': nf / (r32['host_roundtrip_ms_per_step'] * 1e-3),
