@@ -103,7 +103,8 @@ __global__ void k_dt_final(Ctrl *ctrl, double gamma_c, double gamma_f, double *h
     double v[kDtQ] = {INFINITY, -INFINITY, 0.0, 0.0};
     // eight partials per thread per round, their loads issued together (the
     // fused pair pass leaves one per workgroup: ~5,000 at 1M particles, one
-    // round; four per round: 6.64 against 6.38 us)
+    // round; four per round: 6.64 against 6.38 us; 512 threads in two
+    // rounds 6.85, 256 threads with sixteen per round 12.0)
     constexpr int U = 8;
     for (int k0 = threadIdx.x; k0 < R; k0 += U * blockDim.x) {
         double u[U][kDtQ];
@@ -444,8 +445,11 @@ __device__ bool derive_grid(Ctrl *ctrl, double scale, long long cap, const doubl
 }
 
 // Fold the R bounds partials and derive the grid; a velocity bound that the
-// scan found (>= 0) becomes ctrl->vbound. One workgroup; every load, the
-// active flag's included, is issued before the fold.
+// scan found (>= 0) becomes ctrl->vbound. One workgroup of kGridThreads
+// (256: 6.6 us on the 1M dam break against 7.9 with 1,024, four partials per
+// thread); every load, the active flag's included, is issued before the
+// fold.
+constexpr int kGridThreads = 256;
 __global__ void k_grid(Ctrl *ctrl, const double *part, int R, double scale, long long cap) {
     const int active = ctrl->active;
     __shared__ double lds[kBoundsVQ * 16];
@@ -1397,7 +1401,7 @@ static int nn_update(sph_ctx *c, hipStream_t s, int pred = 0, int lean = 0) {
     // the border-cell bounds only in the step's update with PEC's predictor
     const int *border = c->border_bounds && pred == 1 ? c->cellStart : nullptr;
     k_bounds<<<R, kBlock, 0, s>>>(c->ctrl, a, n, pred, c->part_b, hm_const(c), border);
-    k_grid<<<1, 1024, 0, s>>>(c->ctrl, c->part_b, R, c->P.scale, c->cap_cells);
+    k_grid<<<1, kGridThreads, 0, s>>>(c->ctrl, c->part_b, R, c->P.scale, c->cap_cells);
     k_keys<<<nblk(n), kBlock, 0, s>>>(c->ctrl, a, n, pred, c->counts, c->bin, c->rank, c->binflag);
     long long tiles_cap = (c->cap_cells + 1 + kScanTile - 1) / kScanTile;
     int tiles = (int)(tiles_cap < 1024 ? tiles_cap : 1024);
@@ -2290,7 +2294,7 @@ extern "C" int sph_loop_noalloc(const sph_params_t *params, const sph_state_t *s
     k_na_init<<<nblk(ninit), kBlock, 0, s>>>(ctrl, counts, binflag, id, nn, kNaCells);
     const int R = redblk(n) < kBoundsBlocks ? redblk(n) : kBoundsBlocks;
     k_bounds<<<R, kBlock, 0, s>>>(ctrl, a, nn, 0, part, HmConst{}, nullptr);
-    k_grid<<<1, 1024, 0, s>>>(ctrl, part, R, params->scale, kNaCells);
+    k_grid<<<1, kGridThreads, 0, s>>>(ctrl, part, R, params->scale, kNaCells);
     k_keys<<<nblk(n), kBlock, 0, s>>>(ctrl, a, nn, 0, counts, bin, rank, binflag);
     const int tiles = (int)((kNaCells + 1 + kScanTile - 1) / kScanTile);
     k_scan_reduce<<<tiles, kBlock, 0, s>>>(ctrl, counts, tileSums);
